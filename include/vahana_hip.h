@@ -1,0 +1,256 @@
+/*
+ * vahana_hip.h — C ABI of the MI355X (gfx950) execution backend for
+ * Vahana.jl's transition step (`apply!`).
+ *
+ * Every entry point returns an int status (VH_OK == 0, negative on error);
+ * the message of the last error on the calling thread is returned by
+ * vh_last_error().  Handles are opaque.  Host pointers passed in are
+ * borrowed for the duration of the call (the library copies what it keeps);
+ * output buffers are owned by the caller and carry an explicit capacity.
+ *
+ * Reference paths below are relative to the Vahana.jl repository root.
+ */
+#ifndef VAHANA_HIP_H
+#define VAHANA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define VH_OK               0
+#define VH_ERR_ASSERT      -1  /* reference: AssertionError (@assert / @mayassert) */
+#define VH_ERR_UNREGISTERED -2 /* reference: error() for unregistered types/functions */
+#define VH_ERR_OOM         -3  /* device allocation failed */
+#define VH_ERR_BUDGET      -4  /* HBM budget exceeded (serving admission) */
+#define VH_ERR_DEVICE      -5  /* HIP runtime error */
+#define VH_ERR_CAPACITY    -6  /* caller buffer too small / fixed capacity exceeded */
+#define VH_ERR_ARG         -7  /* invalid argument */
+
+/* ------------------------------------------------------------ AgentID codec
+ * src/Agent.jl:30-118: id = type << 56 | rank << 36 | nr, nr 1-based.
+ * This backend runs one rank, so rank == 0 and ids are deterministic. */
+#define VH_BITS_TYPE     8
+#define VH_BITS_PROCESS  20
+#define VH_BITS_AGENTNR  36
+#define VH_SHIFT_TYPE    (VH_BITS_PROCESS + VH_BITS_AGENTNR)
+#define VH_NR_MASK       ((((uint64_t)1) << VH_BITS_AGENTNR) - 1)
+
+#define VH_MAX_AGENT_TYPES 8
+#define VH_MAX_EDGE_TYPES  24
+#define VH_MAX_CALL        8
+
+/* ----------------------------------------------------------------- hints
+ * Agent hints: src/ModelTypes.jl:94-101.  Edge hints: src/ModelTypes.jl:165-228
+ * (NumEdgesOnly and HasEdgeOnly expand to their components, SingleType is
+ * implied by a target type, SingleType+SingleEdge needs Stateless+IgnoreFrom). */
+#define VH_HINT_IMMORTAL          (1u << 0)
+#define VH_HINT_INDEPENDENT       (1u << 1)
+
+#define VH_HINT_STATELESS         (1u << 0)
+#define VH_HINT_IGNOREFROM        (1u << 1)
+#define VH_HINT_SINGLEEDGE        (1u << 2)
+#define VH_HINT_SINGLETYPE        (1u << 3)
+#define VH_HINT_IGNORESOURCESTATE (1u << 4)
+#define VH_HINT_NUMEDGESONLY      (1u << 5)
+#define VH_HINT_HASEDGEONLY       (1u << 6)
+
+/* ----------------------------------------------------- registered transitions
+ * The reference's transition function is an arbitrary Julia closure
+ * (src/Simulation.jl:720-730).  Here each transition body is a HIP device
+ * function compiled into the library and selected by id; role slots bind the
+ * body's edge/agent type parameters to the model's registered types. */
+enum vh_transition {
+    VH_TR_NOOP = 0,          /* return the state unchanged                        */
+    VH_TR_SUM_NEIGHBORS = 1, /* foo = sum(n.foo for n in neighborstates_flexible)  */
+    VH_TR_DIE_IF = 2,        /* return nothing if predicate(state) else state      */
+    VH_TR_ADD_SELF_EDGE = 3, /* add_edge!(sim, id, id, E(params))                  */
+    VH_TR_READD_EDGES = 4,   /* add_edges!(sim, id, edges(sim, id, E))              */
+    VH_TR_NEIGHBOR_FOO_EDGES = 5, /* test/independent.jl:55-68 body                */
+    VH_TR_SPAWN = 6,         /* add_agent!(T(foo + k)) (+ edge newborn -> id)      */
+    VH_TR_COUNT_EDGES = 7,   /* foo = num_edges(sim, id, E)                         */
+    VH_TR_SIR = 16,          /* SIR contagion (Stateless contact edges)            */
+    VH_TR_OPINION_F64 = 17,  /* weighted bounded confidence, Float64 opinions      */
+    VH_TR_OPINION_F32 = 18,  /* same, Float32 opinions                              */
+    VH_TR_CONTACT = 19,      /* dynamic contact network with deaths/births         */
+    VH_TR_COUNT_ = 20
+};
+
+/* predicates of VH_TR_DIE_IF (applied to the Int64 field at offset 0) */
+enum vh_cmp {
+    VH_CMP_ALWAYS = 0, VH_CMP_NEVER = 1, VH_CMP_LT = 2, VH_CMP_GE = 3,
+    VH_CMP_EQ = 4, VH_CMP_ODD = 5, VH_CMP_EVEN = 6, VH_CMP_NUM_EDGES_EQ = 7
+};
+
+typedef struct vh_test_params {      /* params of the test bodies (ids 0-7) */
+    int64_t value;                   /* comparison value / spawn offset     */
+    int32_t cmp;                     /* enum vh_cmp                          */
+    uint32_t pad;
+    uint8_t edge_state[16];          /* edge state bytes for ADD_SELF_EDGE  */
+} vh_test_params;
+
+typedef struct vh_sir_params {       /* SIR: state {UInt8 health, UInt16 days} */
+    double p_inf;                    /* per infected in-neighbour, per step    */
+    uint64_t seed;
+    uint32_t recovery;               /* days until I -> R                      */
+    uint32_t pad;
+} vh_sir_params;
+
+typedef struct vh_opinion_params {   /* state Float64 (or Float32), edge Float32 */
+    double eps;
+} vh_opinion_params;
+
+typedef struct vh_contact_params {   /* state {UInt8 health, Float32 susceptibility} */
+    double beta;                     /* infection prob. scale per infected contact  */
+    double gamma;                    /* recovery probability per step               */
+    double p_death;                  /* death (and 1:1 birth) probability per step  */
+    uint64_t seed;
+    uint32_t n_contacts;             /* partners drawn per agent and step           */
+    uint32_t max_redraw;             /* redraws per partner when the draw is dead   */
+} vh_contact_params;
+
+/* model-state enums shared by SIR and contact */
+#define VH_S 0
+#define VH_I 1
+#define VH_R 2
+
+/* ------------------------------------------------------------- apply! desc
+ * apply!(sim, func, call, read, write; add_existing, with_edge)
+ * src/Simulation.jl:720-821.  Type lists are bit masks over registered type
+ * ids (agent type ids are 1-based like src/ModelTypes.jl:86; bit i <=> id i),
+ * except `call`, whose order fixes the side-effect order. */
+typedef struct vh_apply_desc {
+    uint32_t transition;             /* enum vh_transition                    */
+    uint32_t n_call;
+    uint8_t  call[VH_MAX_CALL];      /* agent type ids in call order           */
+    uint32_t read_agents, read_edges;    /* masks                              */
+    uint32_t write_agents, write_edges;
+    uint32_t add_existing_agents, add_existing_edges;
+    int32_t  with_edge;              /* edge type id or -1                     */
+    int8_t   role_agent[4];          /* body's agent-type parameters (-1 = unused) */
+    int8_t   role_edge[4];           /* body's edge-type parameters  (-1 = unused) */
+    const void* params;              /* body parameter blob (copied)           */
+    uint32_t params_bytes;
+    uint32_t pad;
+} vh_apply_desc;
+
+/* ------------------------------------------------------------- mapreduce
+ * src/AgentMethods.jl:533-565 (agents), src/EdgeMethods.jl:972-1001 (edges),
+ * init values of val4empty src/Helpers.jl:44-81. */
+enum vh_op { VH_OP_ADD = 0, VH_OP_MUL = 1, VH_OP_MAX = 2, VH_OP_MIN = 3,
+             VH_OP_AND = 4, VH_OP_OR = 5 };
+enum vh_dtype { VH_DT_I64 = 0, VH_DT_F64 = 1, VH_DT_BOOL = 2, VH_DT_F32 = 3,
+                VH_DT_U8 = 4, VH_DT_U16 = 5, VH_DT_U32 = 6, VH_DT_I32 = 7,
+                VH_DT_U64 = 8 };
+enum vh_map_kind { VH_MAP_FIELD = 0,      /* f(x) = x.field                     */
+                   VH_MAP_FIELD_EQ = 1,   /* f(x) = x.field == value (Bool/Int) */
+                   VH_MAP_ONE = 2 };      /* f(x) = 1                           */
+typedef struct vh_map_desc {
+    uint32_t kind;                   /* enum vh_map_kind                      */
+    uint32_t offset;                 /* byte offset of the field in the state */
+    uint32_t field_dtype;            /* enum vh_dtype of the field            */
+    uint32_t pad;
+    int64_t  value;                  /* comparison value for FIELD_EQ         */
+} vh_map_desc;
+
+typedef struct vh_model* vh_model_t;
+typedef struct vh_sim*   vh_sim_t;
+
+/* ------------------------------------------------------------- model */
+/* ModelTypes() src/ModelTypes.jl:1-74 */
+int vh_model_create(vh_model_t* out);
+int vh_model_destroy(vh_model_t m);
+/* register_agenttype! src/ModelTypes.jl:81-108 */
+int vh_register_agenttype(vh_model_t m, uint32_t hints, uint32_t state_bytes,
+                          uint8_t* type_id);
+/* register_edgetype! src/ModelTypes.jl:158-230 (target_type < 0: none) */
+int vh_register_edgetype(vh_model_t m, uint32_t hints, uint32_t state_bytes,
+                         int32_t target_type, int64_t size, uint8_t* edge_type_id);
+/* resolved hints after normalisation (src/ModelTypes.jl:181-205) */
+int vh_edgetype_hints(vh_model_t m, uint8_t edge_type_id, uint32_t* hints);
+
+/* ------------------------------------------------------------- simulation */
+/* create_simulation src/Simulation.jl:261-339 (allocates nothing on the
+ * device until finish_init!) */
+int vh_sim_create(vh_model_t m, vh_sim_t* out);
+/* finish_simulation! src/Simulation.jl:551-585 */
+int vh_sim_destroy(vh_sim_t s);
+/* copy_simulation src/Simulation.jl:500-510 (device deep copy) */
+int vh_sim_copy(vh_sim_t s, vh_sim_t* out);
+/* add_agent! / add_agents! during initialisation
+ * src/AgentMethods.jl:65-89, src/Agent.jl:174-180 */
+int vh_add_agents(vh_sim_t s, uint8_t type_id, const void* states, uint64_t n,
+                  uint64_t* ids_out);
+/* add_edge! / add_edges! during initialisation, in call order
+ * src/EdgeMethods.jl:388-523, src/Edge.jl:81-97 (states may be NULL for
+ * stateless types) */
+int vh_add_edges(vh_sim_t s, uint8_t edge_type_id, const uint64_t* from,
+                 const uint64_t* to, const void* states, uint64_t n);
+/* finish_init! src/Simulation.jl:403-476: upload and build the CSR */
+int vh_finish_init(vh_sim_t s);
+/* apply! src/Simulation.jl:720-821, enqueued on the simulation's stream */
+int vh_apply(vh_sim_t s, const vh_apply_desc* desc);
+/* apply! repeated `steps` times with the same desc (one host call) */
+int vh_apply_n(vh_sim_t s, const vh_apply_desc* desc, uint32_t steps);
+/* wait for the simulation's stream */
+int vh_sync(vh_sim_t s);
+/* attach an external hipStream_t (NULL: the library's own stream) */
+int vh_set_stream(vh_sim_t s, void* hip_stream);
+/* sim.num_transitions (src/Simulation.jl:473,816) */
+int vh_num_transitions(vh_sim_t s, uint64_t* out);
+/* mapreduce(sim, f, op, T; init, datatype) — kind 0: agents, 1: edges */
+int vh_mapreduce(vh_sim_t s, int kind, uint8_t type_id, const vh_map_desc* f,
+                 int op, int dtype, const void* init, void* out);
+/* num_agents src/Agent.jl:324-343 */
+int vh_num_agents(vh_sim_t s, uint8_t type_id, uint64_t* out);
+/* num_edges(sim, T) src/Edge.jl:373-389 */
+int vh_num_edges(vh_sim_t s, uint8_t edge_type_id, uint64_t* out);
+/* raw slot view of read state + died flags (all_agents src/Agent.jl:234-260
+ * filters it); *nslots receives length(readstate) */
+int vh_read_agents(vh_sim_t s, uint8_t type_id, void* states, uint8_t* died,
+                   uint64_t cap, uint64_t* nslots);
+/* every edge as (to, from, state) in container order (all_edges
+ * src/EdgeMethods.jl:1004-1027; order of src/EdgeIterator.jl:45-106 with
+ * ascending (type, nr) target keys); from/states may be NULL */
+int vh_read_edges(vh_sim_t s, uint8_t edge_type_id, uint64_t* to,
+                  uint64_t* from, void* states, uint64_t cap, uint64_t* n);
+/* edges / neighborids / edgestates / num_edges of one target outside a
+ * transition (src/EdgeMethods.jl:704-892) */
+int vh_edges_of(vh_sim_t s, uint8_t edge_type_id, uint64_t to, uint64_t* from,
+                void* states, uint64_t cap, uint64_t* n);
+/* agentstate outside a transition src/AgentMethods.jl:91-154 */
+int vh_agentstate(vh_sim_t s, uint64_t id, uint8_t type_id, void* out);
+/* bytes of device memory held by the simulation */
+int vh_device_bytes(vh_sim_t s, uint64_t* out);
+/* last error message on this thread */
+const char* vh_last_error(void);
+/* enable_asserts src/Vahana.jl:42-48: with asserts on (the default) every
+ * apply! waits for its kernels and reports device-side assertion flags
+ * (permissions, removed agents, capacity) as VH_ERR_ASSERT / VH_ERR_CAPACITY;
+ * with asserts off apply! stays asynchronous unless it has births or deaths */
+int vh_enable_asserts(int enable);
+
+/* ------------------------------------------------------ graph generators
+ * Host-side sources for add_graph! (src/GraphsSupport.jl:34-57) with a
+ * counter-hash stream so both the device path and any checker receive the
+ * same edge list.  Pairs are 1-based agent numbers.
+ * Directed G(n, p): for each target ascending, sources ascending. */
+int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from,
+                uint32_t* to, uint64_t cap, uint64_t* m);
+/* Barabasi-Albert, m0 = m clique, undirected pairs in creation order
+ * (count = m(m-1)/2 + (n-m)m) */
+int vh_graph_ba(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b,
+                uint64_t cap, uint64_t* count);
+
+/* ------------------------------------------------------ counter-hash RNG
+ * SplitMix64 finaliser chained over (seed, a, b, c); the same function is
+ * compiled into the device bodies. */
+uint64_t vh_hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VAHANA_HIP_H */

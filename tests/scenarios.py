@@ -1,0 +1,452 @@
+"""The reference's known-answer tests, restated against the Python API.
+
+Every scenario takes an API module `V` (the product package, or the checker
+binding in oracle_api) and returns its observations; tests compare them with
+the answers the reference's own test files assert and with each other.
+"""
+
+
+def _T(V):
+    class NS:
+        pass
+    t = NS()
+    # test/core.jl:8-36
+    t.AMortal = V.struct("AMortal", [("foo", "<i8")])
+    t.AMortalFixed = V.struct("AMortalFixed", [("foo", "<i8")])
+    t.AImm = V.struct("AImm", [("foo", "<i8")])
+    t.AImmFixed = V.struct("AImmFixed", [("foo", "<i8")])
+    t.AImmFixedOversize = V.struct("AImmFixedOversize", [("foo", "<i8")])
+    t.ADefault = V.struct("ADefault", [("foo", "<i8"), ("bool", "?")])
+    t.ESDict = V.struct("ESDict", [("foo", "<i8")])
+    t.ESLDict1 = V.struct("ESLDict1")
+    t.ESLDict2 = V.struct("ESLDict2")
+    return t
+
+
+def core_model(V, t):
+    return (V.ModelTypes()
+            | V.register_agenttype_(t.AMortal)
+            | V.register_agenttype_(t.AMortalFixed)
+            | V.register_agenttype_(t.AImm, "Immortal")
+            | V.register_agenttype_(t.AImmFixed, "Immortal")
+            | V.register_agenttype_(t.AImmFixedOversize, "Immortal")
+            | V.register_agenttype_(t.ADefault)
+            | V.register_edgetype_(t.ESDict)
+            | V.register_edgetype_(t.ESLDict1, "Stateless")
+            | V.register_edgetype_(t.ESLDict2, "Stateless", "SingleType", target=t.AImm)
+            | V.create_model("Test Core"))
+
+
+def add_example_network(V, sim, t, extra=()):
+    """test/core.jl:38-69 (+ edges the transition testset adds right after
+    finish_init!, appended here at the end of the initialisation)."""
+    a1 = V.add_agent_(sim, t.AMortal(1))
+    a2, a3 = V.add_agents_(sim, t.AMortal(2), t.AMortal(3))
+    avids = V.add_agents_(sim, [t.AImm(i) for i in range(1, 11)])
+    avfids = V.add_agents_(sim, [t.AImmFixed(i) for i in range(1, 11)])
+    V.add_agents_(sim, [t.AImmFixedOversize(i) for i in range(1, 11)])
+    V.add_agents_(sim, [t.AMortalFixed(i) for i in range(1, 11)])
+    V.add_agents_(sim, [t.ADefault(i, True) for i in range(1, 11)])
+    V.add_edge_(sim, a2, a1, t.ESDict(1))
+    V.add_edge_(sim, a3, a1, t.ESDict(2))
+    V.add_edge_(sim, avids[0], a1, t.ESDict(3))
+    V.add_edge_(sim, avfids[9], a1, t.ESDict(4))
+    V.add_edges_(sim, a1, [V.Edge(avids[i], (t.ESLDict1, None)) for i in range(10)])
+    for i in range(10):
+        V.add_edge_(sim, avfids[i], avids[i], (t.ESLDict2, None))
+    ids = dict(a1=a1, a2=a2, a3=a3, avids=avids, avfids=avfids)
+    for fn in extra:
+        fn(ids)
+    return ids
+
+
+def core_aggregate(V):
+    """test/core.jl:85-107, 396-440: mapreduce over foo = 1..10."""
+    t = _T(V)
+    sim = V.create_simulation(core_model(V, t))
+    add_example_network(V, sim, t)
+    V.finish_init_(sim)
+    out = {}
+    for T in (t.AImmFixed, t.AImmFixedOversize, t.AMortalFixed, t.ADefault):
+        out[T.name] = (V.mapreduce(sim, "foo", "+", T), V.mapreduce(sim, "foo", "*", T),
+                       V.mapreduce(sim, "foo", "&", T, datatype="i8"),
+                       V.mapreduce(sim, "foo", "|", T, datatype="i8"),
+                       V.mapreduce(sim, "foo", "max", T), V.mapreduce(sim, "foo", "min", T))
+    out["bool"] = (V.mapreduce(sim, "bool", "&", t.ADefault), V.mapreduce(sim, "bool", "|", t.ADefault))
+    # test_aggregate_mortal: every agent of the type dies, mapreduce gives init
+    for T in (t.AMortalFixed, t.ADefault):
+        V.apply_(sim, V.transitions.die_if("always"), [T], [T], [T])
+        out["empty_" + T.name] = V.mapreduce(sim, "foo", "+", T)
+        out["n_" + T.name] = V.num_agents(sim, T)
+    return out
+
+
+def core_sum_neighbors(V):
+    """test/core.jl:289-394: synchronous neighbour sums over 2-3 steps."""
+    t = _T(V)
+    allagenttypes = [t.AMortal, t.AImm, t.AImmFixed]
+    out = {}
+
+    def extra1(ids):
+        V.add_edge_(sim, ids["a1"], ids["a1"], (t.ESLDict1, None))
+        V.add_edge_(sim, ids["avids"][0], ids["avids"][0], (t.ESLDict2, None))
+        V.add_edge_(sim, ids["avfids"][0], ids["avfids"][0], (t.ESLDict1, None))
+        V.add_edge_(sim, ids["avfids"][1], ids["avfids"][0], (t.ESLDict1, None))
+        V.add_edge_(sim, ids["avfids"][1], ids["avfids"][1], (t.ESLDict1, None))
+
+    sim = V.create_simulation(core_model(V, t))
+    ids = add_example_network(V, sim, t, [extra1])
+    V.finish_init_(sim)
+    f = V.transitions.sum_neighbors(t.ESLDict1)
+    V.apply_(sim, f, [t.AMortal], allagenttypes + [t.ESLDict1], [t.AMortal])
+    a = [int(V.agentstate(sim, ids["a1"], t.AMortal)["foo"])]
+    V.apply_(sim, f, [t.AMortal], allagenttypes + [t.ESLDict1], [t.AMortal])
+    a.append(int(V.agentstate(sim, ids["a1"], t.AMortal)["foo"]))
+    out["AMortal"] = a
+
+    def extra2(ids):
+        V.add_edge_(sim, ids["avids"][0], ids["avids"][0], (t.ESLDict2, None))
+
+    sim = V.create_simulation(core_model(V, t))
+    ids = add_example_network(V, sim, t, [extra2])
+    V.finish_init_(sim)
+    f = V.transitions.sum_neighbors(t.ESLDict2)
+    b = []
+    for _ in range(3):
+        V.apply_(sim, f, [t.AImm], allagenttypes + [t.ESLDict2], [t.AImm])
+        b.append(int(V.agentstate(sim, ids["avids"][0], t.AImm)["foo"]))
+    out["AImm"] = b
+
+    def extra3(ids):
+        V.add_edge_(sim, ids["avfids"][0], ids["avfids"][0], (t.ESLDict1, None))
+        V.add_edge_(sim, ids["avfids"][1], ids["avfids"][0], (t.ESLDict1, None))
+        V.add_edge_(sim, ids["avfids"][1], ids["avfids"][1], (t.ESLDict1, None))
+
+    sim = V.create_simulation(core_model(V, t))
+    ids = add_example_network(V, sim, t, [extra3])
+    V.finish_init_(sim)
+    f = V.transitions.sum_neighbors(t.ESLDict1)
+    c = []
+    for _ in range(3):
+        V.apply_(sim, f, [t.AImmFixed], allagenttypes + [t.ESLDict1], [t.AImmFixed])
+        c.append(int(V.agentstate(sim, ids["avfids"][0], t.AImmFixed)["foo"]))
+    out["AImmFixed"] = c
+    # a read permission violation (AImmFixed missing) is an AssertionError
+    sim = V.create_simulation(core_model(V, t))
+    add_example_network(V, sim, t)
+    V.finish_init_(sim)
+    try:
+        V.apply_(sim, V.transitions.sum_neighbors(t.ESLDict2), [t.AImm], [t.ESLDict2], [])
+        out["assert_missing_read"] = False
+    except AssertionError:
+        out["assert_missing_read"] = True
+    return out
+
+
+def core_queries(V):
+    """test/core.jl:165-277: accessors and deaths."""
+    t = _T(V)
+    sim = V.create_simulation(core_model(V, t))
+    ids = add_example_network(V, sim, t)
+    V.finish_init_(sim)
+    out = {}
+    out["edges_a1_ESDict"] = len(V.edges(sim, ids["a1"], t.ESDict))
+    out["neighbors_a1_ESLDict1"] = V.neighborids(sim, ids["a1"], t.ESLDict1)
+    out["edges_a2_ESDict"] = V.edges(sim, ids["a2"], t.ESDict)
+    out["neighbors_avids10"] = V.neighborids(sim, ids["avids"][9], t.ESLDict2)
+    out["avids"] = ids["avids"]
+    out["avfids"] = ids["avfids"]
+    out["num_edges"] = (V.num_edges(sim, ids["a1"], t.ESDict), V.num_edges(sim, ids["a2"], t.ESDict),
+                        V.num_edges(sim, ids["avids"][0], t.ESLDict2))
+    out["state_a2"] = int(V.agentstate(sim, ids["a2"], t.AMortal)["foo"])
+    out["state_avfids10"] = int(V.agentstate(sim, ids["avfids"][9], t.AImmFixed)["foo"])
+    try:
+        V.agentstate(sim, ids["a2"], t.AImm)
+        out["assert_wrong_type"] = False
+    except AssertionError:
+        out["assert_wrong_type"] = True
+    c = V.copy_simulation(sim)
+    V.apply_(c, V.transitions.die_if("ge", 6), [t.AMortalFixed], [t.AMortalFixed], [t.AMortalFixed])
+    out["all_agents_fixed"] = sorted(int(a["foo"]) for a in V.all_agents(c, t.AMortalFixed))
+    out["num_fixed"] = V.num_agents(c, t.AMortalFixed)
+    out["num_fixed_orig"] = V.num_agents(sim, t.AMortalFixed)
+    # with_edge test/core.jl:133-157
+    s2 = V.create_simulation(core_model(V, t))
+    aids = [V.add_agent_(s2, t.AMortal(i)) for i in range(1, 101)]
+    V.add_edge_(s2, aids[0], aids[1], (t.ESLDict1, None))
+    V.finish_init_(s2)
+    s3 = V.copy_simulation(s2)
+    V.apply_(s2, V.transitions.die_if("always"), t.AMortal, t.AMortal, t.AMortal, with_edge=t.ESLDict1)
+    V.apply_(s3, V.transitions.die_if("always"), t.AMortal, [], t.AMortal, with_edge=t.ESLDict1)
+    out["with_edge"] = (V.num_agents(s2, t.AMortal), V.num_agents(s3, t.AMortal))
+    out["with_edge_dead"] = aids[1] not in V.all_agentids(s2, t.AMortal)
+    # all_agentids test/core.jl:242-277
+    m2 = V.ModelTypes() | V.register_agenttype_(t.AMortal) | V.create_model("all_agentids_test")
+    s4 = V.create_simulation(m2)
+    for i in range(1, 11):
+        V.add_agent_(s4, t.AMortal(i))
+    V.finish_init_(s4)
+    V.apply_(s4, V.transitions.die_if("odd"), t.AMortal, t.AMortal, t.AMortal)
+    out["all_agentids_even"] = len(V.all_agentids(s4, t.AMortal))
+    return out
+
+
+EDGE_HINTS = {
+    "EdgeD": (), "EdgeS": ("Stateless",), "EdgeE": ("SingleEdge",), "EdgeT": ("SingleType",),
+    "EdgeI": ("IgnoreFrom",), "EdgeSE": ("Stateless", "SingleEdge"),
+    "EdgeST": ("Stateless", "SingleType"), "EdgeSI": ("Stateless", "IgnoreFrom"),
+    "EdgeEI": ("SingleEdge", "IgnoreFrom"), "EdgeTI": ("SingleType", "IgnoreFrom"),
+    "EdgeSEI": ("Stateless", "SingleEdge", "IgnoreFrom"),
+    "EdgeSTI": ("Stateless", "SingleType", "IgnoreFrom"),
+    "EdgeSETI": ("Stateless", "SingleEdge", "SingleType", "IgnoreFrom"),
+    "EdgeTs": ("SingleType",), "EdgeTsI": ("SingleType", "IgnoreFrom"),
+    "EdgeSTs": ("Stateless", "SingleType"), "EdgeSTsI": ("Stateless", "SingleType", "IgnoreFrom"),
+    "EdgeSETsI": ("Stateless", "SingleEdge", "SingleType", "IgnoreFrom"),
+}
+STATELESS = ["EdgeS", "EdgeSE", "EdgeST", "EdgeSI", "EdgeSEI", "EdgeSTI", "EdgeSETI", "EdgeSTs",
+             "EdgeSTsI", "EdgeSETsI"]
+STATEFUL = ["EdgeD", "EdgeE", "EdgeT", "EdgeI", "EdgeEI", "EdgeTI", "EdgeTs", "EdgeTsI"]
+
+
+def hashint(name, h):
+    return h in name[4:]
+
+
+def edges_model(V):
+    """test/edges.jl:12-64"""
+    Agent = V.struct("Agent", [("foo", "<i8")])
+    AgentB = V.struct("AgentB", [("foo", "<i8")])
+    ets = {}
+    mt = V.ModelTypes() | V.register_agenttype_(Agent) | V.register_agenttype_(AgentB)
+    for name, hints in EDGE_HINTS.items():
+        T = V.struct(name) if name in STATELESS else V.struct(name, [("foo", "<i8")])
+        ets[name] = T
+        kw = {}
+        if "SingleType" in hints:
+            kw["target"] = Agent
+            if name.startswith("EdgeSTs") or name.startswith("EdgeTs") or name == "EdgeSETsI":
+                kw["size"] = 10
+        mt = mt | V.register_edgetype_(T, *hints, **kw)
+    return mt | V.create_model("Test Edges"), Agent, AgentB, ets
+
+
+def edges_matrix(V):
+    """test/edges.jl:66-237: every hint combination."""
+    model, Agent, AgentB, ets = edges_model(V)
+    sim = V.create_simulation(model)
+    a1, a2, a3 = V.add_agents_(sim, Agent(1), Agent(2), Agent(3))
+    out = {"ids": (a1, a2, a3), "second_add_asserts": {}}
+
+    def st(name, v):
+        T = ets[name]
+        return (T, None) if name in STATELESS else (T, T(v))
+
+    for name in STATELESS + STATEFUL:
+        V.add_edge_(sim, a2, a3, st(name, 1))
+        multi_assert = hashint(name, "E") and not hashint(name, "T") and \
+            not (hashint(name, "S") and hashint(name, "I"))
+        if multi_assert:
+            try:
+                V.add_edge_(sim, a1, a3, st(name, 2))
+                out["second_add_asserts"][name] = False
+            except AssertionError:
+                out["second_add_asserts"][name] = True
+        elif not hashint(name, "E"):
+            V.add_edge_(sim, a1, a3, st(name, 2))
+        V.add_edge_(sim, a1, V.Edge(a3, st(name, 3)))
+    V.finish_init_(sim)
+    res = {}
+    for name, T in ets.items():
+        r = {}
+        for k, fn in (("edges", V.edges), ("neighborids", V.neighborids),
+                      ("edgestates", V.edgestates), ("has_edge", V.has_edge)):
+            vals = []
+            for a in (a1, a2, a3):
+                try:
+                    v = fn(sim, a, T)
+                    if k == "edges" and v is not None:
+                        v = [(e.from_, None if e.state is None else int(e.state["foo"]))
+                             for e in (v if isinstance(v, list) else [v])]
+                    if k == "edgestates" and v is not None:
+                        v = [int(x["foo"]) for x in (v if isinstance(v, list) else [v])]
+                    if k == "neighborids" and v is not None and not isinstance(v, list):
+                        v = [v]
+                    vals.append(v)
+                except AssertionError:
+                    vals.append("assert")
+            r[k] = vals
+        try:
+            r["num_edges"] = [V.num_edges(sim, a, T) for a in (a1, a2, a3)]
+        except AssertionError:
+            r["num_edges"] = "assert"
+        try:
+            r["mapreduce"] = V.mapreduce(sim, "foo", "+", T)
+        except (AssertionError, KeyError):
+            r["mapreduce"] = "assert"
+        r["total"] = V.num_edges(sim, T)
+        res[name] = r
+    out["res"] = res
+    return out
+
+
+def edges_transition(V):
+    """test/edges.jl:319-389: rebuild versus add_existing on K_n."""
+    out = {}
+    for name in ["EdgeD", "EdgeT", "EdgeI", "EdgeTI"]:
+        model, Agent, AgentB, ets = edges_model(V)
+        ET = ets[name]
+        sim = V.create_simulation(model)
+        n = 4
+        ids = V.add_agents_(sim, [Agent(i) for i in range(1, n + 1)])
+        # add_graph! of complete_graph(n): both directions of every pair
+        for i in range(n):
+            for j in range(i + 1, n):
+                V.add_edge_(sim, ids[i], ids[j], (ET, ET(j + 1)))
+                V.add_edge_(sim, ids[j], ids[i], (ET, ET(i + 1)))
+        V.finish_init_(sim)
+        seq = []
+        cnt = lambda: [V.num_edges(sim, a, ET) for a in ids]  # noqa: E731
+        V.apply_(sim, V.transitions.noop(), [Agent], [ET], [])
+        seq.append(cnt())
+        V.apply_(sim, V.transitions.noop(), [Agent], [], [ET], add_existing=[ET])
+        seq.append(cnt())
+        if name == "EdgeD":
+            V.apply_(sim, V.transitions.readd_edges(ET), [Agent], [ET], [ET])
+            seq.append(cnt())
+            V.apply_(sim, V.transitions.noop(), [Agent], [], [])
+            V.apply_(sim, V.transitions.readd_edges(ET), [Agent], [ET], [ET], add_existing=[ET])
+            seq.append(cnt())
+            V.apply_(sim, V.transitions.noop(), [Agent], [], [ET])
+            seq.append(cnt())
+        out[name] = seq
+    return out
+
+
+def remove_agents(V):
+    """test/remove_agents.jl:5-119"""
+    DAgent = V.struct("DAgent", [("idx", "<i8")])
+    DAgentRemove = V.struct("DAgentRemove")
+    DEdgeState = V.struct("DEdgeState", [("state", "<i8")])
+    DEdge = V.struct("DEdge")
+    DSingleEdge = V.struct("DSingleEdge")
+    DEdgeST = V.struct("DEdgeST")
+
+    def model():
+        return (V.ModelTypes() | V.register_agenttype_(DAgent) | V.register_agenttype_(DAgentRemove)
+                | V.register_edgetype_(DEdge) | V.register_edgetype_(DEdgeState)
+                | V.register_edgetype_(DSingleEdge, "SingleEdge")
+                | V.register_edgetype_(DEdgeST, "SingleType", target=DAgent)
+                | V.create_model("remove_agents"))
+
+    out = {}
+    for E in (DEdge, DEdgeST):
+        for times in (1, 2):
+            sim = V.create_simulation(model())
+            ids = V.add_agents_(sim, [DAgent(i) for i in range(1, 4)])
+            rids = sim._add_agents(DAgentRemove, [DAgentRemove()])
+            for _ in range(times):
+                for i in ids:
+                    V.add_edge_(sim, i, ids[1], (E, None))
+                for r in rids:
+                    V.add_edge_(sim, r, ids[1], (DEdgeState, DEdgeState(0)))
+            for _ in range(times):
+                V.add_edge_(sim, ids[1], ids[2], (E, None))
+            V.finish_init_(sim)
+            seq = [(V.num_edges(sim, E), V.num_edges(sim, DEdgeState))]
+            V.apply_(sim, V.transitions.die_if("always"), [DAgentRemove], [], [DAgentRemove])
+            seq.append((V.num_edges(sim, E), V.num_edges(sim, DEdgeState)))
+            V.apply_(sim, V.transitions.die_if("num_edges_eq", 0, E), [DAgent], [DAgent, E], [DAgent])
+            seq.append((V.num_edges(sim, E), V.num_edges(sim, DEdgeState)))
+            out[(E.name, times)] = seq
+    return out
+
+
+def independent(V):
+    """test/independent.jl:25-222: :Independent, plain and mortal-independent
+    types must give the same counts, including in-transition births/edges."""
+    res = {}
+    for kind, hints in (("indep", ("Independent",)), ("not", ()), ("immortal", ("Independent",))):
+        A = V.struct("A_" + kind, [("foo", "<i8")])
+        AFooEdge = V.struct("AFooEdge", [("foo", "<i8")])
+        AEdge = V.struct("AEdge")
+        model = (V.ModelTypes() | V.register_agenttype_(A, *hints)
+                 | V.register_edgetype_(AFooEdge) | V.register_edgetype_(AEdge)
+                 | V.create_model("Test Independent"))
+        sim = V.create_simulation(model)
+        ids = [V.add_agent_(sim, A(i)) for i in range(1, 11)]
+        for f in ids:
+            for to in ids:
+                V.add_edge_(sim, f, to, (AEdge, None))
+        V.finish_init_(sim)
+        seq = [(V.num_agents(sim, A), V.num_edges(sim, AEdge))]
+        V.apply_(sim, V.transitions.neighbor_foo_edges(AEdge, AFooEdge, 1), A, [A, AEdge], [A, AFooEdge])
+        seq.append((V.num_agents(sim, A), V.num_edges(sim, AEdge), V.num_edges(sim, AFooEdge)))
+        V.apply_(sim, V.transitions.neighbor_foo_edges(AEdge, AFooEdge, 2), A, [A, AEdge], [A, AFooEdge])
+        seq.append((V.num_agents(sim, A), V.num_edges(sim, AEdge), V.num_edges(sim, AFooEdge)))
+        V.apply_(sim, V.transitions.spawn(A, 2, AFooEdge), A, [A, AEdge], [A, AFooEdge],
+                 add_existing=[AFooEdge])
+        seq.append((V.num_agents(sim, A), V.num_edges(sim, AEdge), V.num_edges(sim, AFooEdge)))
+        res[kind] = seq
+    return res
+
+
+def addexisting(V):
+    """test/addexisting.jl:5-111"""
+    Compute = V.struct("ComputeAgent")
+    Constructed = V.struct("ConstructedAgent")
+    Connection = V.struct("Connection")
+    out = {}
+    for variant, ch, kh in (("plain", (), ()), ("imm_comp", ("Immortal",), ())):
+        model = (V.ModelTypes() | V.register_agenttype_(Compute, *ch)
+                 | V.register_agenttype_(Constructed, *kh)
+                 | V.register_edgetype_(Connection, "Stateless") | V.create_model("Test add_existing"))
+        sim = V.create_simulation(model)
+        cid = sim._add_agents(Compute, [Compute()])[0]
+        kid = sim._add_agents(Constructed, [Constructed()])[0]
+        V.add_edge_(sim, kid, cid, (Connection, None))
+        V.finish_init_(sim)
+        seq = []
+        V.apply_(sim, V.transitions.noop(), Compute, [], Connection, add_existing=Connection)
+        seq.append((V.num_agents(sim, Compute), V.num_agents(sim, Constructed), V.num_edges(sim, Connection)))
+        V.apply_(sim, V.transitions.noop(), [Compute], [], [Constructed, Connection])
+        seq.append((V.num_agents(sim, Compute), V.num_agents(sim, Constructed), V.num_edges(sim, Connection)))
+        V.apply_(sim, V.transitions.spawn(Constructed, 1, Connection, both_ways=False), [Compute], [],
+                 [Constructed, Connection])
+        seq.append((V.num_agents(sim, Compute), V.num_agents(sim, Constructed), V.num_edges(sim, Connection)))
+        V.apply_(sim, V.transitions.noop(), [Compute], [], [Constructed, Connection],
+                 add_existing=[Constructed, Connection])
+        seq.append((V.num_agents(sim, Compute), V.num_agents(sim, Constructed), V.num_edges(sim, Connection)))
+        out[variant] = seq
+    # test_assertion: an :Immortal type in write without add_existing
+    model = (V.ModelTypes() | V.register_agenttype_(Compute)
+             | V.register_agenttype_(Constructed, "Immortal")
+             | V.register_edgetype_(Connection, "Stateless") | V.create_model("Test add_existing"))
+    sim = V.create_simulation(model)
+    cid = sim._add_agents(Compute, [Compute()])[0]
+    kid = sim._add_agents(Constructed, [Constructed()])[0]
+    V.add_edge_(sim, kid, cid, (Connection, None))
+    V.finish_init_(sim)
+    try:
+        V.apply_(sim, V.transitions.noop(), [Compute], [], [Constructed, Connection])
+        out["assert"] = False
+    except AssertionError:
+        out["assert"] = True
+    return out
+
+
+def graphs_k4(V):
+    """test/graphs.jl:16-46: sum of neighbour ids on K4."""
+    GraphA = V.struct("GraphA", [("id", "<i8")])
+    GraphE = V.struct("GraphE")
+    model = (V.ModelTypes() | V.register_agenttype_(GraphA) | V.register_edgetype_(GraphE)
+             | V.create_model("Test Graph"))
+    sim = V.create_simulation(model)
+    n = 4
+    ids = V.add_agents_(sim, [GraphA(i) for i in range(1, n + 1)])
+    for i in range(n):
+        for j in range(i + 1, n):
+            V.add_edge_(sim, ids[i], ids[j], (GraphE, None))
+            V.add_edge_(sim, ids[j], ids[i], (GraphE, None))
+    V.finish_init_(sim)
+    V.apply_(sim, V.transitions.sum_neighbors(GraphE), [GraphA], [GraphA, GraphE], [GraphA])
+    return V.mapreduce(sim, "id", "+", GraphA)

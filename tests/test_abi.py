@@ -1,0 +1,116 @@
+"""C ABI surface and host-side helpers (CPU only: nothing here launches a kernel)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vahana_hip.h")
+
+# SplitMix64 reference output for seed 1234567 (Vigna, splitmix64.c)
+SPLITMIX_1234567 = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                    4593380528125082431, 16408922859458223821]
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(vh_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(vh):
+    lib = vh._abi.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib.dll, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header(vh):
+    A = vh._abi
+    assert C.sizeof(A.ApplyDesc) == 72
+    assert C.sizeof(A.MapDesc) == 24
+    assert C.sizeof(A.TestParams) == 32
+    assert C.sizeof(A.SirParams) == 24
+    assert C.sizeof(A.ContactParams) == 40
+
+
+def _mix64(z):
+    M = (1 << 64) - 1
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+def test_splitmix_known_answers():
+    G = 0x9E3779B97F4A7C15
+    s, out = 1234567, []
+    for _ in range(5):
+        s = (s + G) & ((1 << 64) - 1)
+        out.append(_mix64(s))
+    assert out == SPLITMIX_1234567
+
+
+def test_hash_agrees_everywhere(vh, ora):
+    # first stage of vh_hash4 is the first SplitMix64 output of the seed
+    G = 0x9E3779B97F4A7C15
+    M = (1 << 64) - 1
+
+    def h4(s, a, b, c):
+        h = _mix64((s + G) & M)
+        for w in (a, b, c):
+            h = _mix64((h + w + G) & M)
+        return h
+
+    assert _mix64((1234567 + G) & M) == SPLITMIX_1234567[0]
+    rng = np.random.default_rng(3)
+    L, O = vh._abi.lib(), ora.lib()
+    for _ in range(200):
+        s, a, b, c = (int(x) for x in rng.integers(0, 2**63, 4, dtype=np.int64))
+        ref = h4(s, a, b, c)
+        assert L.hash4(s, a, b, c) == ref
+        assert O.hash4(s, a, b, c) == ref
+        assert int(vh.inputs.hash4(s, a, b, c)) == ref
+    # the vectorised form
+    a = np.arange(1000, dtype=np.uint64)
+    got = vh.inputs.hash4(7, a, 3, 5)
+    assert all(int(got[i]) == h4(7, i, 3, 5) for i in range(0, 1000, 97))
+
+
+def test_uniform_conversions(vh):
+    assert vh.inputs.u01(np.uint64(0)) == 0.0
+    assert vh.inputs.u01(np.uint64(2**64 - 1)) == 1.0 - 2.0**-53
+    w = vh.inputs.open01f(np.array([0, 2**64 - 1], dtype=np.uint64))
+    assert w[0] == np.float32(2.0**-24) and w[1] == np.float32(1.0 - 2.0**-24)
+
+
+def test_er_generator_properties(vh):
+    n, p = 20000, 10.0 / 19999
+    fr, to = vh.inputs.graph_er(n, p, 5)
+    assert np.all(fr != to)
+    assert np.all(np.diff(to.astype(np.int64)) >= 0)          # targets ascending
+    # sources ascending within a target, no duplicates
+    key = to.astype(np.int64) * (n + 1) + fr
+    assert np.all(np.diff(key) > 0)
+    mean = len(to) / n
+    assert abs(mean - 10.0) < 0.2                              # E[d] = (n-1) p = 10
+    deg = np.bincount(to, minlength=n + 1)[1:]
+    assert abs(deg.var() - 10.0 * (1 - p)) < 0.6               # binomial variance
+    # determinism
+    fr2, to2 = vh.inputs.graph_er(n, p, 5)
+    assert np.array_equal(fr, fr2) and np.array_equal(to, to2)
+
+
+def test_ba_generator_properties(vh):
+    n, m = 5000, 5
+    a, b = vh.inputs.graph_ba(n, m, 11)
+    assert len(a) == m * (m - 1) // 2 + (n - m) * m
+    k0 = m * (m - 1) // 2
+    assert np.all(a[k0:] > b[k0:])                             # new node -> earlier node
+    assert np.all(np.bincount(a[k0:], minlength=n + 1)[m + 1:] == m)
+    pairs = set(zip(a.tolist(), b.tolist()))
+    assert len(pairs) == len(a)                                # no multi-edges
+    deg = np.bincount(np.concatenate([a, b]), minlength=n + 1)[1:]
+    assert deg.min() >= m - 1 and deg.sum() == 2 * len(a)
+    assert deg.max() > 10 * m                                  # heavy tail (hubs)

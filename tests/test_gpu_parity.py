@@ -1,0 +1,106 @@
+"""Parity of the HIP path with the checker (GPU).
+
+Every scenario of the reference's own tests runs through the C ABI on cuda:0
+and through the checker on the CPU; observations must be identical and equal
+the reference's known answers.  The model bodies are compared state-by-state
+(bit-exact, including the Float64/Float32 opinions, whose summation order is
+fixed by the model spec).
+"""
+import numpy as np
+import pytest
+
+import scenarios as SC
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b):
+    return repr(a) == repr(b)
+
+
+@pytest.mark.parametrize("name", ["core_aggregate", "core_sum_neighbors", "core_queries",
+                                  "edges_matrix", "edges_transition", "remove_agents",
+                                  "independent", "addexisting", "graphs_k4"])
+def test_reference_scenarios(vh, ora, gpu_ready, name):
+    fn = getattr(SC, name)
+    print(f"[gpu] scenario {name}", flush=True)
+    got = fn(vh)
+    want = fn(ora)
+    assert _eq(got, want), f"{name}:\n device {got}\n checker {want}"
+
+
+def _agents(m):
+    st, died = m.sim._read_agents(m.T)
+    return np.asarray(st), died
+
+
+def _bytes(st):
+    return np.ascontiguousarray(st).view(np.uint8)
+
+
+def test_sir_parity(vh, ora, gpu_ready):
+    M = vh.models
+    n = 3000
+    edges = vh.inputs.graph_er(n, 8.0 / (n - 1), 21)
+    g = M.build_sir(vh, n, seed=3, n_init=30, edges=edges)
+    c = M.build_sir(ora, n, seed=3, n_init=30, edges=edges)
+    for step in range(25):
+        M.step_sir(vh, g, p_inf=0.15, seed=3)
+        M.step_sir(ora, c, p_inf=0.15, seed=3)
+        a, _ = _agents(g)
+        b, _ = _agents(c)
+        assert np.array_equal(_bytes(a), _bytes(b)), f"step {step}"
+    assert (a["health"] == vh.R).sum() > 0
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_opinion_parity(vh, ora, gpu_ready, f32):
+    M = vh.models
+    n = 4000
+    a, b = vh.inputs.graph_ba(n, 5, 8)
+    edges = vh.inputs.symmetric_weighted(a, b, 9)
+    deg = np.bincount(edges[1], minlength=n + 1)
+    assert deg.max() > 64       # the wave-per-agent path is exercised
+    g = M.build_opinion(vh, n, seed=8, f32=f32, edges=edges)
+    c = M.build_opinion(ora, n, seed=8, f32=f32, edges=edges)
+    for step in range(8):
+        M.step_opinion(vh, g, eps=0.2)
+        M.step_opinion(ora, c, eps=0.2)
+        x, _ = _agents(g)
+        y, _ = _agents(c)
+        assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
+
+
+def test_contact_parity(vh, ora, gpu_ready):
+    M = vh.models
+    n = 3000
+    g = M.build_contact(vh, n, seed=5, n_init=30)
+    c = M.build_contact(ora, n, seed=5, n_init=30)
+    for step in range(10):
+        M.step_contact(vh, g, seed=5, p_death=0.05)
+        M.step_contact(ora, c, seed=5, p_death=0.05)
+        a, da = _agents(g)
+        b, db = _agents(c)
+        assert len(a) == len(b), f"step {step}: slots {len(a)} vs {len(b)}"
+        assert np.array_equal(da, db), f"step {step}: died flags"
+        live = ~da
+        assert np.array_equal(_bytes(a[live]), _bytes(b[live])), f"step {step}: states"
+        tg, fg, _ = g.sim._read_edges(g.E)
+        tc, fc, _ = c.sim._read_edges(c.E)
+        assert np.array_equal(tg, tc) and np.array_equal(fg, fc), f"step {step}: contacts"
+        assert vh.num_agents(g.sim, g.T) == n
+
+
+def test_mapreduce_tree_parity(vh, ora, gpu_ready):
+    M = vh.models
+    n = 20000
+    a, b = vh.inputs.graph_ba(n, 3, 2)
+    edges = vh.inputs.symmetric_weighted(a, b, 4)
+    g = M.build_opinion(vh, n, m_ba=3, seed=2, edges=edges)
+    c = M.build_opinion(ora, n, m_ba=3, seed=2, edges=edges)
+    for op in ("+", "*", "max", "min"):
+        x = vh.mapreduce(g.sim, "opinion", op, g.T)
+        y = ora.mapreduce(c.sim, "opinion", op, c.T)
+        assert x == y or (np.isnan(x) and np.isnan(y)), op
+    xw = vh.mapreduce(g.sim, "w", "+", g.E)
+    assert xw == ora.mapreduce(c.sim, "w", "+", c.E)

@@ -1,0 +1,131 @@
+"""The checker against the reference's own known answers (CPU only).
+
+Each expected value below is the one the reference's test file asserts
+(file:line cited); the scenario code lives in scenarios.py and is shared with
+the GPU parity tests.
+"""
+import numpy as np
+import pytest
+
+import scenarios as SC
+
+
+def test_mapreduce_known_answers(ora):
+    # test/core.jl:87-94: foo = 1..10 with +, *, & (Int), | (Int), max, min
+    out = SC.core_aggregate(ora)
+    expect = (55, 3628800, 0, 15, 10, 1)
+    for T in ("AImmFixed", "AImmFixedOversize", "AMortalFixed", "ADefault"):
+        assert out[T] == expect, T
+    # test/core.jl:421-422: all Bool fields true
+    assert out["bool"] == (True, True)
+    # test/core.jl:96-107: no agent left -> the init value, no assertion
+    assert out["empty_AMortalFixed"] == 0 and out["n_AMortalFixed"] == 0
+    assert out["empty_ADefault"] == 0 and out["n_ADefault"] == 0
+
+
+def test_synchronous_neighbor_sums(ora):
+    out = SC.core_sum_neighbors(ora)
+    s = sum(range(1, 11))
+    assert out["AMortal"] == [s + 1, 2 * s + 1]          # test/core.jl:317-330
+    assert out["AImm"] == [2, 3, 4]                       # test/core.jl:340-360
+    assert out["AImmFixed"] == [3, 5, 7]                  # test/core.jl:375-392
+    assert out["assert_missing_read"]                     # test/core.jl:298-301
+
+
+def test_core_queries(ora):
+    out = SC.core_queries(ora)
+    assert out["edges_a1_ESDict"] == 4                    # test/core.jl:195
+    assert out["neighbors_a1_ESLDict1"][0] == out["avids"][0]
+    assert out["neighbors_a1_ESLDict1"][9] == out["avids"][9]   # test/core.jl:208-209
+    assert out["edges_a2_ESDict"] is None                 # test/core.jl:198
+    assert out["neighbors_avids10"] == [out["avfids"][9]]  # test/core.jl:210-211
+    assert out["num_edges"] == (4, 0, 1)                  # test/core.jl:281-283
+    assert out["state_a2"] == 2 and out["state_avfids10"] == 10
+    assert out["assert_wrong_type"]                       # test/core.jl:167
+    assert out["all_agents_fixed"] == [1, 2, 3, 4, 5]     # test/core.jl:229-240
+    assert out["num_fixed"] == 5 and out["num_fixed_orig"] == 10
+    assert out["with_edge"] == (99, 99)                   # test/core.jl:143-155
+    assert out["with_edge_dead"]
+    assert out["all_agentids_even"] == 5                  # test/core.jl:256-259
+
+
+def test_edge_hint_matrix(ora):
+    out = SC.edges_matrix(ora)
+    a1, a2, a3 = out["ids"]
+    res = out["res"]
+    # test/edges.jl:85-97: a second, different edge to a SingleEdge target asserts
+    assert out["second_add_asserts"] == {"EdgeSE": True, "EdgeE": True, "EdgeEI": True}
+    for t in ["EdgeD", "EdgeT", "EdgeTs"]:                # test/edges.jl:126-133
+        assert res[t]["edges"][2] == [(a2, 1), (a1, 2)]
+        assert res[t]["edges"][1] is None
+    assert res["EdgeE"]["edges"][2] == [(a2, 1)]          # test/edges.jl:134-137
+    for t in ["EdgeI", "EdgeEI", "EdgeTI", "EdgeTsI", "EdgeS", "EdgeSE", "EdgeST", "EdgeSI",
+              "EdgeSEI", "EdgeSTI", "EdgeSETI", "EdgeSTs", "EdgeSTsI", "EdgeSETsI"]:
+        assert res[t]["edges"][0] == "assert", t          # test/edges.jl:140-142
+    for t in ["EdgeD", "EdgeT", "EdgeTs", "EdgeS", "EdgeST", "EdgeSTs"]:   # :145-152
+        assert res[t]["neighborids"][2] == [a2, a1]
+        assert res[t]["neighborids"][1] is None
+    for t in ["EdgeE", "EdgeSE"]:
+        assert res[t]["neighborids"][2] == [a2]
+    for t in ["EdgeI", "EdgeTI", "EdgeTsI", "EdgeSI", "EdgeSTI", "EdgeSTsI", "EdgeEI", "EdgeSEI",
+              "EdgeSETI", "EdgeSETsI"]:
+        assert res[t]["neighborids"][0] == "assert", t
+    for t in ["EdgeD", "EdgeT", "EdgeTs", "EdgeI", "EdgeTI", "EdgeTsI"]:   # :166-181
+        assert res[t]["edgestates"][2] == [1, 2]
+        assert res[t]["edgestates"][1] is None
+    for t in ["EdgeE", "EdgeEI"]:
+        assert res[t]["edgestates"][2] == [1]
+    for t in ["EdgeS", "EdgeST", "EdgeSTs", "EdgeSI", "EdgeSTI", "EdgeSTsI", "EdgeSE", "EdgeSEI",
+              "EdgeSETI", "EdgeSETsI"]:
+        assert res[t]["edgestates"][0] == "assert", t
+    for t in ["EdgeD", "EdgeT", "EdgeTs", "EdgeI", "EdgeTI", "EdgeTsI", "EdgeS", "EdgeST",
+              "EdgeSTs", "EdgeSTI", "EdgeSTsI"]:            # :193-199
+        assert res[t]["num_edges"] == [1, 0, 2], t
+    for t in ["EdgeE", "EdgeEI", "EdgeSE", "EdgeSETI", "EdgeSETsI"]:
+        assert res[t]["num_edges"] == "assert", t
+    for t in SC.EDGE_HINTS:                                 # :204-212
+        assert res[t]["has_edge"] == [True, False, True], t
+    for t in ["EdgeD", "EdgeT", "EdgeI", "EdgeTI", "EdgeTs", "EdgeTsI"]:   # :215-220
+        assert res[t]["mapreduce"] == 6, t
+    for t in ["EdgeE", "EdgeEI"]:
+        assert res[t]["mapreduce"] == 4, t
+    for t in ["EdgeS", "EdgeSE", "EdgeST", "EdgeSI", "EdgeSEI", "EdgeSTI", "EdgeSETI", "EdgeSTs",
+              "EdgeSTsI", "EdgeSETsI"]:
+        assert res[t]["mapreduce"] == "assert", t
+
+
+def test_edges_rebuild_vs_add_existing(ora):
+    out = SC.edges_transition(ora)
+    n1 = 3                                                  # complete graph K4: n - 1 in-edges
+    for t in ["EdgeD", "EdgeT", "EdgeI", "EdgeTI"]:        # test/edges.jl:336-349
+        assert out[t][0] == [n1] * 4 and out[t][1] == [n1] * 4, t
+    assert out["EdgeD"][2] == [n1] * 4                      # :352-356
+    assert out["EdgeD"][3] == [2 * n1] * 4                  # :362-370
+    assert out["EdgeD"][4] == [0] * 4                       # :373-378
+
+
+def test_remove_agents(ora):
+    out = SC.remove_agents(ora)
+    for E in ("DEdge", "DEdgeST"):                          # test/remove_agents.jl:37-63
+        assert out[(E, 1)] == [(4, 1), (4, 0), (3, 0)], E
+        assert out[(E, 2)] == [(8, 2), (8, 0), (6, 0)], E   # :82-112
+
+
+def test_independent_equivalence(ora):
+    res = SC.independent(ora)                               # test/independent.jl:77-83,193-198
+    assert res["indep"] == res["not"] == res["immortal"]
+    # first step: agent foo=1 dies, 9 agents each add 2*9 edges (the 2*9 to or
+    # from the dead one are removed with it)
+    assert res["not"][0] == (10, 100)
+    assert res["not"][1][0] == 9
+
+
+def test_addexisting(ora):
+    out = SC.addexisting(ora)                               # test/addexisting.jl:20-63
+    for v in ("plain", "imm_comp"):
+        assert out[v] == [(1, 1, 1), (1, 0, 0), (1, 1, 1), (1, 1, 1)], v
+    assert out["assert"]                                    # :70-81
+
+
+def test_graphs_k4(ora):
+    assert SC.graphs_k4(ora) == sum(range(1, 5)) * 3        # test/graphs.jl:42-43

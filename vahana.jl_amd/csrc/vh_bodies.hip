@@ -1,0 +1,574 @@
+// vh_bodies.hip — transition bodies as HIP device code.
+//
+// Generic bodies run thread-per-slot through k_generic (the agent loop of
+// src/AgentMethods.jl:193-245 with the write policy of :159-181); the model
+// bodies (SIR, weighted bounded confidence, dynamic contacts) have their own
+// kernels over the CSR.
+#include <string.h>
+
+#include "vh_engine.h"
+
+using namespace vh;
+
+namespace {
+
+// ----------------------------------------------------------------- accessors
+// in-edges [b, e) of `to` for edge type et (false: none / not readable)
+__device__ __forceinline__ bool d_row(const DCtx* c, int et, uint64_t to, uint32_t& b, uint32_t& e) {
+    b = e = 0;
+    if (et < 0 || et >= VH_MAXE) {
+        vh_flag(c, VH_DERR_HINT);
+        return false;
+    }
+    const DEdge& E = c->e[et];
+    if (!E.readable) {
+        vh_flag(c, VH_DERR_READ_EDGE);
+        return false;
+    }
+    if (!E.off) return false;
+    uint64_t r = vh_row(E, to);
+    if (r == ~0ull) return false;
+    b = E.off[r];
+    e = E.off[r + 1];
+    return e > b;
+}
+// agentstate / agentstate_flexible (src/AgentMethods.jl:91-123)
+__device__ __forceinline__ const uint8_t* d_state(const DCtx* c, uint64_t id) {
+    uint32_t t = vh_type_of(id);
+    if (t == 0 || t > VH_MAXA) {
+        vh_flag(c, VH_DERR_TARGET);
+        return nullptr;
+    }
+    const DAgent& A = c->a[t];
+    if (!A.readable) {
+        vh_flag(c, VH_DERR_READ_AGENT);
+        return nullptr;
+    }
+    uint64_t nr = vh_nr_of(id);
+    if (nr == 0 || nr > A.nslots) {
+        vh_flag(c, VH_DERR_TARGET);
+        return nullptr;
+    }
+    if (A.mortal && A.died[nr - 1]) {
+        vh_flag(c, VH_DERR_DIED);
+        return nullptr;
+    }
+    return A.rstate + (nr - 1) * A.bytes;
+}
+__device__ __forceinline__ int64_t ld_i64(const uint8_t* p) {
+    int64_t v;
+    memcpy(&v, p, 8);
+    return v;
+}
+__device__ __forceinline__ void st_i64(uint8_t* p, int64_t v) { memcpy(p, &v, 8); }
+
+// add_edge! / add_agent! inside a body: pass 0 counts, pass 1 writes at the
+// position given by the scan of the counts (caller order, then call order)
+struct Emitter {
+    const DCtx* c;
+    uint64_t caller;
+    int pass;
+    uint32_t ne, nb;
+
+    __device__ void edge(uint64_t from, uint64_t to, const void* st) {
+        const DEmitEdge& E = c->ee[0];
+        if (E.et < 0 || !c->e[E.et].writeable) {
+            vh_flag(c, VH_DERR_WRITE);
+            return;
+        }
+        if (pass == 1) {
+            uint64_t p = E.offs[caller] + ne;
+            E.to[p] = to;
+            E.from[p] = from;
+            for (uint32_t k = 0; k < E.bytes; k++) E.st[p * E.bytes + k] = ((const uint8_t*)st)[k];
+        }
+        ne++;
+    }
+    // _get_next_id src/AgentMethods.jl:37-63: LIFO reuse, then fresh slots
+    __device__ uint64_t agent(const uint8_t* st) {
+        const DEmitAgent& B = c->ea[0];
+        if (B.at <= 0 || !c->a[B.at].writeable) {
+            vh_flag(c, VH_DERR_WRITE);
+            return 0;
+        }
+        const DAgent& A = c->a[B.at];
+        uint64_t id = vh_make_id(B.at, 0);
+        if (pass == 1) {
+            uint64_t g = B.offs[caller] + nb;
+            uint64_t slot = g < A.nreuse ? A.reuse[A.nreuse - 1 - g] : A.nextid + (g - A.nreuse);
+            uint8_t* dst = A.wstate + (slot - 1) * A.bytes;
+            for (uint32_t k = 0; k < A.bytes; k++) dst[k] = st[k];
+            id = vh_make_id(B.at, slot);
+        }
+        nb++;
+        return id;
+    }
+    __device__ void finish() {
+        if (pass != 0) return;
+        if (c->ee[0].cnt && ne) c->ee[0].cnt[caller] = ne;
+        if (c->ea[0].cnt && nb) c->ea[0].cnt[caller] = nb;
+    }
+};
+
+// ----------------------------------------------------------------- bodies
+template <int TR>
+struct Body;
+
+template <>
+struct Body<VH_TR_NOOP> {
+    __device__ static bool run(const DCtx*, const uint8_t*, uint64_t, Emitter&, uint8_t*) { return true; }
+};
+// create_sum_state_neighbors test/core.jl:71-83
+template <>
+struct Body<VH_TR_SUM_NEIGHBORS> {
+    __device__ static bool run(const DCtx* c, const uint8_t*, uint64_t id, Emitter&, uint8_t* out) {
+        int et = c->role_e[0];
+        uint32_t b, e;
+        int64_t sum = 0;
+        if (d_row(c, et, id, b, e)) {
+            const DEdge& E = c->e[et];
+            if (VH_HAS(E.hints, VH_HINT_IGNOREFROM)) {
+                vh_flag(c, VH_DERR_HINT);
+                return true;
+            }
+            for (uint32_t k = b; k < e; k++) {
+                const uint8_t* ns = d_state(c, vh_src_id(E, k));
+                if (!ns) return true;
+                sum += ld_i64(ns);
+            }
+        }
+        st_i64(out, sum);
+        return true;
+    }
+};
+__device__ bool pred(const DCtx* c, const uint8_t* st, uint64_t id) {
+    const vh_test_params* p = (const vh_test_params*)c->params;
+    int64_t foo = c->a[vh_type_of(id)].bytes >= 8 ? ld_i64(st) : 0;
+    switch (p->cmp) {
+    case VH_CMP_ALWAYS: return true;
+    case VH_CMP_NEVER: return false;
+    case VH_CMP_LT: return foo < p->value;
+    case VH_CMP_GE: return foo >= p->value;
+    case VH_CMP_EQ: return foo == p->value;
+    case VH_CMP_ODD: return (foo % 2) != 0;
+    case VH_CMP_EVEN: return (foo % 2) == 0;
+    case VH_CMP_NUM_EDGES_EQ: {
+        uint32_t b, e;
+        d_row(c, c->role_e[0], id, b, e);
+        return (int64_t)(e - b) == p->value;
+    }
+    }
+    return false;
+}
+template <>
+struct Body<VH_TR_DIE_IF> {
+    __device__ static bool run(const DCtx* c, const uint8_t* st, uint64_t id, Emitter&, uint8_t*) {
+        return !pred(c, st, id);
+    }
+};
+template <>
+struct Body<VH_TR_ADD_SELF_EDGE> {
+    __device__ static bool run(const DCtx* c, const uint8_t*, uint64_t id, Emitter& em, uint8_t*) {
+        const vh_test_params* p = (const vh_test_params*)c->params;
+        em.edge(id, id, p->edge_state);
+        return true;
+    }
+};
+// add_edges!(sim, id, edges(sim, id, ET)) test/edges.jl:357-359
+template <>
+struct Body<VH_TR_READD_EDGES> {
+    __device__ static bool run(const DCtx* c, const uint8_t*, uint64_t id, Emitter& em, uint8_t*) {
+        int et = c->role_e[0];
+        uint32_t b, e;
+        if (!d_row(c, et, id, b, e)) return true;
+        const DEdge& E = c->e[et];
+        for (uint32_t k = b; k < e; k++) {
+            uint64_t from = VH_HAS(E.hints, VH_HINT_IGNOREFROM) ? 0 : vh_src_id(E, k);
+            em.edge(from, id, E.st + (uint64_t)k * E.bytes);
+        }
+        return true;
+    }
+};
+// test/independent.jl:55-68
+template <>
+struct Body<VH_TR_NEIGHBOR_FOO_EDGES> {
+    __device__ static bool run(const DCtx* c, const uint8_t* st, uint64_t id, Emitter& em, uint8_t*) {
+        const vh_test_params* p = (const vh_test_params*)c->params;
+        int64_t foo = ld_i64(st);
+        if (foo == p->value) return false;
+        int et = c->role_e[0];
+        uint32_t b, e;
+        if (d_row(c, et, id, b, e)) {
+            const DEdge& E = c->e[et];
+            uint8_t es[8];
+            st_i64(es, foo);
+            for (uint32_t k = b; k < e; k++) {
+                uint64_t nid = vh_src_id(E, k);
+                em.edge(nid, id, es);
+                em.edge(id, nid, es);
+            }
+        }
+        return true;
+    }
+};
+// test/independent.jl:165-176 (value births of T(foo + 10j)); cmp selects
+// the edges: 0 both directions with the newborn id as state (independent.jl),
+// 1 newborn -> caller only (test/addexisting.jl:41-44)
+template <>
+struct Body<VH_TR_SPAWN> {
+    __device__ static bool run(const DCtx* c, const uint8_t* st, uint64_t id, Emitter& em, uint8_t*) {
+        const vh_test_params* p = (const vh_test_params*)c->params;
+        int at = c->role_a[0];
+        uint8_t ns[VH_MAX_STATE];
+        uint32_t bytes = at > 0 ? c->a[at].bytes : 0u;
+        uint32_t cbytes = c->a[vh_type_of(id)].bytes;
+        for (int64_t j = 1; j <= p->value; j++) {
+            for (uint32_t k = 0; k < bytes; k++) ns[k] = k < cbytes ? st[k] : 0;
+            if (bytes >= 8 && cbytes >= 8) st_i64(ns, ld_i64(st) + 10 * j);
+            uint64_t nid = em.agent(ns);
+            if (c->role_e[0] >= 0) {
+                uint8_t es[8];
+                st_i64(es, (int64_t)nid);
+                em.edge(nid, id, es);
+                if (p->cmp == 0) em.edge(id, nid, es);
+            }
+        }
+        return true;
+    }
+};
+template <>
+struct Body<VH_TR_COUNT_EDGES> {
+    __device__ static bool run(const DCtx* c, const uint8_t*, uint64_t id, Emitter&, uint8_t* out) {
+        uint32_t b, e;
+        d_row(c, c->role_e[0], id, b, e);
+        st_i64(out, (int64_t)(e - b));
+        return true;
+    }
+};
+
+template <int TR>
+__global__ __launch_bounds__(256) void k_generic(const DCtx* __restrict__ c, int ci, int t, int pass,
+                                                 int with_edge) {
+    uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const DAgent& A = c->a[t];
+    if (idx > A.nslots) return;
+    if (A.mortal && A.died[idx - 1]) return;
+    const uint64_t id = vh_make_id(t, idx);
+    const uint32_t bytes = A.bytes;
+    const uint8_t* rs = A.rstate + (idx - 1) * bytes;
+    if (with_edge >= 0) {
+        // transition_with_read_with_edge! src/AgentMethods.jl:209-228: only
+        // targets of with_edge are called; the others keep their state
+        const DEdge& E = c->e[with_edge];
+        uint64_t r = vh_row(E, id);
+        bool hasedge = E.off && r != ~0ull && E.off[r + 1] > E.off[r];
+        if (!hasedge) {
+            if (pass == 1 && A.writeable && !A.indep) {
+                uint8_t* ws = A.wstate + (idx - 1) * bytes;
+                for (uint32_t k = 0; k < bytes; k++) ws[k] = rs[k];
+            }
+            return;
+        }
+    }
+    uint8_t st[VH_MAX_STATE], out[VH_MAX_STATE];
+    for (uint32_t k = 0; k < bytes; k++) st[k] = out[k] = rs[k];
+    Emitter em{c, c->cbase[ci] + idx - 1, pass, 0u, 0u};
+    bool keep = Body<TR>::run(c, st, id, em, out);
+    if (pass == 0) {
+        em.finish();
+        return;
+    }
+    if (!A.writeable) return;
+    if (!keep) {
+        if (!A.mortal) vh_flag(c, VH_DERR_IMMORTAL);
+        else A.died_now[idx - 1] = 1;
+        return;
+    }
+    uint8_t* ws = A.wstate + (idx - 1) * bytes;
+    for (uint32_t k = 0; k < bytes; k++) ws[k] = out[k];
+}
+
+// ----------------------------------------------------------------- SIR
+// state word: health byte 0, pad byte 1, days bytes 2-3
+__global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
+                                             const uint32_t* __restrict__ src,
+                                             const uint32_t* __restrict__ rs,
+                                             uint32_t* __restrict__ ws, uint64_t n, uint64_t nrows,
+                                             uint64_t rowbase, uint32_t t, double p, uint64_t seed,
+                                             uint32_t rec, uint64_t step) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = rs[i];
+    uint32_t h = s & 0xFFu;
+    if (h == VH_S) {
+        if (i < nrows) {
+            const uint64_t id = vh_make_id(t, i + 1);
+            uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
+            for (uint32_t k = b; k < e; k++) {
+                uint32_t j = src[k];
+                if ((rs[j - 1] & 0xFFu) == VH_I && vh_u01(vh_hash(seed, id, step, k - b + 1)) < p) {
+                    s = (s & 0xFF00u) | VH_I;
+                    break;
+                }
+            }
+        }
+    } else if (h == VH_I) {
+        uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
+        s = (s & 0xFF00u) | (days << 16) | (days >= rec ? VH_R : VH_I);
+    }
+    ws[i] = s;
+}
+
+// ----------------------------------------------------------------- opinion
+// Canonical order: degree <= 64 sequential; degree > 64 lane-strided
+// partials (lane l: k = l, l+64, ...) and an xor butterfly.
+template <class T>
+__global__ __launch_bounds__(256) void k_opinion_light(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ src,
+                                                       const T* __restrict__ x,
+                                                       const float* __restrict__ w,
+                                                       T* __restrict__ xo, uint64_t n,
+                                                       uint64_t nrows, uint64_t rowbase, T eps) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T xi = x[i];
+    T res = xi;
+    if (i < nrows) {
+        uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
+        if (e - b > 64u) return;   // wave-per-agent kernel
+        T num = (T)0, den = (T)0;
+        for (uint32_t k = b; k < e; k++) {
+            T xj = x[src[k] - 1];
+            T wk = (T)w[k];
+            if (fabs(xj - xi) <= eps) {
+                num = num + wk * xj;
+                den = den + wk;
+            }
+        }
+        if (den > (T)0) res = num / den;
+    }
+    xo[i] = res;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_opinion_heavy(const uint32_t* __restrict__ heavy,
+                                                       uint64_t nheavy,
+                                                       const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ src,
+                                                       const T* __restrict__ x,
+                                                       const float* __restrict__ w,
+                                                       T* __restrict__ xo, uint64_t rowbase, T eps) {
+    uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wv >= nheavy) return;
+    uint64_t r = heavy[wv];
+    uint64_t i = r - rowbase;
+    const T xi = x[i];
+    uint32_t b = off[r], e = off[r + 1];
+    T num = (T)0, den = (T)0;
+    for (uint32_t k = b + lane; k < e; k += 64) {
+        T xj = x[src[k] - 1];
+        T wk = (T)w[k];
+        if (fabs(xj - xi) <= eps) {
+            num = num + wk * xj;
+            den = den + wk;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        num = num + __shfl_xor(num, o, 64);
+        den = den + __shfl_xor(den, o, 64);
+    }
+    if (lane == 0) xo[i] = den > (T)0 ? num / den : xi;
+}
+
+__global__ void k_heavy_rows(const uint32_t* __restrict__ off, uint64_t rows, uint64_t rowbase,
+                             uint64_t nrows, uint32_t* __restrict__ list, uint32_t* __restrict__ cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    uint64_t r = rowbase + i;
+    if (off[r + 1] - off[r] > 64u) list[atomicAdd(cnt, 1u)] = (uint32_t)r;
+}
+
+// ----------------------------------------------------------------- contact
+// state: health byte 0, susceptibility float at bytes 4-7
+__global__ __launch_bounds__(256) void k_contact(
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, const uint64_t* __restrict__ rs,
+    uint64_t* __restrict__ ws, const uint8_t* __restrict__ died, uint8_t* __restrict__ died_now,
+    uint64_t n, uint64_t nrows, uint64_t rowbase, uint32_t t, vh_contact_params p, uint64_t step,
+    uint32_t* __restrict__ cnt_e, uint32_t* __restrict__ log, uint8_t* __restrict__ cnt_b,
+    uint64_t* __restrict__ birth_st) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (died[i]) {
+        cnt_e[i] = 0;
+        cnt_b[i] = 0;
+        return;
+    }
+    const uint64_t id = vh_make_id(t, i + 1);
+    const uint64_t s = rs[i];
+    uint32_t h = (uint32_t)(s & 0xFFu);
+    if (h == VH_S) {
+        if (off && i < nrows) {
+            float susc = __uint_as_float((uint32_t)(s >> 32));
+            double pinf = p.beta * (double)susc;
+            uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
+            for (uint32_t k = b; k < e; k++) {
+                uint32_t j = src[k];
+                if ((rs[j - 1] & 0xFFu) == VH_I &&
+                    vh_u01(vh_hash(p.seed, id, step, k - b + 1)) < pinf) {
+                    h = VH_I;
+                    break;
+                }
+            }
+        }
+    } else if (h == VH_I) {
+        if (vh_u01(vh_hash(p.seed, id, step, VH_KEY_RECOVER)) < p.gamma) h = VH_R;
+    }
+    if (vh_u01(vh_hash(p.seed, id, step, VH_KEY_DEATH)) < p.p_death) {
+        died_now[i] = 1;
+        float ns = 0.5f + 0.5f * vh_u01f(vh_hash(p.seed, id, step, VH_KEY_BIRTH));
+        birth_st[i] = (uint64_t)VH_S | ((uint64_t)__float_as_uint(ns) << 32);
+        cnt_b[i] = 1;
+        cnt_e[i] = 0;
+        return;
+    }
+    cnt_b[i] = 0;
+    const uint64_t S = n;
+    uint32_t ne = 0;
+    for (uint32_t cc = 0; cc < p.n_contacts; cc++) {
+        for (uint32_t r = 0; r < p.max_redraw; r++) {
+            uint64_t hh = vh_hash(p.seed, id, step, VH_KEY_PARTNER + (uint64_t)cc * p.max_redraw + r);
+            uint64_t j = 1 + (((hh >> 32) * S) >> 32);
+            if (j == i + 1 || died[j - 1]) continue;
+            log[i * p.n_contacts + ne++] = (uint32_t)j;
+            break;
+        }
+    }
+    cnt_e[i] = ne;
+    ws[i] = (s & ~(uint64_t)0xFFu) | h;
+}
+
+}  // namespace
+
+// ====================================================================== launchers
+namespace vh {
+
+int launch_generic(int tr, const DCtx* d_ctx, int ci, int call_t, uint64_t nslots, int pass,
+                   int with_edge, hipStream_t st) {
+    if (!nslots) return VH_OK;
+    dim3 g(grid_for(nslots, 256)), b(256);
+    switch (tr) {
+#define CASE(X)                                                                            \
+    case X:                                                                                \
+        hipLaunchKernelGGL(k_generic<X>, g, b, 0, st, d_ctx, ci, call_t, pass, with_edge); \
+        break;
+        CASE(VH_TR_NOOP)
+        CASE(VH_TR_SUM_NEIGHBORS)
+        CASE(VH_TR_DIE_IF)
+        CASE(VH_TR_ADD_SELF_EDGE)
+        CASE(VH_TR_READD_EDGES)
+        CASE(VH_TR_NEIGHBOR_FOO_EDGES)
+        CASE(VH_TR_SPAWN)
+        CASE(VH_TR_COUNT_EDGES)
+#undef CASE
+    default:
+        return set_error(VH_ERR_UNREGISTERED, "transition %d has no generic kernel", tr);
+    }
+    HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+static int check_model_layout(Sim& s, int t, int et, uint32_t abytes, uint32_t ebytes, const char* who) {
+    AgentStore& A = s.a[t];
+    EdgeStore& E = s.e[et];
+    if (A.bytes != abytes) return set_error(VH_ERR_ARG, "%s: agent state must be %u bytes", who, abytes);
+    if (E.bytes != ebytes) return set_error(VH_ERR_ARG, "%s: edge state must be %u bytes", who, ebytes);
+    if (VH_HAS(E.hints, VH_HINT_IGNOREFROM))
+        return set_error(VH_ERR_ARG, "%s: the edge type must keep source ids", who);
+    if (E.rows && E.nedges && (E.src64 || (int)E.src_type != t))
+        return set_error(VH_ERR_ARG, "%s: edges must come from the called agent type only "
+                                     "(one-agent-type model)", who);
+    return VH_OK;
+}
+
+int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step) {
+    RC(check_model_layout(s, t, et, 4, 0, "SIR"));
+    AgentStore& A = s.a[t];
+    EdgeStore& E = s.e[et];
+    if (!A.writeable) return set_error(VH_ERR_ASSERT, "SIR: the agent type must be in `write`");
+    if (!A.nslots) return VH_OK;
+    hipLaunchKernelGGL(k_sir, dim3(grid_for(A.nslots, 256)), dim3(256), 0, s.stream,
+                       E.off.as<uint32_t>(), E.src.as<uint32_t>(), (const uint32_t*)A.rstate(),
+                       (uint32_t*)A.wstate(), A.nslots, E.rows ? E.nrows[t] : 0ull, E.rowbase[t],
+                       (uint32_t)t, p.p_inf, p.seed, p.recovery, step);
+    HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
+    if (E.bins_version == E.version) return VH_OK;
+    uint64_t nrows = E.nrows[t];
+    RC(E.heavy.ensure(nrows * 4 + 8, s.stream));
+    uint32_t* cnt = (uint32_t*)s.d_stats.p;
+    fill_u32(cnt, 0, 1, s.stream);
+    if (nrows)
+        hipLaunchKernelGGL(k_heavy_rows, dim3(grid_for(nrows, 256)), dim3(256), 0, s.stream,
+                           E.off.as<uint32_t>(), E.rows, E.rowbase[t], nrows, E.heavy.as<uint32_t>(), cnt);
+    uint32_t h = 0;
+    HIPCHECK(hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    E.nheavy = h;
+    E.bins_version = E.version;
+    return VH_OK;
+}
+
+int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step) {
+    (void)step;
+    RC(check_model_layout(s, t, et, f32 ? 4 : 8, 4, "opinion"));
+    AgentStore& A = s.a[t];
+    EdgeStore& E = s.e[et];
+    if (!A.writeable) return set_error(VH_ERR_ASSERT, "opinion: the agent type must be in `write`");
+    if (!A.nslots) return VH_OK;
+    uint64_t nrows = E.rows ? E.nrows[t] : 0;
+    if (E.rows) RC(build_heavy_bins(s, E, t));
+    dim3 g(grid_for(A.nslots, 256)), b(256);
+    if (f32) {
+        hipLaunchKernelGGL(k_opinion_light<float>, g, b, 0, s.stream, E.off.as<uint32_t>(),
+                           E.src.as<uint32_t>(), (const float*)A.rstate(), E.st.as<float>(),
+                           (float*)A.wstate(), A.nslots, nrows, E.rowbase[t], (float)p.eps);
+        if (E.nheavy)
+            hipLaunchKernelGGL(k_opinion_heavy<float>, dim3(grid_for(E.nheavy * 64, 256)), b, 0, s.stream,
+                               E.heavy.as<uint32_t>(), E.nheavy, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
+                               (const float*)A.rstate(), E.st.as<float>(), (float*)A.wstate(),
+                               E.rowbase[t], (float)p.eps);
+    } else {
+        hipLaunchKernelGGL(k_opinion_light<double>, g, b, 0, s.stream, E.off.as<uint32_t>(),
+                           E.src.as<uint32_t>(), (const double*)A.rstate(), E.st.as<float>(),
+                           (double*)A.wstate(), A.nslots, nrows, E.rowbase[t], p.eps);
+        if (E.nheavy)
+            hipLaunchKernelGGL(k_opinion_heavy<double>, dim3(grid_for(E.nheavy * 64, 256)), b, 0, s.stream,
+                               E.heavy.as<uint32_t>(), E.nheavy, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
+                               (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(),
+                               E.rowbase[t], p.eps);
+    }
+    HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+int launch_contact(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step, const ContactLogs& L) {
+    RC(check_model_layout(s, t, et, 8, 0, "contact"));
+    AgentStore& A = s.a[t];
+    EdgeStore& E = s.e[et];
+    if (!A.nslots) return VH_OK;
+    if (A.nslots >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "contact: too many slots");
+    hipLaunchKernelGGL(k_contact, dim3(grid_for(A.nslots, 256)), dim3(256), 0, s.stream,
+                       E.rows ? E.off.as<uint32_t>() : nullptr, E.src.as<uint32_t>(),
+                       (const uint64_t*)A.rstate(), (uint64_t*)A.wstate(), A.died.as<uint8_t>(),
+                       A.died_now.as<uint8_t>(), A.nslots, E.rows ? E.nrows[t] : 0ull, E.rowbase[t],
+                       (uint32_t)t, p, step, L.cnt_e, L.log_src, L.cnt_b, (uint64_t*)L.birth_st);
+    HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+}  // namespace vh

@@ -1,0 +1,118 @@
+// vh_engine.h — host-side state of models and simulations (internal).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "vh_common.h"
+#include "vh_prims.h"
+
+namespace vh {
+
+// device buffer with grow-only capacity
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    // ensure capacity for `bytes`; keeps the first `keep` bytes when growing
+    int ensure(size_t bytes, hipStream_t st, size_t keep = 0);
+    void release();
+    template <class T> T* as() const { return (T*)p; }
+};
+
+struct Model {
+    int na = 0, ne = 0;
+    struct AT { uint32_t bytes, hints; } a[VH_MAXA + 1];
+    struct ET { uint32_t bytes, hints; int32_t target; int64_t size; } e[VH_MAXE];
+};
+
+struct AgentStore {
+    uint32_t bytes = 0;
+    bool mortal = true, indep = false;
+    std::vector<uint8_t> h_state;   // init-phase states (slot order)
+    uint64_t nextid = 1;            // next fresh slot (1-based)
+    uint64_t nslots = 0;            // length(readstate)
+    uint64_t cap = 0;               // device slot capacity
+    DevBuf state[2];
+    int rd = 0;                     // index of the read buffer
+    DevBuf died, died_now;          // uint8 per slot
+    DevBuf reuse;                   // LIFO stack of free slots (uint32, bottom first)
+    uint64_t nreuse = 0;
+    bool writeable = false;
+    const uint8_t* rstate() const { return state[rd].as<uint8_t>(); }
+    uint8_t* wstate() const { return indep ? state[rd].as<uint8_t>() : state[rd ^ 1].as<uint8_t>(); }
+};
+
+struct EdgeStore {
+    uint32_t bytes = 0, hints = 0;
+    int32_t target = 0;
+    std::vector<uint64_t> h_from, h_to;   // init-phase edges in add_edge! order
+    std::vector<uint8_t> h_st;
+    uint64_t rowbase[VH_MAXA + 1] = {0}, nrows[VH_MAXA + 1] = {0};
+    uint64_t rows = 0;             // total rows of the CSR (0: never built)
+    uint64_t nedges = 0;
+    DevBuf off, src, st;           // CSR: uint32 offsets[rows+1], sources, states
+    bool src64 = true;
+    uint32_t src_type = 0;
+    uint64_t version = 0;          // bumped on every rebuild
+    // degree bins of the opinion kernels, valid for `bins_version`
+    DevBuf heavy;                  // uint32 rows with degree > 64
+    uint64_t nheavy = 0, bins_version = ~0ull;
+    bool writeable = false;
+};
+
+struct Sim {
+    Model* m = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    AgentStore a[VH_MAXA + 1];
+    EdgeStore e[VH_MAXE];
+    bool initialized = false;
+    uint64_t num_transitions = 0;
+    DCtx h_ctx;                    // last uploaded context
+    DevBuf d_ctx;
+    bool ctx_valid = false;
+    DevBuf d_err;                  // uint32 error word
+    DevBuf d_stats;                // small readback area
+    uint64_t* h_stats = nullptr;   // pinned mirror of d_stats
+    Scratch scratch;
+};
+
+// model-body launchers (vh_bodies.hip)
+int launch_generic(int transition, const DCtx* d_ctx, int ci, int call_t, uint64_t nslots, int pass,
+                   int with_edge, hipStream_t st);
+int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step);
+int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step);
+struct ContactLogs {
+    uint32_t* cnt_e;   // per slot
+    uint32_t* log_src; // nslots * n_contacts
+    uint8_t* cnt_b;    // per slot: 0/1 newborn
+    uint8_t* birth_st; // nslots * 8 bytes
+};
+int launch_contact(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step,
+                   const ContactLogs& L);
+// degree bins for the opinion kernels
+int build_heavy_bins(Sim& s, EdgeStore& E, int t);
+
+int set_error(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* where);
+
+#define HIPCHECK(x)                                        \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return vh::hip_fail(e_, #x); \
+    } while (0)
+#define RC(x)              \
+    do {                   \
+        int r_ = (x);      \
+        if (r_) return r_; \
+    } while (0)
+
+}  // namespace vh
+
+struct vh_model {
+    vh::Model m;
+};
+struct vh_sim {
+    vh::Sim s;
+};

@@ -1,0 +1,1184 @@
+// vh_engine.hip — the C ABI: model/simulation registry, finish_init!, the
+// phases of apply! (src/Simulation.jl:720-821) as device passes, CSR builds,
+// death/birth bookkeeping and queries.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <unordered_map>
+
+#include "vh_engine.h"
+
+using namespace vh;
+
+namespace vh {
+
+static thread_local char g_err[1024];
+static int g_asserts = 1;
+
+int set_error(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+int hip_fail(hipError_t e, const char* where) {
+    return set_error(e == hipErrorOutOfMemory ? VH_ERR_OOM : VH_ERR_DEVICE, "%s: %s", where,
+                     hipGetErrorString(e));
+}
+
+int DevBuf::ensure(size_t bytes, hipStream_t st, size_t keep) {
+    if (bytes <= cap) return VH_OK;
+    size_t ncap = std::max(bytes, cap + cap / 4);
+    ncap = (ncap + 255) & ~(size_t)255;
+    void* np = nullptr;
+    hipError_t e = hipMalloc(&np, ncap);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    if (p) {
+        if (keep) {
+            e = hipMemcpyAsync(np, p, std::min(keep, cap), hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+        }
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        (void)hipFree(p);
+    }
+    p = np;
+    cap = ncap;
+    return VH_OK;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+}  // namespace vh
+
+
+static inline bool has(uint32_t h, uint32_t x) { return (h & x) != 0; }
+static inline bool in_mask(uint32_t m, int i) { return (m >> i) & 1u; }
+
+// ====================================================================== model
+extern "C" int vh_model_create(vh_model_t* out) {
+    if (!out) return set_error(VH_ERR_ARG, "null out");
+    *out = new vh_model();
+    return VH_OK;
+}
+extern "C" int vh_model_destroy(vh_model_t m) {
+    delete m;
+    return VH_OK;
+}
+// register_agenttype! src/ModelTypes.jl:81-108
+extern "C" int vh_register_agenttype(vh_model_t mh, uint32_t hints, uint32_t bytes,
+                                     uint8_t* type_id) {
+    Model& m = mh->m;
+    if (m.na + 1 > VH_MAXA)
+        return set_error(VH_ERR_ASSERT, "Can not add new type, maximal number of types already registered");
+    if (hints & ~(VH_HINT_IMMORTAL | VH_HINT_INDEPENDENT))
+        return set_error(VH_ERR_ASSERT, "The agent type hint is unknown");
+    if (bytes > VH_MAX_STATE) return set_error(VH_ERR_ARG, "agent state larger than %d bytes", VH_MAX_STATE);
+    m.na++;
+    m.a[m.na].bytes = bytes;
+    m.a[m.na].hints = hints;
+    *type_id = (uint8_t)m.na;
+    return VH_OK;
+}
+// register_edgetype! src/ModelTypes.jl:158-230
+extern "C" int vh_register_edgetype(vh_model_t mh, uint32_t hints, uint32_t bytes, int32_t target,
+                                    int64_t size, uint8_t* id) {
+    Model& m = mh->m;
+    if (m.ne >= VH_MAXE) return set_error(VH_ERR_ASSERT, "too many edge types");
+    if (has(hints, VH_HINT_NUMEDGESONLY)) hints |= VH_HINT_STATELESS | VH_HINT_IGNOREFROM;
+    if (has(hints, VH_HINT_HASEDGEONLY))
+        hints |= VH_HINT_STATELESS | VH_HINT_IGNOREFROM | VH_HINT_SINGLEEDGE;
+    if (has(hints, VH_HINT_SINGLETYPE) && target <= 0)
+        return set_error(VH_ERR_ASSERT, "the :SingleType hint is set, but the target keyword is missing");
+    if (target > 0) hints |= VH_HINT_SINGLETYPE;
+    if (target > m.na) return set_error(VH_ERR_UNREGISTERED, "unregistered target type");
+    if (has(hints, VH_HINT_SINGLETYPE) && has(hints, VH_HINT_SINGLEEDGE) &&
+        !(has(hints, VH_HINT_STATELESS) && has(hints, VH_HINT_IGNOREFROM)))
+        return set_error(VH_ERR_ASSERT,
+                         "The hints :SingleEdge and :SingleType can be only combined when the type "
+                         "has also the hints :Stateless and :IgnoreFrom.");
+    if (!has(hints, VH_HINT_STATELESS) && bytes > VH_MAX_STATE)
+        return set_error(VH_ERR_ARG, "edge state larger than %d bytes", VH_MAX_STATE);
+    Model::ET& e = m.e[m.ne];
+    e.bytes = has(hints, VH_HINT_STATELESS) ? 0 : bytes;
+    e.hints = hints;
+    e.target = target > 0 ? target : 0;
+    e.size = size;
+    *id = (uint8_t)m.ne++;
+    return VH_OK;
+}
+extern "C" int vh_edgetype_hints(vh_model_t mh, uint8_t et, uint32_t* hints) {
+    if (et >= mh->m.ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+    *hints = mh->m.e[et].hints;
+    return VH_OK;
+}
+extern "C" const char* vh_last_error(void) { return g_err; }
+extern "C" int vh_enable_asserts(int enable) {
+    g_asserts = enable ? 1 : 0;
+    return VH_OK;
+}
+
+// ================================================================== simulation
+extern "C" int vh_sim_create(vh_model_t mh, vh_sim_t* out) {
+    if (!mh || !out) return set_error(VH_ERR_ARG, "null argument");
+    vh_sim* h = new vh_sim();
+    Sim& s = h->s;
+    s.m = &mh->m;
+    hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    s.own_stream = true;
+    for (int t = 1; t <= s.m->na; t++) {
+        s.a[t].bytes = s.m->a[t].bytes;
+        s.a[t].mortal = !has(s.m->a[t].hints, VH_HINT_IMMORTAL);
+        s.a[t].indep = has(s.m->a[t].hints, VH_HINT_INDEPENDENT);
+    }
+    for (int et = 0; et < s.m->ne; et++) {
+        s.e[et].bytes = s.m->e[et].bytes;
+        s.e[et].hints = s.m->e[et].hints;
+        s.e[et].target = s.m->e[et].target;
+        s.e[et].src64 = s.m->na != 1;
+        s.e[et].src_type = 1;
+    }
+    e = hipHostMalloc((void**)&s.h_stats, 64 * sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s.stream);
+        delete h;
+        return hip_fail(e, "hipHostMalloc");
+    }
+    int r = s.d_stats.ensure(64 * sizeof(uint64_t), s.stream);
+    if (!r) r = s.d_err.ensure(sizeof(uint32_t), s.stream);
+    if (!r) r = s.d_ctx.ensure(sizeof(DCtx), s.stream);
+    if (r) {
+        delete h;
+        return r;
+    }
+    fill_u32(s.d_err.as<uint32_t>(), 0, 1, s.stream);
+    *out = h;
+    return VH_OK;
+}
+
+static void sim_release(Sim& s) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (int t = 0; t <= VH_MAXA; t++) {
+        AgentStore& A = s.a[t];
+        A.state[0].release(); A.state[1].release();
+        A.died.release(); A.died_now.release(); A.reuse.release();
+    }
+    for (int et = 0; et < VH_MAXE; et++) {
+        EdgeStore& E = s.e[et];
+        E.off.release(); E.src.release(); E.st.release(); E.heavy.release();
+    }
+    s.d_ctx.release(); s.d_err.release(); s.d_stats.release();
+    s.scratch.release();
+    if (s.h_stats) (void)hipHostFree(s.h_stats);
+    s.h_stats = nullptr;
+    if (s.own_stream && s.stream) (void)hipStreamDestroy(s.stream);
+    s.stream = nullptr;
+}
+
+// finish_simulation! src/Simulation.jl:551-585
+extern "C" int vh_sim_destroy(vh_sim_t h) {
+    if (!h) return VH_OK;
+    sim_release(h->s);
+    delete h;
+    return VH_OK;
+}
+
+extern "C" int vh_set_stream(vh_sim_t h, void* stream) {
+    Sim& s = h->s;
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    if (s.own_stream) (void)hipStreamDestroy(s.stream);
+    if (stream) {
+        s.stream = (hipStream_t)stream;
+        s.own_stream = false;
+    } else {
+        HIPCHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        s.own_stream = true;
+    }
+    return VH_OK;
+}
+
+extern "C" int vh_sync(vh_sim_t h) {
+    HIPCHECK(hipStreamSynchronize(h->s.stream));
+    return VH_OK;
+}
+
+extern "C" int vh_num_transitions(vh_sim_t h, uint64_t* out) {
+    *out = h->s.num_transitions;
+    return VH_OK;
+}
+
+// add_agent! during the initialisation phase src/AgentMethods.jl:65-89
+extern "C" int vh_add_agents(vh_sim_t h, uint8_t t, const void* states, uint64_t n,
+                             uint64_t* ids) {
+    Sim& s = h->s;
+    if (t < 1 || t > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT,
+                         "You can call add_agent! only in the initialization phase (until "
+                         "`finish_init!` is called) or within a transition function called by `apply`.");
+    AgentStore& A = s.a[t];
+    size_t old = A.h_state.size();
+    A.h_state.resize(old + (size_t)n * A.bytes);
+    if (n && A.bytes) memcpy(A.h_state.data() + old, states, (size_t)n * A.bytes);
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t nr = A.nextid++;
+        if (ids) ids[i] = vh_make_id(t, nr);
+    }
+    if (A.nextid - 1 > VH_NR_MASK) return set_error(VH_ERR_ASSERT, "agent nr space exhausted");
+    return VH_OK;
+}
+
+// add_edge! during the initialisation phase src/EdgeMethods.jl:388-523
+extern "C" int vh_add_edges(vh_sim_t h, uint8_t et, const uint64_t* from, const uint64_t* to,
+                            const void* states, uint64_t n) {
+    Sim& s = h->s;
+    if (et >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT,
+                         "You can call add_edge! only in the initialization phase (until "
+                         "`finish_init!` is called) or within a transition function called by `apply`.");
+    EdgeStore& E = s.e[et];
+    const bool ignorefrom = has(E.hints, VH_HINT_IGNOREFROM);
+    const bool stateless = has(E.hints, VH_HINT_STATELESS);
+    const bool singletype = has(E.hints, VH_HINT_SINGLETYPE);
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t tt = vh_type_of(to[i]);
+        if (tt < 1 || (int)tt > s.m->na || vh_nr_of(to[i]) == 0 ||
+            vh_nr_of(to[i]) >= s.a[tt].nextid || (singletype && (int)tt != E.target))
+            return set_error(VH_ERR_ASSERT, "invalid target id %llu for the edge type",
+                             (unsigned long long)to[i]);
+        if (!ignorefrom) {
+            uint32_t ft = vh_type_of(from[i]);
+            if (ft < 1 || (int)ft > s.m->na || vh_nr_of(from[i]) == 0 ||
+                vh_nr_of(from[i]) >= s.a[ft].nextid)
+                return set_error(VH_ERR_ASSERT, "invalid source id %llu", (unsigned long long)from[i]);
+        }
+    }
+    // _can_add src/EdgeMethods.jl:257-294: a second, different edge to the
+    // same target of a :SingleEdge type (not :SingleType, not S+I) asserts
+    if (has(E.hints, VH_HINT_SINGLEEDGE) && !singletype && !(stateless && ignorefrom)) {
+        std::unordered_map<uint64_t, size_t> first;
+        for (size_t k = 0; k < E.h_to.size(); k++) first.emplace(E.h_to[k], k);
+        for (uint64_t i = 0; i < n; i++) {
+            auto it = first.find(to[i]);
+            if (it != first.end()) {
+                size_t k = it->second;
+                bool same = ignorefrom || E.h_from[k] == from[i];
+                if (!stateless && memcmp(E.h_st.data() + k * E.bytes,
+                                         (const uint8_t*)states + i * E.bytes, E.bytes) != 0)
+                    same = false;
+                if (!same)
+                    return set_error(VH_ERR_ASSERT,
+                                     "An edge has already been added to the agent with the id %llu "
+                                     "(and the edgetype hints containing the :SingleEdge hint).",
+                                     (unsigned long long)to[i]);
+            } else {
+                first.emplace(to[i], E.h_to.size() + i);
+            }
+        }
+    }
+    size_t old = E.h_to.size();
+    E.h_to.insert(E.h_to.end(), to, to + n);
+    E.h_from.resize(old + n);
+    if (!ignorefrom) memcpy(E.h_from.data() + old, from, n * 8);
+    if (E.bytes) {
+        size_t ob = E.h_st.size();
+        E.h_st.resize(ob + (size_t)n * E.bytes);
+        if (states) memcpy(E.h_st.data() + ob, states, (size_t)n * E.bytes);
+    }
+    return VH_OK;
+}
+
+// ================================================================ CSR kernels
+namespace {
+
+struct RowLayout {
+    uint64_t base[VH_MAXA + 1];
+    uint64_t nrows[VH_MAXA + 1];
+};
+
+// keys of log entries in the new row layout
+__global__ void k_log_keys(const uint64_t* __restrict__ to, uint64_t n, RowLayout L,
+                           uint32_t* __restrict__ keys, uint32_t* err) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t id = to[i];
+    uint32_t t = vh_type_of(id);
+    uint64_t nr = vh_nr_of(id);
+    if (t == 0 || t > VH_MAXA || nr == 0 || nr > L.nrows[t]) {
+        atomicOr(err, VH_DERR_TARGET);
+        keys[i] = 0;
+        return;
+    }
+    keys[i] = (uint32_t)(L.base[t] + nr - 1);
+}
+
+// explode an existing CSR into (key, source, state) entries in the new layout
+__global__ void k_base_expand(const uint32_t* __restrict__ off, const void* __restrict__ src,
+                              uint32_t src64, uint32_t src_type, const uint8_t* __restrict__ st,
+                              uint32_t bytes, RowLayout oldL, RowLayout newL, uint64_t rows,
+                              uint32_t* __restrict__ keys, uint64_t* __restrict__ from,
+                              uint8_t* __restrict__ stout, uint32_t* err) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    uint32_t t = 0;
+    for (uint32_t q = 1; q <= VH_MAXA; q++)
+        if (oldL.nrows[q] && r >= oldL.base[q] && r < oldL.base[q] + oldL.nrows[q]) t = q;
+    uint64_t nr = r - oldL.base[t] + 1;
+    bool ok = t != 0 && nr <= newL.nrows[t];
+    if (!ok && off[r + 1] > off[r]) atomicOr(err, VH_DERR_TARGET);
+    uint32_t key = ok ? (uint32_t)(newL.base[t] + nr - 1) : 0u;
+    for (uint32_t k = off[r]; k < off[r + 1]; k++) {
+        keys[k] = key;
+        if (from) from[k] = src64 ? ((const uint64_t*)src)[k] : vh_make_id(src_type, ((const uint32_t*)src)[k]);
+        for (uint32_t b = 0; b < bytes; b++) stout[(uint64_t)k * bytes + b] = st[(uint64_t)k * bytes + b];
+    }
+}
+
+// off[r] = first index whose key >= r, for r in [0, rows]
+__global__ void k_offsets_from_sorted(const uint32_t* __restrict__ keys, uint64_t n, uint64_t rows,
+                                      uint32_t* __restrict__ off) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    int64_t prev = i == 0 ? -1 : (int64_t)keys[i - 1];
+    int64_t cur = i == n ? (int64_t)rows : (int64_t)keys[i];
+    for (int64_t r = prev + 1; r <= cur; r++) off[r] = (uint32_t)i;
+}
+
+// row-wise keep flags: SingleEdge keeps the last entry; deaths drop rows of
+// targets and entries of tracked sources that died in this step
+struct DiedNow {
+    const uint8_t* d[VH_MAXA + 1];
+};
+__global__ void k_filter_rows(const uint32_t* __restrict__ off, const void* __restrict__ src,
+                              uint32_t src64, uint32_t src_type, RowLayout L, uint64_t rows,
+                              uint32_t keep_last, uint32_t track_src, DiedNow dn,
+                              uint8_t* __restrict__ keep) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    uint32_t b = off[r], e = off[r + 1];
+    if (b == e) return;
+    uint32_t t = 0;
+    for (uint32_t q = 1; q <= VH_MAXA; q++)
+        if (L.nrows[q] && r >= L.base[q] && r < L.base[q] + L.nrows[q]) t = q;
+    uint64_t nr = r - L.base[t] + 1;
+    bool tdead = dn.d[t] && dn.d[t][nr - 1];
+    for (uint32_t k = b; k < e; k++) {
+        bool kp = !tdead && (!keep_last || k == e - 1);
+        if (kp && track_src) {
+            uint64_t id = src64 ? ((const uint64_t*)src)[k]
+                                : vh_make_id(src_type, ((const uint32_t*)src)[k]);
+            uint32_t ft = vh_type_of(id);
+            if (ft <= VH_MAXA && dn.d[ft] && dn.d[ft][vh_nr_of(id) - 1]) kp = false;
+        }
+        keep[k] = kp ? 1 : 0;
+    }
+}
+
+// compaction of CSR entries by keep flags; pos = exclusive scan of keep
+__global__ void k_compact_entries(const uint8_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                                  uint64_t n, const void* __restrict__ src, void* __restrict__ srco,
+                                  uint32_t srcw, const uint8_t* __restrict__ st,
+                                  uint8_t* __restrict__ sto, uint32_t bytes) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    uint64_t p = pos[i];
+    if (srcw == 8) ((uint64_t*)srco)[p] = ((const uint64_t*)src)[i];
+    else if (srcw == 4) ((uint32_t*)srco)[p] = ((const uint32_t*)src)[i];
+    for (uint32_t b = 0; b < bytes; b++) sto[p * bytes + b] = st[i * bytes + b];
+}
+__global__ void k_remap_offsets(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
+                                uint64_t rows, uint32_t* __restrict__ offo) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= rows) offo[r] = pos[off[r]];
+}
+__global__ void k_u64_to_nr(const uint64_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)vh_nr_of(in[i]);
+}
+__global__ void k_u8_to_u32(const uint8_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+}  // namespace
+
+static RowLayout layout_of(const EdgeStore& E) {
+    RowLayout L;
+    for (int t = 0; t <= VH_MAXA; t++) {
+        L.base[t] = E.rowbase[t];
+        L.nrows[t] = E.nrows[t];
+    }
+    return L;
+}
+
+// row layout of an edge type for the current slot counts
+static void compute_layout(Sim& s, const EdgeStore& E, RowLayout& L, uint64_t& rows) {
+    rows = 0;
+    for (int t = 0; t <= VH_MAXA; t++) { L.base[t] = 0; L.nrows[t] = 0; }
+    for (int t = 1; t <= s.m->na; t++) {
+        bool tgt = has(E.hints, VH_HINT_SINGLETYPE) ? t == E.target : true;
+        uint64_t n = tgt ? s.a[t].nextid - 1 : 0;
+        L.base[t] = rows;
+        L.nrows[t] = n;
+        rows += n;
+    }
+}
+
+static int readback(Sim& s, int n) {
+    HIPCHECK(hipMemcpyAsync(s.h_stats, s.d_stats.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    return VH_OK;
+}
+
+// Build the CSR of E from (optionally) its current entries followed by `nlog`
+// log entries (to, from, states) in add_edge! order: a stable sort by target
+// row keeps, per target, the existing edges first and the new ones in call
+// order (src/EdgeMethods.jl:477-523); :SingleEdge keeps the last one.
+static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
+                     const uint64_t* d_from, const uint8_t* d_st, uint64_t nlog) {
+    hipStream_t st = s.stream;
+    Scratch& sc = s.scratch;
+    RowLayout oldL = layout_of(E), newL;
+    uint64_t rows;
+    compute_layout(s, E, newL, rows);
+    if (rows >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "more than 2^32 target rows");
+    const bool ignorefrom = has(E.hints, VH_HINT_IGNOREFROM);
+    const uint32_t bytes = E.bytes;
+    uint64_t nbase = (keep_base && E.rows) ? E.nedges : 0;
+    uint64_t n = nbase + nlog;
+    if (n >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "more than 2^32 edges of one type");
+
+    uint32_t* keys = (uint32_t*)sc.take(n * 4 + 4);
+    uint32_t* vals = (uint32_t*)sc.take(n * 4 + 4);
+    uint32_t* kalt = (uint32_t*)sc.take(n * 4 + 4);
+    uint32_t* valt = (uint32_t*)sc.take(n * 4 + 4);
+    uint64_t* from = ignorefrom ? nullptr : (uint64_t*)sc.take(n * 8 + 8);
+    uint8_t* sts = bytes ? (uint8_t*)sc.take(n * bytes + 8) : nullptr;
+    uint32_t* flag = (uint32_t*)sc.take(4);
+    if (!keys || !vals || !kalt || !valt || (!ignorefrom && !from) || (bytes && !sts) || !flag)
+        return set_error(VH_ERR_OOM, "scratch allocation failed");
+    uint32_t* err = s.d_err.as<uint32_t>();
+    if (nbase)
+        hipLaunchKernelGGL(k_base_expand, dim3(grid_for(E.rows, 256)), dim3(256), 0, st,
+                           E.off.as<uint32_t>(), E.src.p, (uint32_t)E.src64, E.src_type,
+                           E.st.as<uint8_t>(), bytes, oldL, newL, E.rows, keys, from, sts, err);
+    if (nlog) {
+        hipLaunchKernelGGL(k_log_keys, dim3(grid_for(nlog, 256)), dim3(256), 0, st, d_to, nlog, newL,
+                           keys + nbase, err);
+        if (from) HIPCHECK(hipMemcpyAsync(from + nbase, d_from, nlog * 8, hipMemcpyDeviceToDevice, st));
+        if (bytes)
+            HIPCHECK(hipMemcpyAsync(sts + nbase * bytes, d_st, nlog * bytes, hipMemcpyDeviceToDevice, st));
+    }
+    // already in row order?  (rebuilds whose appends target their callers)
+    bool sorted = nlog == 0;
+    if (!sorted) {
+        check_sorted(keys, n, flag, st);
+        HIPCHECK(hipMemcpyAsync(s.d_stats.p, flag, 4, hipMemcpyDeviceToDevice, st));
+        RC(readback(s, 1));
+        sorted = (uint32_t)s.h_stats[0] == 1u;
+    }
+    uint64_t* from_s = from;
+    uint8_t* st_s = sts;
+    uint32_t* skeys = keys;
+    if (!sorted) {
+        uint32_t bits = 1;
+        while (bits < 32 && (rows >> bits) != 0) bits++;
+        uint32_t* sv = vals;
+        if (radix_sort_pairs(&skeys, &sv, kalt, valt, n, bits, true, sc, st))
+            return set_error(VH_ERR_DEVICE, "radix sort failed");
+        if (from) {
+            from_s = (uint64_t*)sc.take(n * 8 + 8);
+            if (!from_s) return set_error(VH_ERR_OOM, "scratch");
+            gather_records(from_s, from, sv, n, 8, st);
+        }
+        if (bytes) {
+            st_s = (uint8_t*)sc.take(n * bytes + 8);
+            if (!st_s) return set_error(VH_ERR_OOM, "scratch");
+            gather_records(st_s, sts, sv, n, bytes, st);
+        }
+    }
+    // offsets
+    uint32_t* off = (uint32_t*)sc.take((rows + 1) * 4);
+    if (!off) return set_error(VH_ERR_OOM, "scratch");
+    hipLaunchKernelGGL(k_offsets_from_sorted, dim3(grid_for(n + 1, 256)), dim3(256), 0, st, skeys, n,
+                       rows, off);
+    uint64_t nfinal = n;
+    // :SingleEdge keeps the last added edge per target
+    if (has(E.hints, VH_HINT_SINGLEEDGE) && n) {
+        uint8_t* keep = (uint8_t*)sc.take(n + 8);
+        uint32_t* pos = (uint32_t*)sc.take((n + 1) * 4);
+        uint32_t* k32 = (uint32_t*)sc.take((n + 1) * 4);
+        uint32_t* off2 = (uint32_t*)sc.take((rows + 1) * 4);
+        uint64_t* from2 = from_s ? (uint64_t*)sc.take(n * 8 + 8) : nullptr;
+        uint8_t* st2 = bytes ? (uint8_t*)sc.take(n * bytes + 8) : nullptr;
+        if (!keep || !pos || !k32 || !off2) return set_error(VH_ERR_OOM, "scratch");
+        DiedNow dn;
+        memset(&dn, 0, sizeof dn);
+        hipLaunchKernelGGL(k_filter_rows, dim3(grid_for(rows, 256)), dim3(256), 0, st, off,
+                           (const void*)from_s, 1u, 0u, newL, rows, 1u, 0u, dn, keep);
+        hipLaunchKernelGGL(k_u8_to_u32, dim3(grid_for(n, 256)), dim3(256), 0, st, keep, k32, n);
+        if (scan_excl_u32(k32, pos, n, sc, st)) return set_error(VH_ERR_DEVICE, "scan");
+        hipLaunchKernelGGL(k_compact_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, keep, pos, n,
+                           (const void*)from_s, (void*)from2, from_s ? 8u : 0u, st_s, st2, bytes);
+        hipLaunchKernelGGL(k_remap_offsets, dim3(grid_for(rows + 1, 256)), dim3(256), 0, st, off, pos,
+                           rows, off2);
+        off = off2;
+        from_s = from2;
+        st_s = st2;
+        HIPCHECK(hipMemcpyAsync(s.d_stats.p, off + rows, 4, hipMemcpyDeviceToDevice, st));
+        RC(readback(s, 1));
+        nfinal = (uint32_t)s.h_stats[0];
+    }
+    // publish into the store
+    RC(E.off.ensure((rows + 1) * 4, st));
+    HIPCHECK(hipMemcpyAsync(E.off.p, off, (rows + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (!ignorefrom) {
+        if (E.src64) {
+            RC(E.src.ensure(nfinal * 8 + 8, st));
+            HIPCHECK(hipMemcpyAsync(E.src.p, from_s, nfinal * 8, hipMemcpyDeviceToDevice, st));
+        } else {
+            RC(E.src.ensure(nfinal * 4 + 8, st));
+            if (nfinal)
+                hipLaunchKernelGGL(k_u64_to_nr, dim3(grid_for(nfinal, 256)), dim3(256), 0, st, from_s,
+                                   E.src.as<uint32_t>(), nfinal);
+        }
+    }
+    if (bytes) {
+        RC(E.st.ensure(nfinal * bytes + 8, st));
+        HIPCHECK(hipMemcpyAsync(E.st.p, st_s, nfinal * bytes, hipMemcpyDeviceToDevice, st));
+    }
+    for (int t = 0; t <= VH_MAXA; t++) {
+        E.rowbase[t] = newL.base[t];
+        E.nrows[t] = newL.nrows[t];
+    }
+    E.rows = rows;
+    E.nedges = nfinal;
+    E.version++;
+    return VH_OK;
+}
+
+// drop rows of targets and entries of tracked sources that died in this step
+static int filter_deaths(Sim& s, EdgeStore& E, const DiedNow& dn) {
+    if (!E.rows || !E.nedges) return VH_OK;
+    hipStream_t st = s.stream;
+    Scratch& sc = s.scratch;
+    uint64_t n = E.nedges, rows = E.rows;
+    bool track = !has(E.hints, VH_HINT_IGNOREFROM);
+    RowLayout L = layout_of(E);
+    uint8_t* keep = (uint8_t*)sc.take(n + 8);
+    uint32_t* k32 = (uint32_t*)sc.take((n + 1) * 4);
+    uint32_t* pos = (uint32_t*)sc.take((n + 1) * 4);
+    uint32_t* off2 = (uint32_t*)sc.take((rows + 1) * 4);
+    uint32_t srcw = track ? (E.src64 ? 8u : 4u) : 0u;
+    void* src2 = track ? sc.take(n * srcw + 8) : nullptr;
+    uint8_t* st2 = E.bytes ? (uint8_t*)sc.take(n * E.bytes + 8) : nullptr;
+    if (!keep || !k32 || !pos || !off2 || (track && !src2) || (E.bytes && !st2))
+        return set_error(VH_ERR_OOM, "scratch");
+    fill_u8(keep, 1, n, st);
+    hipLaunchKernelGGL(k_filter_rows, dim3(grid_for(rows, 256)), dim3(256), 0, st,
+                       E.off.as<uint32_t>(), E.src.p, (uint32_t)E.src64, E.src_type, L, rows, 0u,
+                       track ? 1u : 0u, dn, keep);
+    hipLaunchKernelGGL(k_u8_to_u32, dim3(grid_for(n, 256)), dim3(256), 0, st, keep, k32, n);
+    if (scan_excl_u32(k32, pos, n, sc, st)) return set_error(VH_ERR_DEVICE, "scan");
+    hipLaunchKernelGGL(k_compact_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, keep, pos, n,
+                       E.src.p, src2, srcw, E.st.as<uint8_t>(), st2, E.bytes);
+    hipLaunchKernelGGL(k_remap_offsets, dim3(grid_for(rows + 1, 256)), dim3(256), 0, st,
+                       E.off.as<uint32_t>(), pos, rows, off2);
+    HIPCHECK(hipMemcpyAsync(s.d_stats.p, off2 + rows, 4, hipMemcpyDeviceToDevice, st));
+    RC(readback(s, 1));
+    uint64_t nn = (uint32_t)s.h_stats[0];
+    HIPCHECK(hipMemcpyAsync(E.off.p, off2, (rows + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (track && nn) HIPCHECK(hipMemcpyAsync(E.src.p, src2, nn * srcw, hipMemcpyDeviceToDevice, st));
+    if (E.bytes && nn) HIPCHECK(hipMemcpyAsync(E.st.p, st2, nn * E.bytes, hipMemcpyDeviceToDevice, st));
+    E.nedges = nn;
+    E.version++;
+    return VH_OK;
+}
+
+// ================================================================ agents
+static int ensure_slots(Sim& s, int t, uint64_t slots) {
+    AgentStore& A = s.a[t];
+    if (slots <= A.cap && A.state[0].p) return VH_OK;
+    uint64_t nc = std::max<uint64_t>(slots, A.cap + A.cap / 4);
+    if (nc < 64) nc = 64;
+    size_t keep = A.nslots * A.bytes;
+    size_t keepw = (std::max(A.nslots, A.nextid - 1)) * A.bytes;
+    RC(A.state[0].ensure(nc * A.bytes + 8, s.stream, A.rd == 0 ? keep : keepw));
+    if (!A.indep) RC(A.state[1].ensure(nc * A.bytes + 8, s.stream, A.rd == 1 ? keep : keepw));
+    if (A.mortal) {
+        RC(A.died.ensure(nc + 8, s.stream, A.nextid - 1));
+        RC(A.died_now.ensure(nc + 8, s.stream, A.nextid - 1));
+        RC(A.reuse.ensure(nc * 4 + 8, s.stream, A.nreuse * 4));
+    }
+    A.cap = nc;
+    return VH_OK;
+}
+
+// finish_init! src/Simulation.jl:403-476 for one process: publish the
+// init-phase agents and build every edge type's CSR.
+extern "C" int vh_finish_init(vh_sim_t h) {
+    Sim& s = h->s;
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT, "You can not call finish_init! twice for the same simulation");
+    hipStream_t st = s.stream;
+    for (int t = 1; t <= s.m->na; t++) {
+        AgentStore& A = s.a[t];
+        uint64_t n = A.nextid - 1;
+        RC(ensure_slots(s, t, n));
+        if (n && A.bytes)
+            HIPCHECK(hipMemcpyAsync(A.state[A.rd].p, A.h_state.data(), n * A.bytes,
+                                    hipMemcpyHostToDevice, st));
+        if (A.mortal) {
+            fill_u8(A.died.as<uint8_t>(), 0, n, st);
+            fill_u8(A.died_now.as<uint8_t>(), 0, n, st);
+        }
+        A.nslots = n;
+        HIPCHECK(hipStreamSynchronize(st));   // h_state is freed next
+        std::vector<uint8_t>().swap(A.h_state);
+    }
+    for (int et = 0; et < s.m->ne; et++) {
+        EdgeStore& E = s.e[et];
+        uint64_t n = E.h_to.size();
+        s.scratch.reset(st);
+        uint64_t* dto = (uint64_t*)s.scratch.take(n * 8 + 8);
+        uint64_t* dfrom = has(E.hints, VH_HINT_IGNOREFROM) ? nullptr : (uint64_t*)s.scratch.take(n * 8 + 8);
+        uint8_t* dst = E.bytes ? (uint8_t*)s.scratch.take(n * E.bytes + 8) : nullptr;
+        if (!dto || (!dfrom && !has(E.hints, VH_HINT_IGNOREFROM)) || (E.bytes && !dst))
+            return set_error(VH_ERR_OOM, "scratch");
+        if (n) {
+            HIPCHECK(hipMemcpyAsync(dto, E.h_to.data(), n * 8, hipMemcpyHostToDevice, st));
+            if (dfrom) HIPCHECK(hipMemcpyAsync(dfrom, E.h_from.data(), n * 8, hipMemcpyHostToDevice, st));
+            if (dst) HIPCHECK(hipMemcpyAsync(dst, E.h_st.data(), n * E.bytes, hipMemcpyHostToDevice, st));
+        }
+        if (s.m->na != 1) E.src64 = true;
+        RC(build_csr(s, E, false, dto, dfrom, dst, n));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<uint64_t>().swap(E.h_to);
+        std::vector<uint64_t>().swap(E.h_from);
+        std::vector<uint8_t>().swap(E.h_st);
+    }
+    s.scratch.reset(st);
+    HIPCHECK(hipStreamSynchronize(st));
+    uint32_t e = 0;
+    HIPCHECK(hipMemcpy(&e, s.d_err.p, 4, hipMemcpyDeviceToHost));
+    if (e) return set_error(VH_ERR_ASSERT, "invalid edge target during finish_init! (flags %u)", e);
+    s.initialized = true;
+    s.num_transitions = 1;
+    return VH_OK;
+}
+
+// ================================================================ apply!
+namespace {
+
+__global__ void k_births_mark(uint8_t* __restrict__ died, const uint32_t* __restrict__ reuse,
+                              uint64_t nreuse, uint64_t nextid, uint64_t nb) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nb) return;
+    uint64_t slot = g < nreuse ? reuse[nreuse - 1 - g] : nextid + (g - nreuse);
+    died[slot - 1] = 0;
+}
+
+// deaths of this step in ascending slot order onto the reuse stack
+// (readreuseable = [readreuseable; writereuseable], src/AgentMethods.jl:430)
+__global__ void k_push_deaths(const uint8_t* __restrict__ died_now, const uint32_t* __restrict__ pos,
+                              uint64_t n, uint32_t* __restrict__ reuse, uint64_t base,
+                              uint8_t* __restrict__ died) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !died_now[i]) return;
+    reuse[base + pos[i]] = (uint32_t)(i + 1);
+    died[i] = 1;
+}
+
+__global__ void k_contact_births(const uint8_t* __restrict__ cnt_b, const uint32_t* __restrict__ pos,
+                                 uint64_t n, const uint8_t* __restrict__ bst,
+                                 const uint32_t* __restrict__ reuse, uint64_t nreuse, uint64_t nextid,
+                                 uint8_t* __restrict__ wstate, uint8_t* __restrict__ died) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !cnt_b[i]) return;
+    uint64_t g = pos[i];
+    uint64_t slot = g < nreuse ? reuse[nreuse - 1 - g] : nextid + (g - nreuse);
+    ((uint64_t*)wstate)[slot - 1] = ((const uint64_t*)bst)[i];
+    died[slot - 1] = 0;
+}
+
+// contact rebuild: rows are the callers (edges target their caller); copy
+// the kept entries of the fixed-capacity log (sources not dead this step)
+__global__ void k_contact_count(const uint32_t* __restrict__ cnt_e, const uint32_t* __restrict__ log,
+                                uint32_t cap, uint64_t n, const uint8_t* __restrict__ died_now,
+                                uint32_t* __restrict__ kept) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = cnt_e[i], k = 0;
+    for (uint32_t j = 0; j < c; j++) k += died_now[log[i * cap + j] - 1] ? 0u : 1u;
+    kept[i] = k;
+}
+__global__ void k_contact_write(const uint32_t* __restrict__ cnt_e, const uint32_t* __restrict__ log,
+                                uint32_t cap, uint64_t n, const uint8_t* __restrict__ died_now,
+                                const uint32_t* __restrict__ off, uint32_t* __restrict__ src) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t i = q / cap;
+    uint32_t j = (uint32_t)(q - i * cap);
+    if (i >= n || j >= cnt_e[i]) return;
+    uint32_t v = log[q];
+    if (died_now[v - 1]) return;
+    // rank of entry j among the kept entries of caller i
+    uint32_t r = 0;
+    for (uint32_t p = 0; p < j; p++) r += died_now[log[i * cap + p] - 1] ? 0u : 1u;
+    src[off[i] + r] = v;
+}
+__global__ void k_extend_offsets(uint32_t* __restrict__ off, uint64_t from_row, uint64_t rows) {
+    uint64_t r = from_row + 1 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= rows) off[r] = off[from_row];
+}
+__global__ void k_copy_nondead(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+}  // namespace
+
+enum Kind { K_PLAIN = 0, K_TWOPASS = 1, K_SIR = 2, K_OPINION = 3, K_CONTACT = 4 };
+static int kind_of(uint32_t tr) {
+    switch (tr) {
+    case VH_TR_NOOP: case VH_TR_SUM_NEIGHBORS: case VH_TR_DIE_IF: case VH_TR_COUNT_EDGES:
+        return K_PLAIN;
+    case VH_TR_ADD_SELF_EDGE: case VH_TR_READD_EDGES: case VH_TR_NEIGHBOR_FOO_EDGES: case VH_TR_SPAWN:
+        return K_TWOPASS;
+    case VH_TR_SIR: return K_SIR;
+    case VH_TR_OPINION_F64: case VH_TR_OPINION_F32: return K_OPINION;
+    case VH_TR_CONTACT: return K_CONTACT;
+    }
+    return -1;
+}
+
+static void fill_ctx(Sim& s, const vh_apply_desc* d, DCtx& c) {
+    memset(&c, 0, sizeof c);
+    for (int t = 1; t <= s.m->na; t++) {
+        AgentStore& A = s.a[t];
+        DAgent& D = c.a[t];
+        D.rstate = A.rstate();
+        D.wstate = A.wstate();
+        D.died = A.mortal ? A.died.as<uint8_t>() : nullptr;
+        D.died_now = A.mortal ? A.died_now.as<uint8_t>() : nullptr;
+        D.nslots = A.nslots;
+        D.bytes = A.bytes;
+        D.mortal = A.mortal;
+        D.indep = A.indep;
+        D.readable = in_mask(d->read_agents, t);
+        D.writeable = A.writeable;
+        D.reuse = A.reuse.as<uint32_t>();
+        D.nreuse = A.nreuse;
+        D.nextid = A.nextid;
+    }
+    for (int et = 0; et < s.m->ne; et++) {
+        EdgeStore& E = s.e[et];
+        DEdge& D = c.e[et];
+        D.off = E.rows ? E.off.as<uint32_t>() : nullptr;
+        D.src = E.src.p;
+        D.st = E.st.as<uint8_t>();
+        for (int t = 0; t <= VH_MAXA; t++) {
+            D.rowbase[t] = E.rowbase[t];
+            D.nrows[t] = E.nrows[t];
+        }
+        D.bytes = E.bytes;
+        D.hints = E.hints;
+        D.src64 = E.src64;
+        D.src_type = E.src_type;
+        D.readable = in_mask(d->read_edges, et);
+        D.writeable = E.writeable;
+    }
+    for (int i = 0; i < 4; i++) {
+        c.role_a[i] = d->role_agent[i];
+        c.role_e[i] = d->role_edge[i];
+    }
+    c.err = s.d_err.as<uint32_t>();
+    c.ee[0].et = c.ee[1].et = -1;
+    c.ea[0].at = -1;
+    if (d->params_bytes) memcpy(c.params, d->params, std::min<uint32_t>(d->params_bytes, VH_PARAMS_BYTES));
+}
+
+static int upload_ctx(Sim& s, const DCtx& c) {
+    if (s.ctx_valid && memcmp(&c, &s.h_ctx, sizeof c) == 0) return VH_OK;
+    s.h_ctx = c;
+    HIPCHECK(hipMemcpyAsync(s.d_ctx.p, &s.h_ctx, sizeof(DCtx), hipMemcpyHostToDevice, s.stream));
+    // the source is pageable host memory: make sure the copy has consumed it
+    // before h_ctx can change again
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    s.ctx_valid = true;
+    return VH_OK;
+}
+
+static int check_errors(Sim& s) {
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    uint32_t e = 0;
+    HIPCHECK(hipMemcpy(&e, s.d_err.p, 4, hipMemcpyDeviceToHost));
+    if (!e) return VH_OK;
+    fill_u32(s.d_err.as<uint32_t>(), 0, 1, s.stream);
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    if (e & VH_DERR_CAP) return set_error(VH_ERR_CAPACITY, "per-caller emission capacity exceeded");
+    if (e & VH_DERR_READ_EDGE)
+        return set_error(VH_ERR_ASSERT, "You try to access an edge of a type that is not in the "
+                                        "`read` argument of the apply! function.");
+    if (e & VH_DERR_READ_AGENT)
+        return set_error(VH_ERR_ASSERT, "an agent type must be in the `read` argument of the "
+                                        "transition function.");
+    if (e & VH_DERR_DIED)
+        return set_error(VH_ERR_ASSERT, "agentstate was requested for an agent that has been removed");
+    if (e & VH_DERR_IMMORTAL)
+        return set_error(VH_ERR_ASSERT, "You can not return `nothing` for immortal agents");
+    if (e & VH_DERR_WRITE)
+        return set_error(VH_ERR_ASSERT, "a type must be in the `write` argument of the transition function.");
+    if (e & VH_DERR_HINT)
+        return set_error(VH_ERR_ASSERT, "the accessor is not defined for the hint combination");
+    if (e & VH_DERR_TARGET)
+        return set_error(VH_ERR_ASSERT, "invalid edge target (type or id)");
+    return set_error(VH_ERR_ASSERT, "device assertion flags %u", e);
+}
+
+static int apply_one(Sim& s, const vh_apply_desc* d) {
+    Model& m = *s.m;
+    hipStream_t st = s.stream;
+    if (!s.initialized) return set_error(VH_ERR_ASSERT, "You must call finish_init! before apply!");
+    if (d->with_edge >= 0 && (d->with_edge >= m.ne || has(m.e[d->with_edge].hints, VH_HINT_SINGLETYPE)))
+        return set_error(VH_ERR_ASSERT, "The `with_edge` keyword can only be used for edgetypes "
+                                        "without the :SingleType hint.");
+    int kind = kind_of(d->transition);
+    if (kind < 0) return set_error(VH_ERR_UNREGISTERED, "unregistered transition %u", d->transition);
+    if (d->n_call > VH_MAX_CALL) return set_error(VH_ERR_ARG, "too many call types");
+    if (d->params_bytes > VH_PARAMS_BYTES) return set_error(VH_ERR_ARG, "params too large");
+    uint32_t callmask = 0;
+    for (uint32_t i = 0; i < d->n_call; i++) {
+        if (d->call[i] < 1 || d->call[i] > m.na) return set_error(VH_ERR_UNREGISTERED, "unregistered call type");
+        callmask |= 1u << d->call[i];
+    }
+    if (callmask & d->add_existing_agents)
+        return set_error(VH_ERR_ASSERT, "a call type can not be element of `add_existing`");
+    if ((d->add_existing_agents & ~d->write_agents) || (d->add_existing_edges & ~d->write_edges))
+        return set_error(VH_ERR_ASSERT, "a type is in `add_existing` but not in `write`");
+    for (int i = 0; i < 4; i++) {
+        if (d->role_edge[i] >= m.ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge role");
+        if (d->role_agent[i] > m.na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent role");
+    }
+    const uint64_t step = s.num_transitions;
+    s.scratch.reset(st);
+
+    // ---- prepare_write! (src/AgentMethods.jl:271-298, src/EdgeMethods.jl:639-663)
+    for (int t = 1; t <= m.na; t++) {
+        AgentStore& A = s.a[t];
+        A.writeable = in_mask(d->write_agents, t);
+        if (!A.writeable) continue;
+        bool ae = in_mask(callmask, t) || in_mask(d->add_existing_agents, t);
+        if (!A.mortal && !ae)
+            return set_error(VH_ERR_ASSERT, "You can add the type to the `write` argument only when it "
+                                            "is also included in the `add_existing` argument, as it "
+                                            "has the hint :Immortal.");
+        if (!ae) {
+            A.nslots = 0;
+            A.nextid = 1;
+            A.nreuse = 0;
+        } else if (!in_mask(callmask, t) && !A.indep && A.nslots && A.bytes) {
+            // retained agents of a written, uncalled type: write = read
+            HIPCHECK(hipMemcpyAsync(A.wstate(), A.rstate(), A.nslots * A.bytes,
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        if (A.mortal && A.nslots) fill_u8(A.died_now.as<uint8_t>(), 0, A.nslots, st);
+    }
+    for (int et = 0; et < m.ne; et++) s.e[et].writeable = in_mask(d->write_edges, et);
+
+    DCtx c;
+    fill_ctx(s, d, c);
+    uint64_t ncallers = 0;
+    for (uint32_t i = 0; i < d->n_call; i++) {
+        c.cbase[i] = ncallers;
+        ncallers += s.a[d->call[i]].nslots;
+    }
+
+    bool deaths_possible = false;
+    for (int t = 1; t <= m.na; t++)
+        if (s.a[t].writeable && s.a[t].mortal && s.a[t].nslots) deaths_possible = true;
+
+    // logs of the two-pass bodies
+    uint64_t ne_tot[2] = {0, 0}, nb_tot = 0;
+    uint64_t* e_to[2] = {nullptr, nullptr};
+    uint64_t* e_from[2] = {nullptr, nullptr};
+    uint8_t* e_st[2] = {nullptr, nullptr};
+    ContactLogs CL = {nullptr, nullptr, nullptr, nullptr};
+    int contact_t = -1;
+
+    if (kind == K_PLAIN) {
+        RC(upload_ctx(s, c));
+        for (uint32_t i = 0; i < d->n_call; i++)
+            RC(launch_generic(d->transition, s.d_ctx.as<DCtx>(), (int)i, d->call[i],
+                              s.a[d->call[i]].nslots, 1, d->with_edge, st));
+    } else if (kind == K_TWOPASS) {
+        // pass 0 counts the add_edge!/add_agent! calls of every caller, an
+        // exclusive scan turns the counts into log positions in call order,
+        // pass 1 runs the body again (it is deterministic) and writes there.
+        // Edges go to role_edge[0] (role_edge[1] for NEIGHBOR_FOO_EDGES),
+        // births to role_agent[0] (SPAWN).
+        int erole = d->transition == VH_TR_NEIGHBOR_FOO_EDGES ? d->role_edge[1] : d->role_edge[0];
+        bool births = d->transition == VH_TR_SPAWN && d->role_agent[0] > 0;
+        uint32_t* ecnt = nullptr;
+        uint64_t* eoff = nullptr;
+        uint32_t* bcnt = nullptr;
+        uint64_t* boff = nullptr;
+        if (erole >= 0) {
+            ecnt = (uint32_t*)s.scratch.take(ncallers * 4 + 8);
+            eoff = (uint64_t*)s.scratch.take((ncallers + 1) * 8);
+            if (!ecnt || !eoff) return set_error(VH_ERR_OOM, "scratch");
+            fill_u32(ecnt, 0, ncallers, st);
+        }
+        if (births) {
+            bcnt = (uint32_t*)s.scratch.take(ncallers * 4 + 8);
+            boff = (uint64_t*)s.scratch.take((ncallers + 1) * 8);
+            if (!bcnt || !boff) return set_error(VH_ERR_OOM, "scratch");
+            fill_u32(bcnt, 0, ncallers, st);
+        }
+        auto set_emit = [&](DCtx& cc) {
+            uint64_t nc = 0;
+            for (uint32_t i = 0; i < d->n_call; i++) {
+                cc.cbase[i] = nc;
+                nc += s.a[d->call[i]].nslots;
+            }
+            if (erole >= 0)
+                cc.ee[0] = DEmitEdge{ecnt, eoff, e_to[0], e_from[0], e_st[0], erole, m.e[erole].bytes};
+            if (births) cc.ea[0] = DEmitAgent{bcnt, boff, d->role_agent[0], 0};
+        };
+        set_emit(c);
+        RC(upload_ctx(s, c));
+        for (uint32_t i = 0; i < d->n_call; i++)
+            RC(launch_generic(d->transition, s.d_ctx.as<DCtx>(), (int)i, d->call[i],
+                              s.a[d->call[i]].nslots, 0, d->with_edge, st));
+        if (ecnt) {
+            if (scan_excl_u64(ecnt, eoff, ncallers, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+            HIPCHECK(hipMemcpyAsync((uint64_t*)s.d_stats.p + 0, eoff + ncallers, 8,
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        if (bcnt) {
+            if (scan_excl_u64(bcnt, boff, ncallers, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+            HIPCHECK(hipMemcpyAsync((uint64_t*)s.d_stats.p + 1, boff + ncallers, 8,
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        RC(readback(s, 2));
+        ne_tot[0] = ecnt ? s.h_stats[0] : 0;
+        nb_tot = bcnt ? s.h_stats[1] : 0;
+        RC(check_errors(s));
+        if (ecnt) {
+            uint32_t b = m.e[erole].bytes;
+            e_to[0] = (uint64_t*)s.scratch.take(ne_tot[0] * 8 + 8);
+            e_from[0] = (uint64_t*)s.scratch.take(ne_tot[0] * 8 + 8);
+            e_st[0] = b ? (uint8_t*)s.scratch.take(ne_tot[0] * b + 8) : nullptr;
+            if (!e_to[0] || !e_from[0] || (b && !e_st[0])) return set_error(VH_ERR_OOM, "scratch");
+        }
+        if (nb_tot) {
+            AgentStore& A = s.a[d->role_agent[0]];
+            if (!A.writeable)
+                return set_error(VH_ERR_ASSERT, "the agent type must be in the `write` argument");
+            uint64_t fresh = nb_tot > A.nreuse ? nb_tot - A.nreuse : 0;
+            RC(ensure_slots(s, d->role_agent[0], A.nextid - 1 + fresh));
+            if (A.mortal && A.nextid - 1 + fresh > A.nslots)
+                fill_u8(A.died_now.as<uint8_t>() + A.nslots, 0, A.nextid - 1 + fresh - A.nslots, st);
+        }
+        fill_ctx(s, d, c);   // buffers may have moved
+        set_emit(c);
+        RC(upload_ctx(s, c));
+        for (uint32_t i = 0; i < d->n_call; i++)
+            RC(launch_generic(d->transition, s.d_ctx.as<DCtx>(), (int)i, d->call[i],
+                              s.a[d->call[i]].nslots, 1, d->with_edge, st));
+        if (nb_tot) {
+            AgentStore& A = s.a[d->role_agent[0]];
+            uint64_t consumed = std::min<uint64_t>(nb_tot, A.nreuse);
+            if (A.mortal)
+                hipLaunchKernelGGL(k_births_mark, dim3(grid_for(nb_tot, 256)), dim3(256), 0, st,
+                                   A.died.as<uint8_t>(), A.reuse.as<uint32_t>(), A.nreuse, A.nextid,
+                                   nb_tot);
+            A.nreuse -= consumed;
+            A.nextid += nb_tot - consumed;
+        }
+    } else if (kind == K_SIR) {
+        if (d->n_call != 1 || d->role_edge[0] < 0 || d->params_bytes < sizeof(vh_sir_params))
+            return set_error(VH_ERR_ARG, "SIR needs one call type, an edge role and vh_sir_params");
+        vh_sir_params p;
+        memcpy(&p, d->params, sizeof p);
+        RC(launch_sir(s, d->call[0], d->role_edge[0], p, step));
+    } else if (kind == K_OPINION) {
+        if (d->n_call != 1 || d->role_edge[0] < 0 || d->params_bytes < sizeof(vh_opinion_params))
+            return set_error(VH_ERR_ARG, "opinion needs one call type, an edge role and vh_opinion_params");
+        vh_opinion_params p;
+        memcpy(&p, d->params, sizeof p);
+        RC(launch_opinion(s, d->call[0], d->role_edge[0], p, d->transition == VH_TR_OPINION_F32, step));
+    } else {  // K_CONTACT
+        if (d->n_call != 1 || d->role_edge[0] < 0 || d->params_bytes < sizeof(vh_contact_params))
+            return set_error(VH_ERR_ARG, "contact needs one call type, an edge role and vh_contact_params");
+        vh_contact_params p;
+        memcpy(&p, d->params, sizeof p);
+        contact_t = d->call[0];
+        AgentStore& A = s.a[contact_t];
+        if (!A.mortal || A.bytes != 8 || !A.writeable || !in_mask(d->write_edges, d->role_edge[0]) ||
+            in_mask(d->add_existing_edges, d->role_edge[0]))
+            return set_error(VH_ERR_ARG, "contact: mortal 8-byte agent type in write, edge type "
+                                         "rebuilt (in write, not add_existing)");
+        uint64_t n = A.nslots;
+        CL.cnt_e = (uint32_t*)s.scratch.take(n * 4 + 8);
+        CL.log_src = (uint32_t*)s.scratch.take(n * (uint64_t)p.n_contacts * 4 + 8);
+        CL.cnt_b = (uint8_t*)s.scratch.take(n + 8);
+        CL.birth_st = (uint8_t*)s.scratch.take(n * 8 + 8);
+        if (!CL.cnt_e || !CL.log_src || !CL.cnt_b || !CL.birth_st) return set_error(VH_ERR_OOM, "scratch");
+        RC(launch_contact(s, contact_t, d->role_edge[0], p, step, CL));
+    }
+
+    // ---- finish_write! for agents: births of the contact body, deaths
+    DiedNow dn;
+    memset(&dn, 0, sizeof dn);
+    bool any_death = false;
+    if (deaths_possible) {
+        // counts: deaths per written mortal type (+ contact births)
+        uint32_t* dpos[VH_MAXA + 1] = {nullptr};
+        uint32_t* bpos = nullptr;
+        int slot = 0;
+        int dslot[VH_MAXA + 1];
+        for (int t = 1; t <= m.na; t++) {
+            AgentStore& A = s.a[t];
+            dslot[t] = -1;
+            if (!(A.writeable && A.mortal && A.nslots)) continue;
+            uint32_t* d32 = (uint32_t*)s.scratch.take(A.nslots * 4 + 8);
+            dpos[t] = (uint32_t*)s.scratch.take((A.nslots + 1) * 4);
+            if (!d32 || !dpos[t]) return set_error(VH_ERR_OOM, "scratch");
+            hipLaunchKernelGGL(k_u8_to_u32, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st,
+                               A.died_now.as<uint8_t>(), d32, A.nslots);
+            if (scan_excl_u32(d32, dpos[t], A.nslots, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+            HIPCHECK(hipMemcpyAsync((uint32_t*)s.d_stats.p + 2 * slot, dpos[t] + A.nslots, 4,
+                                    hipMemcpyDeviceToDevice, st));
+            dslot[t] = slot++;
+        }
+        int bslot = -1;
+        if (contact_t > 0) {
+            AgentStore& A = s.a[contact_t];
+            uint32_t* b32 = (uint32_t*)s.scratch.take(A.nslots * 4 + 8);
+            bpos = (uint32_t*)s.scratch.take((A.nslots + 1) * 4);
+            if (!b32 || !bpos) return set_error(VH_ERR_OOM, "scratch");
+            hipLaunchKernelGGL(k_u8_to_u32, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st, CL.cnt_b,
+                               b32, A.nslots);
+            if (scan_excl_u32(b32, bpos, A.nslots, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+            HIPCHECK(hipMemcpyAsync((uint32_t*)s.d_stats.p + 2 * slot, bpos + A.nslots, 4,
+                                    hipMemcpyDeviceToDevice, st));
+            bslot = slot++;
+        }
+        RC(readback(s, slot));
+        const uint32_t* h32 = (const uint32_t*)s.h_stats;
+        uint64_t nbc = bslot >= 0 ? h32[2 * bslot] : 0;
+        uint64_t ndeath[VH_MAXA + 1] = {0};
+        for (int t = 1; t <= m.na; t++) ndeath[t] = dslot[t] >= 0 ? h32[2 * dslot[t]] : 0;
+        if (contact_t > 0 && nbc) {
+            AgentStore& A = s.a[contact_t];
+            uint64_t consumed = std::min<uint64_t>(nbc, A.nreuse);
+            uint64_t fresh = nbc - consumed;
+            RC(ensure_slots(s, contact_t, A.nextid - 1 + fresh));
+            hipLaunchKernelGGL(k_contact_births, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st,
+                               CL.cnt_b, bpos, A.nslots, CL.birth_st, A.reuse.as<uint32_t>(), A.nreuse,
+                               A.nextid, A.wstate(), A.died.as<uint8_t>());
+            A.nreuse -= consumed;
+            A.nextid += fresh;
+        }
+        for (int t = 1; t <= m.na; t++) {
+            AgentStore& A = s.a[t];
+            if (!ndeath[t]) continue;
+            any_death = true;
+            RC(A.reuse.ensure((A.nreuse + ndeath[t]) * 4 + 8, st, A.nreuse * 4));
+            hipLaunchKernelGGL(k_push_deaths, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st,
+                               A.died_now.as<uint8_t>(), dpos[t], A.nslots, A.reuse.as<uint32_t>(),
+                               A.nreuse, A.died.as<uint8_t>());
+            A.nreuse += ndeath[t];
+            dn.d[t] = A.died_now.as<uint8_t>();
+        }
+    }
+
+    // ---- finish_write! for edges: rebuild written types
+    for (int et = 0; et < m.ne; et++) {
+        EdgeStore& E = s.e[et];
+        if (!E.writeable) continue;
+        bool keep_base = in_mask(d->add_existing_edges, et);
+        if (kind == K_CONTACT && et == d->role_edge[0]) {
+            // rows = caller slots; the sources of this step's deaths are dropped
+            // here, the rows of dead callers are empty by construction
+            AgentStore& A = s.a[contact_t];
+            uint64_t n = A.nslots;
+            vh_contact_params p;
+            memcpy(&p, d->params, sizeof p);
+            uint32_t* kept = (uint32_t*)s.scratch.take(n * 4 + 8);
+            uint32_t* off = (uint32_t*)s.scratch.take((n + 1) * 4 + 8);
+            if (!kept || !off) return set_error(VH_ERR_OOM, "scratch");
+            hipLaunchKernelGGL(k_contact_count, dim3(grid_for(n, 256)), dim3(256), 0, st, CL.cnt_e,
+                               CL.log_src, p.n_contacts, n, A.died_now.as<uint8_t>(), kept);
+            if (scan_excl_u32(kept, off, n, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+            HIPCHECK(hipMemcpyAsync(s.d_stats.p, off + n, 4, hipMemcpyDeviceToDevice, st));
+            RC(readback(s, 1));
+            uint64_t ne = (uint32_t)s.h_stats[0];
+            uint64_t rows = std::max(A.nslots, A.nextid - 1);
+            RC(E.off.ensure((rows + 1) * 4, st));
+            RC(E.src.ensure(ne * 4 + 8, st));
+            HIPCHECK(hipMemcpyAsync(E.off.p, off, (n + 1) * 4, hipMemcpyDeviceToDevice, st));
+            if (rows > n)
+                hipLaunchKernelGGL(k_extend_offsets, dim3(grid_for(rows - n, 256)), dim3(256), 0, st,
+                                   E.off.as<uint32_t>(), n, rows);
+            if (n * p.n_contacts)
+                hipLaunchKernelGGL(k_contact_write, dim3(grid_for(n * p.n_contacts, 256)), dim3(256), 0,
+                                   st, CL.cnt_e, CL.log_src, p.n_contacts, n, A.died_now.as<uint8_t>(),
+                                   E.off.as<uint32_t>(), E.src.as<uint32_t>());
+            for (int t = 0; t <= VH_MAXA; t++) { E.rowbase[t] = 0; E.nrows[t] = 0; }
+            E.nrows[contact_t] = rows;
+            E.rows = rows;
+            E.nedges = ne;
+            E.src64 = false;
+            E.src_type = (uint32_t)contact_t;
+            E.version++;
+            continue;
+        }
+        const uint64_t* lto = nullptr;
+        const uint64_t* lfrom = nullptr;
+        const uint8_t* lst = nullptr;
+        uint64_t nlog = 0;
+        if (kind == K_TWOPASS && c.ee[0].cnt && c.ee[0].et == et) {
+            lto = e_to[0]; lfrom = e_from[0]; lst = e_st[0]; nlog = ne_tot[0];
+        }
+        if (!keep_base || nlog || E.rows == 0)
+            RC(build_csr(s, E, keep_base, lto, lfrom, lst, nlog));
+    }
+    // deaths remove edges of every edge type (src/AgentMethods.jl:314-345)
+    if (any_death)
+        for (int et = 0; et < m.ne; et++)
+            if (!(kind == K_CONTACT && et == d->role_edge[0])) RC(filter_deaths(s, s.e[et], dn));
+
+    // ---- publish (double-buffer swap instead of the reference's memcpy)
+    for (int t = 1; t <= m.na; t++) {
+        AgentStore& A = s.a[t];
+        if (A.writeable) A.nslots = std::max(A.nslots, A.nextid - 1);
+        if (A.writeable && !A.indep) A.rd ^= 1;
+        A.writeable = false;
+    }
+    for (int et = 0; et < m.ne; et++) s.e[et].writeable = false;
+    s.num_transitions++;
+    if (g_asserts) RC(check_errors(s));
+    return VH_OK;
+}
+
+extern "C" int vh_apply(vh_sim_t h, const vh_apply_desc* d) {
+    if (!h || !d) return set_error(VH_ERR_ARG, "null argument");
+    return apply_one(h->s, d);
+}
+extern "C" int vh_apply_n(vh_sim_t h, const vh_apply_desc* d, uint32_t steps) {
+    for (uint32_t i = 0; i < steps; i++) RC(apply_one(h->s, d));
+    return VH_OK;
+}

@@ -1,0 +1,113 @@
+// vh_graph.cpp — host-side graph sources for add_graph! (src/GraphsSupport.jl:34-57)
+// and input fills driven by the counter hash, so every consumer of a seed sees
+// the same edge list and initial state.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/vahana_hip.h"
+
+namespace {
+constexpr uint64_t G = 0x9E3779B97F4A7C15ull;
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline uint64_t hash4(uint64_t s, uint64_t a, uint64_t b, uint64_t c) {
+    uint64_t h = mix64(s + G);
+    h = mix64(h + a + G);
+    h = mix64(h + b + G);
+    return mix64(h + c + G);
+}
+inline double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+constexpr uint64_t KEY_ER = 9ull << 40;
+constexpr uint64_t KEY_BA = 10ull << 40;
+}  // namespace
+
+extern "C" uint64_t vh_hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+    return hash4(seed, a, b, c);
+}
+
+// Directed G(n, p) by geometric skipping over the n-1 candidate sources of
+// each target (Batagelj & Brandes 2005), targets ascending.
+extern "C" int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from, uint32_t* to,
+                           uint64_t cap, uint64_t* m) {
+    uint64_t k = 0;
+    if (n >= 0xFFFFFFFFull) return VH_ERR_ARG;
+    double lq = p < 1.0 ? log1p(-p) : 0.0;
+    for (uint64_t t = 1; t <= n && p > 0.0; t++) {
+        int64_t x = -1;
+        uint64_t draw = 0;
+        for (;;) {
+            uint64_t skip = 0;
+            if (p < 1.0) {
+                double u = u01(hash4(seed, t, draw++, KEY_ER));
+                double q = floor(log1p(-u) / lq);
+                skip = q > 1e18 ? (uint64_t)1e18 : (uint64_t)q;
+            }
+            x += 1 + (int64_t)skip;
+            if (x >= (int64_t)n - 1 || x < 0) break;
+            uint64_t s = (uint64_t)x + 1;          // 1-based among candidates
+            uint64_t src = s < t ? s : s + 1;      // skip the target itself
+            if (k < cap) {
+                from[k] = (uint32_t)src;
+                to[k] = (uint32_t)t;
+            }
+            k++;
+        }
+    }
+    *m = k;
+    return k > cap ? VH_ERR_CAPACITY : VH_OK;
+}
+
+// Barabasi-Albert preferential attachment: a clique on nodes 1..m, then each
+// node v = m+1..n links to m distinct earlier nodes drawn from the endpoint
+// list (probability proportional to degree).  Undirected pairs (v, u) are
+// returned in creation order.
+extern "C" int vh_graph_ba(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b,
+                           uint64_t cap, uint64_t* count) {
+    if (m < 1 || n < m || n >= 0xFFFFFFFFull) return VH_ERR_ARG;
+    uint64_t need = (uint64_t)m * (m - 1) / 2 + (n - m) * (uint64_t)m;
+    *count = need;
+    if (cap < need) return VH_ERR_CAPACITY;
+    std::vector<uint32_t> ends;
+    ends.reserve(2 * need);
+    uint64_t k = 0;
+    for (uint32_t i = 1; i <= m; i++)
+        for (uint32_t j = i + 1; j <= m; j++) {
+            a[k] = j;
+            b[k] = i;
+            k++;
+            ends.push_back(i);
+            ends.push_back(j);
+        }
+    std::vector<uint32_t> chosen(m);
+    for (uint64_t v = (uint64_t)m + 1; v <= n; v++) {
+        uint32_t got = 0;
+        uint64_t draw = 0;
+        uint64_t L = ends.size();
+        while (got < m) {
+            uint32_t u;
+            if (L == 0) {
+                u = (uint32_t)(got + 1);   // m == 1 start: attach to node 1
+            } else {
+                uint64_t h = hash4(seed, v, draw++, KEY_BA);
+                u = ends[((h >> 32) * L) >> 32];
+            }
+            bool dup = false;
+            for (uint32_t q = 0; q < got; q++) dup |= chosen[q] == u;
+            if (dup) continue;
+            chosen[got++] = u;
+        }
+        for (uint32_t q = 0; q < m; q++) {
+            a[k] = (uint32_t)v;
+            b[k] = chosen[q];
+            k++;
+            ends.push_back((uint32_t)v);
+            ends.push_back(chosen[q]);
+        }
+    }
+    return VH_OK;
+}

@@ -1,0 +1,99 @@
+"""The three benchmark models built through the Vahana-style API.
+
+Every builder takes the API module `V` (this package) and returns a small
+namespace with the simulation and its types; the step functions issue one
+apply! each.
+
+* SIR contagion: `SIR` agents (:Immortal) {health::UInt8, days::UInt16},
+  `Contact` edges (:Stateless, :SingleType) on a directed Erdos-Renyi graph.
+* Weighted bounded confidence: `HK` agents {opinion::Float64 (or Float32)},
+  `Knows` edges {w::Float32} (:SingleType) on a symmetrised Barabasi-Albert
+  graph.
+* Dynamic contacts: mortal `Person` agents {health::UInt8, susc::Float32},
+  `Met` edges (:Stateless, :SingleType) rebuilt by every step.
+"""
+import types
+
+import numpy as np
+
+from . import inputs
+
+
+def _types(V):
+    SIR = V.struct("SIR", [("health", "u1"), ("pad", "u1"), ("days", "<u2")])
+    Contact = V.struct("Contact")
+    HK = V.struct("HK", [("opinion", "<f8")])
+    HK32 = V.struct("HK32", [("opinion", "<f4")])
+    Knows = V.struct("Knows", [("w", "<f4")])
+    Person = V.struct("Person", [("health", "u1"), ("susc", "<f4")])
+    Met = V.struct("Met")
+    return SIR, Contact, HK, HK32, Knows, Person, Met
+
+
+def build_sir(V, n, mean_deg=10.0, seed=1, n_init=100, graph_seed=None, edges=None):
+    SIR, Contact, *_ = _types(V)
+    model = (V.ModelTypes()
+             | V.register_agenttype_(SIR, "Immortal")
+             | V.register_edgetype_(Contact, "Stateless", target=SIR)
+             | V.create_model("sir"))
+    sim = V.create_simulation(model)
+    ids = sim._add_agents_array(SIR, inputs.sir_init(n, seed, n_init))
+    if edges is None:
+        p = mean_deg / max(n - 1, 1)
+        edges = inputs.graph_er(n, p, seed if graph_seed is None else graph_seed)
+    fr, to = edges
+    tid = np.uint64(1) << np.uint64(56)
+    sim._add_edges_array(Contact, tid | fr.astype(np.uint64), tid | to.astype(np.uint64))
+    V.finish_init_(sim)
+    return types.SimpleNamespace(sim=sim, T=SIR, E=Contact, ids=ids, nedges=len(to))
+
+
+def step_sir(V, m, p_inf=0.05, seed=1, recovery=7):
+    V.apply_(m.sim, V.transitions.sir(m.E, p_inf, seed, recovery), m.T, [m.T, m.E], m.T)
+
+
+def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None):
+    _, _, HK, HK32, Knows, *_ = _types(V)
+    A = HK32 if f32 else HK
+    model = (V.ModelTypes()
+             | V.register_agenttype_(A, "Immortal")
+             | V.register_edgetype_(Knows, target=A)
+             | V.create_model("opinion"))
+    sim = V.create_simulation(model)
+    x = inputs.opinion_init(n, seed, f32)
+    st = np.zeros(n, dtype=A.dtype)
+    st["opinion"] = x
+    ids = sim._add_agents_array(A, st)
+    if edges is None:
+        a, b = inputs.graph_ba(n, m_ba, seed)
+        edges = inputs.symmetric_weighted(a, b, seed + 1)
+    fr, to, w = edges
+    tid = np.uint64(1) << np.uint64(56)
+    ws = np.zeros(len(w), dtype=Knows.dtype)
+    ws["w"] = w
+    sim._add_edges_array(Knows, tid | fr.astype(np.uint64), tid | to.astype(np.uint64), ws)
+    V.finish_init_(sim)
+    return types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=ids, nedges=len(to), f32=f32)
+
+
+def step_opinion(V, m, eps=0.2):
+    V.apply_(m.sim, V.transitions.opinion(m.E, eps, m.f32), m.T, [m.T, m.E], m.T)
+
+
+def build_contact(V, n, seed=1, n_init=None):
+    *_, Person, Met = _types(V)
+    model = (V.ModelTypes()
+             | V.register_agenttype_(Person)
+             | V.register_edgetype_(Met, "Stateless", target=Person)
+             | V.create_model("contact"))
+    sim = V.create_simulation(model)
+    if n_init is None:
+        n_init = max(1, n // 1000)
+    ids = sim._add_agents_array(Person, inputs.contact_init(n, seed, n_init))
+    V.finish_init_(sim)
+    return types.SimpleNamespace(sim=sim, T=Person, E=Met, ids=ids)
+
+
+def step_contact(V, m, beta=0.05, gamma=0.1, p_death=0.02, seed=1, n_contacts=20, max_redraw=16):
+    V.apply_(m.sim, V.transitions.contact(m.E, beta, gamma, p_death, seed, n_contacts, max_redraw),
+             m.T, [m.T, m.E], [m.T, m.E])

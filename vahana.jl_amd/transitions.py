@@ -1,0 +1,106 @@
+"""Device transition bodies compiled into libvahana_hip.so.
+
+Each factory returns a :class:`Transition` naming the body, the agent/edge
+types bound to its role parameters and its parameter blob — the counterpart
+of the Julia closures the reference passes to apply! (src/Simulation.jl:720).
+"""
+import ctypes as C
+
+from . import _abi
+
+
+class Transition:
+    def __init__(self, id_, name, role_agents=(), role_edges=(), params=None):
+        self.id = id_
+        self.name = name
+        self.role_agents = list(role_agents)
+        self.role_edges = list(role_edges)
+        self.params = params
+
+    def params_bytes(self):
+        if self.params is None:
+            return b""
+        return bytes(C.string_at(C.addressof(self.params), C.sizeof(self.params)))
+
+    def __repr__(self):
+        return f"Transition({self.name})"
+
+
+def _tp(value=0, cmp="always", edge_state=b""):
+    p = _abi.TestParams()
+    p.value = int(value)
+    p.cmp = _abi.CMP[cmp]
+    for i, b in enumerate(bytes(edge_state)[:16]):
+        p.edge_state[i] = b
+    return p
+
+
+def noop():
+    """Returns the state unchanged."""
+    return Transition(_abi.TR_NOOP, "noop", params=_tp())
+
+
+def sum_neighbors(edgetype):
+    """foo = sum of neighborstates_flexible(sim, id, edgetype) .foo (test/core.jl:71-83)."""
+    return Transition(_abi.TR_SUM_NEIGHBORS, "sum_neighbors", role_edges=[edgetype], params=_tp())
+
+
+def die_if(cmp, value=0, edgetype=None):
+    """`nothing` (death) when the predicate on the Int64 field `foo` holds
+    (cmp: always, never, lt, ge, eq, odd, even, num_edges_eq)."""
+    return Transition(_abi.TR_DIE_IF, f"die_if_{cmp}", role_edges=[edgetype],
+                      params=_tp(value, cmp))
+
+
+def add_self_edge(edgetype, state=None):
+    """add_edge!(sim, id, id, state)."""
+    raw = b"" if state is None else bytes(state.tobytes() if hasattr(state, "tobytes") else state)
+    return Transition(_abi.TR_ADD_SELF_EDGE, "add_self_edge", role_edges=[edgetype],
+                      params=_tp(edge_state=raw))
+
+
+def readd_edges(edgetype):
+    """add_edges!(sim, id, edges(sim, id, edgetype)) (test/edges.jl:357-359)."""
+    return Transition(_abi.TR_READD_EDGES, "readd_edges", role_edges=[edgetype], params=_tp())
+
+
+def neighbor_foo_edges(read_edgetype, write_edgetype, die_value):
+    """test/independent.jl:55-68: die when foo == die_value, else add foo-valued
+    edges to and from every neighbour."""
+    return Transition(_abi.TR_NEIGHBOR_FOO_EDGES, "neighbor_foo_edges",
+                      role_edges=[read_edgetype, write_edgetype], params=_tp(die_value))
+
+
+def spawn(agenttype, count, edgetype=None, both_ways=True):
+    """test/independent.jl:165-176: `count` newborns T(foo + 10j), each linked
+    with an edge carrying the newborn id (newborn -> caller, and caller ->
+    newborn when both_ways)."""
+    p = _tp(count)
+    p.cmp = 0 if both_ways else 1
+    return Transition(_abi.TR_SPAWN, "spawn", role_agents=[agenttype], role_edges=[edgetype],
+                      params=p)
+
+
+def count_edges(edgetype):
+    """foo = num_edges(sim, id, edgetype)."""
+    return Transition(_abi.TR_COUNT_EDGES, "count_edges", role_edges=[edgetype], params=_tp())
+
+
+def sir(edgetype, p_inf, seed, recovery=7):
+    """SIR contagion over Stateless in-edges."""
+    p = _abi.SirParams(p_inf=float(p_inf), seed=int(seed), recovery=int(recovery))
+    return Transition(_abi.TR_SIR, "sir", role_edges=[edgetype], params=p)
+
+
+def opinion(edgetype, eps, f32=False):
+    """Weighted bounded confidence (Float64 opinions, or Float32 with f32=True)."""
+    p = _abi.OpinionParams(eps=float(eps))
+    return Transition(_abi.TR_OPINION_F32 if f32 else _abi.TR_OPINION_F64,
+                      "opinion_f32" if f32 else "opinion_f64", role_edges=[edgetype], params=p)
+
+
+def contact(edgetype, beta, gamma, p_death, seed, n_contacts=20, max_redraw=16):
+    """Dynamic contact network with deaths, 1:1 births and rebuilt contacts."""
+    p = _abi.ContactParams(beta=float(beta), gamma=float(gamma), p_death=float(p_death),
+                           seed=int(seed), n_contacts=int(n_contacts), max_redraw=int(max_redraw))
+    return Transition(_abi.TR_CONTACT, "contact", role_edges=[edgetype], params=p)

@@ -1,0 +1,187 @@
+"""Benchmark of apply! on one MI355X.
+
+Workload (default): weighted bounded-confidence opinion dynamics, 10^7 agents
+on a symmetrised Barabasi-Albert graph (m = 5, ~10^8 directed edges), Float64
+opinions, Float32 edge weights, eps = 0.2.  A step is one apply! over every
+agent.  Inputs are synthetic (seeded counter hash), resident in HBM before the
+timed region starts.
+
+Prints one JSON line: agent-updates/s (value), edges/s, the roofline of the
+transition kernel (algorithmic bytes / HIP-event time of the launch on the
+simulation's stream) and the CPU baseline (the checker's apply! restatement on
+one host core over a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def opinion_bytes(n, e, f32):
+    """Compulsory traffic of one step: read + write the state, the offsets,
+    one source index and one Float32 weight per edge; every neighbour state
+    read once (as part of the state read)."""
+    sb = 4 if f32 else 8
+    return n * (sb + sb) + (n + 1) * 4 + e * (4 + 4)
+
+
+def sir_bytes(n, e):
+    return n * (4 + 4) + (n + 1) * 4 + e * 4
+
+
+def cpu_baseline(workload, f32, seconds_cap=25.0):
+    """The checker's apply! (a restatement of the reference's data structures)
+    on one core over a bounded sample: same model, 10^6 agents."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as ora
+    from __graft_entry__ import load_package
+    vh = load_package()
+    M = vh.models
+    n = 1_000_000
+    if workload.startswith("opinion"):
+        a, b = vh.inputs.graph_ba(n, 5, 1)
+        edges = vh.inputs.symmetric_weighted(a, b, 2)
+        m = M.build_opinion(ora, n, seed=1, f32=f32, edges=edges)
+        step = lambda: M.step_opinion(ora, m, eps=0.2)  # noqa: E731
+        ne = len(edges[0])
+    else:
+        edges = vh.inputs.graph_er(n, 10.0 / (n - 1), 1)
+        m = M.build_sir(ora, n, seed=1, edges=edges)
+        step = lambda: M.step_sir(ora, m)  # noqa: E731
+        ne = len(edges[0])
+    step()   # warm-up
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 5 and time.perf_counter() - t_all < seconds_cap:
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": n / med, "unit": "agent-updates/s", "cores": 1, "kind": "port",
+            "edges_per_s": ne / med,
+            "sample": f"{workload} N={n} E={ne}, median of {len(times)} apply! steps after 1 warm-up, "
+                      "1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--workload", default="opinion", choices=["opinion", "opinion32", "sir"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: every simulation here lives on a single GPU "
+              "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from __graft_entry__ import load_package
+
+    vh = load_package()
+    M = vh.models
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    n = args.n
+    f32 = args.workload == "opinion32"
+    t0 = time.perf_counter()
+    if args.workload.startswith("opinion"):
+        a, b = vh.inputs.graph_ba(n, 5, 1)
+        edges = vh.inputs.symmetric_weighted(a, b, 2)
+        del a, b
+        m = M.build_opinion(vh, n, seed=1, f32=f32, edges=edges)
+        step = lambda: M.step_opinion(vh, m, eps=0.2)  # noqa: E731
+        nbytes = opinion_bytes(n, m.nedges, f32)
+        dtype = "f32" if f32 else "f64"
+        workload = (f"opinion dynamics (weighted bounded confidence), Barabasi-Albert m=5 symmetrised, "
+                    f"N={n}, E={m.nedges}, eps=0.2, {'Float32' if f32 else 'Float64'} opinions")
+    else:
+        edges = vh.inputs.graph_er(n, 10.0 / (n - 1), 1)
+        m = M.build_sir(vh, n, seed=1, edges=edges)
+        step = lambda: M.step_sir(vh, m)  # noqa: E731
+        nbytes = sir_bytes(n, m.nedges)
+        dtype = "u8"
+        workload = f"SIR contagion, directed G(n,p), N={n}, E={m.nedges}, p_inf=0.05"
+    del edges
+    log(f"init {time.perf_counter() - t0:.1f}s: {workload}")
+
+    stream = torch.cuda.Stream(device=dev)
+    L = vh._abi.lib()
+    L.set_stream(m.sim.handle, stream.cuda_stream)
+    vh.enable_asserts(False)       # production mode (enable_asserts(false) in the reference)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        step()
+        ends[k].record(stream)
+    L.sync(m.sim.handle)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
+    ev_mean = sum(ev) / len(ev)
+    ms = wall * 1e3 / args.steps
+    value = n / (wall / args.steps)
+    edges_s = m.nedges / (wall / args.steps)
+    achieved = nbytes / (ev_mean * 1e-3) / 1e9
+    log(f"step wall {ms:.3f} ms, kernel events mean {ev_mean:.3f} ms (min {ev[0]:.3f} max {ev[-1]:.3f})")
+    vh.enable_asserts(True)
+    num_trans = m.sim._num_transitions()
+    assert num_trans == 1 + args.warmup + args.steps
+
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"{args.workload}_n{n}_traffic.json")
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(args.workload, f32)
+        log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {cpu}")
+
+    out = {
+        "metric": "agent-updates/s per apply! (edges/s alongside)",
+        "value": value,
+        "unit": "agent-updates/s",
+        "edges_per_s": edges_s,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded counter-hash graph and states)",
+        "config": {"workload": workload, "agents": n, "edges": m.nedges, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

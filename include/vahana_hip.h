@@ -245,6 +245,10 @@ int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from,
 int vh_graph_ba(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b,
                 uint64_t cap, uint64_t* count);
 
+/* Float32 weights in (0, 1): out[i] = (2 (h >> 41) + 1) 2^-24 with
+ * h = vh_hash4(seed, i, b, c) */
+int vh_fill_open01f(uint64_t seed, uint64_t b, uint64_t c, uint64_t n, float* out);
+
 /* ------------------------------------------------------ counter-hash RNG
  * SplitMix64 finaliser chained over (seed, a, b, c); the same function is
  * compiled into the device bodies. */

@@ -1092,7 +1092,7 @@ int or_num_edges(or_sim* s, uint8_t et, uint64_t* out) {
 }
 int or_read_agents(or_sim* s, uint8_t t, void* states, uint8_t* died, uint64_t cap, uint64_t* nslots) {
     if (t < 1 || t > s->m->na) return fail(VH_ERR_UNREGISTERED, "unregistered agent type");
-    rw_t* r = &s->a[t].read;
+    rw_t* r = s->initialized ? &s->a[t].read : &s->a[t].write;
     uint64_t len = s->a[t].nextid - 1;
     if (len > r->len) len = r->len;
     *nslots = len;

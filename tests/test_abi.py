@@ -114,3 +114,13 @@ def test_ba_generator_properties(vh):
     deg = np.bincount(np.concatenate([a, b]), minlength=n + 1)[1:]
     assert deg.min() >= m - 1 and deg.sum() == 2 * len(a)
     assert deg.max() > 10 * m                                  # heavy tail (hubs)
+
+
+def test_weight_fill_matches_hash(vh):
+    a, b = vh.inputs.graph_ba(300, 3, 1)
+    fr, to, w = vh.inputs.symmetric_weighted(a, b, 9)
+    idx = np.arange(len(w), dtype=np.uint64)
+    ref = vh.inputs.open01f(vh.inputs.hash4(9, idx, 0, vh.inputs.KEY_WEIGHT))
+    assert np.array_equal(w, ref)
+    assert np.all((w > 0) & (w < 1))
+    assert np.array_equal(fr[0::2], to[1::2]) and np.array_equal(to[0::2], fr[1::2])

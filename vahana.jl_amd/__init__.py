@@ -89,19 +89,29 @@ class StructType:
             return Value(self, None)
         return Value(self, np.array(tuple(args), dtype=self.dtype)[()])
 
+    def pack(self, values):
+        """Contiguous array of this type with zeroed padding bytes (numpy's
+        structured conversions leave padding uninitialised)."""
+        if isinstance(values, np.ndarray) and values.dtype.names:
+            src = values
+        else:
+            src = np.array([v.v if isinstance(v, Value) else v for v in values], dtype=self.dtype)
+        out = np.zeros(len(src), dtype=self.dtype)
+        for name in self.dtype.names:
+            out[name] = src[name]
+        return out
+
     def to_bytes(self, values):
         if not self.fields:
             return b""
-        if isinstance(values, np.ndarray):
-            arr = values.astype(self.dtype, copy=False)
-        else:
-            arr = np.array([v.v if isinstance(v, Value) else v for v in values], dtype=self.dtype)
-        return np.ascontiguousarray(arr).tobytes()
+        return self.pack(values).tobytes()
 
     def from_bytes(self, buf, n):
+        """Array over a private copy of the raw bytes (padding preserved:
+        numpy's structured copies drop padding bytes)."""
         if not self.fields:
             return [None] * n
-        return np.frombuffer(buf, dtype=self.dtype, count=n).copy()
+        return np.frombuffer(bytearray(buf[:n * self.nbytes]), dtype=self.dtype, count=n)
 
     def __repr__(self):
         return f"StructType({self.name})"
@@ -313,7 +323,7 @@ class Simulation:
         return [ids[i] for i in range(n)]
 
     def _add_agents_array(self, T, arr):
-        arr = np.ascontiguousarray(arr, dtype=T.dtype)
+        arr = T.pack(arr) if T.fields else np.zeros(len(arr), dtype=np.uint8)
         n = len(arr)
         ids = np.empty(n, dtype=np.uint64)
         _check(self.lib.add_agents(self.handle, self._aid(T), arr.ctypes.data_as(C.c_void_p), n,
@@ -326,9 +336,7 @@ class Simulation:
         n = len(to)
         sp = None
         if T.fields and states is not None:
-            if not isinstance(states, np.ndarray):
-                states = [x.v if isinstance(x, Value) else x for x in states]
-            states = np.ascontiguousarray(np.array(states, dtype=T.dtype))
+            states = T.pack(states)
             sp = states.ctypes.data_as(C.c_void_p)
         _check(self.lib.add_edges(self.handle, self._eid(T),
                                   from_.ctypes.data_as(C.POINTER(C.c_uint64)),

@@ -121,6 +121,8 @@ DEVICE_ONLY = [
     ("graph_er", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
     ("graph_ba", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
     ("hash4", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
+    ("fill_open01f", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                               C.POINTER(C.c_float)]),
 ]
 MAPREDUCE = ("mapreduce", C.c_int, [_vp, C.c_int, C.c_uint8, C.POINTER(MapDesc), C.c_int,
                                      C.c_int, _vp, _vp])

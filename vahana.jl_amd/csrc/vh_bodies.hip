@@ -321,47 +321,93 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
 
 // ----------------------------------------------------------------- opinion
 // Canonical order: degree <= 64 sequential; degree > 64 lane-strided
-// partials (lane l: k = l, l+64, ...) and an xor butterfly.
+// partials (lane l: k = l, l+64, ...) and an xor butterfly.  One launch per
+// step: blocks [0, nlight) run thread-per-agent over all agents (skipping the
+// heavy ones), the remaining blocks run one wave per heavy agent.
+// Light agents (degree <= 64) of a 256-agent block: their in-edges are
+// concatenated in agent order and processed in tiles of OP_TILE edge slots.
+// Phase 1 (all threads, coalesced src/w loads, parallel gathers) writes each
+// edge's two terms to LDS; phase 2 lets every thread add its own agent's terms
+// in edge order, so each agent's sums are the sequential canonical sums.
+constexpr int OP_TILE = 2048;
+
 template <class T>
-__global__ __launch_bounds__(256) void k_opinion_light(const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ src,
-                                                       const T* __restrict__ x,
-                                                       const float* __restrict__ w,
-                                                       T* __restrict__ xo, uint64_t n,
-                                                       uint64_t nrows, uint64_t rowbase, T eps) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const T xi = x[i];
-    T res = xi;
-    if (i < nrows) {
-        uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
-        if (e - b > 64u) return;   // wave-per-agent kernel
-        T num = (T)0, den = (T)0;
-        for (uint32_t k = b; k < e; k++) {
-            T xj = x[src[k] - 1];
-            T wk = (T)w[k];
-            if (fabs(xj - xi) <= eps) {
-                num = num + wk * xj;
-                den = den + wk;
-            }
+__device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
+                                              const uint32_t* __restrict__ src,
+                                              const T* __restrict__ x, const float* __restrict__ w,
+                                              T* __restrict__ xo, uint64_t n, uint64_t nrows,
+                                              uint64_t rowbase, T eps) {
+    __shared__ T s_num[OP_TILE];
+    __shared__ float s_den[OP_TILE];
+    __shared__ uint8_t s_own[OP_TILE];
+    __shared__ uint32_t s_b[256], s_p[256], s_wsum[4];
+    __shared__ T s_xi[256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + tid;
+    const bool valid = i < n;
+    uint32_t b = 0, d = 0;
+    T xi = (T)0;
+    if (valid) {
+        xi = x[i];
+        if (i < nrows) {
+            b = off[rowbase + i];
+            d = off[rowbase + i + 1] - b;
         }
-        if (den > (T)0) res = num / den;
     }
-    xo[i] = res;
+    const bool light = valid && d <= 64u;
+    const uint32_t ld = light ? d : 0u;
+    // block exclusive scan of the light degrees
+    uint32_t incl = ld;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    uint32_t pre = 0, total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t v = s_wsum[q];
+        pre += q < wv ? v : 0u;
+        total += v;
+    }
+    const uint32_t P = pre + incl - ld;
+    s_b[tid] = b;
+    s_p[tid] = P;
+    s_xi[tid] = xi;
+    T num = (T)0, den = (T)0;
+    for (uint32_t t0 = 0; t0 < total; t0 += OP_TILE) {
+        const uint32_t lo = P > t0 ? P : t0;
+        const uint32_t hi = (P + ld) < (t0 + OP_TILE) ? (P + ld) : (t0 + OP_TILE);
+        for (uint32_t p = lo; p < hi; p++) s_own[p - t0] = (uint8_t)tid;
+        __syncthreads();
+        const uint32_t cnt = (total - t0) < (uint32_t)OP_TILE ? (total - t0) : (uint32_t)OP_TILE;
+        for (uint32_t q = tid; q < cnt; q += 256) {
+            const uint32_t a = s_own[q];
+            const uint32_t k = s_b[a] + (t0 + q - s_p[a]);
+            const T xj = x[src[k] - 1];
+            const float wk = w[k];
+            const bool acc = fabs(xj - s_xi[a]) <= eps;
+            s_num[q] = acc ? (T)wk * xj : (T)0;
+            s_den[q] = acc ? wk : 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t p = lo; p < hi; p++) {
+            num = num + s_num[p - t0];
+            den = den + (T)s_den[p - t0];
+        }
+        __syncthreads();
+    }
+    if (light) xo[i] = den > (T)0 ? num / den : xi;
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_opinion_heavy(const uint32_t* __restrict__ heavy,
-                                                       uint64_t nheavy,
-                                                       const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ src,
-                                                       const T* __restrict__ x,
-                                                       const float* __restrict__ w,
-                                                       T* __restrict__ xo, uint64_t rowbase, T eps) {
-    uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
+                                              const uint32_t* __restrict__ src,
+                                              const T* __restrict__ x, const float* __restrict__ w,
+                                              T* __restrict__ xo, uint64_t r, uint64_t rowbase, T eps) {
     const int lane = threadIdx.x & 63;
-    if (wv >= nheavy) return;
-    uint64_t r = heavy[wv];
     uint64_t i = r - rowbase;
     const T xi = x[i];
     uint32_t b = off[r], e = off[r + 1];
@@ -380,6 +426,22 @@ __global__ __launch_bounds__(256) void k_opinion_heavy(const uint32_t* __restric
         den = den + __shfl_xor(den, o, 64);
     }
     if (lane == 0) xo[i] = den > (T)0 ? num / den : xi;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ src,
+                                                 const T* __restrict__ x, const float* __restrict__ w,
+                                                 T* __restrict__ xo, uint64_t n, uint64_t nrows,
+                                                 uint64_t rowbase, T eps,
+                                                 const uint32_t* __restrict__ heavy, uint64_t nheavy,
+                                                 uint32_t nlight) {
+    if (blockIdx.x < nlight) {
+        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps);
+    } else {
+        uint64_t wv = ((uint64_t)(blockIdx.x - nlight) * blockDim.x + threadIdx.x) >> 6;
+        if (wv < nheavy) opinion_heavy<T>(off, src, x, w, xo, heavy[wv], rowbase, eps);
+    }
 }
 
 __global__ void k_heavy_rows(const uint32_t* __restrict__ off, uint64_t rows, uint64_t rowbase,
@@ -532,26 +594,18 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     if (!A.nslots) return VH_OK;
     uint64_t nrows = E.rows ? E.nrows[t] : 0;
     if (E.rows) RC(build_heavy_bins(s, E, t));
-    dim3 g(grid_for(A.nslots, 256)), b(256);
-    if (f32) {
-        hipLaunchKernelGGL(k_opinion_light<float>, g, b, 0, s.stream, E.off.as<uint32_t>(),
-                           E.src.as<uint32_t>(), (const float*)A.rstate(), E.st.as<float>(),
-                           (float*)A.wstate(), A.nslots, nrows, E.rowbase[t], (float)p.eps);
-        if (E.nheavy)
-            hipLaunchKernelGGL(k_opinion_heavy<float>, dim3(grid_for(E.nheavy * 64, 256)), b, 0, s.stream,
-                               E.heavy.as<uint32_t>(), E.nheavy, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
-                               (const float*)A.rstate(), E.st.as<float>(), (float*)A.wstate(),
-                               E.rowbase[t], (float)p.eps);
-    } else {
-        hipLaunchKernelGGL(k_opinion_light<double>, g, b, 0, s.stream, E.off.as<uint32_t>(),
-                           E.src.as<uint32_t>(), (const double*)A.rstate(), E.st.as<float>(),
-                           (double*)A.wstate(), A.nslots, nrows, E.rowbase[t], p.eps);
-        if (E.nheavy)
-            hipLaunchKernelGGL(k_opinion_heavy<double>, dim3(grid_for(E.nheavy * 64, 256)), b, 0, s.stream,
-                               E.heavy.as<uint32_t>(), E.nheavy, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
-                               (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(),
-                               E.rowbase[t], p.eps);
-    }
+    uint64_t nheavy = E.rows ? E.nheavy : 0;
+    uint32_t nlight = grid_for(A.nslots, 256);
+    uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+    dim3 g(nlight + nh), b(256);
+    if (f32)
+        hipLaunchKernelGGL(k_opinion<float>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
+                           (const float*)A.rstate(), E.st.as<float>(), (float*)A.wstate(), A.nslots,
+                           nrows, E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nlight);
+    else
+        hipLaunchKernelGGL(k_opinion<double>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
+                           (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(), A.nslots,
+                           nrows, E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nlight);
     HIPCHECK(hipGetLastError());
     return VH_OK;
 }

@@ -30,6 +30,16 @@ extern "C" uint64_t vh_hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) 
     return hash4(seed, a, b, c);
 }
 
+// out[i] = odd multiple of 2^-24 in (0, 1) from hash(seed, i, b, c): the
+// Float32 edge weights of the opinion model
+extern "C" int vh_fill_open01f(uint64_t seed, uint64_t b, uint64_t c, uint64_t n, float* out) {
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t h = hash4(seed, i, b, c);
+        out[i] = (float)((double)((h >> 41) * 2 + 1) * 0x1.0p-24);
+    }
+    return VH_OK;
+}
+
 // Directed G(n, p) by geometric skipping over the n-1 candidate sources of
 // each target (Batagelj & Brandes 2005), targets ascending.
 extern "C" int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from, uint32_t* to,

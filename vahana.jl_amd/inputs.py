@@ -86,8 +86,8 @@ def symmetric_weighted(a, b, wseed):
     to = np.empty(2 * k, dtype=np.uint32)
     fr[0::2], to[0::2] = a, b
     fr[1::2], to[1::2] = b, a
-    idx = np.arange(2 * k, dtype=np.uint64)
-    w = open01f(hash4(wseed, idx, 0, KEY_WEIGHT))
+    w = np.empty(2 * k, dtype=np.float32)
+    _abi.lib().fill_open01f(wseed, 0, KEY_WEIGHT, 2 * k, w.ctypes.data_as(C.POINTER(C.c_float)))
     return fr, to, w
 
 

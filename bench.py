@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--workload", default="opinion", choices=["opinion", "opinion32", "sir"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="run a 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: every simulation here lives on a single GPU "
@@ -124,6 +126,12 @@ def main():
     L.set_stream(m.sim.handle, stream.cuda_stream)
     vh.enable_asserts(False)       # production mode (enable_asserts(false) in the reference)
 
+    if args.calib:
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

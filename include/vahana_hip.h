@@ -233,6 +233,40 @@ const char* vh_last_error(void);
  * with asserts off apply! stays asynchronous unless it has births or deaths */
 int vh_enable_asserts(int enable);
 
+/* ------------------------------------------------------ ensembles
+ * Many independent SIR simulations advanced by one segmented launch per
+ * step: parameter sweeps (every member bit-identical to the same simulation
+ * run alone through apply!) and the step batches of the serving loop.
+ * Members own their device buffers and may join or leave between steps.
+ * No counterpart in the reference (it runs one simulation per MPI job). */
+typedef struct vh_ensemble* vh_ensemble_t;
+typedef struct vh_sir_member {
+    uint64_t n;            /* agents                                          */
+    double mean_deg;       /* G(n, p) with p = mean_deg / (n - 1)            */
+    double p_inf;          /* infection probability per infected in-neighbour */
+    uint64_t seed;         /* transition and initial-state seed              */
+    uint64_t graph_seed;   /* vh_graph_gnp seed                              */
+    uint32_t recovery;     /* days until I -> R                              */
+    uint32_t n_init;       /* expected initial I: each agent with n_init / n */
+} vh_sir_member;
+int vh_ensemble_create(vh_ensemble_t* out);
+int vh_ensemble_destroy(vh_ensemble_t e);
+int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* m, uint32_t* member_id);
+int vh_ensemble_remove(vh_ensemble_t e, uint32_t member_id);
+/* block size (128..1024) and XCD-aware block order of the step launch */
+int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t xcd_remap);
+/* advance every member; use_graph: capture the steps into one hipGraph */
+int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph);
+int vh_ensemble_sync(vh_ensemble_t e);
+/* member state (4-byte SIR records) and its next step number */
+int vh_ensemble_read(vh_ensemble_t e, uint32_t member_id, void* states, uint64_t cap, uint64_t* n,
+                     uint64_t* step);
+int vh_ensemble_info(vh_ensemble_t e, uint64_t* members, uint64_t* agents, uint64_t* edges,
+                     uint64_t* bytes);
+/* times every (block, order) candidate on a snapshot, keeps the fastest */
+int vh_ensemble_autotune(vh_ensemble_t e, uint32_t trial_steps, uint32_t* block, uint32_t* xcd_remap,
+                         float* ms_per_step);
+
 /* ------------------------------------------------------ graph generators
  * Host-side sources for add_graph! (src/GraphsSupport.jl:34-57) with a
  * counter-hash stream so both the device path and any checker receive the
@@ -240,6 +274,12 @@ int vh_enable_asserts(int enable);
  * Directed G(n, p): for each target ascending, sources ascending. */
 int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from,
                 uint32_t* to, uint64_t cap, uint64_t* m);
+/* Directed G(n, p), p = mean_deg / (n - 1): target t draws its in-degree
+ * from Binomial(n - 1, p) by an inverse-CDF table, then that many distinct
+ * uniform sources; edges per target in draw order, targets ascending.  The
+ * ensemble builds the identical graph on the device. */
+int vh_graph_gnp(uint64_t n, double mean_deg, uint64_t seed, uint32_t* from, uint32_t* to,
+                 uint64_t cap, uint64_t* m);
 /* Barabasi-Albert, m0 = m clique, undirected pairs in creation order
  * (count = m(m-1)/2 + (n-m)m) */
 int vh_graph_ba(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b,

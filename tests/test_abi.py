@@ -124,3 +124,20 @@ def test_weight_fill_matches_hash(vh):
     assert np.array_equal(w, ref)
     assert np.all((w > 0) & (w < 1))
     assert np.array_equal(fr[0::2], to[1::2]) and np.array_equal(to[0::2], fr[1::2])
+
+
+def test_gnp_generator_properties(vh):
+    n, md = 30000, 12.0
+    fr, to = vh.inputs.graph_gnp(n, md, 4)
+    assert np.all(fr != to) and np.all((fr >= 1) & (fr <= n))
+    assert np.all(np.diff(to.astype(np.int64)) >= 0)
+    deg = np.bincount(to, minlength=n + 1)[1:]
+    p = md / (n - 1)
+    assert abs(deg.mean() - md) < 0.1                      # E = (n-1) p
+    assert abs(deg.var() - md * (1 - p)) < 0.5             # Binomial variance
+    key = to.astype(np.int64) * (n + 1) + fr
+    assert len(np.unique(key)) == len(key)                  # distinct sources per target
+    # sources are uniform: mean source id ~ (n + 1) / 2
+    assert abs(fr.mean() / ((n + 1) / 2) - 1) < 0.01
+    fr2, to2 = vh.inputs.graph_gnp(n, md, 4)
+    assert np.array_equal(fr, fr2) and np.array_equal(to, to2)

@@ -85,6 +85,12 @@ class ContactParams(C.Structure):
                 ("seed", C.c_uint64), ("n_contacts", C.c_uint32), ("max_redraw", C.c_uint32)]
 
 
+class SirMember(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("mean_deg", C.c_double), ("p_inf", C.c_double),
+                ("seed", C.c_uint64), ("graph_seed", C.c_uint64), ("recovery", C.c_uint32),
+                ("n_init", C.c_uint32)]
+
+
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
@@ -123,6 +129,17 @@ DEVICE_ONLY = [
     ("hash4", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
     ("fill_open01f", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                C.POINTER(C.c_float)]),
+    ("graph_gnp", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
+    ("ensemble_create", C.c_int, [C.POINTER(_vp)]),
+    ("ensemble_destroy", C.c_int, [_vp]),
+    ("ensemble_add_sir", C.c_int, [_vp, C.POINTER(SirMember), _u32p]),
+    ("ensemble_remove", C.c_int, [_vp, C.c_uint32]),
+    ("ensemble_set_config", C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    ("ensemble_step", C.c_int, [_vp, C.c_uint32, C.c_int]),
+    ("ensemble_sync", C.c_int, [_vp]),
+    ("ensemble_read", C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64, _u64p, _u64p]),
+    ("ensemble_info", C.c_int, [_vp, _u64p, _u64p, _u64p, _u64p]),
+    ("ensemble_autotune", C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_float)]),
 ]
 MAPREDUCE = ("mapreduce", C.c_int, [_vp, C.c_int, C.c_uint8, C.POINTER(MapDesc), C.c_int,
                                      C.c_int, _vp, _vp])

@@ -330,28 +330,29 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
 // edge's two terms to LDS; phase 2 lets every thread add its own agent's terms
 // in edge order, so each agent's sums are the sequential canonical sums.
 constexpr int OP_TILE = 2048;
+constexpr int OP_U = 8;   // gathers in flight per thread
 
 template <class T>
 __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
                                               const T* __restrict__ x, const float* __restrict__ w,
                                               T* __restrict__ xo, uint64_t n, uint64_t nrows,
-                                              uint64_t rowbase, T eps) {
+                                              uint64_t rowbase, T eps, uint32_t blk) {
     __shared__ T s_num[OP_TILE];
     __shared__ float s_den[OP_TILE];
     __shared__ uint8_t s_own[OP_TILE];
     __shared__ uint32_t s_b[256], s_p[256], s_wsum[4];
     __shared__ T s_xi[256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + tid;
+    const uint64_t i = (uint64_t)blk * 256 + tid;
     const bool valid = i < n;
     uint32_t b = 0, d = 0;
     T xi = (T)0;
     if (valid) {
-        xi = x[i];
+        xi = __builtin_nontemporal_load(x + i);
         if (i < nrows) {
-            b = off[rowbase + i];
-            d = off[rowbase + i + 1] - b;
+            b = __builtin_nontemporal_load(off + rowbase + i);
+            d = __builtin_nontemporal_load(off + rowbase + i + 1) - b;
         }
     }
     const bool light = valid && d <= 64u;
@@ -383,14 +384,35 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
         for (uint32_t p = lo; p < hi; p++) s_own[p - t0] = (uint8_t)tid;
         __syncthreads();
         const uint32_t cnt = (total - t0) < (uint32_t)OP_TILE ? (total - t0) : (uint32_t)OP_TILE;
-        for (uint32_t q = tid; q < cnt; q += 256) {
-            const uint32_t a = s_own[q];
-            const uint32_t k = s_b[a] + (t0 + q - s_p[a]);
-            const T xj = x[src[k] - 1];
-            const float wk = w[k];
-            const bool acc = fabs(xj - s_xi[a]) <= eps;
-            s_num[q] = acc ? (T)wk * xj : (T)0;
-            s_den[q] = acc ? wk : 0.0f;
+        // batches of OP_U slots per thread: all index loads, then all
+        // gathers, then the terms (keeps OP_U gathers in flight per lane)
+        for (uint32_t q0 = 0; q0 < cnt; q0 += 256u * OP_U) {
+            uint32_t kk[OP_U], sj[OP_U], ag[OP_U];
+            float wk[OP_U];
+            T xj[OP_U];
+#pragma unroll
+            for (int u = 0; u < OP_U; u++) {
+                const uint32_t q = q0 + (uint32_t)u * 256u + tid;
+                const uint32_t a = q < cnt ? s_own[q] : 0u;
+                ag[u] = a;
+                kk[u] = q < cnt ? s_b[a] + (t0 + q - s_p[a]) : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int u = 0; u < OP_U; u++) {
+                sj[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(src + kk[u]) : 1u;
+                wk[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(w + kk[u]) : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < OP_U; u++) xj[u] = kk[u] != 0xFFFFFFFFu ? x[sj[u] - 1] : (T)0;
+#pragma unroll
+            for (int u = 0; u < OP_U; u++) {
+                const uint32_t q = q0 + (uint32_t)u * 256u + tid;
+                if (q < cnt) {
+                    const bool acc = fabs(xj[u] - s_xi[ag[u]]) <= eps;
+                    s_num[q] = acc ? (T)wk[u] * xj[u] : (T)0;
+                    s_den[q] = acc ? wk[u] : 0.0f;
+                }
+            }
         }
         __syncthreads();
         for (uint32_t p = lo; p < hi; p++) {
@@ -399,7 +421,7 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
         }
         __syncthreads();
     }
-    if (light) xo[i] = den > (T)0 ? num / den : xi;
+    if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
 }
 
 template <class T>
@@ -412,7 +434,24 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
     const T xi = x[i];
     uint32_t b = off[r], e = off[r + 1];
     T num = (T)0, den = (T)0;
-    for (uint32_t k = b + lane; k < e; k += 64) {
+    // lane l adds k = l, l+64, ... in order; four strides in flight per lane
+    uint32_t k = b + lane;
+    for (; k + 192 < e; k += 256) {
+        uint32_t s0 = src[k], s1 = src[k + 64], s2 = src[k + 128], s3 = src[k + 192];
+        float w0 = w[k], w1 = w[k + 64], w2 = w[k + 128], w3 = w[k + 192];
+        T x0 = x[s0 - 1], x1 = x[s1 - 1], x2 = x[s2 - 1], x3 = x[s3 - 1];
+        bool a0 = fabs(x0 - xi) <= eps, a1 = fabs(x1 - xi) <= eps;
+        bool a2 = fabs(x2 - xi) <= eps, a3 = fabs(x3 - xi) <= eps;
+        num = num + (a0 ? (T)w0 * x0 : (T)0);
+        den = den + (a0 ? (T)w0 : (T)0);
+        num = num + (a1 ? (T)w1 * x1 : (T)0);
+        den = den + (a1 ? (T)w1 : (T)0);
+        num = num + (a2 ? (T)w2 * x2 : (T)0);
+        den = den + (a2 ? (T)w2 : (T)0);
+        num = num + (a3 ? (T)w3 * x3 : (T)0);
+        den = den + (a3 ? (T)w3 : (T)0);
+    }
+    for (; k < e; k += 64) {
         T xj = x[src[k] - 1];
         T wk = (T)w[k];
         if (fabs(xj - xi) <= eps) {
@@ -436,10 +475,13 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
                                                  uint64_t rowbase, T eps,
                                                  const uint32_t* __restrict__ heavy, uint64_t nheavy,
                                                  uint32_t nlight) {
-    if (blockIdx.x < nlight) {
-        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps);
+    // hub waves first: they are the longest, so they start before the light
+    // blocks instead of forming the tail of the launch
+    const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+    if (blockIdx.x >= nh) {
+        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps, blockIdx.x - nh);
     } else {
-        uint64_t wv = ((uint64_t)(blockIdx.x - nlight) * blockDim.x + threadIdx.x) >> 6;
+        uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
         if (wv < nheavy) opinion_heavy<T>(off, src, x, w, xo, heavy[wv], rowbase, eps);
     }
 }

@@ -1,0 +1,461 @@
+// vh_ensemble.hip — many SIR simulations advanced by one segmented launch
+// per step (ensembles / parameter sweeps, and the step batches of the serving
+// loop).  Every member keeps its own device buffers; a per-step member table
+// (member descriptors + a block map) lets members join and leave between
+// steps.  Each member's result is bit-identical to the same simulation run
+// alone through apply!: the body, its counter-hash keys (seed, AgentID, step,
+// edge position) and the CSR order are the same.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "vh_engine.h"
+
+using namespace vh;
+
+namespace {
+
+constexpr uint64_t KEY_DEG = 11ull << 40;
+constexpr uint64_t KEY_SRC = 12ull << 40;
+
+// inverse-CDF table of Binomial(n - 1, p); the last entry is forced to 2.0
+std::vector<double> binom_cdf(uint64_t n, double p) {
+    std::vector<double> cdf;
+    if (n < 2 || p <= 0.0) {
+        cdf.push_back(2.0);
+        return cdf;
+    }
+    double N = (double)(n - 1);
+    double lp = log(p), lq = log1p(-p), acc = 0.0;
+    for (uint64_t k = 0; k <= n - 1; k++) {
+        double lpmf = lgamma(N + 1.0) - lgamma((double)k + 1.0) - lgamma(N - (double)k + 1.0) +
+                      (double)k * lp + (N - (double)k) * lq;
+        acc += exp(lpmf);
+        cdf.push_back(acc);
+        if (k > (uint64_t)(N * p) && 1.0 - acc < 1e-18) break;
+        if (cdf.size() >= 4096) break;
+    }
+    cdf.back() = 2.0;
+    return cdf;
+}
+
+__host__ __device__ inline uint32_t gnp_degree(const double* cdf, uint32_t ncdf, uint64_t seed,
+                                               uint64_t t) {
+    double u = vh_u01(vh_hash(seed, t, 0, KEY_DEG));
+    uint32_t d = 0;
+    while (d + 1 < ncdf && !(u < cdf[d])) d++;
+    return d;
+}
+// the d distinct sources of target t (1-based), in draw order
+__host__ __device__ inline void gnp_sources(uint64_t n, uint64_t seed, uint64_t t, uint32_t d,
+                                            uint32_t* out) {
+    uint64_t draw = 0;
+    for (uint32_t j = 0; j < d;) {
+        uint64_t h = vh_hash(seed, t, ++draw, KEY_SRC);
+        uint64_t s = 1 + (((h >> 32) * (n - 1)) >> 32);
+        if (s >= t) s++;
+        bool dup = false;
+        for (uint32_t q = 0; q < j; q++) dup |= out[q] == (uint32_t)s;
+        if (dup) continue;
+        out[j++] = (uint32_t)s;
+    }
+}
+
+__global__ void k_gnp_degree(const double* cdf, uint32_t ncdf, uint64_t seed, uint64_t n,
+                             uint32_t* deg) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) deg[i] = gnp_degree(cdf, ncdf, seed, i + 1);
+}
+__global__ void k_gnp_fill(uint64_t n, uint64_t seed, const uint32_t* off, uint32_t* src) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) gnp_sources(n, seed, i + 1, off[i + 1] - off[i], src + off[i]);
+}
+// initial state: I with probability n_init / n (hash keyed like the solo input)
+__global__ void k_sir_init(uint32_t* st, uint64_t n, uint64_t seed, double frac, uint32_t type) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t id = vh_make_id(type, i + 1);
+    st[i] = vh_u01(vh_hash(seed, id, 0, VH_KEY_INIT)) < frac ? (uint32_t)VH_I : (uint32_t)VH_S;
+}
+
+struct MemberDesc {
+    const uint32_t* off;
+    const uint32_t* src;
+    const uint32_t* rs;
+    uint32_t* ws;
+    uint64_t n;
+    uint64_t step;
+    uint64_t seed;
+    double p;
+    uint32_t rec, type;
+};
+
+// one launch for every member: logical block b belongs to member
+// bmember[b] and covers agents [bstart[b], bstart[b] + block) of it
+__global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
+                               const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap) {
+    uint32_t b = blockIdx.x;
+    if (remap) {
+        // XCD-aware order: consecutive blocks of one member land on one XCD
+        // (blocks are dispatched round-robin over the 8 XCDs)
+        uint32_t per = (nblocks + 7) / 8;
+        uint32_t x = b % 8, q = b / 8;
+        b = x * per + q;
+        if (b >= nblocks) return;
+    }
+    const MemberDesc& m = md[bmember[b]];
+    uint64_t i = (uint64_t)bstart[b] + threadIdx.x;
+    if (i >= m.n) return;
+    uint32_t s = m.rs[i];
+    uint32_t h = s & 0xFFu;
+    if (h == VH_S) {
+        const uint64_t id = vh_make_id(m.type, i + 1);
+        uint32_t bb = m.off[i], e = m.off[i + 1];
+        for (uint32_t k = bb; k < e; k++) {
+            uint32_t j = m.src[k];
+            if ((m.rs[j - 1] & 0xFFu) == VH_I && vh_u01(vh_hash(m.seed, id, m.step, k - bb + 1)) < m.p) {
+                s = (s & 0xFF00u) | VH_I;
+                break;
+            }
+        }
+    } else if (h == VH_I) {
+        uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
+        s = (s & 0xFF00u) | (days << 16) | (days >= m.rec ? VH_R : VH_I);
+    }
+    m.ws[i] = s;
+}
+
+}  // namespace
+
+// ====================================================================== host side
+struct Member {
+    uint32_t id;
+    uint64_t n, nedges, step;
+    uint64_t seed;
+    double p;
+    uint32_t rec;
+    DevBuf off, src, st[2];
+    int cur = 0;
+};
+
+struct vh_ensemble {
+    std::map<uint32_t, Member*> members;
+    uint32_t next_id = 1;
+    hipStream_t stream = nullptr;
+    uint32_t block = 256, remap = 1;
+    DevBuf d_desc, d_bmember, d_bstart;
+    std::vector<MemberDesc> h_desc;
+    std::vector<uint32_t> h_bmember, h_bstart;
+    uint32_t nblocks = 0;
+    bool table_valid = false;
+    uint32_t table_block = 0;
+    Scratch scratch;
+    DevBuf d_cdf;
+};
+
+static int build_table(vh_ensemble* e) {
+    e->h_bmember.clear();
+    e->h_bstart.clear();
+    e->h_desc.clear();
+    uint32_t mi = 0;
+    for (auto& kv : e->members) {
+        Member* m = kv.second;
+        for (uint64_t a = 0; a < m->n; a += e->block) {
+            e->h_bmember.push_back(mi);
+            e->h_bstart.push_back((uint32_t)a);
+        }
+        mi++;
+    }
+    e->nblocks = (uint32_t)e->h_bmember.size();
+    RC(e->d_bmember.ensure(e->h_bmember.size() * 4 + 8, e->stream));
+    RC(e->d_bstart.ensure(e->h_bstart.size() * 4 + 8, e->stream));
+    RC(e->d_desc.ensure((e->members.size() + 1) * sizeof(MemberDesc), e->stream));
+    if (e->nblocks) {
+        HIPCHECK(hipMemcpyAsync(e->d_bmember.p, e->h_bmember.data(), e->nblocks * 4, hipMemcpyHostToDevice, e->stream));
+        HIPCHECK(hipMemcpyAsync(e->d_bstart.p, e->h_bstart.data(), e->nblocks * 4, hipMemcpyHostToDevice, e->stream));
+        HIPCHECK(hipStreamSynchronize(e->stream));
+    }
+    e->table_valid = true;
+    e->table_block = e->block;
+    return VH_OK;
+}
+
+// member descriptors for one step (read/write buffers and step numbers)
+static int upload_desc(vh_ensemble* e) {
+    e->h_desc.clear();
+    for (auto& kv : e->members) {
+        Member* m = kv.second;
+        MemberDesc d;
+        d.off = m->off.as<uint32_t>();
+        d.src = m->src.as<uint32_t>();
+        d.rs = m->st[m->cur].as<uint32_t>();
+        d.ws = m->st[m->cur ^ 1].as<uint32_t>();
+        d.n = m->n;
+        d.step = m->step;
+        d.seed = m->seed;
+        d.p = m->p;
+        d.rec = m->rec;
+        d.type = 1;
+        e->h_desc.push_back(d);
+    }
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_create(vh_ensemble_t* out) {
+    vh_ensemble* e = new vh_ensemble();
+    hipError_t r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (r != hipSuccess) {
+        delete e;
+        return hip_fail(r, "hipStreamCreate");
+    }
+    *out = e;
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_destroy(vh_ensemble_t e) {
+    if (!e) return VH_OK;
+    (void)hipStreamSynchronize(e->stream);
+    for (auto& kv : e->members) {
+        Member* m = kv.second;
+        m->off.release(); m->src.release(); m->st[0].release(); m->st[1].release();
+        delete m;
+    }
+    e->d_desc.release(); e->d_bmember.release(); e->d_bstart.release(); e->d_cdf.release();
+    e->scratch.release();
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return VH_OK;
+}
+
+// a SIR member on a device-generated G(n, p) (p = mean_deg / (n - 1)); the
+// same edge list comes from vh_graph_gnp on the host
+extern "C" int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* sm, uint32_t* member_id) {
+    if (!e || !sm || sm->n == 0 || sm->n >= 0xFFFFFFFFull) return set_error(VH_ERR_ARG, "bad member");
+    Member* m = new Member();
+    m->id = e->next_id++;
+    m->n = sm->n;
+    m->seed = sm->seed;
+    m->p = sm->p_inf;
+    m->rec = sm->recovery;
+    m->step = 1;
+    hipStream_t st = e->stream;
+    double pe = sm->n > 1 ? sm->mean_deg / (double)(sm->n - 1) : 0.0;
+    std::vector<double> cdf = binom_cdf(sm->n, pe);
+    int rc = e->d_cdf.ensure(cdf.size() * 8, st);
+    if (!rc) rc = m->off.ensure((sm->n + 1) * 4, st);
+    if (!rc) rc = m->st[0].ensure(sm->n * 4, st);
+    if (!rc) rc = m->st[1].ensure(sm->n * 4, st);
+    if (rc) { delete m; return rc; }
+    e->scratch.reset(st);
+    uint32_t* deg = (uint32_t*)e->scratch.take(sm->n * 4 + 8);
+    if (!deg) { delete m; return set_error(VH_ERR_OOM, "scratch"); }
+    HIPCHECK(hipMemcpyAsync(e->d_cdf.p, cdf.data(), cdf.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gnp_degree, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, e->d_cdf.as<double>(),
+                       (uint32_t)cdf.size(), sm->graph_seed, sm->n, deg);
+    if (scan_excl_u32(deg, m->off.as<uint32_t>(), sm->n, e->scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+    uint32_t ne = 0;
+    HIPCHECK(hipMemcpyAsync(&ne, m->off.as<uint32_t>() + sm->n, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    m->nedges = ne;
+    RC(m->src.ensure((uint64_t)ne * 4 + 8, st));
+    hipLaunchKernelGGL(k_gnp_fill, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, sm->n, sm->graph_seed,
+                       m->off.as<uint32_t>(), m->src.as<uint32_t>());
+    double frac = (double)sm->n_init / (double)sm->n;
+    hipLaunchKernelGGL(k_sir_init, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, m->st[0].as<uint32_t>(),
+                       sm->n, sm->seed, frac, 1u);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    e->members[m->id] = m;
+    e->table_valid = false;
+    *member_id = m->id;
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_remove(vh_ensemble_t e, uint32_t id) {
+    auto it = e->members.find(id);
+    if (it == e->members.end()) return set_error(VH_ERR_ARG, "unknown member %u", id);
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    Member* m = it->second;
+    m->off.release(); m->src.release(); m->st[0].release(); m->st[1].release();
+    delete m;
+    e->members.erase(it);
+    e->table_valid = false;
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t remap) {
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+        return set_error(VH_ERR_ARG, "block must be 64..1024 (power of two)");
+    e->block = block;
+    e->remap = remap ? 1 : 0;
+    e->table_valid = false;
+    return VH_OK;
+}
+
+// advance every member `steps` steps; with use_graph the steps are captured
+// once into a hipGraph and launched as one unit
+extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) {
+    if (e->members.empty() || !steps) return VH_OK;
+    if (!e->table_valid || e->table_block != e->block) RC(build_table(e));
+    size_t nm = e->members.size();
+    // one descriptor table per step (buffers alternate, step numbers grow)
+    RC(e->d_desc.ensure((size_t)steps * nm * sizeof(MemberDesc), e->stream));
+    std::vector<MemberDesc> all;
+    all.reserve((size_t)steps * nm);
+    for (uint32_t s = 0; s < steps; s++) {
+        upload_desc(e);
+        all.insert(all.end(), e->h_desc.begin(), e->h_desc.end());
+        for (auto& kv : e->members) {
+            kv.second->cur ^= 1;
+            kv.second->step++;
+        }
+    }
+    HIPCHECK(hipMemcpyAsync(e->d_desc.p, all.data(), all.size() * sizeof(MemberDesc), hipMemcpyHostToDevice,
+                            e->stream));
+    HIPCHECK(hipStreamSynchronize(e->stream));   // `all` is pageable
+    // with the XCD-aware order the grid is padded to a multiple of 8 so that
+    // every (xcd, slot) pair, and with it every logical block, exists
+    uint32_t grid = e->remap ? 8u * ((e->nblocks + 7u) / 8u) : e->nblocks;
+    dim3 g(grid), b(e->block);
+    if (use_graph && steps > 1) {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t s = 0; s < steps; s++)
+            hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, e->d_desc.as<MemberDesc>() + (size_t)s * nm,
+                               e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(), e->nblocks, e->remap);
+        HIPCHECK(hipStreamEndCapture(e->stream, &graph));
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphLaunch(exec, e->stream));
+        HIPCHECK(hipStreamSynchronize(e->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    } else {
+        for (uint32_t s = 0; s < steps; s++)
+            hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, e->d_desc.as<MemberDesc>() + (size_t)s * nm,
+                               e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(), e->nblocks, e->remap);
+        HIPCHECK(hipGetLastError());
+    }
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_sync(vh_ensemble_t e) {
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_read(vh_ensemble_t e, uint32_t id, void* states, uint64_t cap, uint64_t* n,
+                                uint64_t* step) {
+    auto it = e->members.find(id);
+    if (it == e->members.end()) return set_error(VH_ERR_ARG, "unknown member %u", id);
+    Member* m = it->second;
+    *n = m->n;
+    if (step) *step = m->step;
+    if (cap < m->n) return set_error(VH_ERR_CAPACITY, "capacity");
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    HIPCHECK(hipMemcpy(states, m->st[m->cur].p, m->n * 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_info(vh_ensemble_t e, uint64_t* members, uint64_t* agents, uint64_t* edges,
+                                uint64_t* bytes) {
+    uint64_t a = 0, ed = 0, by = 0;
+    for (auto& kv : e->members) {
+        Member* m = kv.second;
+        a += m->n;
+        ed += m->nedges;
+        by += m->off.cap + m->src.cap + m->st[0].cap + m->st[1].cap;
+    }
+    *members = e->members.size();
+    *agents = a;
+    *edges = ed;
+    *bytes = by;
+    return VH_OK;
+}
+
+// autotuner over result-invariant launch knobs (block size, XCD-aware block
+// order): times `trial_steps` steps per candidate on the members' current
+// state and keeps the fastest.  Trial steps advance a snapshot, not the
+// members: their buffers are restored afterwards.
+extern "C" int vh_ensemble_autotune(vh_ensemble_t e, uint32_t trial_steps, uint32_t* best_block,
+                                    uint32_t* best_remap, float* best_ms) {
+    if (e->members.empty()) return set_error(VH_ERR_ARG, "empty ensemble");
+    if (trial_steps == 0) trial_steps = 3;
+    // snapshot of the current state of every member
+    std::vector<DevBuf> snap(e->members.size());
+    size_t k = 0;
+    std::vector<std::pair<int, uint64_t>> saved;
+    for (auto& kv : e->members) {
+        Member* m = kv.second;
+        RC(snap[k].ensure(m->n * 4, e->stream));
+        HIPCHECK(hipMemcpyAsync(snap[k].p, m->st[m->cur].p, m->n * 4, hipMemcpyDeviceToDevice, e->stream));
+        saved.push_back({m->cur, m->step});
+        k++;
+    }
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    float best = 1e30f;
+    uint32_t bb = e->block, br = e->remap;
+    const uint32_t blocks[] = {128, 256, 512, 1024};
+    for (uint32_t blk : blocks)
+        for (uint32_t rm = 0; rm < 2; rm++) {
+            RC(vh_ensemble_set_config(e, blk, rm));
+            RC(vh_ensemble_step(e, 1, 0));   // warm-up, table build
+            HIPCHECK(hipEventRecord(a, e->stream));
+            RC(vh_ensemble_step(e, trial_steps, 0));
+            HIPCHECK(hipEventRecord(b, e->stream));
+            HIPCHECK(hipEventSynchronize(b));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, a, b));
+            ms /= trial_steps;
+            if (ms < best) {
+                best = ms;
+                bb = blk;
+                br = rm;
+            }
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    // restore
+    k = 0;
+    for (auto& kv : e->members) {
+        Member* m = kv.second;
+        m->cur = saved[k].first;
+        m->step = saved[k].second;
+        HIPCHECK(hipMemcpyAsync(m->st[m->cur].p, snap[k].p, m->n * 4, hipMemcpyDeviceToDevice, e->stream));
+        k++;
+    }
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    for (auto& s : snap) s.release();
+    RC(vh_ensemble_set_config(e, bb, br));
+    if (best_block) *best_block = bb;
+    if (best_remap) *best_remap = br;
+    if (best_ms) *best_ms = best;
+    return VH_OK;
+}
+
+// host restatement of the device generator (same table, same hash draws)
+extern "C" int vh_graph_gnp(uint64_t n, double mean_deg, uint64_t seed, uint32_t* from, uint32_t* to,
+                            uint64_t cap, uint64_t* m) {
+    if (n >= 0xFFFFFFFFull) return VH_ERR_ARG;
+    double pe = n > 1 ? mean_deg / (double)(n - 1) : 0.0;
+    std::vector<double> cdf = binom_cdf(n, pe);
+    uint64_t k = 0;
+    std::vector<uint32_t> buf;
+    for (uint64_t t = 1; t <= n; t++) {
+        uint32_t d = gnp_degree(cdf.data(), (uint32_t)cdf.size(), seed, t);
+        buf.resize(d);
+        gnp_sources(n, seed, t, d, buf.data());
+        for (uint32_t j = 0; j < d; j++, k++)
+            if (k < cap) {
+                from[k] = buf[j];
+                to[k] = (uint32_t)t;
+            }
+    }
+    *m = k;
+    return k > cap ? VH_ERR_CAPACITY : VH_OK;
+}

@@ -63,6 +63,22 @@ def graph_er(n, p, seed):
     return fr[:cap], to[:cap]
 
 
+def graph_gnp(n, mean_deg, seed):
+    """Directed G(n, p) with exact Binomial(n-1, p) in-degrees, the host twin of
+    the ensemble's device generator: (from, to), targets ascending."""
+    L = _abi.lib()
+    m = C.c_uint64()
+    L.graph_gnp(n, mean_deg, seed, None, None, 0, C.byref(m))
+    cap = m.value
+    fr = np.empty(max(cap, 1), dtype=np.uint32)
+    to = np.empty(max(cap, 1), dtype=np.uint32)
+    rc = L.graph_gnp(n, mean_deg, seed, fr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                     to.ctypes.data_as(C.POINTER(C.c_uint32)), cap, C.byref(m))
+    if rc != 0:
+        raise RuntimeError(f"vh_graph_gnp failed ({rc})")
+    return fr[:cap], to[:cap]
+
+
 def graph_ba(n, m, seed):
     """Barabasi-Albert undirected pairs (a, b) in creation order."""
     L = _abi.lib()
@@ -104,6 +120,15 @@ def sir_init(n, seed, n_init, type_id=1):
     if k:
         sel = np.argsort(h, kind="stable")[:k]
         st["health"][sel] = _abi.I
+    return st
+
+
+def sir_init_frac(n, seed, n_init, type_id=1):
+    """All S except the agents with u01(hash(seed, id, 0, INIT)) < n_init / n
+    (the ensemble's initial state)."""
+    st = np.zeros(n, dtype=SIR_DTYPE)
+    ids = (np.uint64(type_id) << np.uint64(56)) | np.arange(1, n + 1, dtype=np.uint64)
+    st["health"][u01(hash4(seed, ids, 0, KEY_INIT)) < n_init / n] = _abi.I
     return st
 
 

@@ -30,14 +30,17 @@ def _types(V):
     return SIR, Contact, HK, HK32, Knows, Person, Met
 
 
-def build_sir(V, n, mean_deg=10.0, seed=1, n_init=100, graph_seed=None, edges=None):
+def build_sir(V, n, mean_deg=10.0, seed=1, n_init=100, graph_seed=None, edges=None,
+              init="smallest"):
     SIR, Contact, *_ = _types(V)
     model = (V.ModelTypes()
              | V.register_agenttype_(SIR, "Immortal")
              | V.register_edgetype_(Contact, "Stateless", target=SIR)
              | V.create_model("sir"))
     sim = V.create_simulation(model)
-    ids = sim._add_agents_array(SIR, inputs.sir_init(n, seed, n_init))
+    st0 = (inputs.sir_init(n, seed, n_init) if init == "smallest"
+           else inputs.sir_init_frac(n, seed, n_init))
+    ids = sim._add_agents_array(SIR, st0)
     if edges is None:
         p = mean_deg / max(n - 1, 1)
         edges = inputs.graph_er(n, p, seed if graph_seed is None else graph_seed)

@@ -253,6 +253,8 @@ int vh_ensemble_create(vh_ensemble_t* out);
 int vh_ensemble_destroy(vh_ensemble_t e);
 int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* m, uint32_t* member_id);
 int vh_ensemble_remove(vh_ensemble_t e, uint32_t member_id);
+/* the member applies steps up to last_step and then stays (0: no limit) */
+int vh_ensemble_set_horizon(vh_ensemble_t e, uint32_t member_id, uint64_t last_step);
 /* block size (128..1024) and XCD-aware block order of the step launch */
 int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t xcd_remap);
 /* advance every member; use_graph: capture the steps into one hipGraph */

@@ -134,6 +134,7 @@ DEVICE_ONLY = [
     ("ensemble_destroy", C.c_int, [_vp]),
     ("ensemble_add_sir", C.c_int, [_vp, C.POINTER(SirMember), _u32p]),
     ("ensemble_remove", C.c_int, [_vp, C.c_uint32]),
+    ("ensemble_set_horizon", C.c_int, [_vp, C.c_uint32, C.c_uint64]),
     ("ensemble_set_config", C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     ("ensemble_step", C.c_int, [_vp, C.c_uint32, C.c_int]),
     ("ensemble_sync", C.c_int, [_vp]),

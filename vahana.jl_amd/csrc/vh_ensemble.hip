@@ -134,6 +134,7 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
 struct Member {
     uint32_t id;
     uint64_t n, nedges, step;
+    uint64_t horizon = 0;   // last step to apply (0: unbounded)
     uint64_t seed;
     double p;
     uint32_t rec;
@@ -286,6 +287,13 @@ extern "C" int vh_ensemble_remove(vh_ensemble_t e, uint32_t id) {
     return VH_OK;
 }
 
+extern "C" int vh_ensemble_set_horizon(vh_ensemble_t e, uint32_t id, uint64_t last_step) {
+    auto it = e->members.find(id);
+    if (it == e->members.end()) return set_error(VH_ERR_ARG, "unknown member %u", id);
+    it->second->horizon = last_step;
+    return VH_OK;
+}
+
 extern "C" int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t remap) {
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return set_error(VH_ERR_ARG, "block must be 64..1024 (power of two)");
@@ -307,11 +315,18 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
     all.reserve((size_t)steps * nm);
     for (uint32_t s = 0; s < steps; s++) {
         upload_desc(e);
-        all.insert(all.end(), e->h_desc.begin(), e->h_desc.end());
+        size_t k = 0;
         for (auto& kv : e->members) {
-            kv.second->cur ^= 1;
-            kv.second->step++;
+            Member* m = kv.second;
+            if (m->horizon && m->step > m->horizon) {
+                e->h_desc[k].n = 0;          // finished: its blocks exit at once
+            } else {
+                m->cur ^= 1;
+                m->step++;
+            }
+            k++;
         }
+        all.insert(all.end(), e->h_desc.begin(), e->h_desc.end());
     }
     HIPCHECK(hipMemcpyAsync(e->d_desc.p, all.data(), all.size() * sizeof(MemberDesc), hipMemcpyHostToDevice,
                             e->stream));

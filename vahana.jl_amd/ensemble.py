@@ -35,6 +35,10 @@ class Ensemble:
         self._check(self.lib.ensemble_remove(self.handle, mid))
         self.members.pop(mid, None)
 
+    def set_horizon(self, mid, last_step):
+        """The member applies steps up to `last_step` (its step numbers start at 1)."""
+        self._check(self.lib.ensemble_set_horizon(self.handle, mid, int(last_step)))
+
     def set_config(self, block, xcd_remap):
         self._check(self.lib.ensemble_set_config(self.handle, block, 1 if xcd_remap else 0))
 

@@ -80,7 +80,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=10_000_000)
-    ap.add_argument("--workload", default="opinion", choices=["opinion", "opinion32", "sir"])
+    ap.add_argument("--workload", default="opinion",
+                    choices=["opinion", "opinion32", "sir", "ensemble", "serve"])
+    ap.add_argument("--replicas", type=int, default=1024)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
@@ -97,6 +101,10 @@ def main():
     vh = load_package()
     M = vh.models
     torch.cuda.init()
+    if args.workload == "ensemble":
+        return bench_ensemble(vh, args)
+    if args.workload == "serve":
+        return bench_serve(vh, args)
     dev = torch.device("cuda:0")
     n = args.n
     f32 = args.workload == "opinion32"
@@ -188,6 +196,89 @@ def main():
                      "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_ensemble(vh, args):
+    """1024 SIR replicas (sizes log-uniform in [1e3, 1e6], mean in-degree
+    U[4, 20], p_inf U[0.01, 0.1], distinct seeds), 200 steps in one
+    graph-captured launch sequence, against the same replicas one at a time."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    R = args.replicas
+    specs = [(int(10 ** rng.uniform(3, 6)), float(rng.uniform(4, 20)), float(rng.uniform(0.01, 0.1)),
+              1 + i) for i in range(R)]
+    E = vh.ensemble.Ensemble()
+    t0 = time.perf_counter()
+    ids = [E.add_sir(n, md, p, seed, 10_000 + seed, 7, max(1, n // 1000)) for n, md, p, seed in specs]
+    info = E.info()
+    log(f"ensemble built in {time.perf_counter() - t0:.1f}s: {info}")
+    block, remap, tms = E.autotune(2)
+    log(f"autotune: block {block}, xcd_remap {remap}, {tms:.3f} ms/step")
+    steps = args.steps if args.steps != 50 else 200
+    E.step(1, use_graph=False)
+    E.sync()
+    t0 = time.perf_counter()
+    E.step(steps, use_graph=True)
+    E.sync()
+    wall = time.perf_counter() - t0
+    agent_steps = info["agents"] * steps
+    # one at a time: a sample of the replicas, each alone (same kernel, same
+    # graph-captured step sequence), scaled to all replicas by agent count
+    sample = list(range(0, R, max(1, R // 64)))
+    solo_time, solo_agents = 0.0, 0
+    for i in sample:
+        n, md, p, seed = specs[i]
+        S = vh.ensemble.Ensemble()
+        S.add_sir(n, md, p, seed, 10_000 + seed, 7, max(1, n // 1000))
+        S.step(1, use_graph=False)
+        S.sync()
+        t1 = time.perf_counter()
+        S.step(steps, use_graph=True)
+        S.sync()
+        solo_time += time.perf_counter() - t1
+        solo_agents += n
+        S.close()
+    solo_rate = solo_agents * steps / solo_time
+    solo_total = len(specs) * (solo_time / len(sample))
+    out = {"metric": "ensemble agent-updates/s (1024 SIR replicas, one launch per step)",
+           "value": agent_steps / wall, "unit": "agent-updates/s",
+           "edges_per_s": info["edges"] * steps / wall, "steps": steps,
+           "ms_per_step": wall * 1e3 / steps, "replicas": R, "agents": info["agents"],
+           "edges": info["edges"], "autotune": {"block": block, "xcd_remap": remap},
+           "one_at_a_time": {"agent_updates_per_s": solo_rate, "sample_replicas": len(sample),
+                             "est_total_s": solo_total},
+           "replicas_per_s": R / wall, "one_at_a_time_replicas_per_s": R / solo_total,
+           "speedup_vs_one_at_a_time": solo_total / wall, "dtype": "u8", "n_gpus": 1,
+           "higher_is_better": True}
+    print(json.dumps(out), flush=True)
+    E.close()
+
+
+def bench_serve(vh, args):
+    """Poisson stream of mixed SIR / opinion requests (sizes log-uniform in
+    [1e4, 1e7], horizons U[50, 500]) served with continuous batching under an
+    HBM budget, against one-at-a-time execution of the same stream."""
+    import torch
+    S = vh.serve
+    reqs = S.make_requests(args.requests, rate=args.rate, seed=11)
+    free, total = torch.cuda.mem_get_info()
+    budget = int(0.9 * free)
+    srv = S.Server(vh, hbm_budget=budget, quantum=10)
+    done, wall = srv.run(reqs)
+    srv.close()
+    s1 = S.summarize(done, wall)
+    log(f"served: {s1}")
+    seq, wall2 = S.run_sequential(vh, reqs)
+    s2 = S.summarize(seq, wall2)
+    log(f"one at a time: {s2}")
+    same = {r.id: r.digest for r in done} == {r.id: r.digest for r in seq}
+    out = {"metric": "served requests/s with p50/p99 latency (SIR + opinion mix)",
+           "value": s1["throughput_rps"], "unit": "requests/s", "served": s1,
+           "one_at_a_time": s2, "throughput_ratio": s1["throughput_rps"] / s2["throughput_rps"],
+           "p99_ratio": s1["p99_s"] / s2["p99_s"], "results_identical": same,
+           "hbm_budget_bytes": budget, "rate_per_s": args.rate, "n_gpus": 1,
+           "higher_is_better": True}
     print(json.dumps(out), flush=True)
 
 

@@ -154,16 +154,19 @@ def run_sequential(vh, requests, time_scale=1.0, keep_results=False):
         if r.arrival * time_scale > now:
             time.sleep(r.arrival * time_scale - now)
         if r.kind == "sir":
-            m = models.build_sir(vh, r.n, seed=r.seed, n_init=max(1, r.n // 1000),
-                                 edges=inputs.graph_gnp(r.n, r.mean_deg, r.seed + 1), init="frac")
-            for _ in range(r.horizon):
-                models.step_sir(vh, m, p_inf=r.p_inf, seed=r.seed, recovery=7)
+            # the same device-built graph and launch as in the mix, alone
+            e = Ensemble()
+            mid = e.add_sir(r.n, r.mean_deg, r.p_inf, r.seed, r.seed + 1, recovery=7,
+                            n_init=max(1, r.n // 1000))
+            e.step(r.horizon, use_graph=True)
+            st, _ = e.read(mid)
+            e.close()
         else:
             m = models.build_opinion(vh, r.n, seed=r.seed, edges=_opinion_edges(r))
             for _ in range(r.horizon):
                 models.step_opinion(vh, m, eps=0.2)
-        st, _ = m.sim._read_agents(m.T)
-        m.sim._destroy()
+            st, _ = m.sim._read_agents(m.T)
+            m.sim._destroy()
         r2 = types.SimpleNamespace(**{k: v for k, v in vars(r).items()})
         r2.digest = _digest(st)
         if keep_results:

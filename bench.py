@@ -41,7 +41,8 @@ def sir_bytes(n, e):
 
 def cpu_baseline(workload, f32, seconds_cap=25.0):
     """The checker's apply! (a restatement of the reference's data structures)
-    on one core over a bounded sample: same model, 10^6 agents."""
+    on one core over a bounded sample: same model, 10^6 agents (2*10^5 for
+    the contact model)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as ora
     from __graft_entry__ import load_package
@@ -54,6 +55,11 @@ def cpu_baseline(workload, f32, seconds_cap=25.0):
         m = M.build_opinion(ora, n, seed=1, f32=f32, edges=edges)
         step = lambda: M.step_opinion(ora, m, eps=0.2)  # noqa: E731
         ne = len(edges[0])
+    elif workload == "contact":
+        n = 200_000
+        m = M.build_contact(ora, n, seed=1)
+        step = lambda: M.step_contact(ora, m, seed=1)  # noqa: E731
+        ne = 20 * n
     else:
         edges = vh.inputs.graph_er(n, 10.0 / (n - 1), 1)
         m = M.build_sir(ora, n, seed=1, edges=edges)
@@ -81,7 +87,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--workload", default="opinion",
-                    choices=["opinion", "opinion32", "sir", "ensemble", "serve"])
+                    choices=["opinion", "opinion32", "sir", "contact", "ensemble", "serve",
+                             "gather_floor"])
     ap.add_argument("--replicas", type=int, default=1024)
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=4.0)
@@ -105,6 +112,8 @@ def main():
         return bench_ensemble(vh, args)
     if args.workload == "serve":
         return bench_serve(vh, args)
+    if args.workload == "gather_floor":
+        return bench_gather_floor(vh)
     dev = torch.device("cuda:0")
     n = args.n
     f32 = args.workload == "opinion32"
@@ -119,6 +128,17 @@ def main():
         dtype = "f32" if f32 else "f64"
         workload = (f"opinion dynamics (weighted bounded confidence), Barabasi-Albert m=5 symmetrised, "
                     f"N={n}, E={m.nedges}, eps=0.2, {'Float32' if f32 else 'Float64'} opinions")
+    elif args.workload == "contact":
+        edges = None
+        m = M.build_contact(vh, n, seed=1)
+        step = lambda: M.step_contact(vh, m, seed=1)  # noqa: E731
+        m.nedges = 20 * n
+        # survivors' state r/w, died flags, previous contacts read, new contacts
+        # written (offsets + sources)
+        nbytes = n * (8 + 8 + 1) + 2 * (n + 1) * 4 + 4 * 20 * n + 4 * 20 * n
+        dtype = "u8"
+        workload = (f"dynamic contact network, N={n}, 20 new contacts per agent and step, "
+                    f"2% deaths with 1:1 births")
     else:
         edges = vh.inputs.graph_er(n, 10.0 / (n - 1), 1)
         m = M.build_sir(vh, n, seed=1, edges=edges)
@@ -170,6 +190,24 @@ def main():
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get("bytes_per_launch")
 
+    drift = None
+    if f32:
+        # the Float32 variant against the Float64 run of the same inputs
+        a, b = vh.inputs.graph_ba(n, 5, 1)
+        edges = vh.inputs.symmetric_weighted(a, b, 2)
+        del a, b
+        m64 = M.build_opinion(vh, n, seed=1, f32=False, edges=edges)
+        del edges
+        for _ in range(1 + args.warmup + args.steps - 1):
+            M.step_opinion(vh, m64, eps=0.2)
+        x64 = np.asarray(m64.sim._read_agents(m64.T)[0]["opinion"])
+        x32 = np.asarray(m.sim._read_agents(m.T)[0]["opinion"]).astype(np.float64)
+        d = np.abs(x32 - x64)
+        drift = {"steps": args.warmup + args.steps, "max_abs": float(d.max()),
+                 "mean_abs": float(d.mean()), "frac_above_1e-3": float((d > 1e-3).mean())}
+        log(f"f32 drift vs f64: {drift}")
+        m64.sim._destroy()
+
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
@@ -196,7 +234,29 @@ def main():
                      "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},
         "cpu_baseline": cpu,
     }
+    if drift is not None:
+        out["f32_drift_vs_f64"] = drift
     print(json.dumps(out), flush=True)
+
+
+def bench_gather_floor(vh):
+    """Random 8-byte gathers versus footprint, and a streaming read: the
+    memory-system bounds of the neighbour-state gather."""
+    import ctypes as C
+    L = vh._abi.lib()
+    n = 100_000_000
+    rows = []
+    for mb in (2, 4, 16, 40, 80, 160, 400, 1024):
+        ms = C.c_float()
+        L.probe_memory(0, mb << 20, n, 10, C.byref(ms))
+        rows.append({"footprint_MiB": mb, "ms": ms.value, "gathers_per_s": n / (ms.value * 1e-3)})
+        log(f"gather footprint {mb:5d} MiB: {ms.value:.3f} ms  {n / ms.value / 1e6:.1f} G/s")
+    ms = C.c_float()
+    L.probe_memory(1, 800 << 20, 0, 10, C.byref(ms))
+    stream = (800 << 20) / (ms.value * 1e-3) / 1e9
+    log(f"stream read 800 MiB: {ms.value:.3f} ms  {stream:.0f} GB/s")
+    print(json.dumps({"metric": "random 8-byte gathers/s by footprint", "gathers": n,
+                      "rows": rows, "stream_read_GBps": stream}), flush=True)
 
 
 def bench_ensemble(vh, args):

@@ -291,6 +291,12 @@ int vh_graph_ba(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b,
  * h = vh_hash4(seed, i, b, c) */
 int vh_fill_open01f(uint64_t seed, uint64_t b, uint64_t c, uint64_t n, float* out);
 
+/* ------------------------------------------------------ diagnostics
+ * Memory-system probe (no reference counterpart): mode 0 times n random
+ * 8-byte gathers over `footprint` bytes, mode 1 a streaming read of
+ * `footprint` bytes; *ms is the mean launch time over `reps` launches. */
+int vh_probe_memory(int mode, uint64_t footprint, uint64_t n, uint32_t reps, float* ms);
+
 /* ------------------------------------------------------ counter-hash RNG
  * SplitMix64 finaliser chained over (seed, a, b, c); the same function is
  * compiled into the device bodies. */

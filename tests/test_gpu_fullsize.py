@@ -1,0 +1,104 @@
+"""Full-size runs of the benchmark configurations on the GPU.
+
+* SIR at 1e5 agents x 100 steps: bit-exact against the checker.
+* Opinion at 1e7 agents (BA m=5, ~1e8 edges): every sampled agent's update
+  (light and hub agents) recomputed from the pre-step state in numpy, in the
+  canonical order, must be bit-identical; opinions stay in range.
+* Contact at 1e6 agents: size-independent invariants of births, deaths and
+  rebuilt contacts.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_sir_config0_bit_exact(vh, ora, gpu_ready):
+    M = vh.models
+    n = 100_000
+    edges = vh.inputs.graph_er(n, 10.0 / (n - 1), 1)
+    g = M.build_sir(vh, n, seed=1, n_init=100, edges=edges)
+    c = M.build_sir(ora, n, seed=1, n_init=100, edges=edges)
+    for step in range(100):
+        M.step_sir(vh, g, p_inf=0.05, seed=1)
+        M.step_sir(ora, c, p_inf=0.05, seed=1)
+        if step % 10 == 9:
+            a, _ = g.sim._read_agents(g.T)
+            b, _ = c.sim._read_agents(c.T)
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), step
+    a, _ = g.sim._read_agents(g.T)
+    h = np.asarray(a)["health"]
+    print(f"[gpu] SIR 1e5 x 100: S {int((h == 0).sum())} I {int((h == 1).sum())} R {int((h == 2).sum())}",
+          flush=True)
+
+
+def _canon(nt, dt):
+    d = len(nt)
+    if d <= 64:
+        a = np.cumsum(nt)[-1] if d else 0.0
+        b = np.cumsum(dt)[-1] if d else 0.0
+        return a, b
+    pa = np.array([np.cumsum(nt[l::64])[-1] if len(nt[l::64]) else 0.0 for l in range(64)])
+    pb = np.array([np.cumsum(dt[l::64])[-1] if len(dt[l::64]) else 0.0 for l in range(64)])
+    idx = np.arange(64)
+    for o in (32, 16, 8, 4, 2, 1):
+        pa = pa + pa[idx ^ o]
+        pb = pb + pb[idx ^ o]
+    return pa[0], pb[0]
+
+
+def test_opinion_config1_sampled_exact(vh, gpu_ready):
+    M = vh.models
+    n = 10_000_000
+    a, b = vh.inputs.graph_ba(n, 5, 1)
+    fr, to, w = vh.inputs.symmetric_weighted(a, b, 2)
+    del a, b
+    m = M.build_opinion(vh, n, seed=1, edges=(fr, to, w))
+    # CSR of the input (insertion order within each target)
+    order = np.argsort(to, kind="stable")
+    src = fr[order]
+    ws = w[order]
+    off = np.zeros(n + 1, dtype=np.int64)
+    off[1:] = np.cumsum(np.bincount(to, minlength=n + 1)[1:])
+    del order
+    deg = np.diff(off)
+    rng = np.random.default_rng(0)
+    hubs = np.argsort(deg)[-200:]
+    sample = np.concatenate([rng.choice(n, 3000, replace=False), hubs])
+    x0 = np.asarray(m.sim._read_agents(m.T)[0]["opinion"]).copy()
+    lo, hi = x0.min(), x0.max()
+    for step in range(3):
+        M.step_opinion(vh, m, eps=0.2)
+        x1 = np.asarray(m.sim._read_agents(m.T)[0]["opinion"]).copy()
+        for i in sample:
+            s, e = off[i], off[i + 1]
+            xj = x0[src[s:e].astype(np.int64) - 1]
+            acc = np.abs(xj - x0[i]) <= 0.2
+            nt = np.where(acc, ws[s:e].astype(np.float64) * xj, 0.0)
+            dt = np.where(acc, ws[s:e].astype(np.float64), 0.0)
+            num, den = _canon(nt, dt)
+            want = num / den if den > 0 else x0[i]
+            assert x1[i] == want, (step, int(i), int(deg[i]))
+        assert x1.min() >= lo and x1.max() <= hi
+        x0 = x1
+    print(f"[gpu] opinion 1e7: {len(sample)} sampled agents (max degree {int(deg.max())}) exact over 3 steps",
+          flush=True)
+
+
+def test_contact_invariants_large(vh, gpu_ready):
+    M = vh.models
+    n = 1_000_000
+    m = M.build_contact(vh, n, seed=9)
+    for step in range(4):
+        M.step_contact(vh, m, seed=9)
+        st, died = m.sim._read_agents(m.T)
+        assert int((~died).sum()) == n
+        to, fr, _ = m.sim._read_edges(m.E)
+        ton = (to & vh._abi.VH_NR_MASK).astype(np.int64)
+        frn = (fr & vh._abi.VH_NR_MASK).astype(np.int64)
+        assert not died[ton - 1].any() and not died[frn - 1].any()
+        assert not (ton == frn).any()
+        cnt = np.bincount(ton, minlength=len(died) + 1)
+        assert cnt.max() <= 20
+        # about 20 contacts per survivor, minus the ~2% that died as partners
+        assert 0.9 * 20 * n * 0.98 < len(to) <= 20 * n

@@ -129,6 +129,8 @@ DEVICE_ONLY = [
     ("hash4", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
     ("fill_open01f", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                C.POINTER(C.c_float)]),
+    ("probe_memory", C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32,
+                               C.POINTER(C.c_float)]),
     ("graph_gnp", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
     ("ensemble_create", C.c_int, [C.POINTER(_vp)]),
     ("ensemble_destroy", C.c_int, [_vp]),

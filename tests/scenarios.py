@@ -450,3 +450,79 @@ def graphs_k4(V):
     V.finish_init_(sim)
     V.apply_(sim, V.transitions.sum_neighbors(GraphE), [GraphA], [GraphA, GraphE], [GraphA])
     return V.mapreduce(sim, "id", "+", GraphA)
+
+
+def edges_iter_removal(V):
+    """test/edgesiterator.jl:6-221: edge counts, edge mapreduce after target
+    deaths and death-driven removal for every hint combination."""
+    out = {"iter": {}, "agg": {}, "remove": {}}
+    for name in STATELESS + STATEFUL:                        # :8-49
+        if hashint(name, "S") and hashint(name, "I"):
+            continue
+        model, Agent, AgentB, ets = edges_model(V)
+        ET = ets[name]
+        sim = V.create_simulation(model)
+        aids = V.add_agents_(sim, [Agent(i) for i in range(1, 11)])
+        for i in aids:
+            st = (ET, None) if name in STATELESS else (ET, ET(i))
+            V.add_edge_(sim, aids[0], i, st)
+            if not hashint(name, "E"):
+                V.add_edge_(sim, i, aids[0], st)
+        before = V.num_edges(sim, ET)
+        V.finish_init_(sim)
+        out["iter"][name] = (before, V.num_edges(sim, ET))
+    for name in STATEFUL:                                     # :53-104
+        model, Agent, AgentB, ets = edges_model(V)
+        ET = ets[name]
+        sim = V.create_simulation(model)
+        aids = V.add_agents_(sim, [Agent(i) for i in range(1, 11)])
+        for i in aids:
+            V.add_edge_(sim, aids[0], i, ET(V.agent_nr(i)))
+        V.finish_init_(sim)
+        r = [V.mapreduce(sim, "foo", "+", ET)]
+        V.apply_(sim, V.transitions.die_if("always"), [Agent], [], [Agent])
+        r.append(V.mapreduce(sim, "foo", "+", ET))
+        model, Agent, AgentB, ets = edges_model(V)
+        ET = ets[name]
+        sim = V.create_simulation(model)
+        aids = V.add_agents_(sim, [Agent(i) for i in range(1, 11)])
+        bids = V.add_agents_(sim, [AgentB(i) for i in range(1, 11)])
+        for i in range(10):
+            V.add_edge_(sim, bids[i], aids[i], ET(i + 1))
+        V.finish_init_(sim)
+        r.append(V.mapreduce(sim, "foo", "+", ET))
+        V.apply_(sim, V.transitions.die_if("always"), [Agent], [], [Agent])
+        r.append(V.mapreduce(sim, "foo", "+", ET))
+        out["agg"][name] = r
+    for name in STATEFUL + STATELESS:                         # :109-216
+        model, Agent, AgentB, ets = edges_model(V)
+        ET = ets[name]
+        stl = name in STATELESS
+        sim = V.create_simulation(model)
+        aids = V.add_agents_(sim, [Agent(i) for i in range(1, 11)])
+        bids = V.add_agents_(sim, [AgentB(i) for i in range(1, 11)])
+        for i in range(10):
+            V.add_edge_(sim, bids[i], aids[i], ET() if stl else ET(i + 1))
+        V.finish_init_(sim)
+        r = [V.num_edges(sim, ET)]
+        V.apply_(sim, V.transitions.die_if("always"), [Agent], [], [Agent])
+        r.append(V.num_edges(sim, ET))
+        hints = EDGE_HINTS[name]
+        if "IgnoreFrom" not in hints:
+            sim = V.create_simulation(model)
+            aids = V.add_agents_(sim, [Agent(i) for i in range(1, 11)])
+            bids = V.add_agents_(sim, [AgentB(i) for i in range(1, 11)])
+            for i in range(10):
+                st = ET() if stl else ET(i + 1)
+                if "SingleEdge" not in hints:
+                    V.add_edge_(sim, aids[i], aids[i], st)
+                    if "SingleType" not in hints:
+                        V.add_edge_(sim, aids[i], bids[i], st)
+                V.add_edge_(sim, aids[i], V.Edge(bids[i], st))
+                if "SingleType" not in hints:
+                    V.add_edge_(sim, bids[i], V.Edge(bids[i], st))
+            V.finish_init_(sim)
+            V.apply_(sim, V.transitions.die_if("odd"), AgentB, AgentB, AgentB)
+            r.append(V.num_edges(sim, ET))
+        out["remove"][name] = r
+    return out

@@ -20,7 +20,8 @@ def _eq(a, b):
 
 @pytest.mark.parametrize("name", ["core_aggregate", "core_sum_neighbors", "core_queries",
                                   "edges_matrix", "edges_transition", "remove_agents",
-                                  "independent", "addexisting", "graphs_k4"])
+                                  "independent", "addexisting", "graphs_k4",
+                                  "edges_iter_removal"])
 def test_reference_scenarios(vh, ora, gpu_ready, name):
     fn = getattr(SC, name)
     print(f"[gpu] scenario {name}", flush=True)

@@ -129,3 +129,26 @@ def test_addexisting(ora):
 
 def test_graphs_k4(ora):
     assert SC.graphs_k4(ora) == sum(range(1, 5)) * 3        # test/graphs.jl:42-43
+
+
+def test_edges_iterator_and_removal(ora):
+    out = SC.edges_iter_removal(ora)
+    for name, (before, after) in out["iter"].items():         # test/edgesiterator.jl:33-43
+        expected = 10 if SC.hashint(name, "E") else 20
+        assert before == after == expected, name
+    for name, r in out["agg"].items():                        # :66-101
+        assert r == [55, 0, 55, 0], name
+    for name, r in out["remove"].items():                     # :131-210
+        assert r[:2] == [10, 0], name
+        hints = SC.EDGE_HINTS[name]
+        if "IgnoreFrom" in hints:
+            assert len(r) == 2
+            continue
+        count = 5
+        if "SingleEdge" not in hints:
+            count += 10
+        if "SingleType" not in hints:
+            count += 5
+        if "SingleEdge" not in hints and "SingleType" not in hints:
+            count += 5
+        assert r[2] == count, (name, r)

@@ -251,12 +251,18 @@ def bench_gather_floor(vh):
         L.probe_memory(0, mb << 20, n, 10, C.byref(ms))
         rows.append({"footprint_MiB": mb, "ms": ms.value, "gathers_per_s": n / (ms.value * 1e-3)})
         log(f"gather footprint {mb:5d} MiB: {ms.value:.3f} ms  {n / ms.value / 1e6:.1f} G/s")
+    hc = {}
+    for mode, name in ((2, "hot22_cold_normal"), (3, "hot22_cold_nontemporal")):
+        ms = C.c_float()
+        L.probe_memory(mode, 80 << 20, n, 10, C.byref(ms))
+        hc[name] = ms.value
+        log(f"{name} (80 MiB): {ms.value:.3f} ms  {n / ms.value / 1e6:.1f} G/s")
     ms = C.c_float()
     L.probe_memory(1, 800 << 20, 0, 10, C.byref(ms))
     stream = (800 << 20) / (ms.value * 1e-3) / 1e9
     log(f"stream read 800 MiB: {ms.value:.3f} ms  {stream:.0f} GB/s")
     print(json.dumps({"metric": "random 8-byte gathers/s by footprint", "gathers": n,
-                      "rows": rows, "stream_read_GBps": stream}), flush=True)
+                      "rows": rows, "hot_cold_ms": hc, "stream_read_GBps": stream}), flush=True)
 
 
 def bench_ensemble(vh, args):

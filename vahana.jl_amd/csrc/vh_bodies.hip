@@ -4,6 +4,7 @@
 // src/AgentMethods.jl:193-245 with the write policy of :159-181); the model
 // bodies (SIR, weighted bounded confidence, dynamic contacts) have their own
 // kernels over the CSR.
+#include <stdlib.h>
 #include <string.h>
 
 #include "vh_engine.h"
@@ -330,6 +331,14 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
 // edge's two terms to LDS; phase 2 lets every thread add its own agent's terms
 // in edge order, so each agent's sums are the sequential canonical sums.
 constexpr int OP_TILE = 2048;
+
+// neighbour state load: sources up to `hot` (1-based) with normal loads, the
+// others non-temporal, so that the hot prefix can stay in L2 while the cold
+// gathers stream past it
+template <class T>
+__device__ __forceinline__ T ld_state(const T* __restrict__ x, uint32_t s, uint32_t hot) {
+    return s <= hot ? x[s - 1] : __builtin_nontemporal_load(x + (s - 1));
+}
 constexpr int OP_U = 8;   // gathers in flight per thread
 
 template <class T>
@@ -337,7 +346,7 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
                                               const T* __restrict__ x, const float* __restrict__ w,
                                               T* __restrict__ xo, uint64_t n, uint64_t nrows,
-                                              uint64_t rowbase, T eps, uint32_t blk) {
+                                              uint64_t rowbase, T eps, uint32_t blk, uint32_t hot) {
     __shared__ T s_num[OP_TILE];
     __shared__ float s_den[OP_TILE];
     __shared__ uint8_t s_own[OP_TILE];
@@ -403,7 +412,7 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                 wk[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(w + kk[u]) : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < OP_U; u++) xj[u] = kk[u] != 0xFFFFFFFFu ? x[sj[u] - 1] : (T)0;
+            for (int u = 0; u < OP_U; u++) xj[u] = kk[u] != 0xFFFFFFFFu ? ld_state(x, sj[u], hot) : (T)0;
 #pragma unroll
             for (int u = 0; u < OP_U; u++) {
                 const uint32_t q = q0 + (uint32_t)u * 256u + tid;
@@ -428,7 +437,8 @@ template <class T>
 __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
                                               const T* __restrict__ x, const float* __restrict__ w,
-                                              T* __restrict__ xo, uint64_t r, uint64_t rowbase, T eps) {
+                                              T* __restrict__ xo, uint64_t r, uint64_t rowbase, T eps,
+                                              uint32_t hot) {
     const int lane = threadIdx.x & 63;
     uint64_t i = r - rowbase;
     const T xi = x[i];
@@ -439,7 +449,8 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
     for (; k + 192 < e; k += 256) {
         uint32_t s0 = src[k], s1 = src[k + 64], s2 = src[k + 128], s3 = src[k + 192];
         float w0 = w[k], w1 = w[k + 64], w2 = w[k + 128], w3 = w[k + 192];
-        T x0 = x[s0 - 1], x1 = x[s1 - 1], x2 = x[s2 - 1], x3 = x[s3 - 1];
+        T x0 = ld_state(x, s0, hot), x1 = ld_state(x, s1, hot);
+        T x2 = ld_state(x, s2, hot), x3 = ld_state(x, s3, hot);
         bool a0 = fabs(x0 - xi) <= eps, a1 = fabs(x1 - xi) <= eps;
         bool a2 = fabs(x2 - xi) <= eps, a3 = fabs(x3 - xi) <= eps;
         num = num + (a0 ? (T)w0 * x0 : (T)0);
@@ -452,7 +463,7 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
         den = den + (a3 ? (T)w3 : (T)0);
     }
     for (; k < e; k += 64) {
-        T xj = x[src[k] - 1];
+        T xj = ld_state(x, src[k], hot);
         T wk = (T)w[k];
         if (fabs(xj - xi) <= eps) {
             num = num + wk * xj;
@@ -474,15 +485,15 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
                                                  T* __restrict__ xo, uint64_t n, uint64_t nrows,
                                                  uint64_t rowbase, T eps,
                                                  const uint32_t* __restrict__ heavy, uint64_t nheavy,
-                                                 uint32_t nlight) {
+                                                 uint32_t nlight, uint32_t hot) {
     // hub waves first: they are the longest, so they start before the light
     // blocks instead of forming the tail of the launch
     const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
     if (blockIdx.x >= nh) {
-        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps, blockIdx.x - nh);
+        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps, blockIdx.x - nh, hot);
     } else {
         uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        if (wv < nheavy) opinion_heavy<T>(off, src, x, w, xo, heavy[wv], rowbase, eps);
+        if (wv < nheavy) opinion_heavy<T>(off, src, x, w, xo, heavy[wv], rowbase, eps, hot);
     }
 }
 
@@ -639,15 +650,22 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     uint64_t nheavy = E.rows ? E.nheavy : 0;
     uint32_t nlight = grid_for(A.nslots, 256);
     uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+    // hot prefix of the state kept in L2 (VH_OPINION_HOT_BYTES; default: off)
+    static long long hot_bytes = -2;
+    if (hot_bytes == -2) {
+        const char* v = getenv("VH_OPINION_HOT_BYTES");
+        hot_bytes = v ? atoll(v) : -1;
+    }
+    uint32_t hot = hot_bytes < 0 ? 0xFFFFFFFFu : (uint32_t)(hot_bytes / (f32 ? 4 : 8));
     dim3 g(nlight + nh), b(256);
     if (f32)
         hipLaunchKernelGGL(k_opinion<float>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
                            (const float*)A.rstate(), E.st.as<float>(), (float*)A.wstate(), A.nslots,
-                           nrows, E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nlight);
+                           nrows, E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nlight, hot);
     else
         hipLaunchKernelGGL(k_opinion<double>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
                            (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(), A.nslots,
-                           nrows, E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nlight);
+                           nrows, E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nlight, hot);
     HIPCHECK(hipGetLastError());
     return VH_OK;
 }

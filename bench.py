@@ -186,9 +186,11 @@ def main():
     assert num_trans == 1 + args.warmup + args.steps
 
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"{args.workload}_n{n}_traffic.json")
-    if os.path.exists(pmc):
-        traffic = json.load(open(pmc)).get("bytes_per_launch")
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # runs of this exact workload (scaled by the copy-kernel calibration)
+    pmc = {("opinion", 10_000_000): "k_opinion_f64_ba10M_traffic.json"}.get((args.workload, n))
+    if pmc and os.path.exists(os.path.join(ROOT, "profiles", pmc)):
+        traffic = json.load(open(os.path.join(ROOT, "profiles", pmc))).get("bytes_per_launch")
 
     drift = None
     if f32:

@@ -506,62 +506,218 @@ __global__ void k_heavy_rows(const uint32_t* __restrict__ off, uint64_t rows, ui
 }
 
 // ----------------------------------------------------------------- contact
-// state: health byte 0, susceptibility float at bytes 4-7
-__global__ __launch_bounds__(256) void k_contact(
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, const uint64_t* __restrict__ rs,
-    uint64_t* __restrict__ ws, const uint8_t* __restrict__ died, uint8_t* __restrict__ died_now,
-    uint64_t n, uint64_t nrows, uint64_t rowbase, uint32_t t, vh_contact_params p, uint64_t step,
-    uint32_t* __restrict__ cnt_e, uint32_t* __restrict__ log, uint8_t* __restrict__ cnt_b,
-    uint64_t* __restrict__ birth_st) {
+// state: health byte 0, susceptibility float at bytes 4-7.
+// The step runs on per-slot flag bitmaps (64 slots per word): `sp` marks
+// slots that are dead in the read state or die in this step, `dead` the
+// former, `inf` slots infected in the read state. A partner draw reads one
+// bit of `sp` and, for the ~4% of partners it marks, one of `dead`. At 10^7
+// slots each map is 1.25 MB, so the two hot ones stay in one XCD's 4 MB L2
+// while the draws and neighbour checks gather from them (the state vector
+// itself is 80 MB and is streamed past the cache).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bool cbit(const uint64_t* __restrict__ b, uint64_t i) {
+    return (b[i >> 6] >> (i & 63)) & 1ull;
+}
+
+// one thread per slot, whole waves per word (grid = nwords * 64 threads)
+__global__ __launch_bounds__(256) void k_contact_flags(const uint64_t* __restrict__ rs,
+                                                       const uint8_t* __restrict__ died, uint64_t n,
+                                                       uint32_t t, vh_contact_params p, uint64_t step,
+                                                       ContactBits B) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (died[i]) {
-        cnt_e[i] = 0;
-        cnt_b[i] = 0;
-        return;
+    uint64_t w = i >> 6;
+    if (w >= B.nwords) return;   // uniform per wave
+    bool in = i < n;
+    bool dead = !in || died[i] != 0;
+    bool inf = in && (__builtin_nontemporal_load(rs + i) & 0xFFu) == VH_I;
+    bool dn = !dead && vh_u01(vh_hash(p.seed, vh_make_id(t, i + 1), step, VH_KEY_DEATH)) < p.p_death;
+    uint64_t md = __ballot(dead), mi = __ballot(inf), mn = __ballot(dn);
+    if ((threadIdx.x & 63u) == 0) {
+        B.sp[w] = md | mn;
+        B.dead[w] = md;
+        B.inf[w] = mi;
+        B.dcnt[w] = (uint32_t)__popcll(mn);
     }
-    const uint64_t id = vh_make_id(t, i + 1);
-    const uint64_t s = rs[i];
-    uint32_t h = (uint32_t)(s & 0xFFu);
-    if (h == VH_S) {
-        if (off && i < nrows) {
-            float susc = __uint_as_float((uint32_t)(s >> 32));
-            double pinf = p.beta * (double)susc;
-            uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
-            for (uint32_t k = b; k < e; k++) {
-                uint32_t j = src[k];
-                if ((rs[j - 1] & 0xFFu) == VH_I &&
-                    vh_u01(vh_hash(p.seed, id, step, k - b + 1)) < pinf) {
-                    h = VH_I;
-                    break;
+}
+
+// Health update, death flag, partner draws. Rows of the previous contact
+// CSR are staged through LDS when the block's range fits (coalesced reads);
+// the kept partners of every slot (accepted draws whose partner does not die
+// in this step, in draw order) go to a fixed-stride log, row i at
+// log[i * n_contacts], written through LDS as one contiguous block range.
+// Dynamic LDS: 256 * n_contacts words when `staged`.
+__global__ __launch_bounds__(256) void k_contact_main(
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, const uint64_t* __restrict__ rs,
+    uint64_t* __restrict__ ws, uint8_t* __restrict__ died_now, uint64_t n, uint64_t nrows,
+    uint64_t rowbase, uint32_t t, vh_contact_params p, uint64_t step, ContactBits B,
+    uint32_t* __restrict__ kept, uint32_t* __restrict__ log, uint32_t staged) {
+    extern __shared__ u32x4 lds_v[];   // 16-byte aligned
+    uint32_t* lds = (uint32_t*)lds_v;
+    __shared__ uint32_t s_b, s_e;
+    const uint32_t C = p.n_contacts;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t i0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t i = i0 + tid;
+    const uint64_t cap = staged ? 256ull * C : 0;
+    // the block's in-edge range
+    if (tid == 0) {
+        uint32_t b = 0, e = 0;
+        if (off && i0 < nrows) {
+            b = off[rowbase + i0];
+            e = off[rowbase + (i0 + 256 < nrows ? i0 + 256 : nrows)];
+        }
+        s_b = b;
+        s_e = e;
+    }
+    __syncthreads();
+    const uint32_t rb = s_b;
+    const bool edges_in_lds = (uint64_t)(s_e - rb) <= cap;
+    if (edges_in_lds)
+        for (uint32_t q = tid; q < s_e - rb; q += 256) lds[q] = __builtin_nontemporal_load(src + rb + q);
+    __syncthreads();
+
+    bool live = i < n && !cbit(B.dead, i);
+    bool survive = false;
+    uint32_t nk = 0;
+    if (live) {
+        const uint64_t id = vh_make_id(t, i + 1);
+        const uint64_t pre = vh_hash_pre(p.seed, id, step);
+        const uint64_t s = __builtin_nontemporal_load(rs + i);
+        uint32_t h = (uint32_t)(s & 0xFFu);
+        if (h == VH_S) {
+            if (off && i < nrows) {
+                float susc = __uint_as_float((uint32_t)(s >> 32));
+                double pinf = p.beta * (double)susc;
+                uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
+                // "some in-edge k has an infected source and u_k < pinf" is
+                // order-free: test the draw first (ALU) and look up the
+                // source's flag only for the few draws below pinf
+                for (uint32_t k = b; k < e; k++) {
+                    if (vh_u01(vh_hash_fin(pre, k - b + 1)) >= pinf) continue;
+                    uint32_t j = edges_in_lds ? lds[k - rb] : src[k];
+                    if (cbit(B.inf, j - 1)) {
+                        h = VH_I;
+                        break;
+                    }
                 }
             }
+        } else if (h == VH_I) {
+            if (vh_u01(vh_hash_fin(pre, VH_KEY_RECOVER)) < p.gamma) h = VH_R;
         }
-    } else if (h == VH_I) {
-        if (vh_u01(vh_hash(p.seed, id, step, VH_KEY_RECOVER)) < p.gamma) h = VH_R;
+        if (cbit(B.sp, i))   // live and marked: dies in this step
+            died_now[i] = 1;
+        else {
+            survive = true;
+            __builtin_nontemporal_store((s & ~(uint64_t)0xFFu) | h, ws + i);
+        }
     }
-    if (vh_u01(vh_hash(p.seed, id, step, VH_KEY_DEATH)) < p.p_death) {
-        died_now[i] = 1;
-        float ns = 0.5f + 0.5f * vh_u01f(vh_hash(p.seed, id, step, VH_KEY_BIRTH));
-        birth_st[i] = (uint64_t)VH_S | ((uint64_t)__float_as_uint(ns) << 32);
-        cnt_b[i] = 1;
-        cnt_e[i] = 0;
+    __syncthreads();   // LDS now holds the log rows of this block
+    if (survive) {
+        const uint64_t pre = vh_hash_pre(p.seed, vh_make_id(t, i + 1), step);
+        uint32_t* row = staged ? lds + (uint64_t)tid * C : log + i * C;
+        uint64_t key = VH_KEY_PARTNER;
+        for (uint32_t cc = 0; cc < C; cc++, key += p.max_redraw) {
+            for (uint32_t r = 0; r < p.max_redraw; r++) {
+                uint64_t hh = vh_hash_fin(pre, key + r);
+                uint64_t j = 1 + (((hh >> 32) * n) >> 32);
+                if (j == i + 1) continue;
+                if (cbit(B.sp, j - 1)) {
+                    if (cbit(B.dead, j - 1)) continue;   // dead in the read state: redraw
+                    break;                               // dies in this step: not kept
+                }
+                row[nk++] = (uint32_t)j;
+                break;
+            }
+        }
+    }
+    if (i < n) kept[i] = nk;
+    if (staged) {
+        __syncthreads();
+        uint64_t rows_here = n - i0 < 256 ? n - i0 : 256;
+        uint64_t len = rows_here * C;
+        uint32_t* dst = log + i0 * C;   // 16-byte aligned: i0 is a multiple of 256
+        uint64_t len4 = len >> 2;
+        for (uint64_t q = tid; q < len4; q += 256)
+            __builtin_nontemporal_store(*(const u32x4*)(lds + 4 * q), (u32x4*)(dst + 4 * q));
+        for (uint64_t q = 4 * len4 + tid; q < len; q += 256) dst[q] = lds[q];
+    }
+}
+
+// newborns of this step's deaths, in slot order of the dying callers: the
+// top of the reuse stack first, then fresh slots (src/AgentMethods.jl:37-89)
+__global__ void k_contact_births(uint64_t n, uint32_t t, vh_contact_params p, uint64_t step,
+                                 ContactBits B, const uint32_t* __restrict__ dpos,
+                                 const uint32_t* __restrict__ reuse, uint64_t nreuse, uint64_t nextid,
+                                 uint64_t* __restrict__ ws, uint8_t* __restrict__ died) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t word = B.sp[i >> 6] & ~B.dead[i >> 6];   // dying in this step
+    uint32_t lane = (uint32_t)(i & 63);
+    if (!((word >> lane) & 1ull)) return;
+    uint64_t g = dpos[i >> 6] + (uint64_t)__popcll(word & ((1ull << lane) - 1ull));
+    uint64_t slot = g < nreuse ? reuse[nreuse - 1 - g] : nextid + (g - nreuse);
+    float ns = 0.5f + 0.5f * vh_u01f(vh_hash(p.seed, vh_make_id(t, i + 1), step, VH_KEY_BIRTH));
+    ws[slot - 1] = (uint64_t)VH_S | ((uint64_t)__float_as_uint(ns) << 32);
+    died[slot - 1] = 0;
+}
+
+// this step's deaths onto the reuse stack in slot order (src/AgentMethods.jl:314-345)
+__global__ void k_contact_deaths(uint64_t n, ContactBits B, const uint32_t* __restrict__ dpos,
+                                 uint32_t* __restrict__ reuse, uint64_t base, uint8_t* __restrict__ died) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t word = B.sp[i >> 6] & ~B.dead[i >> 6];   // dying in this step
+    uint32_t lane = (uint32_t)(i & 63);
+    if (!((word >> lane) & 1ull)) return;
+    uint64_t g = dpos[i >> 6] + (uint64_t)__popcll(word & ((1ull << lane) - 1ull));
+    reuse[base + g] = (uint32_t)(i + 1);
+    died[i] = 1;
+}
+
+// rows past the last caller (fresh newborn slots) are empty
+__global__ void k_contact_extend(uint32_t* __restrict__ off, uint64_t from_row, uint64_t rows) {
+    uint64_t r = from_row + 1 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= rows) off[r] = off[from_row];
+}
+
+// The rebuilt contact CSR (row i = caller slot i, kept partners in draw
+// order): the block's log rows are read as one contiguous range, compacted
+// in LDS and written as the block's contiguous CSR range. Dynamic LDS:
+// 2 * 256 * n_contacts words when `staged`.
+__global__ __launch_bounds__(256) void k_contact_edges(uint64_t n, uint32_t C,
+                                                       const uint32_t* __restrict__ kept,
+                                                       const uint32_t* __restrict__ log,
+                                                       const uint32_t* __restrict__ off,
+                                                       uint32_t* __restrict__ src, uint32_t staged) {
+    extern __shared__ u32x4 lds_v[];   // 16-byte aligned
+    uint32_t* lds = (uint32_t*)lds_v;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t i0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t i = i0 + tid;
+    if (!staged) {   // uniform
+        if (i >= n) return;
+        uint32_t k = kept[i], o = off[i];
+        for (uint32_t r = 0; r < k; r++) src[o + r] = log[i * C + r];
         return;
     }
-    cnt_b[i] = 0;
-    const uint64_t S = n;
-    uint32_t ne = 0;
-    for (uint32_t cc = 0; cc < p.n_contacts; cc++) {
-        for (uint32_t r = 0; r < p.max_redraw; r++) {
-            uint64_t hh = vh_hash(p.seed, id, step, VH_KEY_PARTNER + (uint64_t)cc * p.max_redraw + r);
-            uint64_t j = 1 + (((hh >> 32) * S) >> 32);
-            if (j == i + 1 || died[j - 1]) continue;
-            log[i * p.n_contacts + ne++] = (uint32_t)j;
-            break;
-        }
+    uint32_t* lin = lds;
+    uint32_t* lout = lds + 256 * C;
+    const uint64_t rows_here = n - i0 < 256 ? n - i0 : 256;
+    const uint64_t len = rows_here * C;
+    const uint32_t* lsrc = log + i0 * C;   // 16-byte aligned: i0 is a multiple of 256
+    const uint64_t len4 = len >> 2;
+    for (uint64_t q = tid; q < len4; q += 256)
+        *(u32x4*)(lin + 4 * q) = __builtin_nontemporal_load((const u32x4*)(lsrc + 4 * q));
+    for (uint64_t q = 4 * len4 + tid; q < len; q += 256) lin[q] = lsrc[q];
+    const uint32_t base = off[i0], end = off[i0 + rows_here];
+    __syncthreads();
+    if (i < n) {
+        uint32_t k = kept[i], o = off[i] - base;
+        for (uint32_t r = 0; r < k; r++) lout[o + r] = lin[tid * C + r];
     }
-    cnt_e[i] = ne;
-    ws[i] = (s & ~(uint64_t)0xFFu) | h;
+    __syncthreads();
+    for (uint32_t q = tid; q < end - base; q += 256) __builtin_nontemporal_store(lout[q], src + base + q);
 }
 
 }  // namespace
@@ -670,18 +826,97 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     return VH_OK;
 }
 
-int launch_contact(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step, const ContactLogs& L) {
+// One apply! of the contact model: flags, health update with the kept
+// partners logged, one readback (deaths, contacts), newborns and deaths on
+// the reuse stack, then the contact CSR rebuilt in place from the log (its
+// previous entries were last read by k_contact_main). Deaths of this step
+// are returned for the edge filtering of other edge types.
+int contact_apply(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step, uint64_t* ndeaths) {
     RC(check_model_layout(s, t, et, 8, 0, "contact"));
     AgentStore& A = s.a[t];
     EdgeStore& E = s.e[et];
-    if (!A.nslots) return VH_OK;
-    if (A.nslots >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "contact: too many slots");
-    hipLaunchKernelGGL(k_contact, dim3(grid_for(A.nslots, 256)), dim3(256), 0, s.stream,
-                       E.rows ? E.off.as<uint32_t>() : nullptr, E.src.as<uint32_t>(),
-                       (const uint64_t*)A.rstate(), (uint64_t*)A.wstate(), A.died.as<uint8_t>(),
-                       A.died_now.as<uint8_t>(), A.nslots, E.rows ? E.nrows[t] : 0ull, E.rowbase[t],
-                       (uint32_t)t, p, step, L.cnt_e, L.log_src, L.cnt_b, (uint64_t*)L.birth_st);
+    hipStream_t st = s.stream;
+    const uint64_t n = A.nslots;
+    const uint32_t C = p.n_contacts;
+    *ndeaths = 0;
+    if (n >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "contact: too many slots");
+    if ((uint64_t)n * C >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "contact: too many contacts");
+    // LDS staging of 256 rows of C contacts (main: one buffer, CSR copy: two)
+    const uint32_t staged = C >= 1 && C <= 32;
+    const uint64_t nw = (n + 63) / 64;
+    ContactBits B;
+    B.nwords = nw;
+    B.sp = (uint64_t*)s.scratch.take(nw * 8 + 8);
+    B.dead = (uint64_t*)s.scratch.take(nw * 8 + 8);
+    B.inf = (uint64_t*)s.scratch.take(nw * 8 + 8);
+    B.dcnt = (uint32_t*)s.scratch.take(nw * 4 + 8);
+    uint32_t* dpos = (uint32_t*)s.scratch.take((nw + 1) * 4 + 8);
+    uint32_t* kept = (uint32_t*)s.scratch.take(n * 4 + 8);
+    uint32_t* log = (uint32_t*)s.scratch.take(n * C * 4 + 8);
+    if (!B.sp || !B.dead || !B.inf || !B.dcnt || !dpos || !kept || !log) return set_error(VH_ERR_OOM, "scratch");
+    uint64_t nd = 0, ne = 0;
+    if (n) {
+        hipLaunchKernelGGL(k_contact_flags, dim3(grid_for(nw * 64, 256)), dim3(256), 0, st,
+                           (const uint64_t*)A.rstate(), A.died.as<uint8_t>(), n, (uint32_t)t, p, step, B);
+        if (scan_excl_u32(B.dcnt, dpos, nw, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+        hipLaunchKernelGGL(k_contact_main, dim3(grid_for(n, 256)), dim3(256), staged ? 256 * C * 4 : 0, st,
+                           E.rows ? E.off.as<uint32_t>() : nullptr, E.src.as<uint32_t>(),
+                           (const uint64_t*)A.rstate(), (uint64_t*)A.wstate(), A.died_now.as<uint8_t>(), n,
+                           E.rows ? E.nrows[t] : 0ull, E.rowbase[t], (uint32_t)t, p, step, B, kept, log,
+                           staged);
+        HIPCHECK(hipGetLastError());
+        RC(E.off.ensure((n + 1) * 4 + 8, st));
+        if (scan_excl_u32(kept, E.off.as<uint32_t>(), n, s.scratch, st))
+            return set_error(VH_ERR_DEVICE, "scan");
+        HIPCHECK(hipMemcpyAsync((uint32_t*)s.d_stats.p, dpos + nw, 4, hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync((uint32_t*)s.d_stats.p + 1, E.off.as<uint32_t>() + n, 4,
+                                hipMemcpyDeviceToDevice, st));
+        RC(readback(s, 1));
+        nd = ((const uint32_t*)s.h_stats)[0];
+        ne = ((const uint32_t*)s.h_stats)[1];
+    }
+    if (nd) {
+        uint64_t consumed = std::min<uint64_t>(nd, A.nreuse);
+        uint64_t fresh = nd - consumed;
+        if (fresh) {
+            RC(ensure_slots(s, t, A.nextid - 1 + fresh));
+            if (A.nextid - 1 + fresh > n)
+                fill_u8(A.died_now.as<uint8_t>() + n, 0, A.nextid - 1 + fresh - n, st);
+        }
+        hipLaunchKernelGGL(k_contact_births, dim3(grid_for(n, 256)), dim3(256), 0, st, n, (uint32_t)t, p,
+                           step, B, dpos, A.reuse.as<uint32_t>(), A.nreuse, A.nextid, (uint64_t*)A.wstate(),
+                           A.died.as<uint8_t>());
+        A.nreuse -= consumed;
+        A.nextid += fresh;
+        RC(A.reuse.ensure((A.nreuse + nd) * 4 + 8, st, A.nreuse * 4));
+        hipLaunchKernelGGL(k_contact_deaths, dim3(grid_for(n, 256)), dim3(256), 0, st, n, B, dpos,
+                           A.reuse.as<uint32_t>(), A.nreuse, A.died.as<uint8_t>());
+        A.nreuse += nd;
+        HIPCHECK(hipGetLastError());
+    }
+    // rows = caller slots; rows of dead callers and of newborns are empty
+    uint64_t rows = std::max(n, A.nextid - 1);
+    RC(E.off.ensure((rows + 1) * 4 + 8, st, (n + 1) * 4));
+    if (!n) fill_u32(E.off.as<uint32_t>(), 0, 1, st);
+    if (rows > n)
+        hipLaunchKernelGGL(k_contact_extend, dim3(grid_for(rows - n, 256)), dim3(256), 0, st,
+                           E.off.as<uint32_t>(), n, rows);
+    RC(E.src.ensure(ne * 4 + 8, st));
+    if (ne)
+        hipLaunchKernelGGL(k_contact_edges, dim3(grid_for(n, 256)), dim3(256), staged ? 2 * 256 * C * 4 : 0,
+                           st, n, C, kept, log, E.off.as<uint32_t>(), E.src.as<uint32_t>(), staged);
     HIPCHECK(hipGetLastError());
+    for (int u = 0; u <= VH_MAXA; u++) {
+        E.rowbase[u] = 0;
+        E.nrows[u] = 0;
+    }
+    E.nrows[t] = rows;
+    E.rows = rows;
+    E.nedges = ne;
+    E.src64 = false;
+    E.src_type = (uint32_t)t;
+    E.version++;
+    *ndeaths = nd;
     return VH_OK;
 }
 

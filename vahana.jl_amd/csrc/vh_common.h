@@ -42,12 +42,20 @@ __host__ __device__ __forceinline__ uint64_t vh_mix64(uint64_t z) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
-__host__ __device__ __forceinline__ uint64_t vh_hash(uint64_t s, uint64_t a, uint64_t b,
-                                                     uint64_t c) {
+// The first three stages depend only on (seed, a, b): a body drawing many
+// `c` for one agent and step computes them once (vh_hash_pre) and finishes
+// each draw with one stage (vh_hash_fin); the value is vh_hash's.
+__host__ __device__ __forceinline__ uint64_t vh_hash_pre(uint64_t s, uint64_t a, uint64_t b) {
     uint64_t h = vh_mix64(s + VH_GOLDEN);
     h = vh_mix64(h + a + VH_GOLDEN);
-    h = vh_mix64(h + b + VH_GOLDEN);
-    return vh_mix64(h + c + VH_GOLDEN);
+    return vh_mix64(h + b + VH_GOLDEN);
+}
+__host__ __device__ __forceinline__ uint64_t vh_hash_fin(uint64_t pre, uint64_t c) {
+    return vh_mix64(pre + c + VH_GOLDEN);
+}
+__host__ __device__ __forceinline__ uint64_t vh_hash(uint64_t s, uint64_t a, uint64_t b,
+                                                     uint64_t c) {
+    return vh_hash_fin(vh_hash_pre(s, a, b), c);
 }
 // uniform double in [0,1) from the top 53 bits; float from the top 24
 __host__ __device__ __forceinline__ double vh_u01(uint64_t h) {

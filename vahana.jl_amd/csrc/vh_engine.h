@@ -83,14 +83,19 @@ int launch_generic(int transition, const DCtx* d_ctx, int ci, int call_t, uint64
                    int with_edge, hipStream_t st);
 int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step);
 int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step);
-struct ContactLogs {
-    uint32_t* cnt_e;   // per slot
-    uint32_t* log_src; // nslots * n_contacts
-    uint8_t* cnt_b;    // per slot: 0/1 newborn
-    uint8_t* birth_st; // nslots * 8 bytes
+// contact model: per-step flag bitmaps, one bit per slot (64 per word)
+struct ContactBits {
+    uint64_t* sp;     // dead in the read state or dying in this step (padding set)
+    uint64_t* dead;   // dead in the read state (padding set)
+    uint64_t* inf;    // infected in the read state
+    uint32_t* dcnt;   // deaths per word
+    uint64_t nwords;
 };
-int launch_contact(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step,
-                   const ContactLogs& L);
+// one apply! of the contact model incl. its births, deaths and contact CSR
+int contact_apply(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step, uint64_t* ndeaths);
+// engine services the bodies use
+int readback(Sim& s, int n);                           // d_stats -> h_stats, synchronous
+int ensure_slots(Sim& s, int t, uint64_t slots);
 // degree bins for the opinion kernels
 int build_heavy_bins(Sim& s, EdgeStore& E, int t);
 

@@ -345,14 +345,20 @@ __global__ void k_base_expand(const uint32_t* __restrict__ off, const void* __re
     }
 }
 
-// off[r] = first index whose key >= r, for r in [0, rows]
+// off[r] = first index whose key >= r, for r in [0, rows]: one thread per
+// row, binary search over the sorted keys (rows without keys cost no more
+// than rows with keys)
 __global__ void k_offsets_from_sorted(const uint32_t* __restrict__ keys, uint64_t n, uint64_t rows,
                                       uint32_t* __restrict__ off) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    int64_t prev = i == 0 ? -1 : (int64_t)keys[i - 1];
-    int64_t cur = i == n ? (int64_t)rows : (int64_t)keys[i];
-    for (int64_t r = prev + 1; r <= cur; r++) off[r] = (uint32_t)i;
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > rows) return;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)keys[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    off[r] = (uint32_t)lo;
 }
 
 // row-wise keep flags: SingleEdge keeps the last entry; deaths drop rows of
@@ -435,7 +441,7 @@ static void compute_layout(Sim& s, const EdgeStore& E, RowLayout& L, uint64_t& r
     }
 }
 
-static int readback(Sim& s, int n) {
+int vh::readback(Sim& s, int n) {
     HIPCHECK(hipMemcpyAsync(s.h_stats, s.d_stats.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost,
                             s.stream));
     HIPCHECK(hipStreamSynchronize(s.stream));
@@ -512,7 +518,7 @@ static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
     // offsets
     uint32_t* off = (uint32_t*)sc.take((rows + 1) * 4);
     if (!off) return set_error(VH_ERR_OOM, "scratch");
-    hipLaunchKernelGGL(k_offsets_from_sorted, dim3(grid_for(n + 1, 256)), dim3(256), 0, st, skeys, n,
+    hipLaunchKernelGGL(k_offsets_from_sorted, dim3(grid_for(rows + 1, 256)), dim3(256), 0, st, skeys, n,
                        rows, off);
     uint64_t nfinal = n;
     // :SingleEdge keeps the last added edge per target
@@ -608,7 +614,7 @@ static int filter_deaths(Sim& s, EdgeStore& E, const DiedNow& dn) {
 }
 
 // ================================================================ agents
-static int ensure_slots(Sim& s, int t, uint64_t slots) {
+int vh::ensure_slots(Sim& s, int t, uint64_t slots) {
     AgentStore& A = s.a[t];
     if (slots <= A.cap && A.state[0].p) return VH_OK;
     uint64_t nc = std::max<uint64_t>(slots, A.cap + A.cap / 4);
@@ -701,47 +707,9 @@ __global__ void k_push_deaths(const uint8_t* __restrict__ died_now, const uint32
     died[i] = 1;
 }
 
-__global__ void k_contact_births(const uint8_t* __restrict__ cnt_b, const uint32_t* __restrict__ pos,
-                                 uint64_t n, const uint8_t* __restrict__ bst,
-                                 const uint32_t* __restrict__ reuse, uint64_t nreuse, uint64_t nextid,
-                                 uint8_t* __restrict__ wstate, uint8_t* __restrict__ died) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !cnt_b[i]) return;
-    uint64_t g = pos[i];
-    uint64_t slot = g < nreuse ? reuse[nreuse - 1 - g] : nextid + (g - nreuse);
-    ((uint64_t*)wstate)[slot - 1] = ((const uint64_t*)bst)[i];
-    died[slot - 1] = 0;
-}
 
 // contact rebuild: rows are the callers (edges target their caller); copy
 // the kept entries of the fixed-capacity log (sources not dead this step)
-__global__ void k_contact_count(const uint32_t* __restrict__ cnt_e, const uint32_t* __restrict__ log,
-                                uint32_t cap, uint64_t n, const uint8_t* __restrict__ died_now,
-                                uint32_t* __restrict__ kept) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t c = cnt_e[i], k = 0;
-    for (uint32_t j = 0; j < c; j++) k += died_now[log[i * cap + j] - 1] ? 0u : 1u;
-    kept[i] = k;
-}
-__global__ void k_contact_write(const uint32_t* __restrict__ cnt_e, const uint32_t* __restrict__ log,
-                                uint32_t cap, uint64_t n, const uint8_t* __restrict__ died_now,
-                                const uint32_t* __restrict__ off, uint32_t* __restrict__ src) {
-    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t i = q / cap;
-    uint32_t j = (uint32_t)(q - i * cap);
-    if (i >= n || j >= cnt_e[i]) return;
-    uint32_t v = log[q];
-    if (died_now[v - 1]) return;
-    // rank of entry j among the kept entries of caller i
-    uint32_t r = 0;
-    for (uint32_t p = 0; p < j; p++) r += died_now[log[i * cap + p] - 1] ? 0u : 1u;
-    src[off[i] + r] = v;
-}
-__global__ void k_extend_offsets(uint32_t* __restrict__ off, uint64_t from_row, uint64_t rows) {
-    uint64_t r = from_row + 1 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r <= rows) off[r] = off[from_row];
-}
 __global__ void k_copy_nondead(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
@@ -914,8 +882,8 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
     uint64_t* e_to[2] = {nullptr, nullptr};
     uint64_t* e_from[2] = {nullptr, nullptr};
     uint8_t* e_st[2] = {nullptr, nullptr};
-    ContactLogs CL = {nullptr, nullptr, nullptr, nullptr};
     int contact_t = -1;
+    uint64_t contact_deaths = 0;
 
     if (kind == K_PLAIN) {
         RC(upload_ctx(s, c));
@@ -1030,29 +998,26 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
             in_mask(d->add_existing_edges, d->role_edge[0]))
             return set_error(VH_ERR_ARG, "contact: mortal 8-byte agent type in write, edge type "
                                          "rebuilt (in write, not add_existing)");
-        uint64_t n = A.nslots;
-        CL.cnt_e = (uint32_t*)s.scratch.take(n * 4 + 8);
-        CL.log_src = (uint32_t*)s.scratch.take(n * (uint64_t)p.n_contacts * 4 + 8);
-        CL.cnt_b = (uint8_t*)s.scratch.take(n + 8);
-        CL.birth_st = (uint8_t*)s.scratch.take(n * 8 + 8);
-        if (!CL.cnt_e || !CL.log_src || !CL.cnt_b || !CL.birth_st) return set_error(VH_ERR_OOM, "scratch");
-        RC(launch_contact(s, contact_t, d->role_edge[0], p, step, CL));
+        RC(contact_apply(s, contact_t, d->role_edge[0], p, step, &contact_deaths));
     }
 
-    // ---- finish_write! for agents: births of the contact body, deaths
+    // ---- finish_write! for agents: deaths (the contact body did its own
+    // births and deaths above)
     DiedNow dn;
     memset(&dn, 0, sizeof dn);
     bool any_death = false;
+    if (contact_t > 0 && contact_deaths) {
+        any_death = true;
+        dn.d[contact_t] = s.a[contact_t].died_now.as<uint8_t>();
+    }
     if (deaths_possible) {
-        // counts: deaths per written mortal type (+ contact births)
         uint32_t* dpos[VH_MAXA + 1] = {nullptr};
-        uint32_t* bpos = nullptr;
         int slot = 0;
         int dslot[VH_MAXA + 1];
         for (int t = 1; t <= m.na; t++) {
             AgentStore& A = s.a[t];
             dslot[t] = -1;
-            if (!(A.writeable && A.mortal && A.nslots)) continue;
+            if (!(A.writeable && A.mortal && A.nslots) || t == contact_t) continue;
             uint32_t* d32 = (uint32_t*)s.scratch.take(A.nslots * 4 + 8);
             dpos[t] = (uint32_t*)s.scratch.take((A.nslots + 1) * 4);
             if (!d32 || !dpos[t]) return set_error(VH_ERR_OOM, "scratch");
@@ -1063,44 +1028,18 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
                                     hipMemcpyDeviceToDevice, st));
             dslot[t] = slot++;
         }
-        int bslot = -1;
-        if (contact_t > 0) {
-            AgentStore& A = s.a[contact_t];
-            uint32_t* b32 = (uint32_t*)s.scratch.take(A.nslots * 4 + 8);
-            bpos = (uint32_t*)s.scratch.take((A.nslots + 1) * 4);
-            if (!b32 || !bpos) return set_error(VH_ERR_OOM, "scratch");
-            hipLaunchKernelGGL(k_u8_to_u32, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st, CL.cnt_b,
-                               b32, A.nslots);
-            if (scan_excl_u32(b32, bpos, A.nslots, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
-            HIPCHECK(hipMemcpyAsync((uint32_t*)s.d_stats.p + 2 * slot, bpos + A.nslots, 4,
-                                    hipMemcpyDeviceToDevice, st));
-            bslot = slot++;
-        }
-        RC(readback(s, slot));
+        if (slot) RC(readback(s, slot));
         const uint32_t* h32 = (const uint32_t*)s.h_stats;
-        uint64_t nbc = bslot >= 0 ? h32[2 * bslot] : 0;
-        uint64_t ndeath[VH_MAXA + 1] = {0};
-        for (int t = 1; t <= m.na; t++) ndeath[t] = dslot[t] >= 0 ? h32[2 * dslot[t]] : 0;
-        if (contact_t > 0 && nbc) {
-            AgentStore& A = s.a[contact_t];
-            uint64_t consumed = std::min<uint64_t>(nbc, A.nreuse);
-            uint64_t fresh = nbc - consumed;
-            RC(ensure_slots(s, contact_t, A.nextid - 1 + fresh));
-            hipLaunchKernelGGL(k_contact_births, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st,
-                               CL.cnt_b, bpos, A.nslots, CL.birth_st, A.reuse.as<uint32_t>(), A.nreuse,
-                               A.nextid, A.wstate(), A.died.as<uint8_t>());
-            A.nreuse -= consumed;
-            A.nextid += fresh;
-        }
         for (int t = 1; t <= m.na; t++) {
             AgentStore& A = s.a[t];
-            if (!ndeath[t]) continue;
+            uint64_t nd = dslot[t] >= 0 ? h32[2 * dslot[t]] : 0;
+            if (!nd) continue;
             any_death = true;
-            RC(A.reuse.ensure((A.nreuse + ndeath[t]) * 4 + 8, st, A.nreuse * 4));
+            RC(A.reuse.ensure((A.nreuse + nd) * 4 + 8, st, A.nreuse * 4));
             hipLaunchKernelGGL(k_push_deaths, dim3(grid_for(A.nslots, 256)), dim3(256), 0, st,
                                A.died_now.as<uint8_t>(), dpos[t], A.nslots, A.reuse.as<uint32_t>(),
                                A.nreuse, A.died.as<uint8_t>());
-            A.nreuse += ndeath[t];
+            A.nreuse += nd;
             dn.d[t] = A.died_now.as<uint8_t>();
         }
     }
@@ -1110,42 +1049,7 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
         EdgeStore& E = s.e[et];
         if (!E.writeable) continue;
         bool keep_base = in_mask(d->add_existing_edges, et);
-        if (kind == K_CONTACT && et == d->role_edge[0]) {
-            // rows = caller slots; the sources of this step's deaths are dropped
-            // here, the rows of dead callers are empty by construction
-            AgentStore& A = s.a[contact_t];
-            uint64_t n = A.nslots;
-            vh_contact_params p;
-            memcpy(&p, d->params, sizeof p);
-            uint32_t* kept = (uint32_t*)s.scratch.take(n * 4 + 8);
-            uint32_t* off = (uint32_t*)s.scratch.take((n + 1) * 4 + 8);
-            if (!kept || !off) return set_error(VH_ERR_OOM, "scratch");
-            hipLaunchKernelGGL(k_contact_count, dim3(grid_for(n, 256)), dim3(256), 0, st, CL.cnt_e,
-                               CL.log_src, p.n_contacts, n, A.died_now.as<uint8_t>(), kept);
-            if (scan_excl_u32(kept, off, n, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
-            HIPCHECK(hipMemcpyAsync(s.d_stats.p, off + n, 4, hipMemcpyDeviceToDevice, st));
-            RC(readback(s, 1));
-            uint64_t ne = (uint32_t)s.h_stats[0];
-            uint64_t rows = std::max(A.nslots, A.nextid - 1);
-            RC(E.off.ensure((rows + 1) * 4, st));
-            RC(E.src.ensure(ne * 4 + 8, st));
-            HIPCHECK(hipMemcpyAsync(E.off.p, off, (n + 1) * 4, hipMemcpyDeviceToDevice, st));
-            if (rows > n)
-                hipLaunchKernelGGL(k_extend_offsets, dim3(grid_for(rows - n, 256)), dim3(256), 0, st,
-                                   E.off.as<uint32_t>(), n, rows);
-            if (n * p.n_contacts)
-                hipLaunchKernelGGL(k_contact_write, dim3(grid_for(n * p.n_contacts, 256)), dim3(256), 0,
-                                   st, CL.cnt_e, CL.log_src, p.n_contacts, n, A.died_now.as<uint8_t>(),
-                                   E.off.as<uint32_t>(), E.src.as<uint32_t>());
-            for (int t = 0; t <= VH_MAXA; t++) { E.rowbase[t] = 0; E.nrows[t] = 0; }
-            E.nrows[contact_t] = rows;
-            E.rows = rows;
-            E.nedges = ne;
-            E.src64 = false;
-            E.src_type = (uint32_t)contact_t;
-            E.version++;
-            continue;
-        }
+        if (kind == K_CONTACT && et == d->role_edge[0]) continue;   // rebuilt by contact_apply
         const uint64_t* lto = nullptr;
         const uint64_t* lfrom = nullptr;
         const uint8_t* lst = nullptr;

@@ -75,14 +75,26 @@ def test_opinion_parity(vh, ora, gpu_ready, f32):
         assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
 
 
-def test_contact_parity(vh, ora, gpu_ready):
+# (contacts, redraws, beta, p_death): the default model; more contacts than
+# one LDS row holds (the unstaged path); no contacts; a single draw per
+# contact under heavy deaths (draws exhausted on dead partners); certain
+# transmission (every infected neighbour is looked up)
+CONTACT_CASES = [(20, 16, 0.05, 0.05), (40, 4, 0.05, 0.05), (0, 16, 0.05, 0.05), (5, 1, 0.05, 0.3),
+                 (20, 16, 1.0, 0.02)]
+
+
+@pytest.mark.parametrize("case", CONTACT_CASES, ids=lambda c: "C%d_R%d_b%g_d%g" % c)
+def test_contact_parity(vh, ora, gpu_ready, case):
     M = vh.models
+    C, R, beta, pd = case
     n = 3000
+    print(f"[gpu] contact parity {case}", flush=True)
     g = M.build_contact(vh, n, seed=5, n_init=30)
     c = M.build_contact(ora, n, seed=5, n_init=30)
+    kw = dict(seed=5, beta=beta, p_death=pd, n_contacts=C, max_redraw=R)
     for step in range(10):
-        M.step_contact(vh, g, seed=5, p_death=0.05)
-        M.step_contact(ora, c, seed=5, p_death=0.05)
+        M.step_contact(vh, g, **kw)
+        M.step_contact(ora, c, **kw)
         a, da = _agents(g)
         b, db = _agents(c)
         assert len(a) == len(b), f"step {step}: slots {len(a)} vs {len(b)}"

@@ -541,12 +541,11 @@ __global__ __launch_bounds__(256) void k_contact_flags(const uint64_t* __restric
     }
 }
 
-// Health update, death flag, partner draws. Rows of the previous contact
-// CSR are staged through LDS when the block's range fits (coalesced reads);
-// the kept partners of every slot (accepted draws whose partner does not die
-// in this step, in draw order) go to a fixed-stride log, row i at
-// log[i * n_contacts], written through LDS as one contiguous block range.
-// Dynamic LDS: 256 * n_contacts words when `staged`.
+// Health update, death flag, partner draws. The kept partners of every
+// slot (accepted draws whose partner does not die in this step, in draw
+// order) go to a fixed-stride log, row i at log[i * n_contacts], written
+// through LDS as one contiguous block range. Dynamic LDS: 256 * n_contacts
+// words when `staged`.
 __global__ __launch_bounds__(256) void k_contact_main(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, const uint64_t* __restrict__ rs,
     uint64_t* __restrict__ ws, uint8_t* __restrict__ died_now, uint64_t n, uint64_t nrows,
@@ -554,28 +553,10 @@ __global__ __launch_bounds__(256) void k_contact_main(
     uint32_t* __restrict__ kept, uint32_t* __restrict__ log, uint32_t staged) {
     extern __shared__ u32x4 lds_v[];   // 16-byte aligned
     uint32_t* lds = (uint32_t*)lds_v;
-    __shared__ uint32_t s_b, s_e;
     const uint32_t C = p.n_contacts;
     const uint32_t tid = threadIdx.x;
     const uint64_t i0 = (uint64_t)blockIdx.x * 256;
     const uint64_t i = i0 + tid;
-    const uint64_t cap = staged ? 256ull * C : 0;
-    // the block's in-edge range
-    if (tid == 0) {
-        uint32_t b = 0, e = 0;
-        if (off && i0 < nrows) {
-            b = off[rowbase + i0];
-            e = off[rowbase + (i0 + 256 < nrows ? i0 + 256 : nrows)];
-        }
-        s_b = b;
-        s_e = e;
-    }
-    __syncthreads();
-    const uint32_t rb = s_b;
-    const bool edges_in_lds = (uint64_t)(s_e - rb) <= cap;
-    if (edges_in_lds)
-        for (uint32_t q = tid; q < s_e - rb; q += 256) lds[q] = __builtin_nontemporal_load(src + rb + q);
-    __syncthreads();
 
     bool live = i < n && !cbit(B.dead, i);
     bool survive = false;
@@ -595,8 +576,7 @@ __global__ __launch_bounds__(256) void k_contact_main(
                 // source's flag only for the few draws below pinf
                 for (uint32_t k = b; k < e; k++) {
                     if (vh_u01(vh_hash_fin(pre, k - b + 1)) >= pinf) continue;
-                    uint32_t j = edges_in_lds ? lds[k - rb] : src[k];
-                    if (cbit(B.inf, j - 1)) {
+                    if (cbit(B.inf, src[k] - 1)) {
                         h = VH_I;
                         break;
                     }
@@ -612,7 +592,6 @@ __global__ __launch_bounds__(256) void k_contact_main(
             __builtin_nontemporal_store((s & ~(uint64_t)0xFFu) | h, ws + i);
         }
     }
-    __syncthreads();   // LDS now holds the log rows of this block
     if (survive) {
         const uint64_t pre = vh_hash_pre(p.seed, vh_make_id(t, i + 1), step);
         uint32_t* row = staged ? lds + (uint64_t)tid * C : log + i * C;

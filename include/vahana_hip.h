@@ -189,6 +189,18 @@ int vh_add_agents(vh_sim_t s, uint8_t type_id, const void* states, uint64_t n,
  * stateless types) */
 int vh_add_edges(vh_sim_t s, uint8_t edge_type_id, const uint64_t* from,
                  const uint64_t* to, const void* states, uint64_t n);
+
+/* Bulk init-phase edges (no single reference counterpart: the edges that
+ * add_edge!, src/EdgeMethods.jl:388-523, would add one by one for a directed
+ * G(n, p) graph, target by target). Edge type `et` of a model with one agent
+ * type gets, at finish_init!, a Binomial(n - 1, p) number of in-edges per
+ * agent t (p = mean_deg / (n - 1), n = agents added) from distinct uniform
+ * sources != t, in draw order; a Float32 edge state is the weight
+ * (2 (h >> 41) + 1) 2^-24, h = vh_hash4(weight_seed, k, 0, 6 << 40) for
+ * the k-th edge (0-based, target order). vh_graph_gnp and vh_fill_open01f
+ * give the same edges on the host. No other edges may be added to `et`. */
+int vh_add_edges_gnp(vh_sim_t sim, uint8_t et, double mean_deg, uint64_t graph_seed,
+                     uint64_t weight_seed);
 /* finish_init! src/Simulation.jl:403-476: upload and build the CSR */
 int vh_finish_init(vh_sim_t s);
 /* apply! src/Simulation.jl:720-821, enqueued on the simulation's stream */
@@ -199,6 +211,15 @@ int vh_apply_n(vh_sim_t s, const vh_apply_desc* desc, uint32_t steps);
 int vh_sync(vh_sim_t s);
 /* attach an external hipStream_t (NULL: the library's own stream) */
 int vh_set_stream(vh_sim_t s, void* hip_stream);
+/* a non-blocking hipStream_t at default priority for vh_set_stream,
+ * vh_set_io_stream and vh_ensemble_set_streams (no reference counterpart) */
+int vh_stream_create(void** out);
+int vh_stream_destroy(void* hip_stream);
+/* stream for vh_read_agents (no reference counterpart): the copies wait for
+ * the simulation's last apply! only, so a caller that runs many simulations
+ * on one shared compute stream reads finished ones while the others run;
+ * NULL: reads use the compute stream */
+int vh_set_io_stream(vh_sim_t s, void* hip_stream);
 /* sim.num_transitions (src/Simulation.jl:473,816) */
 int vh_num_transitions(vh_sim_t s, uint64_t* out);
 /* mapreduce(sim, f, op, T; init, datatype) — kind 0: agents, 1: edges */
@@ -251,16 +272,26 @@ typedef struct vh_sir_member {
 } vh_sir_member;
 int vh_ensemble_create(vh_ensemble_t* out);
 int vh_ensemble_destroy(vh_ensemble_t e);
+/* builds the member's graph and state on a second stream of the ensemble,
+ * so it may run while launched steps of the other members are in flight */
 int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* m, uint32_t* member_id);
+/* waits for the member's last launched step only */
 int vh_ensemble_remove(vh_ensemble_t e, uint32_t member_id);
 /* the member applies steps up to last_step and then stays (0: no limit) */
 int vh_ensemble_set_horizon(vh_ensemble_t e, uint32_t member_id, uint64_t last_step);
 /* block size (128..1024) and XCD-aware block order of the step launch */
 int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t xcd_remap);
-/* advance every member; use_graph: capture the steps into one hipGraph */
+/* advance every member; use_graph: capture the steps into one hipGraph.
+ * Asynchronous: returns once the launches are enqueued (the call first waits
+ * for this ensemble's previous launches, not for other work on its stream) */
 int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph);
 int vh_ensemble_sync(vh_ensemble_t e);
-/* member state (4-byte SIR records) and its next step number */
+/* run the step launches on `step_stream` and member builds and reads on
+ * `aux_stream` (both caller-owned hipStream_t; both NULL: the ensemble's own) */
+int vh_ensemble_set_streams(vh_ensemble_t e, void* step_stream, void* aux_stream);
+/* member state (4-byte SIR records) and its next step number; waits for the
+ * member's last launched step only (a member past its horizon is read while
+ * later steps of the others run) */
 int vh_ensemble_read(vh_ensemble_t e, uint32_t member_id, void* states, uint64_t cap, uint64_t* n,
                      uint64_t* step);
 int vh_ensemble_info(vh_ensemble_t e, uint64_t* members, uint64_t* agents, uint64_t* edges,
@@ -291,10 +322,18 @@ int vh_graph_ba(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b,
  * h = vh_hash4(seed, i, b, c) */
 int vh_fill_open01f(uint64_t seed, uint64_t b, uint64_t c, uint64_t n, float* out);
 
+/* Float64 in [0, 1) per agent: out[i] = (h >> 11) 2^-53 with
+ * h = vh_hash4(seed, AgentID(type, i + 1), b, c) (the opinion and other
+ * per-agent initial values) */
+int vh_fill_agent_u01(uint64_t seed, uint32_t type, uint64_t b, uint64_t c, uint64_t n, double* out);
+
 /* ------------------------------------------------------ diagnostics
  * Memory-system probe (no reference counterpart): mode 0 times n random
  * 8-byte gathers over `footprint` bytes, mode 1 a streaming read of
- * `footprint` bytes; *ms is the mean launch time over `reps` launches. */
+ * `footprint` bytes, modes 2/3 gathers with a 2 MiB hot prefix (normal or
+ * non-temporal cold loads), modes 4/5 mode 0 over uncached / fine-grained
+ * device memory (allocated once, kept until the process ends); *ms is the
+ * mean launch time over `reps` launches. */
 int vh_probe_memory(int mode, uint64_t footprint, uint64_t n, uint32_t reps, float* ms);
 
 /* ------------------------------------------------------ counter-hash RNG

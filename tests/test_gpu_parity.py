@@ -120,3 +120,57 @@ def test_mapreduce_tree_parity(vh, ora, gpu_ready):
         assert x == y or (np.isnan(x) and np.isnan(y)), op
     xw = vh.mapreduce(g.sim, "w", "+", g.E)
     assert xw == ora.mapreduce(c.sim, "w", "+", c.E)
+
+
+def test_opinion_device_gnp_graph(vh, ora, gpu_ready):
+    """vh_add_edges_gnp builds on the device the CSR that its host twin
+    (graph_gnp + gnp_weights, added edge by edge) gives; the model then
+    steps bit-identically, through apply_n as through apply!, and equals the
+    checker."""
+    M = vh.models
+    n = 5000
+    print("[gpu] opinion on device G(n, p)", flush=True)
+    d = M.build_opinion_gnp(vh, n, mean_deg=10.0, seed=3, graph_seed=4, weight_seed=5)
+    h = M.build_opinion_gnp(vh, n, mean_deg=10.0, seed=3, graph_seed=4, weight_seed=5, device_graph=False)
+    c = M.build_opinion_gnp(ora, n, mean_deg=10.0, seed=3, graph_seed=4, weight_seed=5, device_graph=False)
+    td, fd, wd = d.sim._read_edges(d.E)
+    th, fh, wh = h.sim._read_edges(h.E)
+    assert len(td) > 9 * n
+    assert np.array_equal(td, th) and np.array_equal(fd, fh)
+    assert np.array_equal(_bytes(wd), _bytes(wh))
+    for step in range(6):
+        M.step_opinion_n(vh, d, 1, eps=0.25)
+        M.step_opinion(vh, h, eps=0.25)
+        M.step_opinion(ora, c, eps=0.25)
+        x, _ = _agents(d)
+        y, _ = _agents(h)
+        z, _ = _agents(c)
+        assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}: device graph vs host twin"
+        assert np.array_equal(_bytes(x), _bytes(z)), f"step {step}: vs checker"
+    M.step_opinion_n(vh, d, 5, eps=0.25)
+    for _ in range(5):
+        M.step_opinion(vh, h, eps=0.25)
+    assert np.array_equal(_bytes(_agents(d)[0]), _bytes(_agents(h)[0])), "apply_n(5)"
+
+
+def test_device_gnp_argument_errors(vh, gpu_ready):
+    M = vh.models
+    d = M.build_opinion_gnp(vh, 100, mean_deg=4.0)
+    with pytest.raises(AssertionError):
+        d.sim._add_edges_gnp(d.E, 4.0, 1, 2)          # after finish_init!
+    _, _, HK, _, Knows, Person, _ = M._types(vh)
+    model = (vh.ModelTypes() | vh.register_agenttype_(HK, "Immortal")
+             | vh.register_agenttype_(Person) | vh.register_edgetype_(Knows, target=HK)
+             | vh.create_model("two types"))
+    sim = vh.create_simulation(model)
+    with pytest.raises(vh.VahanaError):
+        sim._add_edges_gnp(Knows, 4.0, 1, 2)           # more than one agent type
+    model = (vh.ModelTypes() | vh.register_agenttype_(HK, "Immortal")
+             | vh.register_edgetype_(Knows, target=HK) | vh.create_model("twice"))
+    sim = vh.create_simulation(model)
+    sim._add_edges_gnp(Knows, 4.0, 1, 2)
+    with pytest.raises(vh.VahanaError):
+        sim._add_edges_gnp(Knows, 4.0, 1, 2)           # generated twice
+    with pytest.raises(vh.VahanaError):
+        sim._add_edges_array(Knows, np.array([1 << 56 | 1], dtype=np.uint64),
+                             np.array([1 << 56 | 1], dtype=np.uint64))

@@ -348,6 +348,37 @@ class Simulation:
 
     # -- apply!
     def _apply(self, func, call, read, write, add_existing=(), with_edge=None):
+        d, keep = self._desc(func, call, read, write, add_existing, with_edge)
+        self._last_desc = (d, keep)
+        _check(self._call_apply(d), self.lib)
+        return self
+
+    def _prepare(self, func, call, read, write, add_existing=(), with_edge=None):
+        """The apply! descriptor of one transition, built once for repeated
+        steps with `_apply_prepared` (same meaning as `_apply`)."""
+        return self._desc(func, call, read, write, add_existing, with_edge)
+
+    def _apply_prepared(self, prepared, steps=1):
+        """`steps` apply! calls of a prepared transition, enqueued in one ABI
+        call (vh_apply_n); asynchronous like apply!."""
+        d, _keep = prepared
+        _check(self.lib.apply_n(self.handle, C.byref(d), steps), self.lib)
+        return self
+
+    def _set_streams(self, stream, io_stream=None):
+        """Run apply! on `stream` (a raw hipStream_t, None: own stream) and
+        reads on `io_stream` (vh_set_io_stream)."""
+        _check(self.lib.set_stream(self.handle, stream), self.lib)
+        _check(self.lib.set_io_stream(self.handle, io_stream), self.lib)
+
+    def _add_edges_gnp(self, T, mean_deg, graph_seed, weight_seed=0):
+        """Directed G(n, p) in-edges of edge type T generated on the device at
+        finish_init! (vh_add_edges_gnp); the host twin is inputs.graph_gnp with
+        weights inputs.gnp_weights."""
+        _check(self.lib.add_edges_gnp(self.handle, self._eid(T), float(mean_deg), int(graph_seed),
+                                      int(weight_seed)), self.lib)
+
+    def _desc(self, func, call, read, write, add_existing=(), with_edge=None):
         tr = _resolve(func)
         call, read, write = _as_list(call), _as_list(read), _as_list(write)
         add_existing = _as_list(add_existing)
@@ -382,9 +413,7 @@ class Simulation:
         keep = C.create_string_buffer(pbuf, max(len(pbuf), 1))
         d.params = C.cast(keep, C.c_void_p)
         d.params_bytes = len(pbuf)
-        self._last_desc = (d, keep)
-        _check(self._call_apply(d), self.lib)
-        return self
+        return d, keep
 
     def _call_apply(self, d):
         return self.lib.apply(self.handle, C.byref(d))

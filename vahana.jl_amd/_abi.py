@@ -120,8 +120,12 @@ SIGNATURES = [
 ]
 DEVICE_ONLY = [
     ("apply_n", C.c_int, [_vp, C.POINTER(ApplyDesc), C.c_uint32]),
+    ("add_edges_gnp", C.c_int, [_vp, C.c_uint8, C.c_double, C.c_uint64, C.c_uint64]),
     ("sync", C.c_int, [_vp]),
     ("set_stream", C.c_int, [_vp, _vp]),
+    ("set_io_stream", C.c_int, [_vp, _vp]),
+    ("stream_create", C.c_int, [C.POINTER(_vp)]),
+    ("stream_destroy", C.c_int, [_vp]),
     ("device_bytes", C.c_int, [_vp, _u64p]),
     ("enable_asserts", C.c_int, [C.c_int]),
     ("graph_er", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
@@ -129,6 +133,8 @@ DEVICE_ONLY = [
     ("hash4", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
     ("fill_open01f", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                C.POINTER(C.c_float)]),
+    ("fill_agent_u01", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64,
+                                 C.POINTER(C.c_double)]),
     ("probe_memory", C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32,
                                C.POINTER(C.c_float)]),
     ("graph_gnp", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
@@ -140,6 +146,7 @@ DEVICE_ONLY = [
     ("ensemble_set_config", C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     ("ensemble_step", C.c_int, [_vp, C.c_uint32, C.c_int]),
     ("ensemble_sync", C.c_int, [_vp]),
+    ("ensemble_set_streams", C.c_int, [_vp, _vp, _vp]),
     ("ensemble_read", C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64, _u64p, _u64p]),
     ("ensemble_info", C.c_int, [_vp, _u64p, _u64p, _u64p, _u64p]),
     ("ensemble_autotune", C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_float)]),

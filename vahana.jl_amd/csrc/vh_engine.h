@@ -59,12 +59,18 @@ struct EdgeStore {
     DevBuf heavy;                  // uint32 rows with degree > 64
     uint64_t nheavy = 0, bins_version = ~0ull;
     bool writeable = false;
+    // in-edges requested as a device G(n, p) during init (vh_add_edges_gnp)
+    bool gen = false;
+    double gen_mean_deg = 0.0;
+    uint64_t gen_seed = 0, gen_wseed = 0;
 };
 
 struct Sim {
     Model* m = nullptr;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t io = nullptr;      // reads, when set (vh_set_io_stream)
+    hipEvent_t last = nullptr;     // end of the last apply! on `stream`
     AgentStore a[VH_MAXA + 1];
     EdgeStore e[VH_MAXE];
     bool initialized = false;
@@ -93,6 +99,12 @@ struct ContactBits {
 };
 // one apply! of the contact model incl. its births, deaths and contact CSR
 int contact_apply(Sim& s, int t, int et, const vh_contact_params& p, uint64_t step, uint64_t* ndeaths);
+// device G(n, p) CSR in target order (vh_ensemble.hip): off[n + 1], sources as
+// 1-based numbers; one host synchronisation
+int gnp_device(uint64_t n, double mean_deg, uint64_t seed, DevBuf& off, DevBuf& src, DevBuf& cdf,
+               Scratch& sc, hipStream_t st, uint64_t* nedges);
+// Float32 weights in (0, 1) by CSR position (vh_ensemble.hip)
+void fill_weights(float* w, uint64_t n, uint64_t wseed, hipStream_t st);
 // engine services the bodies use
 int readback(Sim& s, int n);                           // d_stats -> h_stats, synchronous
 int ensure_slots(Sim& s, int t, uint64_t slots);

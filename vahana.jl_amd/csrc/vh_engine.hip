@@ -136,6 +136,12 @@ extern "C" int vh_sim_create(vh_model_t mh, vh_sim_t* out) {
         return hip_fail(e, "hipStreamCreate");
     }
     s.own_stream = true;
+    e = hipEventCreateWithFlags(&s.last, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s.stream);
+        delete h;
+        return hip_fail(e, "hipEventCreate");
+    }
     for (int t = 1; t <= s.m->na; t++) {
         s.a[t].bytes = s.m->a[t].bytes;
         s.a[t].mortal = !has(s.m->a[t].hints, VH_HINT_IMMORTAL);
@@ -150,6 +156,7 @@ extern "C" int vh_sim_create(vh_model_t mh, vh_sim_t* out) {
     }
     e = hipHostMalloc((void**)&s.h_stats, 64 * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
+        (void)hipEventDestroy(s.last);
         (void)hipStreamDestroy(s.stream);
         delete h;
         return hip_fail(e, "hipHostMalloc");
@@ -167,7 +174,10 @@ extern "C" int vh_sim_create(vh_model_t mh, vh_sim_t* out) {
 }
 
 static void sim_release(Sim& s) {
-    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    // on a caller's (possibly shared) stream only this simulation's work
+    // must be complete: its last apply!; other work is synchronous
+    if (s.stream && s.own_stream) (void)hipStreamSynchronize(s.stream);
+    else if (s.last) (void)hipEventSynchronize(s.last);
     for (int t = 0; t <= VH_MAXA; t++) {
         AgentStore& A = s.a[t];
         A.state[0].release(); A.state[1].release();
@@ -183,6 +193,8 @@ static void sim_release(Sim& s) {
     s.h_stats = nullptr;
     if (s.own_stream && s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
+    if (s.last) (void)hipEventDestroy(s.last);
+    s.last = nullptr;
 }
 
 // finish_simulation! src/Simulation.jl:551-585
@@ -204,6 +216,24 @@ extern "C" int vh_set_stream(vh_sim_t h, void* stream) {
         HIPCHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
         s.own_stream = true;
     }
+    return VH_OK;
+}
+extern "C" int vh_stream_create(void** out) {
+    if (!out) return set_error(VH_ERR_ARG, "null argument");
+    hipStream_t st = nullptr;
+    HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *out = st;
+    return VH_OK;
+}
+extern "C" int vh_stream_destroy(void* stream) {
+    if (!stream) return VH_OK;
+    HIPCHECK(hipStreamSynchronize((hipStream_t)stream));
+    HIPCHECK(hipStreamDestroy((hipStream_t)stream));
+    return VH_OK;
+}
+extern "C" int vh_set_io_stream(vh_sim_t h, void* stream) {
+    if (!h) return set_error(VH_ERR_ARG, "null simulation");
+    h->s.io = (hipStream_t)stream;
     return VH_OK;
 }
 
@@ -248,6 +278,7 @@ extern "C" int vh_add_edges(vh_sim_t h, uint8_t et, const uint64_t* from, const 
                          "You can call add_edge! only in the initialization phase (until "
                          "`finish_init!` is called) or within a transition function called by `apply`.");
     EdgeStore& E = s.e[et];
+    if (E.gen) return set_error(VH_ERR_ARG, "the in-edges of this edge type are device-generated");
     const bool ignorefrom = has(E.hints, VH_HINT_IGNOREFROM);
     const bool stateless = has(E.hints, VH_HINT_STATELESS);
     const bool singletype = has(E.hints, VH_HINT_SINGLETYPE);
@@ -632,6 +663,65 @@ int vh::ensure_slots(Sim& s, int t, uint64_t slots) {
     return VH_OK;
 }
 
+// Bulk init-phase edges: the G(n, p) graph that vh_graph_gnp + add_edge!
+// in target order would give, generated on the device at finish_init!.
+extern "C" int vh_add_edges_gnp(vh_sim_t h, uint8_t et, double mean_deg, uint64_t graph_seed,
+                                uint64_t weight_seed) {
+    if (!h) return set_error(VH_ERR_ARG, "null simulation");
+    Sim& s = h->s;
+    if (et >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT, "You can call add_edge! only in the initialization phase");
+    EdgeStore& E = s.e[et];
+    if (s.m->na != 1 || E.target != 1)
+        return set_error(VH_ERR_ARG, "device G(n, p): one agent type, edges targeting it");
+    if (has(E.hints, VH_HINT_IGNOREFROM) || has(E.hints, VH_HINT_SINGLEEDGE))
+        return set_error(VH_ERR_ARG, "device G(n, p): the edge type must keep sources and allow many edges");
+    if (E.bytes != 0 && E.bytes != 4)
+        return set_error(VH_ERR_ARG, "device G(n, p): edge state must be empty or one Float32 weight");
+    if (!E.h_to.empty() || E.gen) return set_error(VH_ERR_ARG, "device G(n, p): edges already added");
+    if (!(mean_deg >= 0.0)) return set_error(VH_ERR_ARG, "device G(n, p): mean degree");
+    E.gen = true;
+    E.gen_mean_deg = mean_deg;
+    E.gen_seed = graph_seed;
+    E.gen_wseed = weight_seed;
+    return VH_OK;
+}
+
+// the CSR of a device-generated edge type: rows = the agents of type 1
+static int build_gnp_csr(Sim& s, EdgeStore& E) {
+    hipStream_t st = s.stream;
+    RowLayout L;
+    uint64_t rows;
+    compute_layout(s, E, L, rows);
+    uint64_t ne = 0;
+    if (rows) {
+        DevBuf cdf;
+        int rc = gnp_device(rows, E.gen_mean_deg, E.gen_seed, E.off, E.src, cdf, s.scratch, st, &ne);
+        if (!rc && E.bytes) {
+            rc = E.st.ensure(ne * 4 + 8, st);
+            if (!rc) fill_weights(E.st.as<float>(), ne, E.gen_wseed, st);
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+        cdf.release();
+        RC(rc);
+    } else {
+        RC(E.off.ensure(8, st));
+        fill_u32(E.off.as<uint32_t>(), 0, 1, st);
+    }
+    for (int t = 0; t <= VH_MAXA; t++) {
+        E.rowbase[t] = L.base[t];
+        E.nrows[t] = L.nrows[t];
+    }
+    E.rows = rows;
+    E.nedges = ne;
+    E.src64 = false;
+    E.src_type = 1;
+    E.version++;
+    E.gen = false;
+    return VH_OK;
+}
+
 // finish_init! src/Simulation.jl:403-476 for one process: publish the
 // init-phase agents and build every edge type's CSR.
 extern "C" int vh_finish_init(vh_sim_t h) {
@@ -669,11 +759,21 @@ extern "C" int vh_finish_init(vh_sim_t h) {
             if (dst) HIPCHECK(hipMemcpyAsync(dst, E.h_st.data(), n * E.bytes, hipMemcpyHostToDevice, st));
         }
         if (s.m->na != 1) E.src64 = true;
-        RC(build_csr(s, E, false, dto, dfrom, dst, n));
+        if (E.gen)
+            RC(build_gnp_csr(s, E));
+        else
+            RC(build_csr(s, E, false, dto, dfrom, dst, n));
         HIPCHECK(hipStreamSynchronize(st));
         std::vector<uint64_t>().swap(E.h_to);
         std::vector<uint64_t>().swap(E.h_from);
         std::vector<uint8_t>().swap(E.h_st);
+    }
+    // degree bins of the neighbour-gathering bodies, so that their first
+    // apply! needs no host round trip (rebuilt when the edges change)
+    for (int et = 0; et < s.m->ne; et++) {
+        EdgeStore& E = s.e[et];
+        if (E.rows && E.target >= 1 && !E.src64 && !has(E.hints, VH_HINT_IGNOREFROM))
+            RC(build_heavy_bins(s, E, E.target));
     }
     s.scratch.reset(st);
     HIPCHECK(hipStreamSynchronize(st));
@@ -1080,9 +1180,13 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
 
 extern "C" int vh_apply(vh_sim_t h, const vh_apply_desc* d) {
     if (!h || !d) return set_error(VH_ERR_ARG, "null argument");
-    return apply_one(h->s, d);
+    RC(apply_one(h->s, d));
+    HIPCHECK(hipEventRecord(h->s.last, h->s.stream));
+    return VH_OK;
 }
 extern "C" int vh_apply_n(vh_sim_t h, const vh_apply_desc* d, uint32_t steps) {
+    if (!h || !d) return set_error(VH_ERR_ARG, "null argument");
     for (uint32_t i = 0; i < steps; i++) RC(apply_one(h->s, d));
+    HIPCHECK(hipEventRecord(h->s.last, h->s.stream));
     return VH_OK;
 }

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "vh_engine.h"
@@ -130,7 +131,57 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
 
 }  // namespace
 
+namespace {
+__global__ void k_fill_weights(float* __restrict__ w, uint64_t n, uint64_t wseed) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) w[k] = (float)((double)((vh_hash(wseed, k, 0, VH_KEY_WEIGHT) >> 41) * 2 + 1) * 0x1.0p-24);
+}
+}  // namespace
+
+namespace vh {
+// Directed G(n, p), p = mean_deg / (n - 1), as a CSR in target order: the
+// in-degree of target t is Binomial(n - 1, p) by inverse CDF, its sources are
+// distinct uniform draws != t in draw order (vh_graph_gnp is the host twin).
+// `cdf` and `sc` are working buffers; one host synchronisation (edge count).
+int gnp_device(uint64_t n, double mean_deg, uint64_t seed, DevBuf& off, DevBuf& src, DevBuf& cdf_buf,
+               Scratch& sc, hipStream_t st, uint64_t* nedges) {
+    if (n == 0 || n >= 0xFFFFFFFFull) return set_error(VH_ERR_ARG, "G(n, p): bad n");
+    double pe = n > 1 ? mean_deg / (double)(n - 1) : 0.0;
+    std::vector<double> cdf = binom_cdf(n, pe);
+    RC(cdf_buf.ensure(cdf.size() * 8, st));
+    RC(off.ensure((n + 1) * 4, st));
+    uint32_t* deg = (uint32_t*)sc.take(n * 4 + 8);
+    if (!deg) return set_error(VH_ERR_OOM, "scratch");
+    HIPCHECK(hipMemcpyAsync(cdf_buf.p, cdf.data(), cdf.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gnp_degree, dim3(grid_for(n, 256)), dim3(256), 0, st, cdf_buf.as<double>(),
+                       (uint32_t)cdf.size(), seed, n, deg);
+    if (scan_excl_u32(deg, off.as<uint32_t>(), n, sc, st)) return set_error(VH_ERR_DEVICE, "scan");
+    uint32_t ne = 0;
+    HIPCHECK(hipMemcpyAsync(&ne, off.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));   // also keeps `cdf` alive until the copy is done
+    RC(src.ensure((uint64_t)ne * 4 + 8, st));
+    hipLaunchKernelGGL(k_gnp_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, n, seed, off.as<uint32_t>(),
+                       src.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    *nedges = ne;
+    return VH_OK;
+}
+// w[k] = (2 (h >> 41) + 1) 2^-24, h = vh_hash(wseed, k, 0, VH_KEY_WEIGHT): the
+// values vh_fill_open01f(wseed, 0, VH_KEY_WEIGHT, n, w) gives on the host
+void fill_weights(float* w, uint64_t n, uint64_t wseed, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_weights, dim3(grid_for(n, 256)), dim3(256), 0, st, w, n, wseed);
+}
+}  // namespace vh
+
 // ====================================================================== host side
+// completion of one vh_ensemble_step call, shared by the members it advanced
+struct StepDone {
+    hipEvent_t ev = nullptr;
+    ~StepDone() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
 struct Member {
     uint32_t id;
     uint64_t n, nedges, step;
@@ -140,12 +191,23 @@ struct Member {
     uint32_t rec;
     DevBuf off, src, st[2];
     int cur = 0;
+    std::shared_ptr<StepDone> done;   // the last launch that wrote this member
 };
 
 struct vh_ensemble {
     std::map<uint32_t, Member*> members;
     uint32_t next_id = 1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the step launches
+    hipStream_t aux = nullptr;      // member builds and reads, beside running steps
+    bool own_streams = true;
+    hipGraphExec_t exec = nullptr;  // last graph launched, destroyed after a sync
+    std::shared_ptr<StepDone> last; // completion of the last step call
+    // pinned staging of the block table and the descriptor tables: the
+    // uploads stay asynchronous (reused after the next call's stream sync)
+    void* h_table = nullptr;
+    size_t h_table_cap = 0;
+    void* h_descs = nullptr;
+    size_t h_descs_cap = 0;
     uint32_t block = 256, remap = 1;
     DevBuf d_desc, d_bmember, d_bstart;
     std::vector<MemberDesc> h_desc;
@@ -156,6 +218,17 @@ struct vh_ensemble {
     Scratch scratch;
     DevBuf d_cdf;
 };
+
+static int pinned(void** p, size_t* cap, size_t bytes) {
+    if (bytes <= *cap) return VH_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t n = std::max(bytes, (size_t)4096);
+    HIPCHECK(hipHostMalloc(p, n, hipHostMallocDefault));
+    *cap = n;
+    return VH_OK;
+}
 
 static int build_table(vh_ensemble* e) {
     e->h_bmember.clear();
@@ -175,9 +248,14 @@ static int build_table(vh_ensemble* e) {
     RC(e->d_bstart.ensure(e->h_bstart.size() * 4 + 8, e->stream));
     RC(e->d_desc.ensure((e->members.size() + 1) * sizeof(MemberDesc), e->stream));
     if (e->nblocks) {
-        HIPCHECK(hipMemcpyAsync(e->d_bmember.p, e->h_bmember.data(), e->nblocks * 4, hipMemcpyHostToDevice, e->stream));
-        HIPCHECK(hipMemcpyAsync(e->d_bstart.p, e->h_bstart.data(), e->nblocks * 4, hipMemcpyHostToDevice, e->stream));
-        HIPCHECK(hipStreamSynchronize(e->stream));
+        // e->stream is idle here (vh_ensemble_step synchronised it)
+        RC(pinned(&e->h_table, &e->h_table_cap, (size_t)e->nblocks * 8));
+        uint32_t* ht = (uint32_t*)e->h_table;
+        memcpy(ht, e->h_bmember.data(), e->nblocks * 4);
+        memcpy(ht + e->nblocks, e->h_bstart.data(), e->nblocks * 4);
+        HIPCHECK(hipMemcpyAsync(e->d_bmember.p, ht, e->nblocks * 4, hipMemcpyHostToDevice, e->stream));
+        HIPCHECK(hipMemcpyAsync(e->d_bstart.p, ht + e->nblocks, e->nblocks * 4, hipMemcpyHostToDevice,
+                                e->stream));
     }
     e->table_valid = true;
     e->table_block = e->block;
@@ -208,6 +286,10 @@ static int upload_desc(vh_ensemble* e) {
 extern "C" int vh_ensemble_create(vh_ensemble_t* out) {
     vh_ensemble* e = new vh_ensemble();
     hipError_t r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (r == hipSuccess) {
+        r = hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking);
+        if (r != hipSuccess) (void)hipStreamDestroy(e->stream);
+    }
     if (r != hipSuccess) {
         delete e;
         return hip_fail(r, "hipStreamCreate");
@@ -219,6 +301,8 @@ extern "C" int vh_ensemble_create(vh_ensemble_t* out) {
 extern "C" int vh_ensemble_destroy(vh_ensemble_t e) {
     if (!e) return VH_OK;
     (void)hipStreamSynchronize(e->stream);
+    (void)hipStreamSynchronize(e->aux);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
     for (auto& kv : e->members) {
         Member* m = kv.second;
         m->off.release(); m->src.release(); m->st[0].release(); m->st[1].release();
@@ -226,7 +310,12 @@ extern "C" int vh_ensemble_destroy(vh_ensemble_t e) {
     }
     e->d_desc.release(); e->d_bmember.release(); e->d_bstart.release(); e->d_cdf.release();
     e->scratch.release();
-    (void)hipStreamDestroy(e->stream);
+    if (e->h_table) (void)hipHostFree(e->h_table);
+    if (e->h_descs) (void)hipHostFree(e->h_descs);
+    if (e->own_streams) {
+        (void)hipStreamDestroy(e->aux);
+        (void)hipStreamDestroy(e->stream);
+    }
     delete e;
     return VH_OK;
 }
@@ -242,28 +331,14 @@ extern "C" int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* sm, uin
     m->p = sm->p_inf;
     m->rec = sm->recovery;
     m->step = 1;
-    hipStream_t st = e->stream;
-    double pe = sm->n > 1 ? sm->mean_deg / (double)(sm->n - 1) : 0.0;
-    std::vector<double> cdf = binom_cdf(sm->n, pe);
-    int rc = e->d_cdf.ensure(cdf.size() * 8, st);
-    if (!rc) rc = m->off.ensure((sm->n + 1) * 4, st);
-    if (!rc) rc = m->st[0].ensure(sm->n * 4, st);
+    hipStream_t st = e->aux;   // builds run beside the step launches
+    int rc = m->st[0].ensure(sm->n * 4, st);
     if (!rc) rc = m->st[1].ensure(sm->n * 4, st);
-    if (rc) { delete m; return rc; }
+    uint64_t ne = 0;
     e->scratch.reset(st);
-    uint32_t* deg = (uint32_t*)e->scratch.take(sm->n * 4 + 8);
-    if (!deg) { delete m; return set_error(VH_ERR_OOM, "scratch"); }
-    HIPCHECK(hipMemcpyAsync(e->d_cdf.p, cdf.data(), cdf.size() * 8, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gnp_degree, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, e->d_cdf.as<double>(),
-                       (uint32_t)cdf.size(), sm->graph_seed, sm->n, deg);
-    if (scan_excl_u32(deg, m->off.as<uint32_t>(), sm->n, e->scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
-    uint32_t ne = 0;
-    HIPCHECK(hipMemcpyAsync(&ne, m->off.as<uint32_t>() + sm->n, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    if (!rc) rc = gnp_device(sm->n, sm->mean_deg, sm->graph_seed, m->off, m->src, e->d_cdf, e->scratch, st, &ne);
+    if (rc) { delete m; return rc; }
     m->nedges = ne;
-    RC(m->src.ensure((uint64_t)ne * 4 + 8, st));
-    hipLaunchKernelGGL(k_gnp_fill, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, sm->n, sm->graph_seed,
-                       m->off.as<uint32_t>(), m->src.as<uint32_t>());
     double frac = (double)sm->n_init / (double)sm->n;
     hipLaunchKernelGGL(k_sir_init, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, m->st[0].as<uint32_t>(),
                        sm->n, sm->seed, frac, 1u);
@@ -278,8 +353,10 @@ extern "C" int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* sm, uin
 extern "C" int vh_ensemble_remove(vh_ensemble_t e, uint32_t id) {
     auto it = e->members.find(id);
     if (it == e->members.end()) return set_error(VH_ERR_ARG, "unknown member %u", id);
-    HIPCHECK(hipStreamSynchronize(e->stream));
     Member* m = it->second;
+    // launches after the member's last step only read its (zero) size from
+    // their descriptor tables, so its buffers are free once that step is done
+    if (m->done) HIPCHECK(hipEventSynchronize(m->done->ev));
     m->off.release(); m->src.release(); m->st[0].release(); m->st[1].release();
     delete m;
     e->members.erase(it);
@@ -307,10 +384,20 @@ extern "C" int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t 
 // once into a hipGraph and launched as one unit
 extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) {
     if (e->members.empty() || !steps) return VH_OK;
+    // the previous call's launches read d_desc and the staging buffers, and
+    // its graph must outlive them: wait for them (only them: the stream may
+    // be shared with other work)
+    if (e->last) HIPCHECK(hipEventSynchronize(e->last->ev));
+    if (e->exec) {
+        (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
     if (!e->table_valid || e->table_block != e->block) RC(build_table(e));
     size_t nm = e->members.size();
     // one descriptor table per step (buffers alternate, step numbers grow)
     RC(e->d_desc.ensure((size_t)steps * nm * sizeof(MemberDesc), e->stream));
+    auto done = std::make_shared<StepDone>();
+    HIPCHECK(hipEventCreateWithFlags(&done->ev, hipEventDisableTiming));
     std::vector<MemberDesc> all;
     all.reserve((size_t)steps * nm);
     for (uint32_t s = 0; s < steps; s++) {
@@ -323,42 +410,74 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
             } else {
                 m->cur ^= 1;
                 m->step++;
+                m->done = done;
             }
             k++;
         }
         all.insert(all.end(), e->h_desc.begin(), e->h_desc.end());
     }
-    HIPCHECK(hipMemcpyAsync(e->d_desc.p, all.data(), all.size() * sizeof(MemberDesc), hipMemcpyHostToDevice,
+    RC(pinned(&e->h_descs, &e->h_descs_cap, all.size() * sizeof(MemberDesc)));
+    memcpy(e->h_descs, all.data(), all.size() * sizeof(MemberDesc));
+    HIPCHECK(hipMemcpyAsync(e->d_desc.p, e->h_descs, all.size() * sizeof(MemberDesc), hipMemcpyHostToDevice,
                             e->stream));
-    HIPCHECK(hipStreamSynchronize(e->stream));   // `all` is pageable
     // with the XCD-aware order the grid is padded to a multiple of 8 so that
     // every (xcd, slot) pair, and with it every logical block, exists
     uint32_t grid = e->remap ? 8u * ((e->nblocks + 7u) / 8u) : e->nblocks;
     dim3 g(grid), b(e->block);
     if (use_graph && steps > 1) {
         hipGraph_t graph;
-        hipGraphExec_t exec;
         HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         for (uint32_t s = 0; s < steps; s++)
             hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, e->d_desc.as<MemberDesc>() + (size_t)s * nm,
                                e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(), e->nblocks, e->remap);
         HIPCHECK(hipStreamEndCapture(e->stream, &graph));
-        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        HIPCHECK(hipGraphLaunch(exec, e->stream));
-        HIPCHECK(hipStreamSynchronize(e->stream));
-        (void)hipGraphExecDestroy(exec);
+        hipError_t r = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
+        if (r != hipSuccess) return hip_fail(r, "hipGraphInstantiate");
+        HIPCHECK(hipGraphLaunch(e->exec, e->stream));   // kept until the next sync
     } else {
         for (uint32_t s = 0; s < steps; s++)
             hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, e->d_desc.as<MemberDesc>() + (size_t)s * nm,
                                e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(), e->nblocks, e->remap);
         HIPCHECK(hipGetLastError());
     }
+    HIPCHECK(hipEventRecord(done->ev, e->stream));
+    e->last = done;
+    return VH_OK;
+}
+
+extern "C" int vh_ensemble_set_streams(vh_ensemble_t e, void* step_stream, void* aux_stream) {
+    if (!e) return set_error(VH_ERR_ARG, "null ensemble");
+    if ((step_stream == nullptr) != (aux_stream == nullptr))
+        return set_error(VH_ERR_ARG, "set both streams or neither");
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    HIPCHECK(hipStreamSynchronize(e->aux));
+    if (e->exec) {
+        (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
+    if (e->own_streams) {
+        (void)hipStreamDestroy(e->aux);
+        (void)hipStreamDestroy(e->stream);
+    }
+    if (step_stream) {
+        e->stream = (hipStream_t)step_stream;
+        e->aux = (hipStream_t)aux_stream;
+        e->own_streams = false;
+    } else {
+        HIPCHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        HIPCHECK(hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking));
+        e->own_streams = true;
+    }
     return VH_OK;
 }
 
 extern "C" int vh_ensemble_sync(vh_ensemble_t e) {
     HIPCHECK(hipStreamSynchronize(e->stream));
+    if (e->exec) {
+        (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
     return VH_OK;
 }
 
@@ -370,8 +489,11 @@ extern "C" int vh_ensemble_read(vh_ensemble_t e, uint32_t id, void* states, uint
     *n = m->n;
     if (step) *step = m->step;
     if (cap < m->n) return set_error(VH_ERR_CAPACITY, "capacity");
-    HIPCHECK(hipStreamSynchronize(e->stream));
-    HIPCHECK(hipMemcpy(states, m->st[m->cur].p, m->n * 4, hipMemcpyDeviceToHost));
+    // only the launches that wrote this member must be complete: a member
+    // past its horizon is read while later steps of the others run
+    if (m->done) HIPCHECK(hipEventSynchronize(m->done->ev));
+    HIPCHECK(hipMemcpyAsync(states, m->st[m->cur].p, m->n * 4, hipMemcpyDeviceToHost, e->aux));
+    HIPCHECK(hipStreamSynchronize(e->aux));
     return VH_OK;
 }
 

@@ -40,6 +40,13 @@ extern "C" int vh_fill_open01f(uint64_t seed, uint64_t b, uint64_t c, uint64_t n
     return VH_OK;
 }
 
+extern "C" int vh_fill_agent_u01(uint64_t seed, uint32_t type, uint64_t b, uint64_t c, uint64_t n,
+                                 double* out) {
+    const uint64_t tid = (uint64_t)type << 56;
+    for (uint64_t i = 0; i < n; i++) out[i] = (double)(hash4(seed, tid | (i + 1), b, c) >> 11) * 0x1.0p-53;
+    return VH_OK;
+}
+
 // Directed G(n, p) by geometric skipping over the n-1 candidate sources of
 // each target (Batagelj & Brandes 2005), targets ascending.
 extern "C" int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from, uint32_t* to,

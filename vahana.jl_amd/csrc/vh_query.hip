@@ -324,13 +324,20 @@ extern "C" int vh_read_agents(vh_sim_t h, uint8_t t, void* states, uint8_t* died
         if (died) memset(died, 0, n);
         return VH_OK;
     }
+    // with an I/O stream the copies wait for this simulation's last apply!
+    // only, not for other work queued on a shared compute stream
+    hipStream_t q = s.stream;
+    if (s.io) {
+        HIPCHECK(hipEventSynchronize(s.last));
+        q = s.io;
+    }
     if (states && A.bytes && n)
-        HIPCHECK(hipMemcpyAsync(states, A.rstate(), n * A.bytes, hipMemcpyDeviceToHost, s.stream));
+        HIPCHECK(hipMemcpyAsync(states, A.rstate(), n * A.bytes, hipMemcpyDeviceToHost, q));
     if (died) {
-        if (A.mortal && n) HIPCHECK(hipMemcpyAsync(died, A.died.p, n, hipMemcpyDeviceToHost, s.stream));
+        if (A.mortal && n) HIPCHECK(hipMemcpyAsync(died, A.died.p, n, hipMemcpyDeviceToHost, q));
         else memset(died, 0, n);
     }
-    HIPCHECK(hipStreamSynchronize(s.stream));
+    HIPCHECK(hipStreamSynchronize(q));
     return VH_OK;
 }
 

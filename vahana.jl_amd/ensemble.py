@@ -48,6 +48,11 @@ class Ensemble:
     def sync(self):
         self._check(self.lib.ensemble_sync(self.handle))
 
+    def set_streams(self, step_stream, aux_stream):
+        """Run step launches on `step_stream` and member builds / reads on
+        `aux_stream` (raw hipStream_t handles; both None: own streams)."""
+        self._check(self.lib.ensemble_set_streams(self.handle, step_stream, aux_stream))
+
     def autotune(self, trial_steps=3):
         b, r, ms = C.c_uint32(), C.c_uint32(), C.c_float()
         self._check(self.lib.ensemble_autotune(self.handle, trial_steps, C.byref(b), C.byref(r),

@@ -94,6 +94,14 @@ def graph_ba(n, m, seed):
     return a[:k], b[:k]
 
 
+def gnp_weights(n_edges, wseed):
+    """Float32 weights in (0, 1) by edge position (target order), the values
+    the device writes for vh_add_edges_gnp."""
+    w = np.empty(max(n_edges, 1), dtype=np.float32)
+    _abi.lib().fill_open01f(wseed, 0, KEY_WEIGHT, n_edges, w.ctypes.data_as(C.POINTER(C.c_float)))
+    return w[:n_edges]
+
+
 def symmetric_weighted(a, b, wseed):
     """Both directions of every undirected pair (u->v then v->u, in pair order)
     with a Float32 weight in (0, 1) per directed edge."""
@@ -133,8 +141,10 @@ def sir_init_frac(n, seed, n_init, type_id=1):
 
 
 def opinion_init(n, seed, f32=False, type_id=1):
-    ids = (np.uint64(type_id) << np.uint64(56)) | np.arange(1, n + 1, dtype=np.uint64)
-    x = u01(hash4(seed, ids, 0, KEY_OPINION))
+    """u01(hash4(seed, id, 0, KEY_OPINION)) per agent (filled in C)."""
+    x = np.empty(max(n, 1), dtype=np.float64)
+    _abi.lib().fill_agent_u01(seed, type_id, 0, KEY_OPINION, n, x.ctypes.data_as(C.POINTER(C.c_double)))
+    x = x[:n]
     return x.astype(np.float32) if f32 else x
 
 

@@ -83,6 +83,46 @@ def step_opinion(V, m, eps=0.2):
     V.apply_(m.sim, V.transitions.opinion(m.E, eps, m.f32), m.T, [m.T, m.E], m.T)
 
 
+def build_opinion_gnp(V, n, mean_deg=10.0, seed=1, graph_seed=1, weight_seed=2, f32=False,
+                      device_graph=True, stream=None):
+    """Opinion dynamics on a directed G(n, p) with Float32 weights. With
+    `device_graph` the graph is generated on the device at finish_init!
+    (vh_add_edges_gnp); otherwise its host twin is added edge by edge (the
+    checker's path), giving the same CSR. `stream`: a raw hipStream_t the
+    simulation is built on (and stays on)."""
+    _, _, HK, HK32, Knows, *_ = _types(V)
+    A = HK32 if f32 else HK
+    model = (V.ModelTypes()
+             | V.register_agenttype_(A, "Immortal")
+             | V.register_edgetype_(Knows, target=A)
+             | V.create_model("opinion"))
+    sim = V.create_simulation(model)
+    if stream is not None:
+        sim._set_streams(stream, stream)
+    st = np.zeros(n, dtype=A.dtype)
+    st["opinion"] = inputs.opinion_init(n, seed, f32)
+    ids = sim._add_agents_array(A, st)
+    if device_graph:
+        sim._add_edges_gnp(Knows, mean_deg, graph_seed, weight_seed)
+    else:
+        fr, to = inputs.graph_gnp(n, mean_deg, graph_seed)
+        tid = np.uint64(1) << np.uint64(56)
+        ws = np.zeros(len(to), dtype=Knows.dtype)
+        ws["w"] = inputs.gnp_weights(len(to), weight_seed)
+        sim._add_edges_array(Knows, tid | fr.astype(np.uint64), tid | to.astype(np.uint64), ws)
+    V.finish_init_(sim)
+    return types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=ids, f32=f32)
+
+
+def step_opinion_n(V, m, steps, eps=0.2):
+    """`steps` opinion apply! calls enqueued through one prepared descriptor."""
+    if not hasattr(m, "prepared"):
+        m.prepared = m.sim._prepare(V.transitions.opinion(m.E, eps, m.f32), m.T, [m.T, m.E], m.T)
+        m.prepared_eps = eps
+    assert m.prepared_eps == eps
+    m.sim._apply_prepared(m.prepared, steps)
+
+
 def build_contact(V, n, seed=1, n_init=None):
     *_, Person, Met = _types(V)
     model = (V.ModelTypes()

@@ -3,21 +3,26 @@
 Requests arrive over time (Poisson), are admitted first-come-first-served
 while their estimated device footprint fits the HBM budget, keep their graph
 and state on the device between steps, and leave when their horizon is done.
+Both models build their graphs on the device (directed G(n, p), mean degree
+from the request): SIR requests as ensemble members, opinion requests as
+simulations whose edges come from vh_add_edges_gnp.
 Continuous batching: every scheduler iteration advances all SIR requests by
 one segmented, graph-captured ensemble launch per step and every opinion
-request by apply! calls on its own stream, so their work overlaps on the GPU;
-requests that arrive meanwhile join at the next iteration as budget frees up.
+request by one vh_apply_n call on its own stream, so their work overlaps on
+the GPU; requests that arrive meanwhile join at the next iteration as budget
+frees up.
 
 `run_sequential` is the one-at-a-time baseline: the same requests, each built,
 run to its horizon and read back before the next one starts.
 """
-import hashlib
+import ctypes as C
 import time
 import types
 
 import numpy as np
+import xxhash
 
-from . import _abi, inputs, models
+from . import _abi, models
 from .ensemble import Ensemble
 
 
@@ -39,134 +44,216 @@ def make_requests(count, rate, seed=1, n_min=10_000, n_max=10_000_000, h_min=50,
 
 def footprint(req):
     """Device bytes a request holds while active (state, CSR, build scratch)."""
+    e = req.n * req.mean_deg
     if req.kind == "sir":
-        e = req.n * req.mean_deg
         return int(req.n * 12 + e * 4 + req.n * 8)
-    e = 2 * req.n * 5
-    return int(req.n * 20 + e * 8 + e * 48)
+    # two Float64 state buffers, offsets, sources + weights, generator scratch
+    return int(req.n * 16 + req.n * 4 + e * 8 + req.n * 8)
 
 
-def _opinion_edges(req):
-    fr, to = inputs.graph_gnp(req.n, 5.0, req.seed)
-    return inputs.symmetric_weighted(fr, to, req.seed + 7)
+def _build_opinion(vh, req, stream=None):
+    return models.build_opinion_gnp(vh, req.n, mean_deg=req.mean_deg, seed=req.seed,
+                                    graph_seed=req.seed + 1, weight_seed=req.seed + 7, stream=stream)
+
+
+def _stream(lib):
+    h = C.c_void_p()
+    rc = lib.stream_create(C.byref(h))
+    if rc != _abi.VH_OK:
+        raise RuntimeError(f"vh_stream_create failed ({rc})")
+    return h
 
 
 def _digest(arr):
-    return hashlib.sha256(np.ascontiguousarray(arr).view(np.uint8)).hexdigest()
+    """128-bit digest of the result bytes, to compare results across runs."""
+    return xxhash.xxh3_128_hexdigest(np.ascontiguousarray(arr).view(np.uint8))
 
 
 class Server:
-    def __init__(self, vh, hbm_budget, quantum=10, keep_results=False):
+    """`solo_agents`: SIR requests with at least this many agents run in an
+    ensemble of their own (their `quantum` steps back to back, so that their
+    graph stays in the 256 MB Infinity Cache from step to step); smaller ones
+    share one ensemble whose launches advance all of them at once."""
+
+    def __init__(self, vh, hbm_budget, quantum=10, keep_results=False, solo_agents=1_000_000):
         self.vh = vh
         self.budget = int(hbm_budget)
         self.quantum = quantum
         self.keep = keep_results
+        self.solo_agents = solo_agents
+        # Two streams in all: every step launch goes to `compute`, every
+        # graph build and result read to `io`. Simulations do not get streams
+        # of their own: with a handful of hardware queues per process, many
+        # streams share queues and a read would wait behind unrelated steps.
+        lib = _abi.lib()
+        self.compute = _stream(lib)
+        self.io = _stream(lib)
         self.ens = Ensemble()
-        _abi.lib().enable_asserts(0)
+        self.ens.set_streams(self.compute, self.io)
+        # host seconds per scheduler phase of the last run (observability)
+        self.stats = {}
+        lib.enable_asserts(0)
 
     def run(self, requests, time_scale=1.0):
         """Serve `requests` (arrival times in seconds, scaled by time_scale);
-        returns per-request records with latency and the final-state digest."""
+        returns per-request records with latency and the final-state digest.
+
+        One scheduler iteration: (1) enqueue `quantum` steps of every active
+        request (asynchronous: opinion requests on their own streams, SIR
+        requests in one graph-captured ensemble launch sequence); while those
+        run, (2) read back and digest the requests that finished in the
+        previous iteration and (3) admit arrivals that fit the budget, their
+        graphs built on streams of their own; (4) wait for the quantum;
+        (5) free what was read back (frees wait for an idle GPU) and set the
+        requests that just finished aside for the next iteration's (2)."""
         vh = self.vh
         pending = sorted(requests, key=lambda r: r.arrival)
         active = {}
+        finished = []
         used = 0
         done = []
         t0 = time.perf_counter()
         qi = 0
-        while qi < len(pending) or active:
-            now = time.perf_counter() - t0
-            # admission: FIFO while the head fits the budget
+        ph = dict(enqueue_opinion=0.0, enqueue_sir=0.0, read=0.0, admit=0.0, wait=0.0, release=0.0,
+                  idle=0.0, iterations=0)
+        self.stats = ph
+        q = self.quantum
+        while qi < len(pending) or active or finished:
+            # (1) one quantum for every active request
+            ta = time.perf_counter()
+            stepping = list(active.values())
+            if any(r.kind == "sir" and r.ens is self.ens for r in stepping):
+                self.ens.step(q, use_graph=True)
+            for r in stepping:
+                if r.kind == "sir" and r.ens is not self.ens:
+                    r.ens.step(min(q, r.left), use_graph=True)
+            ta2 = time.perf_counter()
+            for r in stepping:
+                if r.kind == "opinion":
+                    models.step_opinion_n(vh, r.model, min(q, r.left), eps=0.2)
+            for r in stepping:
+                r.left -= min(q, r.left)
+            if stepping:
+                ph["iterations"] += 1
+            tb = time.perf_counter()
+            # (2) results of the previous iteration, beside the running quantum
+            for r in finished:
+                if r.kind == "sir":
+                    st, step = r.ens.read(r.mid)
+                    assert step == r.horizon + 1
+                else:
+                    st, _ = r.model.sim._read_agents(r.model.T)
+                r.digest = _digest(st)
+                if self.keep:
+                    r.result = np.array(st)
+                r.done = time.perf_counter() - t0
+                r.latency = r.done - r.arrival * time_scale
+                done.append(r)
+            tc = time.perf_counter()
+            # (3) admission: FIFO while the head fits the budget
+            now = tc - t0
             while qi < len(pending) and pending[qi].arrival * time_scale <= now:
                 r = pending[qi]
                 need = footprint(r)
-                if used + need > self.budget and active:
+                if used + need > self.budget and (active or finished):
                     break
                 qi += 1
                 used += need
                 r.admitted = now
                 if r.kind == "sir":
-                    r.mid = self.ens.add_sir(r.n, r.mean_deg, r.p_inf, r.seed, r.seed + 1,
-                                             recovery=7, n_init=max(1, r.n // 1000))
-                    self.ens.set_horizon(r.mid, r.horizon)
-                    r.left = r.horizon
+                    if r.n >= self.solo_agents:
+                        r.ens = Ensemble()
+                        r.ens.set_streams(self.compute, self.io)
+                    else:
+                        r.ens = self.ens
+                    r.mid = r.ens.add_sir(r.n, r.mean_deg, r.p_inf, r.seed, r.seed + 1,
+                                          recovery=7, n_init=max(1, r.n // 1000))
+                    r.ens.set_horizon(r.mid, r.horizon)
                 else:
-                    r.model = models.build_opinion(vh, r.n, seed=r.seed, edges=_opinion_edges(r))
-                    r.left = r.horizon
+                    r.model = _build_opinion(vh, r, stream=self.io)
+                    r.model.sim._set_streams(self.compute, self.io)
+                r.left = r.horizon
                 active[r.id] = r
-            if not active:
-                if qi < len(pending):
-                    wait = pending[qi].arrival * time_scale - (time.perf_counter() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 0.05))
-                continue
-            # one scheduler iteration: `q` steps for everybody (ensemble members
-            # stop exactly at their horizon inside the launch)
-            q = self.quantum
-            # opinion steps are enqueued first (asynchronous, one stream per
-            # simulation), the ensemble launch then overlaps with them
-            for r in active.values():
-                if r.kind == "opinion":
-                    for _ in range(min(q, r.left)):
-                        models.step_opinion(vh, r.model, eps=0.2)
-            if any(r.kind == "sir" for r in active.values()):
-                self.ens.step(q, use_graph=True)
-            for r in active.values():
-                r.left -= min(q, r.left)
+            td = time.perf_counter()
+            # (4) the quantum's end: everything stepped on the compute stream
             self.ens.sync()
-            for r in list(active.values()):
-                if r.kind == "opinion":
-                    _abi.lib().sync(r.model.sim.handle)
-            now = time.perf_counter() - t0
-            for r in list(active.values()):
-                if r.left > 0:
-                    continue
+            te = time.perf_counter()
+            # (5) release what was read back; set aside what just finished
+            for r in finished:
                 if r.kind == "sir":
-                    st, step = self.ens.read(r.mid)
-                    assert step == r.horizon + 1
-                    self.ens.remove(r.mid)
+                    if r.ens is self.ens:
+                        self.ens.remove(r.mid)
+                    else:
+                        r.ens.close()
+                    r.ens = None
                 else:
-                    st, _ = r.model.sim._read_agents(r.model.T)
                     r.model.sim._destroy()
                     r.model = None
-                r.digest = _digest(st)
-                if self.keep:
-                    r.result = np.array(st)
-                r.done = now
-                r.latency = now - r.arrival * time_scale
                 used -= footprint(r)
-                done.append(r)
+            finished = [r for r in stepping if r.left == 0]
+            for r in finished:
                 del active[r.id]
+            tf = time.perf_counter()
+            ph["enqueue_sir"] += ta2 - ta
+            ph["enqueue_opinion"] += tb - ta2
+            ph["read"] += tc - tb
+            ph["admit"] += td - tc
+            ph["wait"] += te - td
+            ph["release"] += tf - te
+            if not active and not finished and qi < len(pending):
+                wait = pending[qi].arrival * time_scale - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                    ph["idle"] += min(wait, 0.05)
         wall = time.perf_counter() - t0
         return done, wall
 
     def close(self):
         self.ens.close()
+        lib = _abi.lib()
+        lib.stream_destroy(self.io)
+        lib.stream_destroy(self.compute)
 
 
-def run_sequential(vh, requests, time_scale=1.0, keep_results=False):
-    """One request at a time on the GPU, in arrival order."""
+def run_sequential(vh, requests, time_scale=1.0, keep_results=False, stats=None):
+    """One request at a time on the GPU, in arrival order. With a `stats`
+    dict, the host seconds spent building, running and reading back are
+    added to it."""
     _abi.lib().enable_asserts(0)
     done = []
+    ph = stats if stats is not None else {}
+    for k in ("build", "run", "read", "idle"):
+        ph.setdefault(k, 0.0)
     t0 = time.perf_counter()
     for r in sorted(requests, key=lambda x: x.arrival):
         now = time.perf_counter() - t0
         if r.arrival * time_scale > now:
             time.sleep(r.arrival * time_scale - now)
+            ph["idle"] += r.arrival * time_scale - now
+        ta = time.perf_counter()
         if r.kind == "sir":
             # the same device-built graph and launch as in the mix, alone
             e = Ensemble()
             mid = e.add_sir(r.n, r.mean_deg, r.p_inf, r.seed, r.seed + 1, recovery=7,
                             n_init=max(1, r.n // 1000))
+            tb = time.perf_counter()
             e.step(r.horizon, use_graph=True)
+            e.sync()
+            tc = time.perf_counter()
             st, _ = e.read(mid)
             e.close()
         else:
-            m = models.build_opinion(vh, r.n, seed=r.seed, edges=_opinion_edges(r))
-            for _ in range(r.horizon):
-                models.step_opinion(vh, m, eps=0.2)
+            m = _build_opinion(vh, r)
+            tb = time.perf_counter()
+            models.step_opinion_n(vh, m, r.horizon, eps=0.2)
+            _abi.lib().sync(m.sim.handle)
+            tc = time.perf_counter()
             st, _ = m.sim._read_agents(m.T)
             m.sim._destroy()
+        td = time.perf_counter()
+        ph["build"] += tb - ta
+        ph["run"] += tc - tb
+        ph["read"] += td - tc
         r2 = types.SimpleNamespace(**{k: v for k, v in vars(r).items()})
         r2.digest = _digest(st)
         if keep_results:

@@ -102,3 +102,27 @@ def test_contact_invariants_large(vh, gpu_ready):
         assert cnt.max() <= 20
         # about 20 contacts per survivor, minus the ~2% that died as partners
         assert 0.9 * 20 * n * 0.98 < len(to) <= 20 * n
+
+
+def test_contact_1e5_bit_exact(vh, ora, gpu_ready):
+    """The contact model at 10^5 agents for 20 steps: states, died flags (the
+    slot map) and contact lists equal the checker's."""
+    M = vh.models
+    n = 100_000
+    g = M.build_contact(vh, n, seed=13)
+    c = M.build_contact(ora, n, seed=13)
+    for step in range(20):
+        M.step_contact(vh, g, seed=13)
+        M.step_contact(ora, c, seed=13)
+        if step % 5 == 4:
+            a, da = g.sim._read_agents(g.T)
+            b, db = c.sim._read_agents(c.T)
+            assert len(a) == len(b) and np.array_equal(da, db), step
+            live = ~da
+            ab = np.ascontiguousarray(a).view(np.uint8).reshape(len(a), -1)[live]
+            bb = np.ascontiguousarray(b).view(np.uint8).reshape(len(b), -1)[live]
+            assert np.array_equal(ab, bb), step
+            tg, fg, _ = g.sim._read_edges(g.E)
+            tc, fc, _ = c.sim._read_edges(c.E)
+            assert np.array_equal(tg, tc) and np.array_equal(fg, fc), step
+    print(f"[gpu] contact 1e5 x 20: {int((~da).sum())} live, {len(tg)} contacts, bit-exact", flush=True)

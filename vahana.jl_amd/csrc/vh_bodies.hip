@@ -303,11 +303,14 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
     uint32_t h = s & 0xFFu;
     if (h == VH_S) {
         if (i < nrows) {
-            const uint64_t id = vh_make_id(t, i + 1);
+            const uint64_t pre = vh_hash_pre(seed, vh_make_id(t, i + 1), step);
             uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
+            // "some in-edge k has an infected source and u_k < p" is
+            // order-free: the draw (ALU) first, the source's state (a random
+            // gather) only for the few draws below p
             for (uint32_t k = b; k < e; k++) {
-                uint32_t j = src[k];
-                if ((rs[j - 1] & 0xFFu) == VH_I && vh_u01(vh_hash(seed, id, step, k - b + 1)) < p) {
+                if (vh_u01(vh_hash_fin(pre, k - b + 1)) >= p) continue;
+                if ((rs[src[k] - 1] & 0xFFu) == VH_I) {
                     s = (s & 0xFF00u) | VH_I;
                     break;
                 }

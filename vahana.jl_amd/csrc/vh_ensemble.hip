@@ -113,11 +113,12 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
     uint32_t s = m.rs[i];
     uint32_t h = s & 0xFFu;
     if (h == VH_S) {
-        const uint64_t id = vh_make_id(m.type, i + 1);
+        const uint64_t pre = vh_hash_pre(m.seed, vh_make_id(m.type, i + 1), m.step);
         uint32_t bb = m.off[i], e = m.off[i + 1];
+        // the draw first, the source's state only below p (as in k_sir)
         for (uint32_t k = bb; k < e; k++) {
-            uint32_t j = m.src[k];
-            if ((m.rs[j - 1] & 0xFFu) == VH_I && vh_u01(vh_hash(m.seed, id, m.step, k - bb + 1)) < m.p) {
+            if (vh_u01(vh_hash_fin(pre, k - bb + 1)) >= m.p) continue;
+            if ((m.rs[m.src[k] - 1] & 0xFFu) == VH_I) {
                 s = (s & 0xFF00u) | VH_I;
                 break;
             }

@@ -39,10 +39,10 @@ def sir_bytes(n, e):
     return n * (4 + 4) + (n + 1) * 4 + e * 4
 
 
-def cpu_baseline(workload, f32, seconds_cap=25.0):
+def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0):
     """The checker's apply! (a restatement of the reference's data structures)
-    on one core over a bounded sample: same model, 10^6 agents (2*10^5 for
-    the contact model)."""
+    over a bounded sample of the workload: same model, 10^6 agents (2*10^5
+    for the contact model); median of up to 5 steps after a warm-up."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as ora
     from __graft_entry__ import load_package
@@ -50,19 +50,19 @@ def cpu_baseline(workload, f32, seconds_cap=25.0):
     M = vh.models
     n = 1_000_000
     if workload.startswith("opinion"):
-        a, b = vh.inputs.graph_ba(n, 5, 1)
-        edges = vh.inputs.symmetric_weighted(a, b, 2)
-        m = M.build_opinion(ora, n, seed=1, f32=f32, edges=edges)
+        a, b = vh.inputs.graph_ba(n, 5, seed)
+        edges = vh.inputs.symmetric_weighted(a, b, seed + 1)
+        m = M.build_opinion(ora, n, seed=seed, f32=f32, edges=edges)
         step = lambda: M.step_opinion(ora, m, eps=0.2)  # noqa: E731
         ne = len(edges[0])
     elif workload == "contact":
         n = 200_000
-        m = M.build_contact(ora, n, seed=1)
-        step = lambda: M.step_contact(ora, m, seed=1)  # noqa: E731
+        m = M.build_contact(ora, n, seed=seed)
+        step = lambda: M.step_contact(ora, m, seed=seed)  # noqa: E731
         ne = 20 * n
     else:
-        edges = vh.inputs.graph_er(n, 10.0 / (n - 1), 1)
-        m = M.build_sir(ora, n, seed=1, edges=edges)
+        edges = vh.inputs.graph_er(n, 10.0 / (n - 1), seed)
+        m = M.build_sir(ora, n, seed=seed, edges=edges)
         step = lambda: M.step_sir(ora, m)  # noqa: E731
         ne = len(edges[0])
     step()   # warm-up
@@ -73,11 +73,55 @@ def cpu_baseline(workload, f32, seconds_cap=25.0):
         step()
         times.append(time.perf_counter() - t0)
     times.sort()
+    return n, ne, times
+
+
+def cpu_baseline(workload, f32):
+    """Setting (i): one host core."""
+    n, ne, times = _cpu_sample(workload, f32)
     med = times[len(times) // 2]
     return {"value": n / med, "unit": "agent-updates/s", "cores": 1, "kind": "port",
             "edges_per_s": ne / med,
             "sample": f"{workload} N={n} E={ne}, median of {len(times)} apply! steps after 1 warm-up, "
                       "1 host core"}
+
+
+def cpu_baseline_all_cores(workload, f32):
+    """Setting (ii): the host cores as P independent processes, each owning a
+    contiguous range of agents (its own sample of the workload, rank-seeded;
+    the exchange between ranks is not modelled), run at the same time. The
+    aggregate rate is all ranks' agents over the slowest rank's median step,
+    as ranks advance in lockstep. P = the CPUs this process may use, at most
+    15. The workers make no GPU calls."""
+    import subprocess
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    P = max(1, min(ncpu, 15))
+    env = dict(os.environ, ROCR_VISIBLE_DEVICES="", HIP_VISIBLE_DEVICES="", OMP_NUM_THREADS="1")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--workload", workload]
+    procs = [subprocess.Popen(cmd + ["--rank", str(r)], stdout=subprocess.PIPE, env=env)
+             for r in range(P)]
+    res = []
+    for pr in procs:
+        out, _ = pr.communicate(timeout=600)
+        if pr.returncode != 0:
+            raise RuntimeError(f"cpu worker failed ({pr.returncode})")
+        res.append(json.loads(out.decode().strip().splitlines()[-1]))
+    slow = max(r["median_s"] for r in res)
+    n = sum(r["n"] for r in res)
+    ne = sum(r["edges"] for r in res)
+    return {"value": n / slow, "unit": "agent-updates/s", "cores": P, "kind": "port",
+            "edges_per_s": ne / slow,
+            "sample": f"{workload}: {P} processes x N={res[0]['n']} (rank-seeded samples, no halo "
+                      f"exchange), slowest rank's median apply! step {slow:.3f} s"}
+
+
+def cpu_worker(args):
+    f32 = args.workload == "opinion32"
+    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank)
+    print(json.dumps({"n": n, "edges": ne, "median_s": times[len(times) // 2]}), flush=True)
 
 
 def main():
@@ -94,9 +138,13 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrivals/s (0: 0.8 x the measured one-at-a-time capacity)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--rank", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calib", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
+    if args.cpu_worker:   # one rank of the all-cores CPU baseline (no GPU use)
+        return cpu_worker(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: every simulation here lives on a single GPU "
               "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
@@ -216,6 +264,9 @@ def main():
         t0 = time.perf_counter()
         cpu = cpu_baseline(args.workload, f32)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {cpu}")
+        t0 = time.perf_counter()
+        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32)
+        log(f"cpu baseline, all cores {time.perf_counter() - t0:.1f}s: {cpu['all_cores']}")
 
     out = {
         "metric": "agent-updates/s per apply! (edges/s alongside)",

@@ -72,6 +72,8 @@ enum vh_transition {
     VH_TR_NEIGHBOR_FOO_EDGES = 5, /* test/independent.jl:55-68 body                */
     VH_TR_SPAWN = 6,         /* add_agent!(T(foo + k)) (+ edge newborn -> id)      */
     VH_TR_COUNT_EDGES = 7,   /* foo = num_edges(sim, id, E)                         */
+    VH_TR_DIFFUSE = 8,       /* Bool field |= any in-neighbour's (test/raster.jl:38-41) */
+    VH_TR_MOVE_TO = 9,       /* value_on_pos: move_to! cell (v,...,v) (test/raster.jl:279-285) */
     VH_TR_SIR = 16,          /* SIR contagion (Stateless contact edges)            */
     VH_TR_OPINION_F64 = 17,  /* weighted bounded confidence, Float64 opinions      */
     VH_TR_OPINION_F32 = 18,  /* same, Float32 opinions                              */
@@ -91,6 +93,16 @@ typedef struct vh_test_params {      /* params of the test bodies (ids 0-7) */
     uint32_t pad;
     uint8_t edge_state[16];          /* edge state bytes for ADD_SELF_EDGE  */
 } vh_test_params;
+
+typedef struct vh_raster_params {    /* raster bodies (ids 8, 9)                    */
+    uint32_t field_offset;           /* DIFFUSE: byte offset of the Bool field       */
+    uint32_t cell_type;              /* MOVE_TO: agent type of the raster's cells    */
+    uint64_t first_nr;               /* MOVE_TO: agent number of cell (1, ..., 1)    */
+    int64_t dims[4];                 /* MOVE_TO: raster size, column-major           */
+    uint32_t ndims;                  /* MOVE_TO: 1..4                                 */
+    uint32_t flags;                  /* MOVE_TO: 1 edge cell -> agent, 2 agent -> cell */
+    uint8_t edge_state[16];          /* MOVE_TO: state of the added edges            */
+} vh_raster_params;
 
 typedef struct vh_sir_params {       /* SIR: state {UInt8 health, UInt16 days} */
     double p_inf;                    /* per infected in-neighbour, per step    */
@@ -201,6 +213,19 @@ int vh_add_edges(vh_sim_t s, uint8_t edge_type_id, const uint64_t* from,
  * give the same edges on the host. No other edges may be added to `et`. */
 int vh_add_edges_gnp(vh_sim_t sim, uint8_t et, double mean_deg, uint64_t graph_seed,
                      uint64_t weight_seed);
+
+/* connect_raster_neighbors! src/Raster.jl:134-164 for a raster whose
+ * prod(dims) cells are the agents first_nr, first_nr + 1, ... of cell_type
+ * in column-major order (as add_raster! adds them): for every cell `org`
+ * and stencil offset `s` an edge org -> org + s (wrapped per dimension when
+ * periodic, dropped outside the raster otherwise), all with `edge_state`.
+ * Stencil as src/Raster.jl:80-96 (metric 0 Chebyshev, 1 Euclidean, 2
+ * Manhattan; at most 128 offsets). Generated on the device at finish_init!;
+ * the CSR equals adding the edges one by one, cell by cell. No other edges
+ * may be added to `et`. */
+int vh_add_edges_stencil(vh_sim_t sim, uint8_t et, uint32_t cell_type, uint64_t first_nr, uint32_t ndims,
+                         const int64_t* dims, double distance, int metric, int periodic,
+                         const uint8_t* edge_state);
 /* finish_init! src/Simulation.jl:403-476: upload and build the CSR */
 int vh_finish_init(vh_sim_t s);
 /* apply! src/Simulation.jl:720-821, enqueued on the simulation's stream */

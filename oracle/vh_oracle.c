@@ -924,6 +924,49 @@ static int body_contact(ctx_t* c, const uint8_t* st, uint64_t id, uint8_t* out) 
     return 1;
 }
 
+/* diffuse / diffuse3D test/raster.jl:38-41,143-146: the Bool field becomes
+ * a.active || mapreduce(n -> n.active, |, neighborstates(sim, id, E)) */
+static int body_diffuse(ctx_t* c, const uint8_t* st, uint64_t id, uint8_t* out) {
+    const vh_raster_params* p = (const vh_raster_params*)c->s->params;
+    uint32_t f = p->field_offset;
+    uint8_t v = st[f] ? 1 : 0;
+    if (!v) {
+        cont_t* k = rd_cont(c, c->d->role_edge[0], id);
+        if (k)
+            for (uint32_t i = 0; i < k->n; i++) {
+                const uint8_t* ns = agentstate_of(c, k->from[i]);
+                if (!ns) return 1;
+                v |= ns[f] ? 1 : 0;
+            }
+    }
+    out[f] = v;
+    return 1;
+}
+/* value_on_pos test/raster.jl:279-285 with move_to! src/Raster.jl:437-463
+ * (distance 0): v = first(neighborstates_flexible(sim, id, E)).ids_sum, then
+ * add_edge!(cell(v,...,v), id) and add_edge!(id, cell(v,...,v)) */
+static int body_move_to(ctx_t* c, const uint8_t* st, uint64_t id, uint8_t* out) {
+    (void)st;
+    const vh_raster_params* p = (const vh_raster_params*)c->s->params;
+    int et = c->d->role_edge[0];
+    cont_t* k = rd_cont(c, et, id);
+    if (!k || k->n == 0) { c->err = fail(VH_ERR_ASSERT, "first(nothing): no neighbour"); return 1; }
+    const uint8_t* ns = agentstate_of(c, k->from[0]);
+    if (!ns) return 1;
+    int64_t v = rd_i64(ns);
+    uint64_t lin = 0, stride = 1;
+    for (uint32_t d = 0; d < p->ndims && d < 4; d++) {
+        if (v < 1 || v > p->dims[d]) { c->err = fail(VH_ERR_ASSERT, "raster position out of bounds"); return 1; }
+        lin += (uint64_t)(v - 1) * stride;
+        stride *= (uint64_t)p->dims[d];
+    }
+    uint64_t cell = agent_id(p->cell_type, p->first_nr + lin);
+    if (p->flags & 1u) ctx_add_edge(c, et, cell, id, p->edge_state);
+    if (p->flags & 2u) ctx_add_edge(c, et, id, cell, p->edge_state);
+    wr_i64(out, v);
+    return 1;
+}
+
 static body_fn body_of(uint32_t tr) {
     switch (tr) {
     case VH_TR_NOOP: return body_noop;
@@ -934,6 +977,8 @@ static body_fn body_of(uint32_t tr) {
     case VH_TR_NEIGHBOR_FOO_EDGES: return body_neighbor_foo_edges;
     case VH_TR_SPAWN: return body_spawn;
     case VH_TR_COUNT_EDGES: return body_count_edges;
+    case VH_TR_DIFFUSE: return body_diffuse;
+    case VH_TR_MOVE_TO: return body_move_to;
     case VH_TR_SIR: return body_sir;
     case VH_TR_OPINION_F64: return body_opinion_f64;
     case VH_TR_OPINION_F32: return body_opinion_f32;

@@ -4,6 +4,7 @@ Every scenario takes an API module `V` (the product package, or the checker
 binding in oracle_api) and returns its observations; tests compare them with
 the answers the reference's own test files assert and with each other.
 """
+import numpy as np
 
 
 def _T(V):
@@ -525,4 +526,139 @@ def edges_iter_removal(V):
             V.apply_(sim, V.transitions.die_if("odd"), AgentB, AgentB, AgentB)
             r.append(V.num_edges(sim, ET))
         out["remove"][name] = r
+    return out
+
+
+# ---------------------------------------------------------------- raster
+def raster_model(V):
+    """raster_model test/raster.jl:1-33."""
+    GridA = V.struct("GridA", [("pos", "<i8", (2,)), ("active", "?")])
+    Grid3D = V.struct("Grid3D", [("pos", "<i8", (3,)), ("active", "?")])
+    GridE = V.struct("GridE")
+    Position = V.struct("Position", [("ids_sum", "<i8")])
+    MovingAgent = V.struct("MovingAgent", [("value", "<i8")])
+    OnPosition = V.struct("OnPosition")
+    model = (V.ModelTypes() | V.register_agenttype_(GridA) | V.register_agenttype_(Grid3D)
+             | V.register_edgetype_(GridE) | V.register_agenttype_(Position)
+             | V.register_agenttype_(MovingAgent) | V.register_edgetype_(OnPosition)
+             | V.create_model("Raster_Test"))
+    return model, GridA, Grid3D, GridE, Position, MovingAgent, OnPosition
+
+
+def _diffusion(V, model, T, GridE, dims, active, steps, uniform=False, **conn):
+    sim = V.create_simulation(model)
+    V.raster.add_raster_(sim, "grid", dims, lambda p: T(p, tuple(p) == tuple(active)))
+    # uniform: the edge state as a value (a device simulation then builds the
+    # stencil on the device), else the reference's constructor form
+    V.raster.connect_raster_neighbors_(sim, "grid", GridE() if uniform else (lambda a, b: GridE()),
+                                       **conn)
+    V.finish_init_(sim)
+    counts = [V.mapreduce(sim, "active", "+", T)]
+    for _ in range(steps):
+        V.apply_(sim, V.transitions.diffuse(GridE, T), [T], [T, GridE], [T])
+        counts.append(V.mapreduce(sim, "active", "+", T))
+    return sim, counts
+
+
+def raster_diffusion(V, uniform=False):
+    """test/raster.jl:36-238: active cells after 0, 1, 2 diffusion steps on 2D
+    and 3D rasters under each metric, distance and periodicity; the
+    calc_rasterstate / calc_raster readouts of the first raster."""
+    model, GridA, Grid3D, GridE, *_ = raster_model(V)
+    out = {}
+    sim, out["2d_10x8"] = _diffusion(V, model, GridA, GridE, (10, 8), (10, 8), 2,
+                                     uniform)                                               # :43-58
+    r = V.raster.calc_rasterstate(sim, "grid", lambda c: bool(c["active"]), bool, GridA)  # :60-63
+    out["rasterstate"] = (bool(r[0, 2]), bool(r[0, 0]), bool(r[3, 3]))
+    r = V.raster.calc_raster(sim, "grid", lambda i: bool(V.agentstate(sim, i, GridA)["active"]),
+                             bool)                                                          # :65-68
+    out["raster"] = (bool(r[0, 2]), bool(r[0, 0]), bool(r[3, 3]))
+    out["pos_of_3_5"] = tuple(int(x) for x in V.agentstate(sim, V.raster.cellid(sim, "grid", (3, 5)),
+                                                           GridA)["pos"])
+    V.finish_simulation_(sim)
+    for key, dims, act in (("2d_7x9", (7, 9), (3, 5)), ("2d_17x6", (17, 6), (9, 1)),
+                           ("2d_6x7", (6, 7), (1, 1))):                                   # :72-139
+        sim, out[key] = _diffusion(V, model, GridA, GridE, dims, act, 2, uniform)
+        V.finish_simulation_(sim)
+    cases = {"3d": dict(), "3d_nonperiodic": dict(periodic=False),                       # :149-186
+             "3d_dist2": dict(periodic=False, distance=2),                                 # :188-208
+             "3d_euclidean": dict(distance=1.5, metric="euclidean"),                       # :210-226
+             "3d_manhatten": dict(distance=1.5, metric="manhatten"),                       # :228-244
+             "3d_manhatten_d2": dict(periodic=False, distance=2, metric="manhatten")}      # :246-262
+    for key, conn in cases.items():
+        steps = 2 if key in ("3d", "3d_nonperiodic", "3d_dist2") else 1
+        sim, out[key] = _diffusion(V, model, Grid3D, GridE, (6, 7, 6), (1, 1, 1), steps, uniform,
+                                   **conn)
+        V.finish_simulation_(sim)
+    return out
+
+
+def raster_nodeid(V):
+    """Raster_NodeID test/raster.jl:268-342: agents summed onto their cells,
+    then moved to the cell (v, v) of their cell's sum."""
+    model, GridA, Grid3D, GridE, Position, MovingAgent, OnPosition = raster_model(V)
+    sim = V.create_simulation(model)
+    V.raster.add_raster_(sim, "raster", (10, 10), lambda p: Position(0))
+    V.raster.connect_raster_neighbors_(sim, "raster", lambda a, b: GridE())
+    p = [V.add_agent_(sim, MovingAgent(v)) for v in (1, 2, 3)]
+    for a, pos in zip(p, ((1, 1), (2, 2), (2, 2))):
+        V.raster.move_to_(sim, "raster", a, pos, OnPosition(), OnPosition())
+    V.finish_init_(sim)
+    sum_on_pos = V.transitions.sum_neighbors(OnPosition)
+    V.apply_(sim, sum_on_pos, [Position], [MovingAgent, OnPosition], [Position])
+    out = {}
+    r = V.raster.calc_rasterstate(sim, "raster", lambda c: int(c["ids_sum"]), int, Position)
+    out["first"] = (int(r[0, 0]), int(r[0, 1]), int(r[1, 1]), int(r[1, 2]))
+    r = V.raster.rastervalues(sim, "raster", "ids_sum")
+    out["first_values"] = (int(r[0, 0]), int(r[0, 1]), int(r[1, 1]), int(r[1, 2]))
+    V.apply_(sim, V.transitions.move_to_value(sim, "raster", OnPosition), [MovingAgent],
+             [Position, OnPosition], [OnPosition, MovingAgent])
+    V.apply_(sim, sum_on_pos, [Position], [MovingAgent, OnPosition], [Position])
+    r = V.raster.calc_rasterstate(sim, "raster", lambda c: int(c["ids_sum"]), int, Position)
+    out["second"] = (int(r[0, 0]), int(r[0, 1]), int(r[1, 1]), int(r[4, 4]))
+    out["values"] = [int(V.agentstate(sim, a, MovingAgent)["value"]) for a in p]
+    V.finish_simulation_(sim)
+    return out
+
+
+def raster_move_to_dist(V):
+    """MoveTo_Dist test/raster.jl:347-412: edges of move_to! with a
+    surrounding under each metric, 2D and 4D."""
+    model, GridA, Grid3D, GridE, Position, MovingAgent, OnPosition = raster_model(V)
+    out = {}
+    sim = V.create_simulation(model)
+    V.raster.add_raster_(sim, "raster", (10, 10), lambda p: Position(0))
+    p = [V.add_agent_(sim, MovingAgent(1)) for _ in range(3)]
+    V.raster.move_to_(sim, "raster", p[0], (4, 4), OnPosition(), None, distance=2)
+    V.raster.move_to_(sim, "raster", p[1], (4, 4), OnPosition(), None, distance=2, metric="manhatten")
+    V.raster.move_to_(sim, "raster", p[2], (4, 4), OnPosition(), None, distance=2.5,
+                      metric="euclidean")
+    V.finish_init_(sim)
+    out["2d"] = [V.num_edges(sim, a, OnPosition) for a in p]
+    V.finish_simulation_(sim)
+    sim = V.create_simulation(model)
+    V.raster.add_raster_(sim, "raster", (10, 10, 10, 10), lambda p: Position(0))
+    p = [V.add_agent_(sim, MovingAgent(1)) for _ in range(2)]
+    V.raster.move_to_(sim, "raster", p[0], (4, 4, 4, 4), OnPosition(), None, distance=1)
+    V.raster.move_to_(sim, "raster", p[1], (4, 4, 4, 4), OnPosition(), None, distance=1,
+                      metric="manhatten")
+    V.finish_init_(sim)
+    out["4d"] = [V.num_edges(sim, a, OnPosition) for a in p]
+    V.finish_simulation_(sim)
+    return out
+
+
+def raster_random(V):
+    """test/raster.jl:417-437: weighted random positions and cells with a
+    one-hot weight array."""
+    model, GridA, Grid3D, *_ = raster_model(V)
+    sim = V.create_simulation(model)
+    V.raster.add_raster_(sim, "raster", (10, 20, 30), lambda p: Grid3D(p, True))
+    V.finish_init_(sim)
+    w = np.zeros((10, 20, 30))
+    w[6, 18, 22] = 1.0
+    pos = V.raster.random_pos(sim, "raster", w)
+    cell = V.raster.random_cell(sim, "raster", w)
+    out = (pos, tuple(int(x) for x in V.agentstate(sim, cell, Grid3D)["pos"]))
+    V.finish_simulation_(sim)
     return out

@@ -34,6 +34,7 @@ def test_struct_layouts_match_header(vh):
     assert C.sizeof(A.TestParams) == 32
     assert C.sizeof(A.SirParams) == 24
     assert C.sizeof(A.ContactParams) == 40
+    assert C.sizeof(A.RasterParams) == 72
 
 
 def _mix64(z):

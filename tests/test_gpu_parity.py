@@ -21,7 +21,8 @@ def _eq(a, b):
 @pytest.mark.parametrize("name", ["core_aggregate", "core_sum_neighbors", "core_queries",
                                   "edges_matrix", "edges_transition", "remove_agents",
                                   "independent", "addexisting", "graphs_k4",
-                                  "edges_iter_removal"])
+                                  "edges_iter_removal", "raster_diffusion", "raster_nodeid",
+                                  "raster_move_to_dist", "raster_random"])
 def test_reference_scenarios(vh, ora, gpu_ready, name):
     fn = getattr(SC, name)
     print(f"[gpu] scenario {name}", flush=True)
@@ -174,3 +175,48 @@ def test_device_gnp_argument_errors(vh, gpu_ready):
     with pytest.raises(vh.VahanaError):
         sim._add_edges_array(Knows, np.array([1 << 56 | 1], dtype=np.uint64),
                              np.array([1 << 56 | 1], dtype=np.uint64))
+
+
+def test_raster_device_stencil_known_answers(vh, ora, gpu_ready):
+    """The raster known answers with the stencil edges generated on the
+    device (vh_add_edges_stencil) equal the checker's host-built run."""
+    got = SC.raster_diffusion(vh, uniform=True)
+    want = SC.raster_diffusion(ora)
+    assert got == want
+    assert got["2d_10x8"] == [1, 9, 25] and got["3d_euclidean"] == [1, 19]
+
+
+STENCIL_CASES = [((10, 8), 1, "chebyshev", True), ((10, 8), 1, "chebyshev", False),
+                 ((2, 3), 1, "chebyshev", True),       # wrapped duplicates
+                 ((1, 5), 2, "chebyshev", True),
+                 ((13,), 3, "chebyshev", True), ((6, 7, 6), 2, "manhatten", False),
+                 ((6, 7, 6), 1.5, "euclidean", True), ((5, 4, 3, 3), 1, "chebyshev", True),
+                 ((40, 30), 2.5, "euclidean", False)]
+
+
+@pytest.mark.parametrize("case", STENCIL_CASES, ids=lambda c: "x".join(map(str, c[0])) + f"_{c[2]}{c[1]}"
+                         + ("_p" if c[3] else ""))
+def test_raster_device_stencil_csr(vh, gpu_ready, case):
+    """The device stencil CSR equals the host path's, edge by edge (targets,
+    sources in insertion order, states), in a model with other agent types
+    before the cells and a stateful edge type."""
+    dims, dist, metric, periodic = case
+    Other = vh.struct("Other", [("x", "<i8")])
+    Cell = vh.struct("Cell", [("v", "<i4")])
+    Link = vh.struct("Link", [("w", "<f4"), ("tag", "<i4")])
+    model = (vh.ModelTypes() | vh.register_agenttype_(Other) | vh.register_agenttype_(Cell)
+             | vh.register_edgetype_(Link) | vh.create_model("stencil"))
+    res = []
+    for device in (True, False):
+        sim = vh.create_simulation(model)
+        vh.add_agents_(sim, [Other(i) for i in range(3)])
+        vh.raster.add_raster_(sim, "r", dims, lambda p: Cell(sum(p)))
+        vh.raster.connect_raster_neighbors_(sim, "r", Link(0.5, 7), distance=dist, metric=metric,
+                                            periodic=periodic, device=device)
+        vh.finish_init_(sim)
+        to, fr, st = sim._read_edges(Link)
+        res.append((to, fr, np.ascontiguousarray(st).view(np.uint8)))
+        vh.finish_simulation_(sim)
+    (a, b, c), (x, y, z) = res
+    assert len(a) > 0
+    assert np.array_equal(a, x) and np.array_equal(b, y) and np.array_equal(c, z)

@@ -152,3 +152,55 @@ def test_edges_iterator_and_removal(ora):
         if "SingleEdge" not in hints and "SingleType" not in hints:
             count += 5
         assert r[2] == count, (name, r)
+
+
+def test_raster_diffusion(ora):
+    out = SC.raster_diffusion(ora)
+    for key in ("2d_10x8", "2d_7x9", "2d_17x6", "2d_6x7"):   # test/raster.jl:54-58, 81-85, ...
+        assert out[key] == [1, 9, 25], key
+    assert out["rasterstate"] == (False, True, False)         # :60-63
+    assert out["raster"] == (False, True, False)              # :65-68
+    assert out["pos_of_3_5"] == (3, 5)
+    assert out["3d"] == [1, 3 * 3 * 3, 5 * 5 * 5]              # :162-166
+    assert out["3d_nonperiodic"] == [1, 2 * 2 * 2, 3 * 3 * 3]  # :182-186
+    assert out["3d_dist2"] == [1, 3 * 3 * 3, 5 * 5 * 5]        # :203-207
+    assert out["3d_euclidean"] == [1, 3 * 3 * 3 - 8]          # :222-224
+    assert out["3d_manhatten"] == [1, 7]                      # :240-242
+    assert out["3d_manhatten_d2"] == [1, 10]                  # :258-260
+
+
+def test_raster_nodeid(ora):
+    out = SC.raster_nodeid(ora)
+    assert out["first"] == (1, 0, 5, 0)                       # test/raster.jl:301-305
+    assert out["first_values"] == (1, 0, 5, 0)
+    assert out["second"] == (1, 0, 0, 10)                     # :327-331
+    assert out["values"] == [1, 5, 5]
+
+
+def test_raster_move_to_dist(ora):
+    out = SC.raster_move_to_dist(ora)
+    assert out["2d"] == [25, 13, 21]                          # test/raster.jl:374-376
+    assert out["4d"] == [3 * 3 * 3 * 3, 1 + 4 * 2]            # :401-402
+
+
+def test_raster_random(ora):
+    pos, cellpos = SC.raster_random(ora)                      # test/raster.jl:427-433
+    assert pos == (7, 19, 23) and cellpos == (7, 19, 23)
+
+
+def test_raster_stencils(vh):
+    """Stencil sizes from their closed forms: the Chebyshev cube (2d+1)^n - 1;
+    the Manhattan ball: 2n points at d = 1, 12 at d = 2 in 2D; the Euclidean
+    ball of radius 1.5 in 3D: the 3x3x3 cube without the origin and the 8
+    corners at sqrt(3); radius 2.5 in 2D: the 5x5 square without the origin
+    and the 4 corners at sqrt(8). Offsets run first component fastest
+    (src/Raster.jl:80-96)."""
+    st = vh.raster.stencil
+    for n in (1, 2, 3, 4):
+        for d in (1, 2):
+            assert len(st("chebyshev", n, d)) == (2 * d + 1) ** n - 1
+        assert len(st("manhatten", n, 1)) == 2 * n
+    assert len(st("manhatten", 2, 2)) == 12
+    assert len(st("euclidean", 3, 1.5)) == 26 - 8
+    assert len(st("euclidean", 2, 2.5)) == 20               # 5x5 minus origin minus 4 corners
+    assert st("chebyshev", 2, 1)[:3] == [(-1, -1), (0, -1), (1, -1)]

@@ -38,6 +38,7 @@ EDGE_HINTS = {"Stateless": HINT_STATELESS, "IgnoreFrom": HINT_IGNOREFROM,
 
 TR_NOOP, TR_SUM_NEIGHBORS, TR_DIE_IF, TR_ADD_SELF_EDGE = 0, 1, 2, 3
 TR_READD_EDGES, TR_NEIGHBOR_FOO_EDGES, TR_SPAWN, TR_COUNT_EDGES = 4, 5, 6, 7
+TR_DIFFUSE, TR_MOVE_TO = 8, 9
 TR_SIR, TR_OPINION_F64, TR_OPINION_F32, TR_CONTACT = 16, 17, 18, 19
 
 CMP = {"always": 0, "never": 1, "lt": 2, "ge": 3, "eq": 4, "odd": 5, "even": 6,
@@ -68,6 +69,12 @@ class MapDesc(C.Structure):
 
 class TestParams(C.Structure):
     _fields_ = [("value", C.c_int64), ("cmp", C.c_int32), ("pad", C.c_uint32),
+                ("edge_state", C.c_uint8 * 16)]
+
+
+class RasterParams(C.Structure):
+    _fields_ = [("field_offset", C.c_uint32), ("cell_type", C.c_uint32), ("first_nr", C.c_uint64),
+                ("dims", C.c_int64 * 4), ("ndims", C.c_uint32), ("flags", C.c_uint32),
                 ("edge_state", C.c_uint8 * 16)]
 
 
@@ -121,6 +128,9 @@ SIGNATURES = [
 DEVICE_ONLY = [
     ("apply_n", C.c_int, [_vp, C.POINTER(ApplyDesc), C.c_uint32]),
     ("add_edges_gnp", C.c_int, [_vp, C.c_uint8, C.c_double, C.c_uint64, C.c_uint64]),
+    ("add_edges_stencil", C.c_int, [_vp, C.c_uint8, C.c_uint32, C.c_uint64, C.c_uint32,
+                                    C.POINTER(C.c_int64), C.c_double, C.c_int, C.c_int,
+                                    C.POINTER(C.c_uint8)]),
     ("sync", C.c_int, [_vp]),
     ("set_stream", C.c_int, [_vp, _vp]),
     ("set_io_stream", C.c_int, [_vp, _vp]),

@@ -247,6 +247,62 @@ struct Body<VH_TR_COUNT_EDGES> {
     }
 };
 
+// diffuse test/raster.jl:38-41 (diffuse3D :143-146): the Bool field at
+// `field_offset` becomes true when it is, or when any in-neighbour's is
+template <>
+struct Body<VH_TR_DIFFUSE> {
+    __device__ static bool run(const DCtx* c, const uint8_t* st, uint64_t id, Emitter&, uint8_t* out) {
+        const vh_raster_params* p = (const vh_raster_params*)c->params;
+        const uint32_t f = p->field_offset;
+        uint8_t v = st[f] ? 1 : 0;
+        int et = c->role_e[0];
+        uint32_t b, e;
+        if (!v && d_row(c, et, id, b, e)) {
+            const DEdge& E = c->e[et];
+            for (uint32_t k = b; k < e && !v; k++) {
+                const uint8_t* ns = d_state(c, vh_src_id(E, k));
+                if (!ns) return true;
+                v = ns[f] ? 1 : 0;
+            }
+        }
+        out[f] = v;
+        return true;
+    }
+};
+// value_on_pos test/raster.jl:279-285: v = the Int64 at offset 0 of the
+// first in-neighbour over E; move_to!(sim, raster, id, (v, ..., v), E(), E())
+// with distance 0 (src/Raster.jl:437-463): edge cell -> id, then id -> cell;
+// the new state is v at offset 0
+template <>
+struct Body<VH_TR_MOVE_TO> {
+    __device__ static bool run(const DCtx* c, const uint8_t*, uint64_t id, Emitter& em, uint8_t* out) {
+        const vh_raster_params* p = (const vh_raster_params*)c->params;
+        int et = c->role_e[0];
+        uint32_t b, e;
+        if (!d_row(c, et, id, b, e)) {   // first(nothing) fails in the reference
+            vh_flag(c, VH_DERR_TARGET);
+            return true;
+        }
+        const uint8_t* ns = d_state(c, vh_src_id(c->e[et], b));
+        if (!ns) return true;
+        const int64_t v = ld_i64(ns);
+        uint64_t lin = 0, stride = 1;
+        for (uint32_t d = 0; d < p->ndims && d < 4; d++) {
+            if (v < 1 || v > p->dims[d]) {   // raster[pos] out of bounds
+                vh_flag(c, VH_DERR_TARGET);
+                return true;
+            }
+            lin += (uint64_t)(v - 1) * stride;
+            stride *= (uint64_t)p->dims[d];
+        }
+        const uint64_t cell = vh_make_id(p->cell_type, p->first_nr + lin);
+        if (p->flags & 1u) em.edge(cell, id, p->edge_state);
+        if (p->flags & 2u) em.edge(id, cell, p->edge_state);
+        st_i64(out, v);
+        return true;
+    }
+};
+
 template <int TR>
 __global__ __launch_bounds__(256) void k_generic(const DCtx* __restrict__ c, int ci, int t, int pass,
                                                  int with_edge) {
@@ -702,6 +758,73 @@ __global__ __launch_bounds__(256) void k_contact_edges(uint64_t n, uint32_t C,
     for (uint32_t q = tid; q < end - base; q += 256) __builtin_nontemporal_store(lout[q], src + base + q);
 }
 
+// ----------------------------------------------------------------- raster stencils
+// Cell i of a raster (column-major) has coordinates c; its in-edges over a
+// stencil come from org = c - off[k] (wrapped per dimension when periodic,
+// else only when inside the raster), one per stencil offset k, ordered as
+// connect_raster_neighbors! inserts them: by org's index, then by k.
+__device__ inline void cell_coords(const StencilSpec& sp, uint64_t i, int64_t* c) {
+    for (uint32_t d = 0; d < sp.ndims; d++) {
+        c[d] = (int64_t)(i % (uint64_t)sp.dims[d]);
+        i /= (uint64_t)sp.dims[d];
+    }
+}
+__device__ inline bool stencil_org(const StencilSpec& sp, const int64_t* c, uint32_t k, uint64_t& org) {
+    uint64_t lin = 0, stride = 1;
+    for (uint32_t d = 0; d < sp.ndims; d++) {
+        int64_t o = c[d] - (int64_t)sp.off[k][d];
+        if (o < 0 || o >= sp.dims[d]) {
+            if (!sp.periodic) return false;
+            o %= sp.dims[d];
+            if (o < 0) o += sp.dims[d];
+        }
+        lin += (uint64_t)o * stride;
+        stride *= (uint64_t)sp.dims[d];
+    }
+    org = lin;
+    return true;
+}
+__global__ void k_stencil_degree(StencilSpec sp, uint64_t ncells, uint64_t rowbase, uint32_t* __restrict__ deg) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells) return;
+    int64_t c[4];
+    cell_coords(sp, i, c);
+    uint32_t n = 0;
+    uint64_t org;
+    for (uint32_t k = 0; k < sp.nst; k++) n += stencil_org(sp, c, k, org) ? 1u : 0u;
+    deg[rowbase + i] = n;
+}
+__global__ void k_stencil_fill(StencilSpec sp, uint64_t ncells, uint64_t rowbase,
+                               const uint32_t* __restrict__ off, void* __restrict__ src, uint32_t src64,
+                               uint8_t* __restrict__ st, uint32_t bytes) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells) return;
+    int64_t c[4];
+    cell_coords(sp, i, c);
+    // keys (org << 8 | k), insertion-sorted; offsets are visited last to
+    // first, so inside the raster (org falling with k) each key appends
+    uint64_t key[VH_STENCIL_MAX];
+    uint32_t n = 0;
+    for (int k = (int)sp.nst - 1; k >= 0; k--) {
+        uint64_t org;
+        if (!stencil_org(sp, c, (uint32_t)k, org)) continue;
+        const uint64_t kk = (org << 8) | (uint64_t)k;
+        uint32_t j = n++;
+        while (j > 0 && key[j - 1] > kk) {
+            key[j] = key[j - 1];
+            j--;
+        }
+        key[j] = kk;
+    }
+    const uint64_t o = off[rowbase + i];
+    for (uint32_t j = 0; j < n; j++) {
+        const uint64_t nr = sp.first_nr + (key[j] >> 8);
+        if (src64) ((uint64_t*)src)[o + j] = vh_make_id(sp.cell_type, nr);
+        else ((uint32_t*)src)[o + j] = (uint32_t)nr;
+        for (uint32_t b = 0; b < bytes; b++) st[(o + j) * bytes + b] = sp.state[b];
+    }
+}
+
 }  // namespace
 
 // ====================================================================== launchers
@@ -724,6 +847,8 @@ int launch_generic(int tr, const DCtx* d_ctx, int ci, int call_t, uint64_t nslot
         CASE(VH_TR_NEIGHBOR_FOO_EDGES)
         CASE(VH_TR_SPAWN)
         CASE(VH_TR_COUNT_EDGES)
+        CASE(VH_TR_DIFFUSE)
+        CASE(VH_TR_MOVE_TO)
 #undef CASE
     default:
         return set_error(VH_ERR_UNREGISTERED, "transition %d has no generic kernel", tr);
@@ -805,6 +930,29 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
                            (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(), A.nslots,
                            nrows, E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nlight, hot);
     HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+int stencil_csr(Sim& s, EdgeStore& E, uint64_t rowbase, uint64_t rows) {
+    const StencilSpec& sp = E.sspec;
+    hipStream_t st = s.stream;
+    uint64_t ncells = 1;
+    for (uint32_t d = 0; d < sp.ndims; d++) ncells *= (uint64_t)sp.dims[d];
+    uint32_t* deg = (uint32_t*)s.scratch.take(rows * 4 + 8);
+    if (!deg) return set_error(VH_ERR_OOM, "scratch");
+    fill_u32(deg, 0, rows, st);
+    hipLaunchKernelGGL(k_stencil_degree, dim3(grid_for(ncells, 256)), dim3(256), 0, st, sp, ncells, rowbase, deg);
+    RC(E.off.ensure((rows + 1) * 4 + 8, st));
+    if (scan_excl_u32(deg, E.off.as<uint32_t>(), rows, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+    HIPCHECK(hipMemcpyAsync(s.d_stats.p, E.off.as<uint32_t>() + rows, 4, hipMemcpyDeviceToDevice, st));
+    RC(readback(s, 1));
+    uint64_t ne = (uint32_t)s.h_stats[0];
+    RC(E.src.ensure(ne * (E.src64 ? 8 : 4) + 8, st));
+    if (E.bytes) RC(E.st.ensure(ne * E.bytes + 8, st));
+    hipLaunchKernelGGL(k_stencil_fill, dim3(grid_for(ncells, 256)), dim3(256), 0, st, sp, ncells, rowbase,
+                       E.off.as<uint32_t>(), E.src.p, (uint32_t)E.src64, E.st.as<uint8_t>(), E.bytes);
+    HIPCHECK(hipGetLastError());
+    E.nedges = ne;
     return VH_OK;
 }
 

@@ -43,6 +43,17 @@ struct AgentStore {
     uint8_t* wstate() const { return indep ? state[rd].as<uint8_t>() : state[rd ^ 1].as<uint8_t>(); }
 };
 
+// a raster stencil (connect_raster_neighbors!): cells are the agents
+// first_nr .. first_nr + prod(dims) - 1 of cell_type in column-major order
+constexpr int VH_STENCIL_MAX = 128;
+struct StencilSpec {
+    int64_t dims[4];
+    uint64_t first_nr;
+    uint32_t cell_type, ndims, nst, periodic;
+    int8_t off[VH_STENCIL_MAX][4];   // offsets in stencil order
+    uint8_t state[16];
+};
+
 struct EdgeStore {
     uint32_t bytes = 0, hints = 0;
     int32_t target = 0;
@@ -59,6 +70,9 @@ struct EdgeStore {
     DevBuf heavy;                  // uint32 rows with degree > 64
     uint64_t nheavy = 0, bins_version = ~0ull;
     bool writeable = false;
+    // in-edges requested as a raster stencil during init (vh_add_edges_stencil)
+    bool stencil = false;
+    StencilSpec sspec;
     // in-edges requested as a device G(n, p) during init (vh_add_edges_gnp)
     bool gen = false;
     double gen_mean_deg = 0.0;
@@ -103,6 +117,9 @@ int contact_apply(Sim& s, int t, int et, const vh_contact_params& p, uint64_t st
 // 1-based numbers; one host synchronisation
 int gnp_device(uint64_t n, double mean_deg, uint64_t seed, DevBuf& off, DevBuf& src, DevBuf& cdf,
                Scratch& sc, hipStream_t st, uint64_t* nedges);
+// CSR of a raster stencil (vh_bodies.hip): off[rows + 1] with the cell rows
+// at rowbase, sources as AgentIDs (src64) or numbers, uniform states
+int stencil_csr(Sim& s, EdgeStore& E, uint64_t rowbase, uint64_t rows);
 // Float32 weights in (0, 1) by CSR position (vh_ensemble.hip)
 void fill_weights(float* w, uint64_t n, uint64_t wseed, hipStream_t st);
 // engine services the bodies use

@@ -278,7 +278,8 @@ extern "C" int vh_add_edges(vh_sim_t h, uint8_t et, const uint64_t* from, const 
                          "You can call add_edge! only in the initialization phase (until "
                          "`finish_init!` is called) or within a transition function called by `apply`.");
     EdgeStore& E = s.e[et];
-    if (E.gen) return set_error(VH_ERR_ARG, "the in-edges of this edge type are device-generated");
+    if (E.gen || E.stencil)
+        return set_error(VH_ERR_ARG, "the in-edges of this edge type are device-generated");
     const bool ignorefrom = has(E.hints, VH_HINT_IGNOREFROM);
     const bool stateless = has(E.hints, VH_HINT_STATELESS);
     const bool singletype = has(E.hints, VH_HINT_SINGLETYPE);
@@ -688,6 +689,91 @@ extern "C" int vh_add_edges_gnp(vh_sim_t h, uint8_t et, double mean_deg, uint64_
     return VH_OK;
 }
 
+// connect_raster_neighbors! (src/Raster.jl:134-164) as one request: the
+// stencil edges of a raster whose cells are consecutive agents of one type,
+// generated on the device at finish_init! (the CSR that adding them one by
+// one gives). Offsets as `_stencil_core` (src/Raster.jl:80-96): the cube
+// [-d, d]^n, d = floor(distance), first component fastest, without the
+// origin, Euclidean / Manhattan filtered by <= distance.
+extern "C" int vh_add_edges_stencil(vh_sim_t h, uint8_t et, uint32_t cell_type, uint64_t first_nr,
+                                    uint32_t ndims, const int64_t* dims, double distance, int metric,
+                                    int periodic, const uint8_t* edge_state) {
+    if (!h || !dims) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (et >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT, "connect_raster_neighbors! before finish_init! only");
+    EdgeStore& E = s.e[et];
+    if (!E.h_to.empty() || E.gen || E.stencil)
+        return set_error(VH_ERR_ARG, "raster stencil: the edge type already has edges");
+    if (has(E.hints, VH_HINT_IGNOREFROM) || has(E.hints, VH_HINT_SINGLEEDGE) || E.bytes > 16)
+        return set_error(VH_ERR_ARG, "raster stencil: sources kept, many edges, state <= 16 bytes");
+    if (cell_type < 1 || (int)cell_type > s.m->na) return set_error(VH_ERR_UNREGISTERED, "cell type");
+    if (has(E.hints, VH_HINT_SINGLETYPE) && E.target != (int32_t)cell_type)
+        return set_error(VH_ERR_ASSERT, "raster stencil: the edge type targets another agent type");
+    if (ndims < 1 || ndims > 4) return set_error(VH_ERR_ARG, "raster stencil: 1 to 4 dimensions");
+    StencilSpec sp;
+    memset(&sp, 0, sizeof sp);
+    uint64_t ncells = 1;
+    for (uint32_t d = 0; d < 4; d++) {
+        sp.dims[d] = d < ndims ? dims[d] : 1;
+        if (sp.dims[d] < 1) return set_error(VH_ERR_ARG, "raster stencil: empty dimension");
+        ncells *= (uint64_t)sp.dims[d];
+    }
+    if (first_nr < 1 || first_nr - 1 + ncells > s.a[cell_type].nextid - 1)
+        return set_error(VH_ERR_ARG, "raster stencil: the cells are not all added");
+    if (metric < 0 || metric > 2 || !(distance >= 0.0) || distance > 127.0)
+        return set_error(VH_ERR_ARG, "raster stencil: metric / distance");
+    const int64_t d = (int64_t)floor(distance);
+    int64_t side = 2 * d + 1, total = 1;
+    for (uint32_t k = 0; k < ndims; k++) total *= side;
+    uint32_t nst = 0;
+    for (int64_t idx = 0; idx < total; idx++) {
+        int64_t o[4] = {0, 0, 0, 0}, q = idx, sq = 0, sa = 0;
+        bool origin = true;
+        for (uint32_t k = 0; k < ndims; k++) {
+            o[k] = q % side - d;
+            q /= side;
+            origin = origin && o[k] == 0;
+            sq += o[k] * o[k];
+            sa += o[k] < 0 ? -o[k] : o[k];
+        }
+        if (origin) continue;
+        if (metric == 1 && sqrt((double)sq) > distance) continue;
+        if (metric == 2 && (double)sa > distance) continue;
+        if (nst >= (uint32_t)VH_STENCIL_MAX)
+            return set_error(VH_ERR_CAPACITY, "raster stencil: more than %d offsets", VH_STENCIL_MAX);
+        for (uint32_t k = 0; k < 4; k++) sp.off[nst][k] = (int8_t)o[k];
+        nst++;
+    }
+    sp.nst = nst;
+    sp.ndims = ndims;
+    sp.first_nr = first_nr;
+    sp.cell_type = cell_type;
+    sp.periodic = periodic ? 1u : 0u;
+    if (edge_state && E.bytes) memcpy(sp.state, edge_state, E.bytes);
+    E.sspec = sp;
+    E.stencil = true;
+    return VH_OK;
+}
+
+static int build_stencil(Sim& s, EdgeStore& E) {
+    RowLayout L;
+    uint64_t rows;
+    compute_layout(s, E, L, rows);
+    if (rows >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "more than 2^32 target rows");
+    const StencilSpec& sp = E.sspec;
+    RC(stencil_csr(s, E, L.base[sp.cell_type] + sp.first_nr - 1, rows));
+    for (int t = 0; t <= VH_MAXA; t++) {
+        E.rowbase[t] = L.base[t];
+        E.nrows[t] = L.nrows[t];
+    }
+    E.rows = rows;
+    E.version++;
+    E.stencil = false;
+    return VH_OK;
+}
+
 // the CSR of a device-generated edge type: rows = the agents of type 1
 static int build_gnp_csr(Sim& s, EdgeStore& E) {
     hipStream_t st = s.stream;
@@ -761,6 +847,8 @@ extern "C" int vh_finish_init(vh_sim_t h) {
         if (s.m->na != 1) E.src64 = true;
         if (E.gen)
             RC(build_gnp_csr(s, E));
+        else if (E.stencil)
+            RC(build_stencil(s, E));
         else
             RC(build_csr(s, E, false, dto, dfrom, dst, n));
         HIPCHECK(hipStreamSynchronize(st));
@@ -821,8 +909,10 @@ enum Kind { K_PLAIN = 0, K_TWOPASS = 1, K_SIR = 2, K_OPINION = 3, K_CONTACT = 4 
 static int kind_of(uint32_t tr) {
     switch (tr) {
     case VH_TR_NOOP: case VH_TR_SUM_NEIGHBORS: case VH_TR_DIE_IF: case VH_TR_COUNT_EDGES:
+    case VH_TR_DIFFUSE:
         return K_PLAIN;
     case VH_TR_ADD_SELF_EDGE: case VH_TR_READD_EDGES: case VH_TR_NEIGHBOR_FOO_EDGES: case VH_TR_SPAWN:
+    case VH_TR_MOVE_TO:
         return K_TWOPASS;
     case VH_TR_SIR: return K_SIR;
     case VH_TR_OPINION_F64: case VH_TR_OPINION_F32: return K_OPINION;

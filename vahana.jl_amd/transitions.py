@@ -104,3 +104,35 @@ def contact(edgetype, beta, gamma, p_death, seed, n_contacts=20, max_redraw=16):
     p = _abi.ContactParams(beta=float(beta), gamma=float(gamma), p_death=float(p_death),
                            seed=int(seed), n_contacts=int(n_contacts), max_redraw=int(max_redraw))
     return Transition(_abi.TR_CONTACT, "contact", role_edges=[edgetype], params=p)
+
+
+def diffuse(edgetype, T, field="active"):
+    """a.field || any(n.field for n in neighborstates(sim, id, edgetype)) on a
+    Bool field of T (test/raster.jl:38-41, 143-146)."""
+    p = _abi.RasterParams()
+    p.field_offset = T.dtype.fields[field][1]
+    return Transition(_abi.TR_DIFFUSE, "diffuse", role_edges=[edgetype], params=p)
+
+
+def move_to_value(sim, raster, edgetype, from_raster=True, to_raster=True, edge_state=None):
+    """value_on_pos (test/raster.jl:279-285): v = the Int64 at offset 0 of the
+    first in-neighbour over `edgetype`; move_to!(sim, raster, id, (v, ..., v),
+    ...) with distance 0; the new state is v. The raster's cells must be
+    consecutive agents of one type (as add_raster_ makes them)."""
+    from .raster import _rasters
+    R = _rasters(sim)[raster]
+    contig = R.contiguous()
+    if contig is None or len(R.dims) > 4:
+        raise AssertionError("move_to_value needs a raster of consecutive cells, <= 4 dimensions")
+    p = _abi.RasterParams()
+    p.cell_type, p.first_nr = contig
+    p.ndims = len(R.dims)
+    for i, d in enumerate(R.dims):
+        p.dims[i] = d
+    p.flags = (1 if from_raster else 0) | (2 if to_raster else 0)
+    raw = b"" if edge_state is None else bytes(edge_state)
+    if len(raw) > 16:
+        raise AssertionError("edge states of more than 16 bytes")
+    for i, b in enumerate(raw):
+        p.edge_state[i] = b
+    return Transition(_abi.TR_MOVE_TO, "move_to_value", role_edges=[edgetype], params=p)

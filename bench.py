@@ -60,6 +60,11 @@ def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0):
         m = M.build_contact(ora, n, seed=seed)
         step = lambda: M.step_contact(ora, m, seed=seed)  # noqa: E731
         ne = 20 * n
+    elif workload == "raster":
+        m = M.build_raster_diffusion(ora, 1000, seed=seed)
+        n = m.n
+        step = lambda: M.step_raster_diffusion(ora, m)  # noqa: E731
+        ne = 8 * n
     else:
         edges = vh.inputs.graph_er(n, 10.0 / (n - 1), seed)
         m = M.build_sir(ora, n, seed=seed, edges=edges)
@@ -131,8 +136,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--workload", default="opinion",
-                    choices=["opinion", "opinion32", "sir", "contact", "ensemble", "serve",
-                             "gather_floor"])
+                    choices=["opinion", "opinion32", "sir", "contact", "raster", "ensemble",
+                             "serve", "gather_floor"])
     ap.add_argument("--replicas", type=int, default=1024)
     ap.add_argument("--requests", type=int, default=60)
     ap.add_argument("--rate", type=float, default=0.0,
@@ -177,6 +182,20 @@ def main():
         dtype = "f32" if f32 else "f64"
         workload = (f"opinion dynamics (weighted bounded confidence), Barabasi-Albert m=5 symmetrised, "
                     f"N={n}, E={m.nedges}, eps=0.2, {'Float32' if f32 else 'Float64'} opinions")
+    elif args.workload == "raster":
+        import math
+        side = int(round(math.sqrt(n)))
+        n = side * side
+        edges = None
+        m = M.build_raster_diffusion(vh, side, seed=1)
+        step = lambda: M.step_raster_diffusion(vh, m)  # noqa: E731
+        m.nedges = 8 * n
+        # cell state read + written (1 B each), offsets, one source per edge;
+        # neighbour states count once as part of the state read
+        nbytes = n * (1 + 1) + (n + 1) * 4 + 4 * 8 * n
+        dtype = "u8"
+        workload = (f"raster diffusion (test/raster.jl diffuse), {side}x{side} periodic Moore "
+                    f"neighbourhood, N={n}, stencil edges built on the device")
     elif args.workload == "contact":
         edges = None
         m = M.build_contact(vh, n, seed=1)

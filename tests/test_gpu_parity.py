@@ -220,3 +220,20 @@ def test_raster_device_stencil_csr(vh, gpu_ready, case):
     (a, b, c), (x, y, z) = res
     assert len(a) > 0
     assert np.array_equal(a, x) and np.array_equal(b, y) and np.array_equal(c, z)
+
+
+def test_raster_diffusion_model_parity(vh, ora, gpu_ready):
+    """The benchmark's raster diffusion (device stencil, diffuse body) equals
+    the checker's (host-added stencil edges) step by step."""
+    M = vh.models
+    g = M.build_raster_diffusion(vh, 300, seed=4, frac=0.002)
+    c = M.build_raster_diffusion(ora, 300, seed=4, frac=0.002)
+    tg, fg, _ = g.sim._read_edges(g.E)
+    tc, fc, _ = c.sim._read_edges(c.E)
+    assert np.array_equal(tg, tc) and np.array_equal(fg, fc)
+    for step in range(6):
+        M.step_raster_diffusion(vh, g)
+        M.step_raster_diffusion(ora, c)
+        a, _ = _agents(g)
+        b, _ = _agents(c)
+        assert np.array_equal(_bytes(a), _bytes(b)), step

@@ -11,6 +11,9 @@ apply! each.
   graph.
 * Dynamic contacts: mortal `Person` agents {health::UInt8, susc::Float32},
   `Met` edges (:Stateless, :SingleType) rebuilt by every step.
+* Raster diffusion: a side x side periodic raster of `Cell` agents
+  (:Immortal) {active::Bool}, Moore neighbourhood `Adj` edges (:Stateless,
+  :SingleType) from the device stencil; a step is test/raster.jl's diffuse.
 """
 import types
 
@@ -140,3 +143,28 @@ def build_contact(V, n, seed=1, n_init=None):
 def step_contact(V, m, beta=0.05, gamma=0.1, p_death=0.02, seed=1, n_contacts=20, max_redraw=16):
     V.apply_(m.sim, V.transitions.contact(m.E, beta, gamma, p_death, seed, n_contacts, max_redraw),
              m.T, [m.T, m.E], [m.T, m.E])
+
+
+def build_raster_diffusion(V, side, seed=1, frac=0.01, device_stencil=True):
+    """`side` x `side` periodic raster, a cell active when
+    u01(hash(seed, id, 0, KEY_INIT)) < frac."""
+    Cell = V.struct("Cell", [("active", "?")])
+    Adj = V.struct("Adj")
+    model = (V.ModelTypes() | V.register_agenttype_(Cell, "Immortal")
+             | V.register_edgetype_(Adj, "Stateless", target=Cell) | V.create_model("raster"))
+    sim = V.create_simulation(model)
+    n = side * side
+    u = np.empty(n, dtype=np.float64)
+    inputs._abi.lib().fill_agent_u01(seed, 1, 0, inputs.KEY_INIT, n,
+                                     u.ctypes.data_as(inputs.C.POINTER(inputs.C.c_double)))
+    st = np.zeros(n, dtype=Cell.dtype)
+    st["active"] = u < frac
+    V.raster.add_raster_(sim, "grid", (side, side), st, T=Cell)
+    # on the device the stencil is generated there (the checker adds the edges)
+    V.raster.connect_raster_neighbors_(sim, "grid", Adj(), device=None if device_stencil else False)
+    V.finish_init_(sim)
+    return types.SimpleNamespace(sim=sim, T=Cell, E=Adj, n=n)
+
+
+def step_raster_diffusion(V, m):
+    V.apply_(m.sim, V.transitions.diffuse(m.E, m.T), m.T, [m.T, m.E], m.T)

@@ -65,13 +65,21 @@ def positions(dims):
         yield tuple(reversed(t))
 
 
-def add_raster_(sim, name, dims, agent_constructor):
+def add_raster_(sim, name, dims, agent_constructor, T=None):
     """add_raster! src/Raster.jl:32-56: one agent per cell, built by
-    `agent_constructor(pos)`, added in column-major order; returns the ids."""
+    `agent_constructor(pos)`, added in column-major order; returns the ids.
+    Bulk form: `agent_constructor` a structured array of T with the cells'
+    states in column-major order."""
     from . import add_agent_, add_agents_, _type_of_value
     if sim.initialized:
         raise AssertionError("add_raster! can be only called before finish_init!")
     dims = tuple(int(d) for d in dims)
+    if isinstance(agent_constructor, np.ndarray):
+        if T is None or len(agent_constructor) != int(np.prod(dims)):
+            raise AssertionError("bulk add_raster_: the type and one state per cell")
+        ids = sim._add_agents_array(T, agent_constructor)
+        _rasters(sim)[name] = Raster(np.asarray(ids, dtype=np.uint64).reshape(dims, order="F"))
+        return ids
     values = [agent_constructor(p) for p in positions(dims)]
     types = {id(_type_of_value(sim, v)) for v in values}
     if len(types) == 1:
@@ -145,23 +153,47 @@ def connect_raster_neighbors_(sim, name, edge_constructor, distance=1, metric="c
                 raise AssertionError("the device stencil needs consecutive cells of one agent "
                                      "type, at most 4 dimensions and an edge state of <= 16 bytes")
             return _device_stencil(sim, T, R, contig, distance, metric, periodic, raw)
-    froms, tos, states, T = [], [], [], None
-    for org in positions(dims):
-        for s in st:
-            shifted = _checkpos(tuple(o + x for o, x in zip(org, s)), dims, periodic)
-            if shifted is None:
-                continue
-            v = edge_constructor(org, shifted) if not uniform else edge_constructor
-            if T is None:
-                T = v.T if isinstance(v, Value) else v[0]
-            froms.append(R.ids[tuple(p - 1 for p in org)])
-            tos.append(R.ids[tuple(p - 1 for p in shifted)])
-            if T.fields:
-                states.append(v.v if isinstance(v, Value) else v[1])
-    if T is None:
+    org, tgt = _stencil_pairs(dims, st, periodic)
+    if len(org) == 0:
         return
-    sim._add_edges_array(T, np.array(froms, dtype=np.uint64), np.array(tos, dtype=np.uint64),
-                         states if T.fields else None)
+    froms = R.ids.reshape(-1, order="F")[org]
+    tos = R.ids.reshape(-1, order="F")[tgt]
+    if uniform:
+        v = edge_constructor
+        T = v.T if isinstance(v, Value) else v[0]
+        one = v.v if isinstance(v, Value) else v[1]
+        states = [one] * len(org) if T.fields else None
+    else:
+        vals = [edge_constructor(_pos(o, dims), _pos(t, dims)) for o, t in zip(org, tgt)]
+        T = vals[0].T if isinstance(vals[0], Value) else vals[0][0]
+        states = [v.v if isinstance(v, Value) else v[1] for v in vals] if T.fields else None
+    sim._add_edges_array(T, froms, tos, states)
+
+
+def _pos(lin, dims):
+    return tuple(int(x) + 1 for x in np.unravel_index(int(lin), dims, order="F"))
+
+
+def _stencil_pairs(dims, st, periodic):
+    """(org, target) cell indices (column-major) of every stencil edge, in
+    insertion order: org ascending, then stencil order (src/Raster.jl:151-161)."""
+    n = int(np.prod(dims))
+    if not st:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    D = np.array(dims, dtype=np.int64)
+    S = np.array(st, dtype=np.int64)                       # (nst, nd)
+    coords = np.stack(np.unravel_index(np.arange(n), dims, order="F"), axis=1)   # (n, nd)
+    sh = coords[:, None, :] + S[None, :, :]                # (n, nst, nd)
+    if periodic:
+        sh %= D
+        valid = np.ones(sh.shape[:2], dtype=bool)
+    else:
+        valid = ((sh >= 0) & (sh < D)).all(axis=2)
+        sh = np.where(valid[:, :, None], sh, 0)
+    strides = np.cumprod(np.concatenate([[1], D[:-1]]))
+    tgt = (sh * strides).sum(axis=2)
+    org = np.broadcast_to(np.arange(n)[:, None], tgt.shape)
+    return org[valid], tgt[valid]
 
 
 def _device_stencil(sim, T, R, contig, distance, metric, periodic, raw):

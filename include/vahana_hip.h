@@ -247,6 +247,13 @@ int vh_stream_destroy(void* hip_stream);
 int vh_set_io_stream(vh_sim_t s, void* hip_stream);
 /* sim.num_transitions (src/Simulation.jl:473,816) */
 int vh_num_transitions(vh_sim_t s, uint64_t* out);
+/* read_snapshot! src/HDF5.jl:1594-1646 (_read_agents_restore! :801-850), after
+ * finish_init!: the died flags of the n slots of agent type t; the reuse
+ * stack becomes the died slots in ascending order, as the reference
+ * rebuilds it */
+int vh_restore_died(vh_sim_t sim, uint8_t t, const uint8_t* died, uint64_t n);
+/* read_snapshot! sets sim.num_transitions to the snapshot's transition */
+int vh_set_num_transitions(vh_sim_t sim, uint64_t n);
 /* mapreduce(sim, f, op, T; init, datatype) — kind 0: agents, 1: edges */
 int vh_mapreduce(vh_sim_t s, int kind, uint8_t type_id, const vh_map_desc* f,
                  int op, int dtype, const void* init, void* out);

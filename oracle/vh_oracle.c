@@ -1112,6 +1112,36 @@ int or_apply(or_sim* s, const vh_apply_desc* d) {
 
 int or_num_transitions(or_sim* s, uint64_t* out) { *out = s->num_transitions; return VH_OK; }
 
+/* read_snapshot! src/HDF5.jl:1594-1646 with _read_agents_restore! :801-850:
+ * the died flags of the n slots of type t; the reuseable slots become the
+ * died ones in ascending order (not the order they died in) */
+int or_restore_died(or_sim* s, uint8_t t, const uint8_t* died, uint64_t n) {
+    if (t < 1 || t > s->m->na) return fail(VH_ERR_UNREGISTERED, "unregistered agent type");
+    if (!s->initialized) return fail(VH_ERR_ASSERT, "restore after finish_init! only");
+    agents_t* A = &s->a[t];
+    if (n != A->read.len) return fail(VH_ERR_ARG, "restore: %llu slots, the type has %llu",
+                                      (unsigned long long)n, (unsigned long long)A->read.len);
+    if (!mortal_t(s, t)) {
+        for (uint64_t i = 0; i < n; i++)
+            if (died[i]) return fail(VH_ERR_ASSERT, "restore: a died agent of an :Immortal type");
+        return VH_OK;
+    }
+    /* the reference reads the flags into write.died and publishes them
+     * (finish_write!), so both buffers hold them */
+    if (A->write.len < n) rw_resize(&A->write, n, s->m->a[t].bytes, 1);
+    A->read.reuse.n = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        A->read.died[i] = died[i] ? 1 : 0;
+        A->write.died[i] = A->read.died[i];
+        if (died[i]) u64_push(&A->read.reuse, i + 1);
+    }
+    return VH_OK;
+}
+int or_set_num_transitions(or_sim* s, uint64_t n) {
+    s->num_transitions = n;
+    return VH_OK;
+}
+
 /* ------------------------------------------------------------ queries */
 /* num_agents src/Agent.jl:324-343 */
 int or_num_agents(or_sim* s, uint8_t t, uint64_t* out) {

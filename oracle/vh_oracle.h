@@ -33,6 +33,8 @@ int or_add_edges(or_sim* s, uint8_t et, const uint64_t* from, const uint64_t* to
 int or_finish_init(or_sim* s);
 int or_apply(or_sim* s, const vh_apply_desc* d);
 int or_num_transitions(or_sim* s, uint64_t* out);
+int or_restore_died(or_sim* s, uint8_t t, const uint8_t* died, uint64_t n);
+int or_set_num_transitions(or_sim* s, uint64_t n);
 /* order: 0 = the reference's sequential left fold, 1 = the device's fixed tree */
 int or_mapreduce(or_sim* s, int kind, uint8_t t, const vh_map_desc* f, int op,
                  int dtype, const void* init, void* out, int order);

@@ -77,6 +77,9 @@ register_agenttype_ = vh.register_agenttype_
 register_edgetype_ = vh.register_edgetype_
 transitions = vh.transitions
 raster = vh.raster
+models = vh.models
+inputs = vh.inputs
+snapshot = vh.snapshot
 for _name in ["add_agent_", "add_agents_", "add_edge_", "add_edges_", "finish_init_", "apply_",
               "apply", "mapreduce", "num_agents", "num_edges", "all_agents", "all_agentids",
               "agentstate", "edges", "neighborids", "edgestates", "has_edge", "copy_simulation",

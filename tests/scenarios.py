@@ -662,3 +662,75 @@ def raster_random(V):
     out = (pos, tuple(int(x) for x in V.agentstate(sim, cell, Grid3D)["pos"]))
     V.finish_simulation_(sim)
     return out
+
+
+# ---------------------------------------------------------------- snapshots
+def sir_model(V):
+    """The SIR benchmark model's types (models.build_sir)."""
+    SIR, Contact, *_ = V.models._types(V)
+    model = (V.ModelTypes() | V.register_agenttype_(SIR, "Immortal")
+             | V.register_edgetype_(Contact, "Stateless", target=SIR) | V.create_model("sir"))
+    return model, SIR, Contact
+
+
+def contact_model(V):
+    """The contact model's types (models.build_contact)."""
+    *_, Person, Met = V.models._types(V)
+    model = (V.ModelTypes() | V.register_agenttype_(Person)
+             | V.register_edgetype_(Met, "Stateless", target=Person) | V.create_model("contact"))
+    return model, Person, Met
+
+
+def snapshot_sir_resume(V, path):
+    """write_snapshot / read_snapshot! (src/HDF5.jl:688-708, 1594-1646) on an
+    immortal model: a run resumed from a snapshot equals the uninterrupted
+    run, and the transition counter resumes."""
+    import types
+    M = V.models
+    edges = V.inputs.graph_er(2000, 8.0 / 1999, 3)
+    a = M.build_sir(V, 2000, seed=2, n_init=20, edges=edges)
+    for _ in range(5):
+        M.step_sir(V, a, seed=2)
+    t = V.snapshot.write_snapshot(a.sim, path, comment="after five")
+    model, SIR, Contact = sir_model(V)
+    b = types.SimpleNamespace(sim=V.create_simulation(model), T=SIR, E=Contact)
+    t2 = V.snapshot.read_snapshot(b.sim, path)
+    resumed = b.sim._num_transitions()
+    for _ in range(5):
+        M.step_sir(V, a, seed=2)
+        M.step_sir(V, b, seed=2)
+    x, _ = a.sim._read_agents(a.T)
+    y, _ = b.sim._read_agents(b.T)
+    same = (np.ascontiguousarray(x).view(np.uint8).tobytes()
+            == np.ascontiguousarray(y).view(np.uint8).tobytes())
+    return {"t": t, "t_read": t2, "resumed": resumed, "same": same,
+            "list": V.snapshot.list_snapshots(path)}
+
+
+def snapshot_contact_restore(V, path):
+    """A mortal model restored from a snapshot: states, died flags and
+    contacts as written; the free slots rebuilt in ascending order
+    (_read_agents_restore! src/HDF5.jl:838-846)."""
+    import types
+    M = V.models
+    a = M.build_contact(V, 3000, seed=8, n_init=30)
+    for _ in range(4):
+        M.step_contact(V, a, seed=8, p_death=0.05)
+    V.snapshot.write_snapshot(a.sim, path)
+    model, Person, Met = contact_model(V)
+    b = types.SimpleNamespace(sim=V.create_simulation(model), T=Person, E=Met)
+    V.snapshot.read_snapshot(b.sim, path)
+    sa, da = a.sim._read_agents(a.T)
+    sb, db = b.sim._read_agents(b.T)
+    ea = a.sim._read_edges(a.E)
+    eb = b.sim._read_edges(b.E)
+    ra = np.ascontiguousarray(sa).view(np.uint8).reshape(len(da), -1)
+    rb = np.ascontiguousarray(sb).view(np.uint8).reshape(len(db), -1)
+    differ = (ra != rb).any(axis=1)
+    # live slots bit-identical; a died slot's bytes are not state (the
+    # device may hold stale buffer contents there, restored with zeroed
+    # padding)
+    return {"died": np.array_equal(da, db),
+            "states": not differ[~da].any() and bool(np.array_equal(differ, differ & da)),
+            "edges": np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1]),
+            "free": [int(i) + 1 for i in np.nonzero(db)[0]], "b": b}

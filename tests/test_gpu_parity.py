@@ -237,3 +237,36 @@ def test_raster_diffusion_model_parity(vh, ora, gpu_ready):
         a, _ = _agents(g)
         b, _ = _agents(c)
         assert np.array_equal(_bytes(a), _bytes(b)), step
+
+
+def test_snapshot_resume_device(vh, gpu_ready, tmp_path):
+    out = SC.snapshot_sir_resume(vh, str(tmp_path / "sir"))
+    assert out["t"] == out["t_read"] == out["resumed"] == 6
+    assert out["same"]
+    assert out["list"] == [(6, "after five")]
+
+
+def test_snapshot_cross_restore_contact(vh, ora, gpu_ready, tmp_path):
+    """A device snapshot of the mortal contact model restored into the
+    device and into the checker (free slots rebuilt ascending on both):
+    four more steps stay bit-identical, slot map included."""
+    import types
+    path = str(tmp_path / "c")
+    out = SC.snapshot_contact_restore(vh, path)
+    assert out["died"] and out["states"] and out["edges"] and out["free"]
+    g = out["b"]
+    model, Person, Met = SC.contact_model(ora)
+    c = types.SimpleNamespace(sim=ora.create_simulation(model), T=Person, E=Met)
+    ora.snapshot.read_snapshot(c.sim, path)
+    assert c.sim.reuse_stack(c.T) == out["free"]
+    M = vh.models
+    for step in range(4):
+        M.step_contact(vh, g, seed=8, p_death=0.05)
+        M.step_contact(ora, c, seed=8, p_death=0.05)
+        a, da = _agents(g)
+        b, db = _agents(c)
+        assert np.array_equal(da, db), step
+        assert np.array_equal(_bytes(a)[~da], _bytes(b)[~db]), step
+        tg, fg, _ = g.sim._read_edges(g.E)
+        tc, fc, _ = c.sim._read_edges(c.E)
+        assert np.array_equal(tg, tc) and np.array_equal(fg, fc), step

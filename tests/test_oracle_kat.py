@@ -204,3 +204,37 @@ def test_raster_stencils(vh):
     assert len(st("euclidean", 3, 1.5)) == 26 - 8
     assert len(st("euclidean", 2, 2.5)) == 20               # 5x5 minus origin minus 4 corners
     assert st("chebyshev", 2, 1)[:3] == [(-1, -1), (0, -1), (1, -1)]
+
+
+def test_snapshot_resume_checker(ora, tmp_path):
+    out = SC.snapshot_sir_resume(ora, str(tmp_path / "sir"))
+    assert out["t"] == out["t_read"] == out["resumed"] == 6   # 1 + five apply!
+    assert out["same"]
+    assert out["list"] == [(6, "after five")]
+
+
+def test_snapshot_contact_restore_checker(ora, tmp_path):
+    out = SC.snapshot_contact_restore(ora, str(tmp_path / "contact"))
+    assert out["died"] and out["states"] and out["edges"]
+    b = out["b"]
+    assert out["free"] and b.sim.reuse_stack(b.T) == out["free"]   # ascending
+    # one step on: the free slots are exactly the died ones (a restored died
+    # slot that no birth took stays died)
+    ora.models.step_contact(ora, b, seed=8, p_death=0.05)
+    _, died = b.sim._read_agents(b.T)
+    assert sorted(b.sim.reuse_stack(b.T)) == [int(i) + 1 for i in np.nonzero(died)[0]]
+
+
+def test_snapshot_raster_ids(ora, tmp_path):
+    model, GridA, Grid3D, GridE, *_ = SC.raster_model(ora)
+    sim = ora.create_simulation(model)
+    ora.raster.add_raster_(sim, "grid", (4, 3), lambda p: GridA(p, p == (2, 2)))
+    ora.raster.connect_raster_neighbors_(sim, "grid", lambda a, b: GridE())
+    ora.finish_init_(sim)
+    ora.snapshot.write_snapshot(sim, str(tmp_path / "r"))
+    sim2 = ora.create_simulation(model)
+    ora.snapshot.read_snapshot(sim2, str(tmp_path / "r"))
+    assert np.array_equal(sim2.rasters["grid"].ids, sim.rasters["grid"].ids)
+    assert ora.raster.cellid(sim2, "grid", (2, 2)) == ora.raster.cellid(sim, "grid", (2, 2))
+    ora.apply_(sim2, ora.transitions.diffuse(GridE, GridA), [GridA], [GridA, GridE], [GridA])
+    assert ora.mapreduce(sim2, "active", "+", GridA) == 9

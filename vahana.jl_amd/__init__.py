@@ -740,4 +740,4 @@ def enable_asserts(enable=True):
     _abi.lib().enable_asserts(1 if enable else 0)
 
 
-from . import inputs, models, ensemble, serve, raster  # noqa: E402,F401
+from . import inputs, models, ensemble, serve, raster, snapshot  # noqa: E402,F401

@@ -242,6 +242,39 @@ extern "C" int vh_sync(vh_sim_t h) {
     return VH_OK;
 }
 
+// read_snapshot! src/HDF5.jl:1594-1646 with _read_agents_restore! :801-850:
+// after finish_init!, the died flags of the n slots of type t; the reuse
+// stack becomes the died slots in ascending order, as the reference rebuilds it
+extern "C" int vh_restore_died(vh_sim_t h, uint8_t t, const uint8_t* died, uint64_t n) {
+    if (!h || (n && !died)) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (t < 1 || t > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+    if (!s.initialized) return set_error(VH_ERR_ASSERT, "restore after finish_init! only");
+    AgentStore& A = s.a[t];
+    if (n != A.nslots) return set_error(VH_ERR_ARG, "restore: %llu slots, the type has %llu",
+                                        (unsigned long long)n, (unsigned long long)A.nslots);
+    std::vector<uint32_t> free_slots;
+    for (uint64_t i = 0; i < n; i++)
+        if (died[i]) free_slots.push_back((uint32_t)(i + 1));
+    if (!A.mortal) {
+        if (!free_slots.empty()) return set_error(VH_ERR_ASSERT, "restore: a died agent of an :Immortal type");
+        return VH_OK;
+    }
+    hipStream_t st = s.stream;
+    HIPCHECK(hipStreamSynchronize(st));
+    if (n) HIPCHECK(hipMemcpy(A.died.p, died, n, hipMemcpyHostToDevice));
+    RC(A.reuse.ensure(free_slots.size() * 4 + 8, st));
+    if (!free_slots.empty())
+        HIPCHECK(hipMemcpy(A.reuse.p, free_slots.data(), free_slots.size() * 4, hipMemcpyHostToDevice));
+    A.nreuse = free_slots.size();
+    return VH_OK;
+}
+extern "C" int vh_set_num_transitions(vh_sim_t h, uint64_t n) {
+    if (!h) return set_error(VH_ERR_ARG, "null simulation");
+    h->s.num_transitions = n;
+    return VH_OK;
+}
+
 extern "C" int vh_num_transitions(vh_sim_t h, uint64_t* out) {
     *out = h->s.num_transitions;
     return VH_OK;

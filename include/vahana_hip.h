@@ -285,6 +285,12 @@ const char* vh_last_error(void);
  * (permissions, removed agents, capacity) as VH_ERR_ASSERT / VH_ERR_CAPACITY;
  * with asserts off apply! stays asynchronous unless it has births or deaths */
 int vh_enable_asserts(int enable);
+/* test hooks (no reference counterpart): the next n device allocations of
+ * the library fail as out of memory (an injected allocation failure); OR
+ * `flags` into a simulation's device error word, as a failing body would
+ * (an injected device error: the next checked apply! reports it) */
+int vh_debug_fail_allocs(uint32_t n);
+int vh_debug_set_device_error(vh_sim_t sim, uint32_t flags);
 
 /* ------------------------------------------------------ ensembles
  * Many independent SIR simulations advanced by one segmented launch per

@@ -37,3 +37,60 @@ def test_served_results_equal_solo_results(vh, ora, gpu_ready):
     want = np.asarray(c.sim._read_agents(c.T)[0])
     assert np.array_equal(np.ascontiguousarray(r.result).view(np.uint8),
                           np.ascontiguousarray(want).view(np.uint8)), (r.n, r.horizon)
+
+
+def test_serving_failures_leave_others_unchanged(vh, gpu_ready):
+    """A request larger than the whole budget is rejected, one is cancelled
+    after its first quantum, one fails to build (an injected device
+    allocation failure); every other request's result equals its
+    one-at-a-time run."""
+    import types
+    S = vh.serve
+    lib = vh._abi.lib()
+    reqs = S.make_requests(10, rate=0, seed=5, n_min=1000, n_max=30000, h_min=12, h_max=40)
+    big = types.SimpleNamespace(id=100, kind="opinion", n=10 ** 9, horizon=10, seed=1, arrival=0.0,
+                                mean_deg=10.0, p_inf=0.05)
+    cancel_id, fail_id = reqs[1].id, reqs[2].id
+
+    def hook(r):
+        if r.id == fail_id:
+            lib.debug_fail_allocs(1)
+
+    srv = S.Server(vh, hbm_budget=10 ** 9, quantum=5, solo_agents=20_000)
+    try:
+        done, wall = srv.run(reqs + [big], cancel={cancel_id: 1}, before_admit=hook)
+    finally:
+        lib.debug_fail_allocs(0)
+        srv.close()
+    status = {r.id: r.status for r in done}
+    print(f"[gpu] statuses {status}", flush=True)
+    assert status[100] == "rejected"
+    assert status[cancel_id] == "cancelled"
+    assert status[fail_id] == "failed"
+    assert len(done) == len(reqs) + 1
+    ok = [r for r in reqs if status[r.id] == "done"]
+    assert len(ok) == len(reqs) - 2
+    seq, _ = S.run_sequential(vh, ok)
+    assert {r.id: r.digest for r in done if r.status == "done"} == {r.id: r.digest for r in seq}
+    s = S.summarize(done, wall)
+    assert s["rejected"] == 1 and s["failed"] == 1 and s["cancelled"] == 1
+
+
+def test_injected_device_error_stays_with_its_simulation(vh, ora, gpu_ready):
+    """A device error flag raised in one simulation (the fault hook) makes its
+    next apply! fail with an assertion; another simulation steps on,
+    equal to the checker."""
+    M = vh.models
+    a = M.build_opinion_gnp(vh, 2000, mean_deg=6.0, seed=1)
+    b = M.build_opinion_gnp(vh, 2000, mean_deg=6.0, seed=2)
+    c = M.build_opinion_gnp(ora, 2000, mean_deg=6.0, seed=2, device_graph=False)
+    vh.enable_asserts(True)
+    vh._abi.lib().debug_set_device_error(a.sim.handle, 1)
+    with pytest.raises(AssertionError):
+        M.step_opinion(vh, a, eps=0.2)
+    for _ in range(3):
+        M.step_opinion(vh, b, eps=0.2)
+        M.step_opinion(ora, c, eps=0.2)
+    x, _ = b.sim._read_agents(b.T)
+    y, _ = c.sim._read_agents(c.T)
+    assert np.array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))

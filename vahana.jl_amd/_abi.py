@@ -140,6 +140,8 @@ DEVICE_ONLY = [
     ("stream_destroy", C.c_int, [_vp]),
     ("device_bytes", C.c_int, [_vp, _u64p]),
     ("enable_asserts", C.c_int, [C.c_int]),
+    ("debug_fail_allocs", C.c_int, [C.c_uint32]),
+    ("debug_set_device_error", C.c_int, [_vp, C.c_uint32]),
     ("graph_er", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
     ("graph_ba", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
     ("hash4", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),

@@ -34,7 +34,7 @@ int DevBuf::ensure(size_t bytes, hipStream_t st, size_t keep) {
     size_t ncap = std::max(bytes, cap + cap / 4);
     ncap = (ncap + 255) & ~(size_t)255;
     void* np = nullptr;
-    hipError_t e = hipMalloc(&np, ncap);
+    hipError_t e = device_alloc(&np, ncap);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
     if (p) {
         if (keep) {
@@ -121,6 +121,20 @@ extern "C" int vh_edgetype_hints(vh_model_t mh, uint8_t et, uint32_t* hints) {
 extern "C" const char* vh_last_error(void) { return g_err; }
 extern "C" int vh_enable_asserts(int enable) {
     g_asserts = enable ? 1 : 0;
+    return VH_OK;
+}
+extern "C" int vh_debug_fail_allocs(uint32_t n) {
+    fail_next_allocs(n);
+    return VH_OK;
+}
+extern "C" int vh_debug_set_device_error(vh_sim_t h, uint32_t flags) {
+    if (!h) return set_error(VH_ERR_ARG, "null simulation");
+    Sim& s = h->s;
+    uint32_t e = 0;
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    HIPCHECK(hipMemcpy(&e, s.d_err.p, 4, hipMemcpyDeviceToHost));
+    e |= flags;
+    HIPCHECK(hipMemcpy(s.d_err.p, &e, 4, hipMemcpyHostToDevice));
     return VH_OK;
 }
 

@@ -14,6 +14,11 @@ namespace vh {
 // the main block spills into extra blocks, which reset() folds into one larger
 // main block (after synchronising the stream) so steady-state steps allocate
 // nothing.
+// hipMalloc for everything the library allocates; fail_next_allocs(n) makes
+// the next n calls return hipErrorOutOfMemory (injected failures for tests)
+hipError_t device_alloc(void** p, size_t bytes);
+void fail_next_allocs(uint32_t n);
+
 struct Scratch {
     void* main = nullptr;
     size_t cap = 0, used = 0, spilled = 0;

@@ -1,5 +1,7 @@
 // vh_prims.hip — device-wide scan, stable LSD radix sort, fills and gathers
 // for gfx950 (wave64).
+#include <atomic>
+
 #include "vh_prims.h"
 
 #include <algorithm>
@@ -7,6 +9,16 @@
 namespace vh {
 
 // ------------------------------------------------------------------ arena
+// every device allocation of the library; a test hook makes the next n fail
+static std::atomic<uint32_t> g_fail_allocs{0};
+hipError_t device_alloc(void** p, size_t bytes) {
+    uint32_t k = g_fail_allocs.load();
+    while (k > 0)
+        if (g_fail_allocs.compare_exchange_weak(k, k - 1)) return hipErrorOutOfMemory;
+    return hipMalloc(p, bytes);
+}
+void fail_next_allocs(uint32_t n) { g_fail_allocs.store(n); }
+
 void* Scratch::take(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (bytes == 0) bytes = 256;
@@ -17,7 +29,7 @@ void* Scratch::take(size_t bytes) {
     }
     if (nextra >= 64) return nullptr;
     void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (device_alloc(&p, bytes) != hipSuccess) return nullptr;
     extra[nextra++] = p;
     spilled += bytes;
     return p;
@@ -34,7 +46,7 @@ void Scratch::reset(hipStream_t st) {
         main = nullptr;
         cap = 0;
         size_t ncap = need + need / 4 + (1 << 20);
-        if (hipMalloc(&main, ncap) == hipSuccess) cap = ncap;
+        if (device_alloc(&main, ncap) == hipSuccess) cap = ncap;
     }
     used = 0;
 }

@@ -87,12 +87,14 @@ def step_opinion(V, m, eps=0.2):
 
 
 def build_opinion_gnp(V, n, mean_deg=10.0, seed=1, graph_seed=1, weight_seed=2, f32=False,
-                      device_graph=True, stream=None):
+                      device_graph=True, stream=None, keep=None):
     """Opinion dynamics on a directed G(n, p) with Float32 weights. With
     `device_graph` the graph is generated on the device at finish_init!
     (vh_add_edges_gnp); otherwise its host twin is added edge by edge (the
     checker's path), giving the same CSR. `stream`: a raw hipStream_t the
-    simulation is built on (and stays on)."""
+    simulation is built on (and stays on). `keep(m)` receives the namespace as
+    soon as the simulation exists (so that a caller can release it when a
+    later build step fails)."""
     _, _, HK, HK32, Knows, *_ = _types(V)
     A = HK32 if f32 else HK
     model = (V.ModelTypes()
@@ -100,6 +102,9 @@ def build_opinion_gnp(V, n, mean_deg=10.0, seed=1, graph_seed=1, weight_seed=2, 
              | V.register_edgetype_(Knows, target=A)
              | V.create_model("opinion"))
     sim = V.create_simulation(model)
+    out = types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=None, f32=f32)
+    if keep is not None:
+        keep(out)
     if stream is not None:
         sim._set_streams(stream, stream)
     st = np.zeros(n, dtype=A.dtype)
@@ -114,7 +119,8 @@ def build_opinion_gnp(V, n, mean_deg=10.0, seed=1, graph_seed=1, weight_seed=2, 
         ws["w"] = inputs.gnp_weights(len(to), weight_seed)
         sim._add_edges_array(Knows, tid | fr.astype(np.uint64), tid | to.astype(np.uint64), ws)
     V.finish_init_(sim)
-    return types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=ids, f32=f32)
+    out.ids = ids
+    return out
 
 
 def step_opinion_n(V, m, steps, eps=0.2):

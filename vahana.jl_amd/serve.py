@@ -22,7 +22,7 @@ import types
 import numpy as np
 import xxhash
 
-from . import _abi, models
+from . import VahanaError, _abi, models
 from .ensemble import Ensemble
 
 
@@ -51,9 +51,10 @@ def footprint(req):
     return int(req.n * 16 + req.n * 4 + e * 8 + req.n * 8)
 
 
-def _build_opinion(vh, req, stream=None):
+def _build_opinion(vh, req, stream=None, keep=None):
     return models.build_opinion_gnp(vh, req.n, mean_deg=req.mean_deg, seed=req.seed,
-                                    graph_seed=req.seed + 1, weight_seed=req.seed + 7, stream=stream)
+                                    graph_seed=req.seed + 1, weight_seed=req.seed + 7, stream=stream,
+                                    keep=keep)
 
 
 def _stream(lib):
@@ -94,9 +95,17 @@ class Server:
         self.stats = {}
         lib.enable_asserts(0)
 
-    def run(self, requests, time_scale=1.0):
+    def run(self, requests, time_scale=1.0, cancel=None, before_admit=None):
         """Serve `requests` (arrival times in seconds, scaled by time_scale);
         returns per-request records with latency and the final-state digest.
+
+        Every record ends with a `status`: "done" (digest, latency), "rejected"
+        (its footprint exceeds the whole budget: never admitted), "failed" (its
+        graph or state could not be built, e.g. out of device memory; what it
+        had allocated is released) or "cancelled" (`cancel` maps a request id
+        to the number of quanta it runs before it is dropped). Other requests
+        are not affected by any of these. `before_admit(r)` is called just
+        before a request is built (a test hook).
 
         One scheduler iteration: (1) enqueue `quantum` steps of every active
         request (asynchronous: opinion requests on their own streams, SIR
@@ -146,32 +155,31 @@ class Server:
                 r.digest = _digest(st)
                 if self.keep:
                     r.result = np.array(st)
-                r.done = time.perf_counter() - t0
-                r.latency = r.done - r.arrival * time_scale
-                done.append(r)
+                self._end(r, "done", t0, time_scale, done)
             tc = time.perf_counter()
             # (3) admission: FIFO while the head fits the budget
             now = tc - t0
             while qi < len(pending) and pending[qi].arrival * time_scale <= now:
                 r = pending[qi]
                 need = footprint(r)
+                if need > self.budget:
+                    qi += 1
+                    self._end(r, "rejected", t0, time_scale, done)
+                    continue
                 if used + need > self.budget and (active or finished):
                     break
                 qi += 1
-                used += need
                 r.admitted = now
-                if r.kind == "sir":
-                    if r.n >= self.solo_agents:
-                        r.ens = Ensemble()
-                        r.ens.set_streams(self.compute, self.io)
-                    else:
-                        r.ens = self.ens
-                    r.mid = r.ens.add_sir(r.n, r.mean_deg, r.p_inf, r.seed, r.seed + 1,
-                                          recovery=7, n_init=max(1, r.n // 1000))
-                    r.ens.set_horizon(r.mid, r.horizon)
-                else:
-                    r.model = _build_opinion(vh, r, stream=self.io)
-                    r.model.sim._set_streams(self.compute, self.io)
+                r.quanta = 0
+                try:
+                    if before_admit is not None:
+                        before_admit(r)
+                    self._build(vh, r)
+                except (VahanaError, AssertionError) as ex:
+                    r.error = str(ex)
+                    self._end(r, "failed", t0, time_scale, done)
+                    continue
+                used += need
                 r.left = r.horizon
                 active[r.id] = r
             td = time.perf_counter()
@@ -180,17 +188,16 @@ class Server:
             te = time.perf_counter()
             # (5) release what was read back; set aside what just finished
             for r in finished:
-                if r.kind == "sir":
-                    if r.ens is self.ens:
-                        self.ens.remove(r.mid)
-                    else:
-                        r.ens.close()
-                    r.ens = None
-                else:
-                    r.model.sim._destroy()
-                    r.model = None
+                self._release(r)
                 used -= footprint(r)
-            finished = [r for r in stepping if r.left == 0]
+            for r in stepping:
+                r.quanta += 1
+                if cancel and r.left > 0 and r.id in cancel and r.quanta >= cancel[r.id]:
+                    del active[r.id]
+                    self._release(r)
+                    used -= footprint(r)
+                    self._end(r, "cancelled", t0, time_scale, done)
+            finished = [r for r in stepping if r.left == 0 and r.id in active]
             for r in finished:
                 del active[r.id]
             tf = time.perf_counter()
@@ -207,6 +214,56 @@ class Server:
                     ph["idle"] += min(wait, 0.05)
         wall = time.perf_counter() - t0
         return done, wall
+
+    def _build(self, vh, r):
+        """Graph and state of a request on the device; on failure nothing of
+        it stays allocated."""
+        if r.kind == "sir":
+            solo = r.n >= self.solo_agents
+            ens = Ensemble() if solo else self.ens
+            try:
+                if solo:
+                    ens.set_streams(self.compute, self.io)
+                r.mid = ens.add_sir(r.n, r.mean_deg, r.p_inf, r.seed, r.seed + 1,
+                                    recovery=7, n_init=max(1, r.n // 1000))
+                ens.set_horizon(r.mid, r.horizon)
+            except Exception:
+                if solo:
+                    ens.close()
+                raise
+            r.ens = ens
+        else:
+            m = None
+            try:
+                m = _build_opinion(vh, r, stream=self.io, keep=lambda mm: setattr(r, "_partial", mm))
+                m.sim._set_streams(self.compute, self.io)
+            except Exception:
+                part = getattr(r, "_partial", None)
+                if part is not None:
+                    part.sim._destroy()
+                raise
+            finally:
+                if hasattr(r, "_partial"):
+                    del r._partial
+            r.model = m
+
+    def _release(self, r):
+        if r.kind == "sir":
+            if r.ens is self.ens:
+                self.ens.remove(r.mid)
+            else:
+                r.ens.close()
+            r.ens = None
+        else:
+            r.model.sim._destroy()
+            r.model = None
+
+    @staticmethod
+    def _end(r, status, t0, time_scale, done):
+        r.status = status
+        r.done = time.perf_counter() - t0
+        r.latency = r.done - r.arrival * time_scale
+        done.append(r)
 
     def close(self):
         self.ens.close()
@@ -255,6 +312,7 @@ def run_sequential(vh, requests, time_scale=1.0, keep_results=False, stats=None)
         ph["run"] += tc - tb
         ph["read"] += td - tc
         r2 = types.SimpleNamespace(**{k: v for k, v in vars(r).items()})
+        r2.status = "done"
         r2.digest = _digest(st)
         if keep_results:
             r2.result = np.array(st)
@@ -265,7 +323,15 @@ def run_sequential(vh, requests, time_scale=1.0, keep_results=False, stats=None)
 
 
 def summarize(done, wall):
-    lat = np.array([r.latency for r in done])
-    return {"requests": len(done), "wall_s": wall, "throughput_rps": len(done) / wall,
-            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "agent_steps_per_s": sum(r.n * r.horizon for r in done) / wall}
+    """Throughput and latency percentiles of the completed requests; the
+    counts of the others by status."""
+    ok = [r for r in done if getattr(r, "status", "done") == "done"]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    out = {"requests": len(ok), "wall_s": wall, "throughput_rps": len(ok) / wall,
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "agent_steps_per_s": sum(r.n * r.horizon for r in ok) / wall}
+    for st in ("rejected", "failed", "cancelled"):
+        k = sum(1 for r in done if getattr(r, "status", "done") == st)
+        if k:
+            out[st] = k
+    return out

@@ -39,7 +39,7 @@ def sir_bytes(n, e):
     return n * (4 + 4) + (n + 1) * 4 + e * 4
 
 
-def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0):
+def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0, relabel=True):
     """The checker's apply! (a restatement of the reference's data structures)
     over a bounded sample of the workload: same model, 10^6 agents (2*10^5
     for the contact model); median of up to 5 steps after a warm-up."""
@@ -52,7 +52,7 @@ def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0):
     if workload.startswith("opinion"):
         a, b = vh.inputs.graph_ba(n, 5, seed)
         edges = vh.inputs.symmetric_weighted(a, b, seed + 1)
-        m = M.build_opinion(ora, n, seed=seed, f32=f32, edges=edges)
+        m = M.build_opinion(ora, n, seed=seed, f32=f32, edges=edges, relabel=relabel)
         step = lambda: M.step_opinion(ora, m, eps=0.2)  # noqa: E731
         ne = len(edges[0])
     elif workload == "contact":
@@ -81,9 +81,9 @@ def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0):
     return n, ne, times
 
 
-def cpu_baseline(workload, f32):
-    """Setting (i): one host core."""
-    n, ne, times = _cpu_sample(workload, f32)
+def cpu_baseline(workload, f32, relabel=True):
+    """Setting (i): one host core (the same agent numbering as the GPU run)."""
+    n, ne, times = _cpu_sample(workload, f32, relabel=relabel)
     med = times[len(times) // 2]
     return {"value": n / med, "unit": "agent-updates/s", "cores": 1, "kind": "port",
             "edges_per_s": ne / med,
@@ -91,7 +91,7 @@ def cpu_baseline(workload, f32):
                       "1 host core"}
 
 
-def cpu_baseline_all_cores(workload, f32):
+def cpu_baseline_all_cores(workload, f32, relabel=True):
     """Setting (ii): the host cores as P independent processes, each owning a
     contiguous range of agents (its own sample of the workload, rank-seeded;
     the exchange between ranks is not modelled), run at the same time. The
@@ -106,6 +106,8 @@ def cpu_baseline_all_cores(workload, f32):
     P = max(1, min(ncpu, 15))
     env = dict(os.environ, ROCR_VISIBLE_DEVICES="", HIP_VISIBLE_DEVICES="", OMP_NUM_THREADS="1")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--workload", workload]
+    if not relabel:
+        cmd.append("--no-relabel")
     procs = [subprocess.Popen(cmd + ["--rank", str(r)], stdout=subprocess.PIPE, env=env)
              for r in range(P)]
     res = []
@@ -125,7 +127,7 @@ def cpu_baseline_all_cores(workload, f32):
 
 def cpu_worker(args):
     f32 = args.workload == "opinion32"
-    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank)
+    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank, relabel=not args.no_relabel)
     print(json.dumps({"n": n, "edges": ne, "median_s": times[len(times) // 2]}), flush=True)
 
 
@@ -143,6 +145,9 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrivals/s (0: 0.8 x the measured one-at-a-time capacity)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-relabel", action="store_true",
+                    help="opinion: keep the generation order of the agents (default: renumbered "
+                         "breadth-first at finish_init!)")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--rank", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calib", action="store_true",
@@ -176,12 +181,15 @@ def main():
         a, b = vh.inputs.graph_ba(n, 5, 1)
         edges = vh.inputs.symmetric_weighted(a, b, 2)
         del a, b
-        m = M.build_opinion(vh, n, seed=1, f32=f32, edges=edges)
+        relabel = not args.no_relabel
+        m = M.build_opinion(vh, n, seed=1, f32=f32, edges=edges, relabel=relabel)
         step = lambda: M.step_opinion(vh, m, eps=0.2)  # noqa: E731
         nbytes = opinion_bytes(n, m.nedges, f32)
         dtype = "f32" if f32 else "f64"
         workload = (f"opinion dynamics (weighted bounded confidence), Barabasi-Albert m=5 symmetrised, "
-                    f"N={n}, E={m.nedges}, eps=0.2, {'Float32' if f32 else 'Float64'} opinions")
+                    f"N={n}, E={m.nedges}, eps=0.2, {'Float32' if f32 else 'Float64'} opinions"
+                    + (", agents renumbered breadth-first at finish_init! (idmapping)" if relabel
+                       else ""))
     elif args.workload == "raster":
         import math
         side = int(round(math.sqrt(n)))
@@ -256,7 +264,9 @@ def main():
     traffic = None
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # runs of this exact workload (scaled by the copy-kernel calibration)
-    pmc = {("opinion", 10_000_000): "k_opinion_f64_ba10M_traffic.json"}.get((args.workload, n))
+    pmc = {("opinion", 10_000_000, False): "k_opinion_f64_ba10M_traffic.json",
+           ("opinion", 10_000_000, True): "k_opinion_f64_ba10M_bfs_traffic.json"}.get(
+        (args.workload, n, not args.no_relabel))
     if pmc and os.path.exists(os.path.join(ROOT, "profiles", pmc)):
         traffic = json.load(open(os.path.join(ROOT, "profiles", pmc))).get("bytes_per_launch")
 
@@ -266,7 +276,7 @@ def main():
         a, b = vh.inputs.graph_ba(n, 5, 1)
         edges = vh.inputs.symmetric_weighted(a, b, 2)
         del a, b
-        m64 = M.build_opinion(vh, n, seed=1, f32=False, edges=edges)
+        m64 = M.build_opinion(vh, n, seed=1, f32=False, edges=edges, relabel=not args.no_relabel)
         del edges
         for _ in range(1 + args.warmup + args.steps - 1):
             M.step_opinion(vh, m64, eps=0.2)
@@ -281,10 +291,10 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        cpu = cpu_baseline(args.workload, f32)
+        cpu = cpu_baseline(args.workload, f32, relabel=not args.no_relabel)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {cpu}")
         t0 = time.perf_counter()
-        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32)
+        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32, relabel=not args.no_relabel)
         log(f"cpu baseline, all cores {time.perf_counter() - t0:.1f}s: {cpu['all_cores']}")
 
     out = {

@@ -228,6 +228,14 @@ int vh_add_edges_stencil(vh_sim_t sim, uint8_t et, uint32_t cell_type, uint64_t 
                          const uint8_t* edge_state);
 /* finish_init! src/Simulation.jl:403-476: upload and build the CSR */
 int vh_finish_init(vh_sim_t s);
+/* finish_init! with the agents of type t renumbered in breadth-first order
+ * of the init edges among them (the reference renumbers agents when it
+ * distributes them, src/Simulation.jl:414-453, and returns the mapping with
+ * `return_idmapping`): new_nr[k] receives the new number of the agent added
+ * as number k + 1. States travel with their agents and every target keeps
+ * its in-edges in insertion order, so results map one to one. AgentIDs
+ * stored inside agent or edge states are not renumbered. */
+int vh_finish_init_relabel(vh_sim_t sim, uint8_t t, uint64_t* new_nr);
 /* apply! src/Simulation.jl:720-821, enqueued on the simulation's stream */
 int vh_apply(vh_sim_t s, const vh_apply_desc* desc);
 /* apply! repeated `steps` times with the same desc (one host call) */
@@ -364,6 +372,13 @@ int vh_fill_open01f(uint64_t seed, uint64_t b, uint64_t c, uint64_t n, float* ou
  * h = vh_hash4(seed, AgentID(type, i + 1), b, c) (the opinion and other
  * per-agent initial values) */
 int vh_fill_agent_u01(uint64_t seed, uint32_t type, uint64_t b, uint64_t c, uint64_t n, double* out);
+
+/* breadth-first numbering of vertices 1..n over m undirected edges
+ * (a[k], b[k]): adjacency in edge order, the first root the vertex of
+ * highest degree (lowest number among equals), further components from their
+ * lowest-numbered vertex; new_nr[v - 1] = 1-based position of v (the order
+ * vh_finish_init_relabel applies) */
+int vh_bfs_order(uint64_t n, uint64_t m, const uint32_t* a, const uint32_t* b, uint64_t* new_nr);
 
 /* ------------------------------------------------------ diagnostics
  * Memory-system probe (no reference counterpart): mode 0 times n random

@@ -142,3 +142,50 @@ def test_gnp_generator_properties(vh):
     assert abs(fr.mean() / ((n + 1) / 2) - 1) < 0.01
     fr2, to2 = vh.inputs.graph_gnp(n, md, 4)
     assert np.array_equal(fr, fr2) and np.array_equal(to, to2)
+
+
+def test_bfs_order_known_answers(vh):
+    bfs = vh.inputs.bfs_order
+    # path 1-2-3-4-5: root 2 (highest degree, lowest number), then 1, 3, 4, 5
+    assert bfs(5, [1, 2, 3, 4], [2, 3, 4, 5]).tolist() == [2, 1, 3, 4, 5]
+    # star around 3 (edges 3-1, 3-2, 3-4): 3 first, its neighbours in edge order
+    assert bfs(4, [3, 3, 3], [1, 2, 4]).tolist() == [2, 3, 1, 4]
+    # two components and an isolated vertex: component by component
+    assert bfs(5, [1, 4], [2, 5]).tolist() == [1, 2, 3, 4, 5]
+    # random graph: a permutation whose first vertex has the highest degree,
+    # and every later vertex of a component has a neighbour numbered before it
+    rng = np.random.default_rng(3)
+    n = 300
+    a = rng.integers(1, n + 1, 900).astype(np.uint32)
+    b = rng.integers(1, n + 1, 900).astype(np.uint32)
+    nn = bfs(n, a, b)
+    assert sorted(nn.tolist()) == list(range(1, n + 1))
+    deg = np.bincount(a, minlength=n + 1) + np.bincount(b, minlength=n + 1)
+    root = int(np.nonzero(nn == 1)[0][0]) + 1
+    assert deg[root] == deg[1:].max()
+    nbr = {v: set() for v in range(1, n + 1)}
+    for x, y in zip(a.tolist(), b.tolist()):
+        nbr[x].add(y)
+        nbr[y].add(x)
+    for v in range(1, n + 1):
+        if nbr[v] and nn[v - 1] != 1:
+            earlier = [u for u in nbr[v] if nn[u - 1] < nn[v - 1]]
+            later_root = all(nn[u - 1] > nn[v - 1] for u in nbr[v])
+            assert earlier or later_root   # v is the root of a later component
+
+
+def test_relabelled_checker_run_maps_to_plain_run(vh, ora):
+    """The opinion model on a breadth-first renumbered graph (the checker's
+    host renumbering) equals the plain run agent by agent: states travel with
+    their agents and every target keeps its in-edge order."""
+    M = vh.models
+    a, b = vh.inputs.graph_ba(2000, 3, 5)
+    edges = vh.inputs.symmetric_weighted(a, b, 6)
+    p = M.build_opinion(ora, 2000, seed=5, edges=edges)
+    r = M.build_opinion(ora, 2000, seed=5, edges=edges, relabel=True)
+    for _ in range(3):
+        M.step_opinion(ora, p, eps=0.3)
+        M.step_opinion(ora, r, eps=0.3)
+    xp = np.asarray(p.sim._read_agents(p.T)[0]["opinion"])
+    xr = np.asarray(r.sim._read_agents(r.T)[0]["opinion"])
+    assert np.array_equal(xr[(r.idmap - 1).astype(np.int64)], xp)

@@ -270,3 +270,25 @@ def test_snapshot_cross_restore_contact(vh, ora, gpu_ready, tmp_path):
         tg, fg, _ = g.sim._read_edges(g.E)
         tc, fc, _ = c.sim._read_edges(c.E)
         assert np.array_equal(tg, tc) and np.array_equal(fg, fc), step
+
+
+def test_relabelled_opinion_device(vh, ora, gpu_ready):
+    """finish_init_(relabel=T) on the device: the mapping equals the
+    checker's host renumbering, the run equals the checker's relabelled run
+    bit for bit and the device's plain run agent by agent."""
+    M = vh.models
+    a, b = vh.inputs.graph_ba(20000, 5, 7)
+    edges = vh.inputs.symmetric_weighted(a, b, 8)
+    d = M.build_opinion(vh, 20000, seed=7, edges=edges, relabel=True)
+    p = M.build_opinion(vh, 20000, seed=7, edges=edges)
+    c = M.build_opinion(ora, 20000, seed=7, edges=edges, relabel=True)
+    assert np.array_equal(d.idmap, c.idmap)
+    for _ in range(4):
+        for m, V in ((d, vh), (p, vh), (c, ora)):
+            M.step_opinion(V, m, eps=0.2)
+    xd = np.asarray(d.sim._read_agents(d.T)[0]["opinion"])
+    xp = np.asarray(p.sim._read_agents(p.T)[0]["opinion"])
+    xc = np.asarray(c.sim._read_agents(c.T)[0]["opinion"])
+    assert np.array_equal(xd, xc)
+    assert np.array_equal(xd[(d.idmap - 1).astype(np.int64)], xp)
+    assert vh.updateids(d.idmap, vh.agent_id(1, 5)) == vh.agent_id(1, int(d.idmap[4]))

@@ -346,6 +346,17 @@ class Simulation:
         _check(self.lib.finish_init(self.handle), self.lib)
         self.initialized = True
 
+    def _finish_init_relabel(self, T):
+        """finish_init! with the agents of type T renumbered breadth-first
+        (vh_finish_init_relabel); returns new_nr[old_nr - 1]."""
+        n = C.c_uint64()
+        self.lib.read_agents(self.handle, self._aid(T), None, None, 0, C.byref(n))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(self.lib.finish_init_relabel(self.handle, self._aid(T),
+                                            out.ctypes.data_as(C.POINTER(C.c_uint64))), self.lib)
+        self.initialized = True
+        return out[:n.value]
+
     # -- apply!
     def _apply(self, func, call, read, write, add_existing=(), with_edge=None):
         d, keep = self._desc(func, call, read, write, add_existing, with_edge)
@@ -564,9 +575,26 @@ def add_edges_(sim, to, edges_):
         add_edge_(sim, to, e)
 
 
-def finish_init_(sim):
-    """finish_init! src/Simulation.jl:403-476."""
-    sim._finish_init()
+def finish_init_(sim, relabel=None):
+    """finish_init! src/Simulation.jl:403-476. With `relabel` = an agent type,
+    its agents are renumbered breadth-first over their edges (as the
+    reference renumbers agents when it distributes them, returning the
+    mapping with `return_idmapping`); returns new_nr[old_nr - 1]."""
+    if relabel is None:
+        sim._finish_init()
+        return None
+    return sim._finish_init_relabel(relabel)
+
+
+def updateids(idmapping, ids):
+    """updateids for a mapping from finish_init_ (the reference's idmapping,
+    src/Simulation.jl:455-475): the new
+    AgentIDs of `ids` (scalar or array) of the renumbered type."""
+    a = np.asarray(ids, dtype=np.uint64)
+    t = a & ~np.uint64(_abi.VH_NR_MASK)
+    nr = (a & np.uint64(_abi.VH_NR_MASK)).astype(np.int64)
+    out = t | np.asarray(idmapping, dtype=np.uint64)[nr - 1]
+    return int(out) if out.ndim == 0 else out
 
 
 def apply_(sim, func, call, read, write, add_existing=(), with_edge=None):

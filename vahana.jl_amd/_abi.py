@@ -130,6 +130,7 @@ SIGNATURES = [
 DEVICE_ONLY = [
     ("apply_n", C.c_int, [_vp, C.POINTER(ApplyDesc), C.c_uint32]),
     ("add_edges_gnp", C.c_int, [_vp, C.c_uint8, C.c_double, C.c_uint64, C.c_uint64]),
+    ("finish_init_relabel", C.c_int, [_vp, C.c_uint8, _u64p]),
     ("add_edges_stencil", C.c_int, [_vp, C.c_uint8, C.c_uint32, C.c_uint64, C.c_uint32,
                                     C.POINTER(C.c_int64), C.c_double, C.c_int, C.c_int,
                                     C.POINTER(C.c_uint8)]),
@@ -147,6 +148,7 @@ DEVICE_ONLY = [
     ("hash4", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
     ("fill_open01f", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                C.POINTER(C.c_float)]),
+    ("bfs_order", C.c_int, [C.c_uint64, C.c_uint64, _u32p, _u32p, _u64p]),
     ("fill_agent_u01", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64,
                                  C.POINTER(C.c_double)]),
     ("probe_memory", C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32,

@@ -122,6 +122,9 @@ int gnp_device(uint64_t n, double mean_deg, uint64_t seed, DevBuf& off, DevBuf& 
 int stencil_csr(Sim& s, EdgeStore& E, uint64_t rowbase, uint64_t rows);
 // Float32 weights in (0, 1) by CSR position (vh_ensemble.hip)
 void fill_weights(float* w, uint64_t n, uint64_t wseed, hipStream_t st);
+// host graph helper (vh_graph.cpp)
+extern "C" int vh_bfs_order(uint64_t n, uint64_t m, const uint32_t* a, const uint32_t* b,
+                            uint64_t* new_nr);
 // engine services the bodies use
 int readback(Sim& s, int n);                           // d_stats -> h_stats, synchronous
 int ensure_slots(Sim& s, int t, uint64_t slots);

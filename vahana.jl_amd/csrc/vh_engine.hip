@@ -855,6 +855,55 @@ static int build_gnp_csr(Sim& s, EdgeStore& E) {
     return VH_OK;
 }
 
+// finish_init! with the agents of type t renumbered for locality, as the
+// reference renumbers them when it distributes them (src/Simulation.jl:414-453,
+// `return_idmapping`): vh_bfs_order over the init edges among agents of
+// type t. States travel with their agents; every edge keeps its place in the
+// insertion order, so each target keeps its in-edges in the same order.
+// new_nr[k] (k = old number - 1) receives the new number. AgentIDs held
+// inside agent or edge states are not renumbered.
+extern "C" int vh_finish_init_relabel(vh_sim_t h, uint8_t t, uint64_t* new_nr) {
+    if (!h || !new_nr) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT, "You can not call finish_init! twice for the same simulation");
+    if (t < 1 || t > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+    AgentStore& A = s.a[t];
+    const uint64_t n = A.nextid - 1;
+    if (n >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "too many agents");
+    // the endpoints of every init edge among agents of type t, in edge-type
+    // and insertion order
+    std::vector<uint32_t> ea, eb;
+    for (int et = 0; et < s.m->ne; et++) {
+        EdgeStore& E = s.e[et];
+        if (has(E.hints, VH_HINT_IGNOREFROM)) continue;
+        for (size_t k = 0; k < E.h_to.size(); k++) {
+            uint64_t to = E.h_to[k], fr = E.h_from[k];
+            if (vh_type_of(to) != t || vh_type_of(fr) != t) continue;
+            ea.push_back((uint32_t)vh_nr_of(fr));
+            eb.push_back((uint32_t)vh_nr_of(to));
+        }
+    }
+    RC(vh_bfs_order(n, ea.size(), ea.data(), eb.data(), new_nr));
+    std::vector<uint32_t>().swap(ea);
+    std::vector<uint32_t>().swap(eb);
+    // states to their new slots
+    if (A.bytes && n) {
+        std::vector<uint8_t> ns(n * A.bytes);
+        for (uint64_t o = 0; o < n; o++)
+            memcpy(ns.data() + (new_nr[o] - 1) * A.bytes, A.h_state.data() + o * A.bytes, A.bytes);
+        A.h_state.swap(ns);
+    }
+    // edge endpoints of type t renumbered, edges in place
+    for (int et = 0; et < s.m->ne; et++) {
+        EdgeStore& E = s.e[et];
+        for (auto* v : {&E.h_to, &E.h_from})
+            for (auto& id : *v)
+                if (id && vh_type_of(id) == t) id = vh_make_id(t, new_nr[vh_nr_of(id) - 1]);
+    }
+    return vh_finish_init(h);
+}
+
 // finish_init! src/Simulation.jl:403-476 for one process: publish the
 // init-phase agents and build every edge type's CSR.
 extern "C" int vh_finish_init(vh_sim_t h) {

@@ -47,6 +47,67 @@ extern "C" int vh_fill_agent_u01(uint64_t seed, uint32_t type, uint64_t b, uint6
     return VH_OK;
 }
 
+// Breadth-first numbering of n vertices over m undirected edges (a_k, b_k)
+// (1-based): adjacency lists in edge order, the first root the vertex of
+// highest degree (the lowest number among equals), each further component
+// from its lowest-numbered vertex. new_nr[v - 1] = position of v + 1.
+extern "C" int vh_bfs_order(uint64_t n, uint64_t m, const uint32_t* a, const uint32_t* b,
+                            uint64_t* new_nr) {
+    if (n >= 0xFFFFFFFFull) return VH_ERR_CAPACITY;
+    for (uint64_t k = 0; k < m; k++)
+        if (a[k] < 1 || a[k] > n || b[k] < 1 || b[k] > n) return VH_ERR_ARG;
+    std::vector<uint64_t> start(n + 2, 0);
+    for (uint64_t k = 0; k < m; k++) {
+        start[a[k]]++;
+        start[b[k]]++;
+    }
+    uint64_t acc = 0;
+    for (uint64_t v = 1; v <= n + 1; v++) {
+        uint64_t c = start[v];
+        start[v] = acc;
+        acc += c;
+    }
+    std::vector<uint32_t> adj(acc);
+    {
+        std::vector<uint64_t> fill(start.begin(), start.end());
+        for (uint64_t k = 0; k < m; k++) {
+            adj[fill[a[k]]++] = b[k];
+            adj[fill[b[k]]++] = a[k];
+        }
+    }
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    std::vector<uint8_t> seen(n + 1, 0);
+    uint64_t root = 1, best = 0;
+    for (uint64_t v = 1; v <= n; v++)
+        if (start[v + 1] - start[v] > best) {
+            best = start[v + 1] - start[v];
+            root = v;
+        }
+    uint64_t next_unvisited = 1;
+    while (order.size() < n) {
+        if (seen[root]) {
+            while (next_unvisited <= n && seen[next_unvisited]) next_unvisited++;
+            root = next_unvisited;
+        }
+        size_t head = order.size();
+        seen[root] = 1;
+        order.push_back((uint32_t)root);
+        while (head < order.size()) {
+            uint32_t v = order[head++];
+            for (uint64_t k = start[v]; k < start[v + 1]; k++) {
+                uint32_t w = adj[k];
+                if (!seen[w]) {
+                    seen[w] = 1;
+                    order.push_back(w);
+                }
+            }
+        }
+    }
+    for (uint64_t k = 0; k < n; k++) new_nr[order[k] - 1] = k + 1;
+    return VH_OK;
+}
+
 // Directed G(n, p) by geometric skipping over the n-1 candidate sources of
 // each target (Batagelj & Brandes 2005), targets ascending.
 extern "C" int vh_graph_er(uint64_t n, double p, uint64_t seed, uint32_t* from, uint32_t* to,

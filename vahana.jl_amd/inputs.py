@@ -94,6 +94,21 @@ def graph_ba(n, m, seed):
     return a[:k], b[:k]
 
 
+def bfs_order(n, a, b):
+    """new_nr[v - 1] of the breadth-first numbering of vertices 1..n over the
+    undirected edges (a[k], b[k]) (vh_bfs_order, the order
+    finish_init_(sim, relabel=T) applies)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    rc = _abi.lib().bfs_order(n, len(a), a.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              b.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    if rc != 0:
+        raise RuntimeError(f"vh_bfs_order failed ({rc})")
+    return out[:n]
+
+
 def gnp_weights(n_edges, wseed):
     """Float32 weights in (0, 1) by edge position (target order), the values
     the device writes for vh_add_edges_gnp."""

@@ -58,7 +58,11 @@ def step_sir(V, m, p_inf=0.05, seed=1, recovery=7):
     V.apply_(m.sim, V.transitions.sir(m.E, p_inf, seed, recovery), m.T, [m.T, m.E], m.T)
 
 
-def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None):
+def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None, relabel=False):
+    """`relabel`: agents renumbered breadth-first at finish_init! (m.idmap =
+    new number of each agent number); a simulation without
+    vh_finish_init_relabel (the checker) gets its inputs renumbered the same
+    way (inputs.bfs_order)."""
     _, _, HK, HK32, Knows, *_ = _types(V)
     A = HK32 if f32 else HK
     model = (V.ModelTypes()
@@ -69,17 +73,30 @@ def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None):
     x = inputs.opinion_init(n, seed, f32)
     st = np.zeros(n, dtype=A.dtype)
     st["opinion"] = x
-    ids = sim._add_agents_array(A, st)
     if edges is None:
         a, b = inputs.graph_ba(n, m_ba, seed)
         edges = inputs.symmetric_weighted(a, b, seed + 1)
     fr, to, w = edges
+    idmap = None
+    host_relabel = relabel and not hasattr(sim.lib, "finish_init_relabel")
+    if host_relabel:
+        idmap = inputs.bfs_order(n, fr, to)
+        st2 = np.zeros_like(st)
+        st2[(idmap - 1).astype(np.int64)] = st
+        st = st2
+        fr = idmap[fr.astype(np.int64) - 1]
+        to = idmap[to.astype(np.int64) - 1]
+    ids = sim._add_agents_array(A, st)
     tid = np.uint64(1) << np.uint64(56)
     ws = np.zeros(len(w), dtype=Knows.dtype)
     ws["w"] = w
     sim._add_edges_array(Knows, tid | fr.astype(np.uint64), tid | to.astype(np.uint64), ws)
-    V.finish_init_(sim)
-    return types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=ids, nedges=len(to), f32=f32)
+    if relabel and not host_relabel:
+        idmap = V.finish_init_(sim, relabel=A)
+    else:
+        V.finish_init_(sim)
+    return types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=ids, nedges=len(to), f32=f32,
+                                 idmap=idmap)
 
 
 def step_opinion(V, m, eps=0.2):

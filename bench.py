@@ -264,9 +264,13 @@ def main():
     traffic = None
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # runs of this exact workload (scaled by the copy-kernel calibration)
-    pmc = {("opinion", 10_000_000, False): "k_opinion_f64_ba10M_traffic.json",
-           ("opinion", 10_000_000, True): "k_opinion_f64_ba10M_bfs_traffic.json"}.get(
-        (args.workload, n, not args.no_relabel))
+    # (the Float64 state of 10^7 agents is gathered in two passes unless
+    # VH_OPINION_PLAN_BYTES turns that off)
+    two_pass = 0 <= int(os.environ.get("VH_OPINION_PLAN_BYTES", 16 << 20)) <= n * 8
+    pmc = {("opinion", 10_000_000, False, False): "k_opinion_f64_ba10M_traffic.json",
+           ("opinion", 10_000_000, True, False): "k_opinion_f64_ba10M_bfs_traffic.json",
+           ("opinion", 10_000_000, True, True): "k_opinion_f64_ba10M_bfs_plan_traffic.json"}.get(
+        (args.workload, n, not args.no_relabel, two_pass))
     if pmc and os.path.exists(os.path.join(ROOT, "profiles", pmc)):
         traffic = json.load(open(os.path.join(ROOT, "profiles", pmc))).get("bytes_per_launch")
 

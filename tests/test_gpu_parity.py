@@ -58,8 +58,20 @@ def test_sir_parity(vh, ora, gpu_ready):
     assert (a["health"] == vh.R).sum() > 0
 
 
+# neighbour-state gather: direct; two-pass plan with 16 or 8 source
+# partitions; two-pass plan with one source per partition (4000 partitions,
+# a two-digit plan sort)
+GATHER_MODES = {"direct": ("-1", None), "plan": ("0", "2048"), "plan_1src": ("0", "1")}
+
+
+@pytest.mark.parametrize("mode", list(GATHER_MODES))
 @pytest.mark.parametrize("f32", [False, True])
-def test_opinion_parity(vh, ora, gpu_ready, f32):
+def test_opinion_parity(vh, ora, gpu_ready, f32, mode, monkeypatch):
+    plan_bytes, part_bytes = GATHER_MODES[mode]
+    monkeypatch.setenv("VH_OPINION_PLAN_BYTES", plan_bytes)
+    if part_bytes:
+        monkeypatch.setenv("VH_OPINION_PART_BYTES", part_bytes)
+    print(f"[gpu] opinion parity f32={f32} gather={mode}", flush=True)
     M = vh.models
     n = 4000
     a, b = vh.inputs.graph_ba(n, 5, 8)

@@ -400,10 +400,22 @@ __device__ __forceinline__ T ld_state(const T* __restrict__ x, uint32_t s, uint3
 }
 constexpr int OP_U = 8;   // gathers in flight per thread
 
-template <class T>
+// neighbour state of in-edge slot `s`: with PLAN, `s` is the edge's position
+// in the gathered plan vector y (k_plan_gather); otherwise the source's
+// 1-based number in x
+template <class T, bool PLAN>
+__device__ __forceinline__ T ld_nb(const T* __restrict__ x, const T* __restrict__ y, uint32_t s, uint32_t hot) {
+    if constexpr (PLAN)
+        return y[s];
+    else
+        return ld_state(x, s, hot);
+}
+
+template <class T, bool PLAN>
 __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
-                                              const T* __restrict__ x, const float* __restrict__ w,
+                                              const T* __restrict__ x, const T* __restrict__ y,
+                                              const float* __restrict__ w,
                                               T* __restrict__ xo, uint64_t n, uint64_t nrows,
                                               uint64_t rowbase, T eps, uint32_t blk, uint32_t hot) {
     __shared__ T s_num[OP_TILE];
@@ -471,7 +483,8 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                 wk[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(w + kk[u]) : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < OP_U; u++) xj[u] = kk[u] != 0xFFFFFFFFu ? ld_state(x, sj[u], hot) : (T)0;
+            for (int u = 0; u < OP_U; u++)
+                xj[u] = kk[u] != 0xFFFFFFFFu ? ld_nb<T, PLAN>(x, y, sj[u], hot) : (T)0;
 #pragma unroll
             for (int u = 0; u < OP_U; u++) {
                 const uint32_t q = q0 + (uint32_t)u * 256u + tid;
@@ -492,10 +505,11 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
     if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
 }
 
-template <class T>
+template <class T, bool PLAN>
 __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
-                                              const T* __restrict__ x, const float* __restrict__ w,
+                                              const T* __restrict__ x, const T* __restrict__ y,
+                                              const float* __restrict__ w,
                                               T* __restrict__ xo, uint64_t r, uint64_t rowbase, T eps,
                                               uint32_t hot) {
     const int lane = threadIdx.x & 63;
@@ -508,8 +522,8 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
     for (; k + 192 < e; k += 256) {
         uint32_t s0 = src[k], s1 = src[k + 64], s2 = src[k + 128], s3 = src[k + 192];
         float w0 = w[k], w1 = w[k + 64], w2 = w[k + 128], w3 = w[k + 192];
-        T x0 = ld_state(x, s0, hot), x1 = ld_state(x, s1, hot);
-        T x2 = ld_state(x, s2, hot), x3 = ld_state(x, s3, hot);
+        T x0 = ld_nb<T, PLAN>(x, y, s0, hot), x1 = ld_nb<T, PLAN>(x, y, s1, hot);
+        T x2 = ld_nb<T, PLAN>(x, y, s2, hot), x3 = ld_nb<T, PLAN>(x, y, s3, hot);
         bool a0 = fabs(x0 - xi) <= eps, a1 = fabs(x1 - xi) <= eps;
         bool a2 = fabs(x2 - xi) <= eps, a3 = fabs(x3 - xi) <= eps;
         num = num + (a0 ? (T)w0 * x0 : (T)0);
@@ -522,7 +536,7 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
         den = den + (a3 ? (T)w3 : (T)0);
     }
     for (; k < e; k += 64) {
-        T xj = ld_state(x, src[k], hot);
+        T xj = ld_nb<T, PLAN>(x, y, src[k], hot);
         T wk = (T)w[k];
         if (fabs(xj - xi) <= eps) {
             num = num + wk * xj;
@@ -537,22 +551,84 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
     if (lane == 0) xo[i] = den > (T)0 ? num / den : xi;
 }
 
-template <class T>
+// ----------------------------------------------------------------- gather plan
+// For a large state vector the neighbour states of a static edge store are
+// gathered in two passes through a plan: the CSR slots sorted (stably) by
+// source partition, a partition being 2^shift consecutive sources sized to
+// sit in one XCD's L2. Pass 1 (k_plan_gather) walks the plan in order,
+// y[j] = x[psrc[j]]: each XCD takes a contiguous range of it, so its gathers
+// hit the one partition its L2 holds, and y is written sequentially. Pass 2
+// (k_opinion with PLAN) reads y[pinv[k]] for CSR slot k: within a partition
+// the plan keeps CSR order, so contiguous agents read y as one sequential
+// stream per partition, which the XCD's L2 serves. The values are the same as
+// x[src[k]] and the sums keep their canonical order.
+constexpr int GP_U = 8;   // plan entries per thread in pass 1
+
+// One launch per step: blocks [0, nh) run one wave per heavy agent (hub
+// waves first: they are the longest, so they start before the light blocks
+// instead of forming the tail of the launch), the rest thread-per-agent
+// blocks. With `remap` (nh a multiple of 8, 8 * ceil(nlight / 8) light
+// blocks) the light blocks of one XCD take one contiguous range of agents:
+// blocks are dispatched round-robin over the 8 XCDs, and contiguous agents
+// read the plan vector as a few sequential streams that one XCD's L2 keeps.
+template <class T, bool PLAN>
 __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ src,
-                                                 const T* __restrict__ x, const float* __restrict__ w,
-                                                 T* __restrict__ xo, uint64_t n, uint64_t nrows,
-                                                 uint64_t rowbase, T eps,
+                                                 const T* __restrict__ x, const T* __restrict__ y,
+                                                 const float* __restrict__ w, T* __restrict__ xo, uint64_t n,
+                                                 uint64_t nrows, uint64_t rowbase, T eps,
                                                  const uint32_t* __restrict__ heavy, uint64_t nheavy,
-                                                 uint32_t nlight, uint32_t hot) {
-    // hub waves first: they are the longest, so they start before the light
-    // blocks instead of forming the tail of the launch
-    const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+                                                 uint32_t nh, uint32_t nlight, uint32_t hot, uint32_t remap) {
     if (blockIdx.x >= nh) {
-        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps, blockIdx.x - nh, hot);
+        uint32_t l = blockIdx.x - nh;
+        if (remap) {
+            const uint32_t per = (nlight + 7) / 8;
+            l = (l % 8) * per + l / 8;
+            if (l >= nlight) return;   // whole block
+        }
+        opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
     } else {
         uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        if (wv < nheavy) opinion_heavy<T>(off, src, x, w, xo, heavy[wv], rowbase, eps, hot);
+        if (wv < nheavy) opinion_heavy<T, PLAN>(off, src, x, y, w, xo, heavy[wv], rowbase, eps, hot);
+    }
+}
+
+
+__global__ void k_plan_keys(const uint32_t* __restrict__ src, uint64_t ne, uint32_t shift,
+                            uint32_t* __restrict__ keys) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ne) keys[k] = (src[k] - 1u) >> shift;
+}
+
+__global__ void k_plan_fill(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint64_t ne,
+                            uint32_t* __restrict__ psrc, uint32_t* __restrict__ pinv) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ne) return;
+    const uint32_t k = perm[j];
+    psrc[j] = src[k];
+    pinv[k] = (uint32_t)j;
+}
+
+// grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U))
+template <class T>
+__global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc, const T* __restrict__ x,
+                                                     T* __restrict__ y, uint64_t ne) {
+    const uint32_t per = gridDim.x / 8;
+    const uint32_t l = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const uint64_t j0 = (uint64_t)l * (256 * GP_U) + threadIdx.x;
+    uint32_t sj[GP_U];
+    T v[GP_U];
+#pragma unroll
+    for (int u = 0; u < GP_U; u++) {
+        const uint64_t j = j0 + (uint64_t)u * 256;
+        sj[u] = j < ne ? __builtin_nontemporal_load(psrc + j) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
+#pragma unroll
+    for (int u = 0; u < GP_U; u++) {
+        const uint64_t j = j0 + (uint64_t)u * 256;
+        if (j < ne) __builtin_nontemporal_store(v[u], y + j);
     }
 }
 
@@ -901,6 +977,41 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
     return VH_OK;
 }
 
+// The plan of the in-edges of E (one target type at row 0; sources 1-based
+// numbers <= nsrc < 2^32).
+int build_gather_plan(Sim& s, EdgeStore& E, uint32_t shift, uint64_t nsrc) {
+    if (E.plan_version == E.version && E.plan_shift == shift) return VH_OK;
+    hipStream_t st = s.stream;
+    const uint64_t ne = E.nedges;
+    RC(E.psrc.ensure(ne * 4 + 8, st));
+    RC(E.pinv.ensure(ne * 4 + 8, st));
+    if (ne) {
+        const uint64_t nparts = (nsrc + (1ull << shift) - 1) >> shift;
+        uint32_t bits = 1;
+        while (bits < 32 && (1ull << bits) < nparts) bits++;
+        uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4);
+        uint32_t* vals = (uint32_t*)s.scratch.take(ne * 4);
+        uint32_t* ka = (uint32_t*)s.scratch.take(ne * 4);
+        uint32_t* va = (uint32_t*)s.scratch.take(ne * 4);
+        if (!keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), ne, shift,
+                           keys);
+        if (radix_sort_pairs(&keys, &vals, ka, va, ne, bits, true, s.scratch, st))
+            return set_error(VH_ERR_DEVICE, "plan sort");
+        hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), vals, ne,
+                           E.psrc.as<uint32_t>(), E.pinv.as<uint32_t>());
+        HIPCHECK(hipGetLastError());
+    }
+    E.plan_version = E.version;
+    E.plan_shift = shift;
+    return VH_OK;
+}
+
+static long long env_ll(const char* name, long long dflt) {
+    const char* v = getenv(name);
+    return v ? atoll(v) : dflt;
+}
+
 int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step) {
     (void)step;
     RC(check_model_layout(s, t, et, f32 ? 4 : 8, 4, "opinion"));
@@ -908,27 +1019,53 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     EdgeStore& E = s.e[et];
     if (!A.writeable) return set_error(VH_ERR_ASSERT, "opinion: the agent type must be in `write`");
     if (!A.nslots) return VH_OK;
+    const uint32_t tb = f32 ? 4 : 8;
     uint64_t nrows = E.rows ? E.nrows[t] : 0;
     if (E.rows) RC(build_heavy_bins(s, E, t));
     uint64_t nheavy = E.rows ? E.nheavy : 0;
     uint32_t nlight = grid_for(A.nslots, 256);
-    uint32_t nh = (uint32_t)((nheavy + 3) / 4);
     // hot prefix of the state kept in L2 (VH_OPINION_HOT_BYTES; default: off)
-    static long long hot_bytes = -2;
-    if (hot_bytes == -2) {
-        const char* v = getenv("VH_OPINION_HOT_BYTES");
-        hot_bytes = v ? atoll(v) : -1;
+    const long long hot_bytes = env_ll("VH_OPINION_HOT_BYTES", -1);
+    // two-pass gather from a state vector of at least VH_OPINION_PLAN_BYTES
+    // (< 0: never) with source partitions of VH_OPINION_PART_BYTES
+    const long long plan_min = env_ll("VH_OPINION_PLAN_BYTES", 16ll << 20);
+    const long long part_bytes = env_ll("VH_OPINION_PART_BYTES", 2ll << 20);
+    uint32_t hot = hot_bytes < 0 ? 0xFFFFFFFFu : (uint32_t)(hot_bytes / tb);
+    // the plan needs one target type at row 0 and 32-bit sources, all of `t`
+    const bool plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
+                      E.rowbase[t] == 0 && nrows == E.rows && (uint64_t)A.nslots * tb >= (uint64_t)plan_min;
+    uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+    uint32_t nl = nlight;
+    if (plan) {
+        uint32_t shift = 0;
+        while ((2ull << shift) * tb <= (uint64_t)part_bytes && shift < 31) shift++;
+        RC(build_gather_plan(s, E, shift, A.nslots));
+        RC(E.pval.ensure(E.nedges * tb + 8, s.stream));
+        uint64_t nb = (E.nedges + 256 * GP_U - 1) / (256 * GP_U);
+        dim3 gg((uint32_t)((nb + 7) / 8 * 8)), bb(256);
+        if (f32)
+            hipLaunchKernelGGL(k_plan_gather<float>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
+                               (const float*)A.rstate(), E.pval.as<float>(), E.nedges);
+        else
+            hipLaunchKernelGGL(k_plan_gather<double>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
+                               (const double*)A.rstate(), E.pval.as<double>(), E.nedges);
+        nh = (nh + 7) / 8 * 8;
+        nl = (nlight + 7) / 8 * 8;
     }
-    uint32_t hot = hot_bytes < 0 ? 0xFFFFFFFFu : (uint32_t)(hot_bytes / (f32 ? 4 : 8));
-    dim3 g(nlight + nh), b(256);
-    if (f32)
-        hipLaunchKernelGGL(k_opinion<float>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
-                           (const float*)A.rstate(), E.st.as<float>(), (float*)A.wstate(), A.nslots,
-                           nrows, E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nlight, hot);
-    else
-        hipLaunchKernelGGL(k_opinion<double>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
-                           (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(), A.nslots,
-                           nrows, E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nlight, hot);
+    dim3 g(nl + nh), b(256);
+    const uint32_t* src = plan ? E.pinv.as<uint32_t>() : E.src.as<uint32_t>();
+    if (f32) {
+        auto k = plan ? k_opinion<float, true> : k_opinion<float, false>;
+        hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const float*)A.rstate(),
+                           E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(), A.nslots, nrows,
+                           E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot,
+                           (uint32_t)plan);
+    } else {
+        auto k = plan ? k_opinion<double, true> : k_opinion<double, false>;
+        hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const double*)A.rstate(),
+                           E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(), A.nslots, nrows,
+                           E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan);
+    }
     HIPCHECK(hipGetLastError());
     return VH_OK;
 }

@@ -69,6 +69,11 @@ struct EdgeStore {
     // degree bins of the opinion kernels, valid for `bins_version`
     DevBuf heavy;                  // uint32 rows with degree > 64
     uint64_t nheavy = 0, bins_version = ~0ull;
+    // gather plan of the opinion kernels (build_gather_plan), valid for
+    // `plan_version` with partitions of 2^plan_shift sources
+    DevBuf psrc, pinv, pval;       // uint32 sources in plan order, plan position per CSR slot, states
+    uint64_t plan_version = ~0ull;
+    uint32_t plan_shift = 0;
     bool writeable = false;
     // in-edges requested as a raster stencil during init (vh_add_edges_stencil)
     bool stencil = false;

@@ -42,13 +42,22 @@ def make_requests(count, rate, seed=1, n_min=10_000, n_max=10_000_000, h_min=50,
     return out
 
 
+# state vectors from this size on are gathered in two passes by the opinion
+# kernels (the library's default of VH_OPINION_PLAN_BYTES)
+OPINION_PLAN_MIN_BYTES = 16 << 20
+
+
 def footprint(req):
     """Device bytes a request holds while active (state, CSR, build scratch)."""
     e = req.n * req.mean_deg
     if req.kind == "sir":
         return int(req.n * 12 + e * 4 + req.n * 8)
     # two Float64 state buffers, offsets, sources + weights, generator scratch
-    return int(req.n * 16 + req.n * 4 + e * 8 + req.n * 8)
+    fp = req.n * 16 + req.n * 4 + e * 8 + req.n * 8
+    if req.n * 8 >= OPINION_PLAN_MIN_BYTES:
+        # the two-pass gather: plan sources, positions and states, sort scratch
+        fp += e * 16 + e * 16
+    return int(fp)
 
 
 def _build_opinion(vh, req, stream=None, keep=None):

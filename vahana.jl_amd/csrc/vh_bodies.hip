@@ -564,6 +564,111 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
 // x[src[k]] and the sums keep their canonical order.
 constexpr int GP_U = 8;   // plan entries per thread in pass 1
 
+// Pass 2 with tile lists: the light in-edges of a 256-agent block are
+// numbered in agent order from lbase[block] (the block's "light edges"), in
+// tiles of OP_TILE. At plan time every tile's entries were sorted by plan
+// position, i.e. by (source partition, CSR slot): entry e of a tile reads
+// y[ty[e]], a run of consecutive plan entries per partition, and lands at
+// slot tslot[e] of the tile in LDS. Weights, terms and the sums in canonical
+// order are as in opinion_block.
+template <class T>
+__device__ __forceinline__ void opinion_block_tiles(const uint32_t* __restrict__ off, const T* __restrict__ x,
+                                                    const T* __restrict__ y, const float* __restrict__ w,
+                                                    T* __restrict__ xo, uint64_t n, uint64_t nrows, T eps,
+                                                    uint32_t blk, const uint32_t* __restrict__ lbase,
+                                                    const uint32_t* __restrict__ ty,
+                                                    const uint16_t* __restrict__ tslot, uint32_t nblkp) {
+    __shared__ T s_val[OP_TILE];   // x_j by slot, then the numerator term
+    __shared__ float s_den[OP_TILE];
+    __shared__ uint8_t s_own[OP_TILE];
+    __shared__ uint32_t s_b[256], s_p[256], s_wsum[4];
+    __shared__ T s_xi[256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint64_t i = (uint64_t)blk * 256 + tid;
+    const bool valid = i < n;
+    uint32_t b = 0, d = 0;
+    T xi = (T)0;
+    if (valid) {
+        xi = __builtin_nontemporal_load(x + i);
+        if (i < nrows) {
+            b = __builtin_nontemporal_load(off + i);
+            d = __builtin_nontemporal_load(off + i + 1) - b;
+        }
+    }
+    const uint32_t lb = blk < nblkp ? lbase[blk] : 0u;
+    const bool light = valid && d <= 64u;
+    const uint32_t ld = light ? d : 0u;
+    uint32_t incl = ld;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y1 = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y1;
+    }
+    if (lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    uint32_t pre = 0, total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t v = s_wsum[q];
+        pre += q < wv ? v : 0u;
+        total += v;
+    }
+    const uint32_t P = pre + incl - ld;
+    s_b[tid] = b;
+    s_p[tid] = P;
+    s_xi[tid] = xi;
+    T num = (T)0, den = (T)0;
+    constexpr int QU = OP_TILE / 256;   // slots of a tile per thread
+    for (uint32_t t0 = 0; t0 < total; t0 += OP_TILE) {
+        const uint32_t lo = P > t0 ? P : t0;
+        const uint32_t hi = (P + ld) < (t0 + OP_TILE) ? (P + ld) : (t0 + OP_TILE);
+        for (uint32_t q = lo; q < hi; q++) s_own[q - t0] = (uint8_t)tid;
+        const uint32_t cnt = (total - t0) < (uint32_t)OP_TILE ? (total - t0) : (uint32_t)OP_TILE;
+        __syncthreads();
+        // weights of this thread's slots q = tid + 256 u (slot order) and
+        // the tile's list entries e = tid + 256 u (plan order), all in flight
+        float wq[QU];
+        uint32_t jj[QU], sl[QU];
+        T v[QU];
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            const uint32_t q = (uint32_t)(tid + 256 * u);
+            wq[u] = 0.0f;
+            jj[u] = 0u;
+            sl[u] = 0xFFFFFFFFu;
+            if (q < cnt) {
+                const uint32_t a = s_own[q];
+                wq[u] = __builtin_nontemporal_load(w + s_b[a] + (t0 + q - s_p[a]));
+                jj[u] = __builtin_nontemporal_load(ty + lb + t0 + q);
+                sl[u] = __builtin_nontemporal_load(tslot + lb + t0 + q);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < QU; u++) v[u] = sl[u] < cnt ? y[jj[u]] : (T)0;
+#pragma unroll
+        for (int u = 0; u < QU; u++)
+            if (sl[u] < cnt) s_val[sl[u]] = v[u];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            const uint32_t q = (uint32_t)(tid + 256 * u);
+            if (q < cnt) {
+                const T xj = s_val[q];
+                const bool acc = fabs(xj - s_xi[s_own[q]]) <= eps;
+                s_val[q] = acc ? (T)wq[u] * xj : (T)0;
+                s_den[q] = acc ? wq[u] : 0.0f;
+            }
+        }
+        __syncthreads();
+        for (uint32_t q = lo; q < hi; q++) {
+            num = num + s_val[q - t0];
+            den = den + (T)s_den[q - t0];
+        }
+        __syncthreads();
+    }
+    if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
+}
+
 // One launch per step: blocks [0, nh) run one wave per heavy agent (hub
 // waves first: they are the longest, so they start before the light blocks
 // instead of forming the tail of the launch), the rest thread-per-agent
@@ -571,14 +676,20 @@ constexpr int GP_U = 8;   // plan entries per thread in pass 1
 // blocks) the light blocks of one XCD take one contiguous range of agents:
 // blocks are dispatched round-robin over the 8 XCDs, and contiguous agents
 // read the plan vector as a few sequential streams that one XCD's L2 keeps.
-template <class T, bool PLAN>
+// MODE 0: one-pass gather from x; 1: y[pinv[k]]; 2: tile lists for the light
+// blocks, y[pinv[k]] for the heavy agents.
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ src,
                                                  const T* __restrict__ x, const T* __restrict__ y,
                                                  const float* __restrict__ w, T* __restrict__ xo, uint64_t n,
                                                  uint64_t nrows, uint64_t rowbase, T eps,
                                                  const uint32_t* __restrict__ heavy, uint64_t nheavy,
-                                                 uint32_t nh, uint32_t nlight, uint32_t hot, uint32_t remap) {
+                                                 uint32_t nh, uint32_t nlight, uint32_t hot, uint32_t remap,
+                                                 const uint32_t* __restrict__ lbase,
+                                                 const uint32_t* __restrict__ ty,
+                                                 const uint16_t* __restrict__ tslot, uint32_t nblkp) {
+    constexpr bool PLAN = MODE != 0;
     if (blockIdx.x >= nh) {
         uint32_t l = blockIdx.x - nh;
         if (remap) {
@@ -586,7 +697,10 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
             l = (l % 8) * per + l / 8;
             if (l >= nlight) return;   // whole block
         }
-        opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
+        if constexpr (MODE == 2)
+            opinion_block_tiles<T>(off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp);
+        else
+            opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
     } else {
         uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
         if (wv < nheavy) opinion_heavy<T, PLAN>(off, src, x, y, w, xo, heavy[wv], rowbase, eps, hot);
@@ -607,6 +721,67 @@ __global__ void k_plan_fill(const uint32_t* __restrict__ src, const uint32_t* __
     const uint32_t k = perm[j];
     psrc[j] = src[k];
     pinv[k] = (uint32_t)j;
+}
+
+// light-edge total of each 256-row block
+__global__ __launch_bounds__(256) void k_light_totals(const uint32_t* __restrict__ off, uint64_t nrows,
+                                                      uint32_t* __restrict__ tot) {
+    __shared__ uint32_t s_w[4];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t d = i < nrows ? off[i + 1] - off[i] : 0u;
+    uint32_t v = d <= 64u ? d : 0u;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) tot[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// per light edge (number lbase[block] + position in the block): sort key
+// (block, tile, source partition), CSR slot and slot within the tile
+__global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ src, uint64_t nrows,
+                                                    const uint32_t* __restrict__ lbase, uint32_t shift,
+                                                    uint32_t pbits, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
+    __shared__ uint32_t s_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + tid;
+    uint32_t b = 0, d = 0;
+    if (i < nrows) {
+        b = off[i];
+        d = off[i + 1] - b;
+    }
+    const uint32_t ld = d <= 64u ? d : 0u;
+    uint32_t incl = ld;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y1 = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y1;
+    }
+    if (lane == 63) s_w[wv] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) pre += q < wv ? s_w[q] : 0u;
+    const uint32_t P = pre + incl - ld, lb = lbase[blockIdx.x];
+    const uint32_t hi = blockIdx.x << (3 + pbits);
+    for (uint32_t e = 0; e < ld; e++) {
+        const uint32_t q = P + e, idx = lb + q;
+        keys[idx] = hi | ((q / (uint32_t)OP_TILE) << pbits) | ((src[b + e] - 1u) >> shift);
+        kidx[idx] = b + e;
+        tsl[idx] = (uint16_t)(q % (uint32_t)OP_TILE);
+    }
+}
+
+__global__ void k_light_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ kidx,
+                             const uint16_t* __restrict__ tsl, const uint32_t* __restrict__ pinv, uint64_t nl,
+                             uint32_t* __restrict__ ty, uint16_t* __restrict__ tslot) {
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nl) return;
+    const uint32_t idx = order[e];
+    ty[e] = pinv[kidx[idx]];
+    tslot[e] = tsl[idx];
 }
 
 // grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U))
@@ -977,6 +1152,52 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
     return VH_OK;
 }
 
+// Tile lists of the light blocks (opinion_block_tiles) from a built plan:
+// light edges numbered per block from lbase, sorted by (block, tile,
+// partition) with a stable radix sort, so each tile lists its entries in
+// plan order. Left out (E.plan_tiles false) when the key needs more than 32
+// bits.
+int build_tile_lists(Sim& s, EdgeStore& E, uint32_t shift, uint32_t nparts) {
+    hipStream_t st = s.stream;
+    E.plan_tiles = false;
+    const uint64_t nrows = E.rows;
+    const uint32_t nblk = (uint32_t)((nrows + 255) / 256);
+    uint32_t bbits = 1, pbits = 1;
+    while ((1ull << bbits) < nblk) bbits++;
+    while ((1ull << pbits) < nparts) pbits++;
+    static_assert(64 * 256 / OP_TILE <= 8, "tiles per block fit 3 key bits");
+    if (bbits + 3 + pbits > 32) return VH_OK;
+    RC(E.lbase.ensure(((uint64_t)nblk + 1) * 4 + 8, st));
+    uint32_t* tot = (uint32_t*)s.scratch.take((uint64_t)nblk * 4 + 4);
+    if (!tot) return set_error(VH_ERR_OOM, "scratch");
+    const uint32_t* off = E.off.as<uint32_t>();
+    hipLaunchKernelGGL(k_light_totals, dim3(nblk), dim3(256), 0, st, off, nrows, tot);
+    if (scan_excl_u32(tot, E.lbase.as<uint32_t>(), nblk, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+    uint32_t nl = 0;
+    HIPCHECK(hipMemcpyAsync(&nl, E.lbase.as<uint32_t>() + nblk, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    RC(E.ty.ensure((uint64_t)nl * 4 + 8, st));
+    RC(E.tslot.ensure((uint64_t)nl * 2 + 8, st));
+    if (nl) {
+        uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
+        uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
+        uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
+        uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
+        uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
+        uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2);
+        if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, E.src.as<uint32_t>(), nrows,
+                           E.lbase.as<uint32_t>(), shift, pbits, keys, kidx, tsl);
+        if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + 3 + pbits, true, s.scratch, st))
+            return set_error(VH_ERR_DEVICE, "tile sort");
+        hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, vals, kidx, tsl,
+                           E.pinv.as<uint32_t>(), (uint64_t)nl, E.ty.as<uint32_t>(), E.tslot.as<uint16_t>());
+        HIPCHECK(hipGetLastError());
+    }
+    E.plan_tiles = true;
+    return VH_OK;
+}
+
 // The plan of the in-edges of E (one target type at row 0; sources 1-based
 // numbers <= nsrc < 2^32).
 int build_gather_plan(Sim& s, EdgeStore& E, uint32_t shift, uint64_t nsrc) {
@@ -1001,6 +1222,7 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t shift, uint64_t nsrc) {
         hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), vals, ne,
                            E.psrc.as<uint32_t>(), E.pinv.as<uint32_t>());
         HIPCHECK(hipGetLastError());
+        RC(build_tile_lists(s, E, shift, (uint32_t)nparts));
     }
     E.plan_version = E.version;
     E.plan_shift = shift;
@@ -1054,17 +1276,22 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     }
     dim3 g(nl + nh), b(256);
     const uint32_t* src = plan ? E.pinv.as<uint32_t>() : E.src.as<uint32_t>();
+    // pass 2 with tile lists unless VH_OPINION_TILES=0
+    const int mode = !plan ? 0 : (E.plan_tiles && env_ll("VH_OPINION_TILES", 1) != 0) ? 2 : 1;
+    const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
     if (f32) {
-        auto k = plan ? k_opinion<float, true> : k_opinion<float, false>;
+        auto k = mode == 2 ? k_opinion<float, 2> : mode == 1 ? k_opinion<float, 1> : k_opinion<float, 0>;
         hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const float*)A.rstate(),
                            E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(), A.nslots, nrows,
                            E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot,
-                           (uint32_t)plan);
+                           (uint32_t)plan, E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(),
+                           nblkp);
     } else {
-        auto k = plan ? k_opinion<double, true> : k_opinion<double, false>;
+        auto k = mode == 2 ? k_opinion<double, 2> : mode == 1 ? k_opinion<double, 1> : k_opinion<double, 0>;
         hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const double*)A.rstate(),
                            E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(), A.nslots, nrows,
-                           E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan);
+                           E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan,
+                           E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(), nblkp);
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

@@ -571,18 +571,31 @@ constexpr int GP_U = 8;   // plan entries per thread in pass 1
 // y[ty[e]], a run of consecutive plan entries per partition, and lands at
 // slot tslot[e] of the tile in LDS. Weights, terms and the sums in canonical
 // order are as in opinion_block.
+// LDS of a tile-list block; a heavy block gives each of its 4 waves a
+// quarter of `val` (OP_TILE / 4 = OPH_CHUNK slots)
 template <class T>
-__device__ __forceinline__ void opinion_block_tiles(const uint32_t* __restrict__ off, const T* __restrict__ x,
-                                                    const T* __restrict__ y, const float* __restrict__ w,
-                                                    T* __restrict__ xo, uint64_t n, uint64_t nrows, T eps,
-                                                    uint32_t blk, const uint32_t* __restrict__ lbase,
+struct TileLDS {
+    T val[OP_TILE];   // x_j by slot, then the numerator term
+    float den[OP_TILE];
+    uint8_t own[OP_TILE];
+    uint32_t b[256], p[256], wsum[4];
+    T xi[256];
+};
+constexpr int OPH_CHUNK = OP_TILE / 4;   // heavy-row slots staged per wave
+
+template <class T>
+__device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32_t* __restrict__ off,
+                                                    const T* __restrict__ x, const T* __restrict__ y,
+                                                    const float* __restrict__ w, T* __restrict__ xo, uint64_t n,
+                                                    uint64_t nrows, T eps, uint32_t blk,
+                                                    const uint32_t* __restrict__ lbase,
                                                     const uint32_t* __restrict__ ty,
                                                     const uint16_t* __restrict__ tslot, uint32_t nblkp) {
-    __shared__ T s_val[OP_TILE];   // x_j by slot, then the numerator term
-    __shared__ float s_den[OP_TILE];
-    __shared__ uint8_t s_own[OP_TILE];
-    __shared__ uint32_t s_b[256], s_p[256], s_wsum[4];
-    __shared__ T s_xi[256];
+    T* s_val = sm.val;
+    float* s_den = sm.den;
+    uint8_t* s_own = sm.own;
+    uint32_t *s_b = sm.b, *s_p = sm.p, *s_wsum = sm.wsum;
+    T* s_xi = sm.xi;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t i = (uint64_t)blk * 256 + tid;
     const bool valid = i < n;
@@ -669,6 +682,70 @@ __device__ __forceinline__ void opinion_block_tiles(const uint32_t* __restrict__
     if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
 }
 
+// A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
+// h are numbered from hbase[h] in CSR order, in chunks of OPH_CHUNK slots,
+// each chunk listed in plan order (hty: plan position, hslot: slot in the
+// chunk). The wave copies a chunk's runs of y into its LDS quarter `buf`,
+// then lane l adds slots l, l + 64, ... in CSR order: the canonical
+// lane-strided partials of opinion_heavy, then the same xor butterfly.
+template <class T>
+__device__ __forceinline__ void opinion_heavy_tiles(const uint32_t* __restrict__ off, const T* __restrict__ x,
+                                                    const T* __restrict__ y, const float* __restrict__ w,
+                                                    T* __restrict__ xo, uint64_t r, uint32_t hb, T eps,
+                                                    const uint32_t* __restrict__ hty,
+                                                    const uint16_t* __restrict__ hslot, T* buf) {
+    constexpr int HU = OPH_CHUNK / 64;
+    const uint32_t lane = threadIdx.x & 63;
+    const T xi = x[r];
+    const uint32_t b = off[r], d = off[r + 1] - b;
+    T num = (T)0, den = (T)0;
+    for (uint32_t c0 = 0; c0 < d; c0 += OPH_CHUNK) {
+        const uint32_t cnt = d - c0 < (uint32_t)OPH_CHUNK ? d - c0 : (uint32_t)OPH_CHUNK;
+        float wq[HU];
+        uint32_t jj[HU], sl[HU];
+        T v[HU];
+#pragma unroll
+        for (int u = 0; u < HU; u++) {
+            const uint32_t q = lane + 64u * (uint32_t)u;
+            wq[u] = 0.0f;
+            jj[u] = 0u;
+            sl[u] = 0xFFFFFFFFu;
+            if (q < cnt) {
+                wq[u] = __builtin_nontemporal_load(w + b + c0 + q);
+                jj[u] = __builtin_nontemporal_load(hty + hb + c0 + q);
+                sl[u] = __builtin_nontemporal_load(hslot + hb + c0 + q);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < HU; u++) v[u] = sl[u] < cnt ? y[jj[u]] : (T)0;
+#pragma unroll
+        for (int u = 0; u < HU; u++)
+            if (sl[u] < cnt) buf[sl[u]] = v[u];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int u = 0; u < HU; u++) {
+            const uint32_t q = lane + 64u * (uint32_t)u;
+            if (q < cnt) {
+                const T xj = buf[q];
+                const bool acc = fabs(xj - xi) <= eps;
+                num = num + (acc ? (T)wq[u] * xj : (T)0);
+                den = den + (acc ? (T)wq[u] : (T)0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        num = num + __shfl_xor(num, o, 64);
+        den = den + __shfl_xor(den, o, 64);
+    }
+    if (lane == 0) xo[r] = den > (T)0 ? num / den : xi;
+}
+
 // One launch per step: blocks [0, nh) run one wave per heavy agent (hub
 // waves first: they are the longest, so they start before the light blocks
 // instead of forming the tail of the launch), the rest thread-per-agent
@@ -688,8 +765,14 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
                                                  uint32_t nh, uint32_t nlight, uint32_t hot, uint32_t remap,
                                                  const uint32_t* __restrict__ lbase,
                                                  const uint32_t* __restrict__ ty,
-                                                 const uint16_t* __restrict__ tslot, uint32_t nblkp) {
+                                                 const uint16_t* __restrict__ tslot, uint32_t nblkp,
+                                                 const uint32_t* __restrict__ hbase,
+                                                 const uint32_t* __restrict__ hty,
+                                                 const uint16_t* __restrict__ hslot) {
     constexpr bool PLAN = MODE != 0;
+    // MODE 2: one LDS area for the light blocks and the heavy waves
+    __shared__ __align__(16) unsigned char smraw[MODE == 2 ? sizeof(TileLDS<T>) : 16];
+    TileLDS<T>& sm = *reinterpret_cast<TileLDS<T>*>(smraw);
     if (blockIdx.x >= nh) {
         uint32_t l = blockIdx.x - nh;
         if (remap) {
@@ -697,12 +780,21 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
             l = (l % 8) * per + l / 8;
             if (l >= nlight) return;   // whole block
         }
-        if constexpr (MODE == 2)
-            opinion_block_tiles<T>(off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp);
-        else
+        if constexpr (MODE == 2) {
+            opinion_block_tiles<T>(sm, off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp);
+        } else {
             opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
+        }
     } else {
         uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        if constexpr (MODE == 2) {
+            if (hbase) {   // one quarter of the block's LDS per wave
+                if (wv < nheavy)
+                    opinion_heavy_tiles<T>(off, x, y, w, xo, heavy[wv], hbase[wv], eps, hty, hslot,
+                                           sm.val + (threadIdx.x >> 6) * OPH_CHUNK);
+                return;
+            }
+        }
         if (wv < nheavy) opinion_heavy<T, PLAN>(off, src, x, y, w, xo, heavy[wv], rowbase, eps, hot);
     }
 }
@@ -782,6 +874,36 @@ __global__ void k_light_fill(const uint32_t* __restrict__ order, const uint32_t*
     const uint32_t idx = order[e];
     ty[e] = pinv[kidx[idx]];
     tslot[e] = tsl[idx];
+}
+
+// in-edge count and OPH_CHUNK-chunk count of each heavy row
+__global__ void k_heavy_sizes(const uint32_t* __restrict__ off, const uint32_t* __restrict__ heavy, uint64_t nheavy,
+                              uint32_t* __restrict__ deg, uint32_t* __restrict__ chunks) {
+    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nheavy) return;
+    const uint32_t r = heavy[h], d = off[r + 1] - off[r];
+    deg[h] = d;
+    chunks[h] = (d + OPH_CHUNK - 1) / OPH_CHUNK;
+}
+
+// one wave per heavy row: sort key (chunk, source partition), CSR slot and
+// slot within the chunk of each of its in-edges (numbered from hbase[h])
+__global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ src,
+                                                    const uint32_t* __restrict__ heavy, uint64_t nheavy,
+                                                    const uint32_t* __restrict__ hbase,
+                                                    const uint32_t* __restrict__ cbase, uint32_t shift,
+                                                    uint32_t pbits, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
+    const uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (h >= nheavy) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t r = heavy[h], b = off[r], d = off[r + 1] - b, hb = hbase[h], cb = cbase[h];
+    for (uint32_t t = lane; t < d; t += 64) {
+        keys[hb + t] = ((cb + t / OPH_CHUNK) << pbits) | ((src[b + t] - 1u) >> shift);
+        kidx[hb + t] = b + t;
+        tsl[hb + t] = (uint16_t)(t % OPH_CHUNK);
+    }
 }
 
 // grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U))
@@ -1195,6 +1317,45 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t shift, uint32_t nparts) {
         HIPCHECK(hipGetLastError());
     }
     E.plan_tiles = true;
+    // heavy rows (E.heavy, in its order): chunk lists
+    E.plan_htiles = false;
+    const uint64_t nheavy = E.nheavy;
+    if (!nheavy) return VH_OK;
+    uint32_t* hd = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
+    uint32_t* hc = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
+    uint32_t* cbase = (uint32_t*)s.scratch.take(nheavy * 4 + 8);
+    if (!hd || !hc || !cbase) return set_error(VH_ERR_OOM, "scratch");
+    RC(E.hbase.ensure(nheavy * 4 + 8, st));
+    hipLaunchKernelGGL(k_heavy_sizes, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off, E.heavy.as<uint32_t>(),
+                       nheavy, hd, hc);
+    if (scan_excl_u32(hd, E.hbase.as<uint32_t>(), nheavy, s.scratch, st) ||
+        scan_excl_u32(hc, cbase, nheavy, s.scratch, st))
+        return set_error(VH_ERR_DEVICE, "scan");
+    uint32_t htot[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(&htot[0], E.hbase.as<uint32_t>() + nheavy, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&htot[1], cbase + nheavy, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const uint32_t nhe = htot[0], nch = htot[1];
+    uint32_t cbits = 1;
+    while ((1ull << cbits) < nch) cbits++;
+    if (cbits + pbits > 32 || !nhe) return VH_OK;
+    RC(E.hty.ensure((uint64_t)nhe * 4 + 8, st));
+    RC(E.hslot.ensure((uint64_t)nhe * 2 + 8, st));
+    uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+    uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+    uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+    uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+    uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+    uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nhe * 2);
+    if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
+    hipLaunchKernelGGL(k_heavy_keys, dim3(grid_for(nheavy * 64, 256)), dim3(256), 0, st, off, E.src.as<uint32_t>(),
+                       E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, shift, pbits, keys, kidx, tsl);
+    if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
+        return set_error(VH_ERR_DEVICE, "chunk sort");
+    hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl,
+                       E.pinv.as<uint32_t>(), (uint64_t)nhe, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
+    HIPCHECK(hipGetLastError());
+    E.plan_htiles = true;
     return VH_OK;
 }
 
@@ -1279,19 +1440,23 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     // pass 2 with tile lists unless VH_OPINION_TILES=0
     const int mode = !plan ? 0 : (E.plan_tiles && env_ll("VH_OPINION_TILES", 1) != 0) ? 2 : 1;
     const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
+    // heavy rows from chunk lists unless VH_OPINION_HEAVY_TILES=0
+    const uint32_t* hb = mode == 2 && E.plan_htiles && env_ll("VH_OPINION_HEAVY_TILES", 1) != 0
+                             ? E.hbase.as<uint32_t>() : nullptr;
     if (f32) {
         auto k = mode == 2 ? k_opinion<float, 2> : mode == 1 ? k_opinion<float, 1> : k_opinion<float, 0>;
         hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const float*)A.rstate(),
                            E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(), A.nslots, nrows,
                            E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot,
                            (uint32_t)plan, E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(),
-                           nblkp);
+                           nblkp, hb, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
     } else {
         auto k = mode == 2 ? k_opinion<double, 2> : mode == 1 ? k_opinion<double, 1> : k_opinion<double, 0>;
         hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const double*)A.rstate(),
                            E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(), A.nslots, nrows,
                            E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan,
-                           E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(), nblkp);
+                           E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(), nblkp, hb,
+                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

@@ -73,7 +73,8 @@ struct EdgeStore {
     // `plan_version` with partitions of 2^plan_shift sources
     DevBuf psrc, pinv, pval;       // uint32 sources in plan order, plan position per CSR slot, states
     DevBuf lbase, ty, tslot;       // light-edge base per block, tile lists (plan position, slot)
-    bool plan_tiles = false;       // tile lists built
+    DevBuf hbase, hty, hslot;      // the same for the heavy rows (base per heavy row, chunk lists)
+    bool plan_tiles = false, plan_htiles = false;   // tile lists, heavy chunk lists built
     uint64_t plan_version = ~0ull;
     uint32_t plan_shift = 0;
     bool writeable = false;

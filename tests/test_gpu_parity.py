@@ -59,22 +59,26 @@ def test_sir_parity(vh, ora, gpu_ready):
 
 
 # neighbour-state gather (VH_OPINION_PLAN_BYTES, VH_OPINION_PART_BYTES,
-# VH_OPINION_TILES, VH_OPINION_HEAVY_TILES): direct; two-pass plan with 16 or
-# 8 source partitions, pass 2 from tile and chunk lists, from tile lists with
-# the heavy agents by y[pinv[k]], or all by y[pinv[k]]; two-pass plan with one
-# source per partition (4000 partitions, a two-digit plan sort)
-GATHER_MODES = {"direct": ("-1", None, "1", "1"), "plan": ("0", "2048", "1", "1"),
-                "plan_heavy_pinv": ("0", "2048", "1", "0"), "plan_pinv": ("0", "2048", "0", "1"),
-                "plan_1src": ("0", "1", "1", "1")}
+# VH_OPINION_TILES, VH_OPINION_HEAVY_TILES, VH_OPINION_PACKED): direct;
+# two-pass plan with 16 (f64) or 8 (f32) source partitions and pass 2 from
+# packed tile lists and chunk lists, from packed tile lists with the heavy
+# agents by y[pinv[k]], from unpacked tile lists, or all by y[pinv[k]];
+# two-pass plan with one source per partition (4000 partitions, a two-digit
+# plan sort, unpacked tile lists)
+GATHER_MODES = {"direct": ("-1", None, "1", "1", "1"), "plan": ("0", "2048", "1", "1", "1"),
+                "plan_heavy_pinv": ("0", "2048", "1", "0", "1"),
+                "plan_unpacked": ("0", "2048", "1", "1", "0"), "plan_pinv": ("0", "2048", "0", "1", "1"),
+                "plan_1src": ("0", "1", "1", "1", "1")}
 
 
 @pytest.mark.parametrize("mode", list(GATHER_MODES))
 @pytest.mark.parametrize("f32", [False, True])
 def test_opinion_parity(vh, ora, gpu_ready, f32, mode, monkeypatch):
-    plan_bytes, part_bytes, tiles, heavy_tiles = GATHER_MODES[mode]
+    plan_bytes, part_bytes, tiles, heavy_tiles, packed = GATHER_MODES[mode]
     monkeypatch.setenv("VH_OPINION_PLAN_BYTES", plan_bytes)
     monkeypatch.setenv("VH_OPINION_TILES", tiles)
     monkeypatch.setenv("VH_OPINION_HEAVY_TILES", heavy_tiles)
+    monkeypatch.setenv("VH_OPINION_PACKED", packed)
     if part_bytes:
         monkeypatch.setenv("VH_OPINION_PART_BYTES", part_bytes)
     print(f"[gpu] opinion parity f32={f32} gather={mode}", flush=True)

@@ -554,7 +554,8 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
 // ----------------------------------------------------------------- gather plan
 // For a large state vector the neighbour states of a static edge store are
 // gathered in two passes through a plan: the CSR slots sorted (stably) by
-// source partition, a partition being 2^shift consecutive sources sized to
+// source partition and, within one, light rows' slots before heavy rows'
+// (degree > 64), a partition being `psize` consecutive sources sized to
 // sit in one XCD's L2. Pass 1 (k_plan_gather) walks the plan in order,
 // y[j] = x[psrc[j]]: each XCD takes a contiguous range of it, so its gathers
 // hit the one partition its L2 holds, and y is written sequentially. Pass 2
@@ -580,17 +581,27 @@ struct TileLDS {
     uint8_t own[OP_TILE];
     uint32_t b[256], p[256], wsum[4];
     T xi[256];
+    uint32_t prun[32];
+    uint16_t pfirst[32];
 };
 constexpr int OPH_CHUNK = OP_TILE / 4;   // heavy-row slots staged per wave
+constexpr int OPP_RUNS = 32;              // partitions of a packed tile list (5 bits beside 11)
+constexpr int OPP_TILES = 64 * 256 / OP_TILE;   // tiles of a light block at most
 
-template <class T>
+// PACKED: the list entry is slot | partition << 11, and its plan position is
+// prun[b][t][p] + (entry - pfirst[b][t][p]) for tile t of block b; ty is not
+// read. Threads 0-31 fetch the first tile's run table with the agents' data
+// and each next tile's while the current one is processed.
+template <class T, bool PACKED>
 __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32_t* __restrict__ off,
                                                     const T* __restrict__ x, const T* __restrict__ y,
                                                     const float* __restrict__ w, T* __restrict__ xo, uint64_t n,
                                                     uint64_t nrows, T eps, uint32_t blk,
                                                     const uint32_t* __restrict__ lbase,
                                                     const uint32_t* __restrict__ ty,
-                                                    const uint16_t* __restrict__ tslot, uint32_t nblkp) {
+                                                    const uint16_t* __restrict__ tslot, uint32_t nblkp,
+                                                    const uint32_t* __restrict__ prun,
+                                                    const uint16_t* __restrict__ pfirst) {
     T* s_val = sm.val;
     float* s_den = sm.den;
     uint8_t* s_own = sm.own;
@@ -609,6 +620,11 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
         }
     }
     const uint32_t lb = blk < nblkp ? lbase[blk] : 0u;
+    uint32_t nx_run = 0, nx_first = 0;   // run table entry `tid` of the next tile
+    if (PACKED && tid < OPP_RUNS && blk < nblkp) {
+        nx_run = prun[(uint64_t)blk * OPP_TILES * OPP_RUNS + tid];
+        nx_first = pfirst[(uint64_t)blk * OPP_TILES * OPP_RUNS + tid];
+    }
     const bool light = valid && d <= 64u;
     const uint32_t ld = light ? d : 0u;
     uint32_t incl = ld;
@@ -637,12 +653,32 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
         const uint32_t hi = (P + ld) < (t0 + OP_TILE) ? (P + ld) : (t0 + OP_TILE);
         for (uint32_t q = lo; q < hi; q++) s_own[q - t0] = (uint8_t)tid;
         const uint32_t cnt = (total - t0) < (uint32_t)OP_TILE ? (total - t0) : (uint32_t)OP_TILE;
+        if (PACKED && tid < OPP_RUNS) {
+            sm.prun[tid] = nx_run;
+            sm.pfirst[tid] = (uint16_t)nx_first;
+            if (t0 + OP_TILE < total) {
+                const uint64_t g = (uint64_t)blk * OPP_TILES + t0 / OP_TILE + 1;
+                nx_run = prun[g * OPP_RUNS + tid];
+                nx_first = pfirst[g * OPP_RUNS + tid];
+            }
+        }
         __syncthreads();
         // weights of this thread's slots q = tid + 256 u (slot order) and
         // the tile's list entries e = tid + 256 u (plan order), all in flight
         float wq[QU];
-        uint32_t jj[QU], sl[QU];
+        uint32_t jj[QU], sl[QU], wi[QU];
         T v[QU];
+        // the weight index of every slot first (LDS only), then all global
+        // loads at once
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            const uint32_t q = (uint32_t)(tid + 256 * u);
+            wi[u] = 0xFFFFFFFFu;
+            if (q < cnt) {
+                const uint32_t a = s_own[q];
+                wi[u] = s_b[a] + (t0 + q - s_p[a]);
+            }
+        }
 #pragma unroll
         for (int u = 0; u < QU; u++) {
             const uint32_t q = (uint32_t)(tid + 256 * u);
@@ -650,10 +686,21 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
             jj[u] = 0u;
             sl[u] = 0xFFFFFFFFu;
             if (q < cnt) {
-                const uint32_t a = s_own[q];
-                wq[u] = __builtin_nontemporal_load(w + s_b[a] + (t0 + q - s_p[a]));
-                jj[u] = __builtin_nontemporal_load(ty + lb + t0 + q);
+                wq[u] = __builtin_nontemporal_load(w + wi[u]);
+                if constexpr (!PACKED) jj[u] = __builtin_nontemporal_load(ty + lb + t0 + q);
                 sl[u] = __builtin_nontemporal_load(tslot + lb + t0 + q);
+            }
+        }
+        if constexpr (PACKED) {
+            // (after all loads are issued) entry -> slot and plan position
+#pragma unroll
+            for (int u = 0; u < QU; u++) {
+                const uint32_t q = (uint32_t)(tid + 256 * u);
+                if (q < cnt) {
+                    const uint32_t pk = sl[u], pp = pk >> 11;
+                    sl[u] = pk & 2047u;
+                    jj[u] = sm.prun[pp] + (q - sm.pfirst[pp]);
+                }
             }
         }
 #pragma unroll
@@ -754,7 +801,8 @@ __device__ __forceinline__ void opinion_heavy_tiles(const uint32_t* __restrict__
 // blocks are dispatched round-robin over the 8 XCDs, and contiguous agents
 // read the plan vector as a few sequential streams that one XCD's L2 keeps.
 // MODE 0: one-pass gather from x; 1: y[pinv[k]]; 2: tile lists for the light
-// blocks, y[pinv[k]] for the heavy agents.
+// blocks (chunk lists or y[pinv[k]] for the heavy agents); 3: as 2 with packed
+// tile lists.
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ src,
@@ -768,10 +816,13 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
                                                  const uint16_t* __restrict__ tslot, uint32_t nblkp,
                                                  const uint32_t* __restrict__ hbase,
                                                  const uint32_t* __restrict__ hty,
-                                                 const uint16_t* __restrict__ hslot) {
+                                                 const uint16_t* __restrict__ hslot,
+                                                 const uint32_t* __restrict__ prun,
+                                                 const uint16_t* __restrict__ pfirst) {
     constexpr bool PLAN = MODE != 0;
-    // MODE 2: one LDS area for the light blocks and the heavy waves
-    __shared__ __align__(16) unsigned char smraw[MODE == 2 ? sizeof(TileLDS<T>) : 16];
+    constexpr bool TILES = MODE >= 2;
+    // tile modes: one LDS area for the light blocks and the heavy waves
+    __shared__ __align__(16) unsigned char smraw[TILES ? sizeof(TileLDS<T>) : 16];
     TileLDS<T>& sm = *reinterpret_cast<TileLDS<T>*>(smraw);
     if (blockIdx.x >= nh) {
         uint32_t l = blockIdx.x - nh;
@@ -780,14 +831,15 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
             l = (l % 8) * per + l / 8;
             if (l >= nlight) return;   // whole block
         }
-        if constexpr (MODE == 2) {
-            opinion_block_tiles<T>(sm, off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp);
+        if constexpr (TILES) {
+            opinion_block_tiles<T, MODE == 3>(sm, off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp,
+                                              prun, pfirst);
         } else {
             opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
         }
     } else {
         uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        if constexpr (MODE == 2) {
+        if constexpr (TILES) {
             if (hbase) {   // one quarter of the block's LDS per wave
                 if (wv < nheavy)
                     opinion_heavy_tiles<T>(off, x, y, w, xo, heavy[wv], hbase[wv], eps, hty, hslot,
@@ -800,10 +852,15 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
 }
 
 
-__global__ void k_plan_keys(const uint32_t* __restrict__ src, uint64_t ne, uint32_t shift,
-                            uint32_t* __restrict__ keys) {
-    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < ne) keys[k] = (src[k] - 1u) >> shift;
+// plan sort key of each CSR slot: source partition, then 1 for the slots of
+// a heavy row (degree > 64), so that within a partition the light slots come
+// first, in CSR order
+__global__ void k_plan_keys(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t nrows,
+                            uint32_t psize, uint32_t* __restrict__ keys) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const uint32_t b = off[r], e = off[r + 1], hv = e - b > 64u ? 1u : 0u;
+    for (uint32_t k = b; k < e; k++) keys[k] = ((src[k] - 1u) / psize) << 1 | hv;
 }
 
 __global__ void k_plan_fill(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint64_t ne,
@@ -833,7 +890,7 @@ __global__ __launch_bounds__(256) void k_light_totals(const uint32_t* __restrict
 // (block, tile, source partition), CSR slot and slot within the tile
 __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ src, uint64_t nrows,
-                                                    const uint32_t* __restrict__ lbase, uint32_t shift,
+                                                    const uint32_t* __restrict__ lbase, uint32_t psize,
                                                     uint32_t pbits, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
     __shared__ uint32_t s_w[4];
@@ -860,7 +917,7 @@ __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__
     const uint32_t hi = blockIdx.x << (3 + pbits);
     for (uint32_t e = 0; e < ld; e++) {
         const uint32_t q = P + e, idx = lb + q;
-        keys[idx] = hi | ((q / (uint32_t)OP_TILE) << pbits) | ((src[b + e] - 1u) >> shift);
+        keys[idx] = hi | ((q / (uint32_t)OP_TILE) << pbits) | ((src[b + e] - 1u) / psize);
         kidx[idx] = b + e;
         tsl[idx] = (uint16_t)(q % (uint32_t)OP_TILE);
     }
@@ -875,6 +932,31 @@ __global__ void k_light_fill(const uint32_t* __restrict__ order, const uint32_t*
     ty[e] = pinv[kidx[idx]];
     tslot[e] = tsl[idx];
 }
+
+// packed tile lists (at most OPP_RUNS partitions): per entry of the sorted
+// light lists, slot | partition << 11; for tile t of block b and each
+// partition p present, the plan position of its run's first entry
+// (prun[(b * OPP_TILES + t) * OPP_RUNS + p]) and that entry's index in the
+// tile (pfirst[...])
+__global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+                            const uint32_t* __restrict__ kidx, const uint16_t* __restrict__ tsl,
+                            const uint32_t* __restrict__ pinv, const uint32_t* __restrict__ lbase,
+                            uint32_t pbits, uint64_t nl, uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
+                            uint16_t* __restrict__ pfirst) {
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nl) return;
+    const uint32_t key = keys[e], p = key & ((1u << pbits) - 1u);
+    const uint32_t blk = key >> (3 + pbits), t = (key >> pbits) & 7u;
+    const uint32_t t0 = lbase[blk] + t * (uint32_t)OP_TILE;
+    const uint32_t idx = order[e];
+    tpk[e] = (uint16_t)(tsl[idx] | (p << 11));
+    if (e == t0 || keys[e - 1] != key) {
+        const uint64_t g = (uint64_t)blk * OPP_TILES + t;
+        prun[g * OPP_RUNS + p] = pinv[kidx[idx]];
+        pfirst[g * OPP_RUNS + p] = (uint16_t)(e - t0);
+    }
+}
+
 
 // in-edge count and OPH_CHUNK-chunk count of each heavy row
 __global__ void k_heavy_sizes(const uint32_t* __restrict__ off, const uint32_t* __restrict__ heavy, uint64_t nheavy,
@@ -892,7 +974,7 @@ __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ src,
                                                     const uint32_t* __restrict__ heavy, uint64_t nheavy,
                                                     const uint32_t* __restrict__ hbase,
-                                                    const uint32_t* __restrict__ cbase, uint32_t shift,
+                                                    const uint32_t* __restrict__ cbase, uint32_t psize,
                                                     uint32_t pbits, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
     const uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -900,7 +982,7 @@ __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t r = heavy[h], b = off[r], d = off[r + 1] - b, hb = hbase[h], cb = cbase[h];
     for (uint32_t t = lane; t < d; t += 64) {
-        keys[hb + t] = ((cb + t / OPH_CHUNK) << pbits) | ((src[b + t] - 1u) >> shift);
+        keys[hb + t] = ((cb + t / OPH_CHUNK) << pbits) | ((src[b + t] - 1u) / psize);
         kidx[hb + t] = b + t;
         tsl[hb + t] = (uint16_t)(t % OPH_CHUNK);
     }
@@ -1279,9 +1361,10 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
 // partition) with a stable radix sort, so each tile lists its entries in
 // plan order. Left out (E.plan_tiles false) when the key needs more than 32
 // bits.
-int build_tile_lists(Sim& s, EdgeStore& E, uint32_t shift, uint32_t nparts) {
+int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
     hipStream_t st = s.stream;
     E.plan_tiles = false;
+    E.plan_packed = false;
     const uint64_t nrows = E.rows;
     const uint32_t nblk = (uint32_t)((nrows + 255) / 256);
     uint32_t bbits = 1, pbits = 1;
@@ -1309,12 +1392,25 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t shift, uint32_t nparts) {
         uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2);
         if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
         hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, E.src.as<uint32_t>(), nrows,
-                           E.lbase.as<uint32_t>(), shift, pbits, keys, kidx, tsl);
+                           E.lbase.as<uint32_t>(), psize, pbits, keys, kidx, tsl);
         if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + 3 + pbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "tile sort");
         hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, vals, kidx, tsl,
                            E.pinv.as<uint32_t>(), (uint64_t)nl, E.ty.as<uint32_t>(), E.tslot.as<uint16_t>());
         HIPCHECK(hipGetLastError());
+        if (nparts <= (uint32_t)OPP_RUNS) {
+            // packed lists: 16-bit entries, a run table per tile (a fixed
+            // OPP_TILES x OPP_RUNS slot per block)
+            const uint64_t nt = (uint64_t)nblk * OPP_TILES * OPP_RUNS;
+            RC(E.prun.ensure(nt * 4 + 8, st));
+            RC(E.pfirst.ensure(nt * 2 + 8, st));
+            RC(E.tpk.ensure((uint64_t)nl * 2 + 8, st));
+            hipLaunchKernelGGL(k_pack_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, tsl,
+                               E.pinv.as<uint32_t>(), E.lbase.as<uint32_t>(), pbits, (uint64_t)nl,
+                               E.tpk.as<uint16_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>());
+            HIPCHECK(hipGetLastError());
+            E.plan_packed = true;
+        }
     }
     E.plan_tiles = true;
     // heavy rows (E.heavy, in its order): chunk lists
@@ -1349,7 +1445,7 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t shift, uint32_t nparts) {
     uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nhe * 2);
     if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
     hipLaunchKernelGGL(k_heavy_keys, dim3(grid_for(nheavy * 64, 256)), dim3(256), 0, st, off, E.src.as<uint32_t>(),
-                       E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, shift, pbits, keys, kidx, tsl);
+                       E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, psize, pbits, keys, kidx, tsl);
     if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
         return set_error(VH_ERR_DEVICE, "chunk sort");
     hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl,
@@ -1361,14 +1457,14 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t shift, uint32_t nparts) {
 
 // The plan of the in-edges of E (one target type at row 0; sources 1-based
 // numbers <= nsrc < 2^32).
-int build_gather_plan(Sim& s, EdgeStore& E, uint32_t shift, uint64_t nsrc) {
-    if (E.plan_version == E.version && E.plan_shift == shift) return VH_OK;
+int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc) {
+    if (E.plan_version == E.version && E.plan_psize == psize) return VH_OK;
     hipStream_t st = s.stream;
     const uint64_t ne = E.nedges;
     RC(E.psrc.ensure(ne * 4 + 8, st));
     RC(E.pinv.ensure(ne * 4 + 8, st));
     if (ne) {
-        const uint64_t nparts = (nsrc + (1ull << shift) - 1) >> shift;
+        const uint64_t nparts = (nsrc + psize - 1) / psize;
         uint32_t bits = 1;
         while (bits < 32 && (1ull << bits) < nparts) bits++;
         uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4);
@@ -1376,17 +1472,17 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t shift, uint64_t nsrc) {
         uint32_t* ka = (uint32_t*)s.scratch.take(ne * 4);
         uint32_t* va = (uint32_t*)s.scratch.take(ne * 4);
         if (!keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
-        hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), ne, shift,
-                           keys);
-        if (radix_sort_pairs(&keys, &vals, ka, va, ne, bits, true, s.scratch, st))
+        hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(E.rows, 256)), dim3(256), 0, st, E.off.as<uint32_t>(),
+                           E.src.as<uint32_t>(), E.rows, psize, keys);
+        if (radix_sort_pairs(&keys, &vals, ka, va, ne, bits + 1, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "plan sort");
         hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), vals, ne,
                            E.psrc.as<uint32_t>(), E.pinv.as<uint32_t>());
         HIPCHECK(hipGetLastError());
-        RC(build_tile_lists(s, E, shift, (uint32_t)nparts));
+        RC(build_tile_lists(s, E, psize, (uint32_t)nparts));
     }
     E.plan_version = E.version;
-    E.plan_shift = shift;
+    E.plan_psize = psize;
     return VH_OK;
 }
 
@@ -1410,8 +1506,11 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     // hot prefix of the state kept in L2 (VH_OPINION_HOT_BYTES; default: off)
     const long long hot_bytes = env_ll("VH_OPINION_HOT_BYTES", -1);
     // two-pass gather from a state vector of at least VH_OPINION_PLAN_BYTES
-    // (< 0: never) with source partitions of VH_OPINION_PART_BYTES
+    // (< 0: never) with source partitions of VH_OPINION_PART_BYTES (default
+    // 2 MiB, widened up to 1.5x to make at most 32 partitions, which the
+    // packed tile lists need)
     const long long plan_min = env_ll("VH_OPINION_PLAN_BYTES", 16ll << 20);
+    const bool part_set = getenv("VH_OPINION_PART_BYTES") != nullptr;
     const long long part_bytes = env_ll("VH_OPINION_PART_BYTES", 2ll << 20);
     uint32_t hot = hot_bytes < 0 ? 0xFFFFFFFFu : (uint32_t)(hot_bytes / tb);
     // the plan needs one target type at row 0 and 32-bit sources, all of `t`
@@ -1420,9 +1519,13 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     uint32_t nh = (uint32_t)((nheavy + 3) / 4);
     uint32_t nl = nlight;
     if (plan) {
-        uint32_t shift = 0;
-        while ((2ull << shift) * tb <= (uint64_t)part_bytes && shift < 31) shift++;
-        RC(build_gather_plan(s, E, shift, A.nslots));
+        uint64_t psize = part_bytes / tb;
+        if (psize < 1) psize = 1;
+        if (psize > 0x7FFFFFFFull) psize = 0x7FFFFFFFull;
+        const uint64_t np = (A.nslots + psize - 1) / psize;
+        if (!part_set && np > (uint64_t)OPP_RUNS && np * 2 <= (uint64_t)OPP_RUNS * 3)
+            psize = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;
+        RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots));
         RC(E.pval.ensure(E.nedges * tb + 8, s.stream));
         uint64_t nb = (E.nedges + 256 * GP_U - 1) / (256 * GP_U);
         dim3 gg((uint32_t)((nb + 7) / 8 * 8)), bb(256);
@@ -1438,25 +1541,36 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     dim3 g(nl + nh), b(256);
     const uint32_t* src = plan ? E.pinv.as<uint32_t>() : E.src.as<uint32_t>();
     // pass 2 with tile lists unless VH_OPINION_TILES=0
-    const int mode = !plan ? 0 : (E.plan_tiles && env_ll("VH_OPINION_TILES", 1) != 0) ? 2 : 1;
+    int mode = !plan ? 0 : (E.plan_tiles && env_ll("VH_OPINION_TILES", 1) != 0) ? 2 : 1;
+    // packed tile lists unless VH_OPINION_PACKED=0
+    if (mode == 2 && E.plan_packed && env_ll("VH_OPINION_PACKED", 1) != 0) mode = 3;
+    const uint16_t* tsl_arg = mode == 3 ? E.tpk.as<uint16_t>() : E.tslot.as<uint16_t>();
     const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
     // heavy rows from chunk lists unless VH_OPINION_HEAVY_TILES=0
-    const uint32_t* hb = mode == 2 && E.plan_htiles && env_ll("VH_OPINION_HEAVY_TILES", 1) != 0
+    const uint32_t* hb = mode >= 2 && E.plan_htiles && env_ll("VH_OPINION_HEAVY_TILES", 1) != 0
                              ? E.hbase.as<uint32_t>() : nullptr;
     if (f32) {
-        auto k = mode == 2 ? k_opinion<float, 2> : mode == 1 ? k_opinion<float, 1> : k_opinion<float, 0>;
+        auto k = mode == 3   ? k_opinion<float, 3>
+                 : mode == 2 ? k_opinion<float, 2>
+                 : mode == 1 ? k_opinion<float, 1>
+                             : k_opinion<float, 0>;
         hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const float*)A.rstate(),
                            E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(), A.nslots, nrows,
                            E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot,
-                           (uint32_t)plan, E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(),
-                           nblkp, hb, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
+                           (uint32_t)plan, E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), tsl_arg, nblkp, hb,
+                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
+                           E.pfirst.as<uint16_t>());
     } else {
-        auto k = mode == 2 ? k_opinion<double, 2> : mode == 1 ? k_opinion<double, 1> : k_opinion<double, 0>;
+        auto k = mode == 3   ? k_opinion<double, 3>
+                 : mode == 2 ? k_opinion<double, 2>
+                 : mode == 1 ? k_opinion<double, 1>
+                             : k_opinion<double, 0>;
         hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const double*)A.rstate(),
                            E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(), A.nslots, nrows,
                            E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan,
-                           E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), E.tslot.as<uint16_t>(), nblkp, hb,
-                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
+                           E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), tsl_arg, nblkp, hb,
+                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
+                           E.pfirst.as<uint16_t>());
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

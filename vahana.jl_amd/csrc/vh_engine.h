@@ -74,9 +74,10 @@ struct EdgeStore {
     DevBuf psrc, pinv, pval;       // uint32 sources in plan order, plan position per CSR slot, states
     DevBuf lbase, ty, tslot;       // light-edge base per block, tile lists (plan position, slot)
     DevBuf hbase, hty, hslot;      // the same for the heavy rows (base per heavy row, chunk lists)
-    bool plan_tiles = false, plan_htiles = false;   // tile lists, heavy chunk lists built
+    DevBuf prun, pfirst, tpk;      // packed tile lists: run tables per block and tile, entries
+    bool plan_tiles = false, plan_htiles = false, plan_packed = false;
     uint64_t plan_version = ~0ull;
-    uint32_t plan_shift = 0;
+    uint32_t plan_psize = 0;       // sources per partition
     bool writeable = false;
     // in-edges requested as a raster stencil during init (vh_add_edges_stencil)
     bool stencil = false;

@@ -55,8 +55,9 @@ def footprint(req):
     # two Float64 state buffers, offsets, sources + weights, generator scratch
     fp = req.n * 16 + req.n * 4 + e * 8 + req.n * 8
     if req.n * 8 >= OPINION_PLAN_MIN_BYTES:
-        # the two-pass gather: plan sources, positions and states, sort scratch
-        fp += e * 16 + e * 16
+        # the two-pass gather: plan sources, positions and states, tile lists
+        # and run tables; the scratch of their one-time sorts
+        fp += e * 24 + req.n * 8 + e * 40
     return int(fp)
 
 

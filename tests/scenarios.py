@@ -530,15 +530,17 @@ def edges_iter_removal(V):
 
 
 # ---------------------------------------------------------------- raster
-def raster_model(V):
-    """raster_model test/raster.jl:1-33."""
+def raster_model(V, immortal=False):
+    """raster_model test/raster.jl:1-33 (`immortal`: the two cell types with
+    the :Immortal hint, which changes nothing the diffusion observes)."""
     GridA = V.struct("GridA", [("pos", "<i8", (2,)), ("active", "?")])
     Grid3D = V.struct("Grid3D", [("pos", "<i8", (3,)), ("active", "?")])
     GridE = V.struct("GridE")
     Position = V.struct("Position", [("ids_sum", "<i8")])
     MovingAgent = V.struct("MovingAgent", [("value", "<i8")])
     OnPosition = V.struct("OnPosition")
-    model = (V.ModelTypes() | V.register_agenttype_(GridA) | V.register_agenttype_(Grid3D)
+    hints = ("Immortal",) if immortal else ()
+    model = (V.ModelTypes() | V.register_agenttype_(GridA, *hints) | V.register_agenttype_(Grid3D, *hints)
              | V.register_edgetype_(GridE) | V.register_agenttype_(Position)
              | V.register_agenttype_(MovingAgent) | V.register_edgetype_(OnPosition)
              | V.create_model("Raster_Test"))
@@ -560,11 +562,11 @@ def _diffusion(V, model, T, GridE, dims, active, steps, uniform=False, **conn):
     return sim, counts
 
 
-def raster_diffusion(V, uniform=False):
+def raster_diffusion(V, uniform=False, immortal=False):
     """test/raster.jl:36-238: active cells after 0, 1, 2 diffusion steps on 2D
     and 3D rasters under each metric, distance and periodicity; the
     calc_rasterstate / calc_raster readouts of the first raster."""
-    model, GridA, Grid3D, GridE, *_ = raster_model(V)
+    model, GridA, Grid3D, GridE, *_ = raster_model(V, immortal)
     out = {}
     sim, out["2d_10x8"] = _diffusion(V, model, GridA, GridE, (10, 8), (10, 8), 2,
                                      uniform)                                               # :43-58

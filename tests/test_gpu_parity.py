@@ -208,6 +208,17 @@ def test_raster_device_stencil_known_answers(vh, ora, gpu_ready):
     assert got["2d_10x8"] == [1, 9, 25] and got["3d_euclidean"] == [1, 19]
 
 
+def test_raster_stencil_diffuse_known_answers(vh, ora, gpu_ready):
+    """With :Immortal cells on a device stencil, diffuse runs the stencil
+    kernel (neighbours from the stencil arithmetic, not the CSR): the known
+    answers of every metric, distance and periodicity case are unchanged."""
+    print("[gpu] raster stencil diffuse", flush=True)
+    got = SC.raster_diffusion(vh, uniform=True, immortal=True)
+    want = SC.raster_diffusion(ora)
+    assert got == want
+    assert got["2d_10x8"] == [1, 9, 25] and got["3d_euclidean"] == [1, 19]
+
+
 STENCIL_CASES = [((10, 8), 1, "chebyshev", True), ((10, 8), 1, "chebyshev", False),
                  ((2, 3), 1, "chebyshev", True),       # wrapped duplicates
                  ((1, 5), 2, "chebyshev", True),

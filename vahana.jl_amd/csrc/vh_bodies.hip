@@ -1280,6 +1280,64 @@ __global__ void k_stencil_fill(StencilSpec sp, uint64_t ncells, uint64_t rowbase
     }
 }
 
+// diffuse (test/raster.jl diffuse) of every slot of the cell type: a slot is
+// active when it is or some in-neighbour is; the in-neighbours of cell c are
+// the cells c - offset of the stencil (wrapped when periodic), the sources
+// k_stencil_fill gives its CSR row. Other slots of the type have no in-edges.
+__global__ __launch_bounds__(256) void k_diffuse_stencil(StencilSpec sp, const uint8_t* __restrict__ rs,
+                                                         uint8_t* __restrict__ ws, uint64_t nslots, uint64_t ncells,
+                                                         uint32_t bytes, uint32_t f) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const uint8_t* r = rs + i * bytes;
+    uint8_t* w = ws + i * bytes;
+    for (uint32_t k = 0; k < bytes; k++) w[k] = r[k];
+    uint8_t v = r[f] ? 1 : 0;
+    const uint64_t nr = i + 1;
+    if (!v && nr >= sp.first_nr && nr - sp.first_nr < ncells) {
+        if (ncells <= 0x7FFFFFFFull) {
+            // 32-bit coordinates; a wrap needs one add or subtract while the
+            // offset is at most the extent (else the general path below)
+            int32_t c[4], dim[4];
+            uint32_t rest = (uint32_t)(nr - sp.first_nr);
+            for (uint32_t d = 0; d < sp.ndims; d++) {
+                dim[d] = (int32_t)sp.dims[d];
+                c[d] = (int32_t)(rest % (uint32_t)dim[d]);
+                rest /= (uint32_t)dim[d];
+            }
+            for (uint32_t k = 0; k < sp.nst && !v; k++) {
+                uint32_t lin = 0, stride = 1;
+                bool in = true;
+                for (uint32_t d = 0; d < sp.ndims; d++) {
+                    int32_t o = c[d] - (int32_t)sp.off[k][d];
+                    if (o < 0 || o >= dim[d]) {
+                        if (!sp.periodic) {
+                            in = false;
+                            break;
+                        }
+                        o = o < 0 ? o + dim[d] : o - dim[d];
+                        if (o < 0 || o >= dim[d]) {
+                            o %= dim[d];
+                            if (o < 0) o += dim[d];
+                        }
+                    }
+                    lin += (uint32_t)o * stride;
+                    stride *= (uint32_t)dim[d];
+                }
+                if (in && rs[(sp.first_nr - 1 + lin) * bytes + f]) v = 1;
+            }
+        } else {
+            int64_t c[4];
+            cell_coords(sp, nr - sp.first_nr, c);
+            for (uint32_t k = 0; k < sp.nst && !v; k++) {
+                uint64_t org;
+                if (stencil_org(sp, c, k, org) && rs[(sp.first_nr - 1 + org) * bytes + f]) v = 1;
+            }
+        }
+    }
+    w[f] = v;
+}
+
 }  // namespace
 
 // ====================================================================== launchers
@@ -1572,6 +1630,18 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
                            E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
                            E.pfirst.as<uint16_t>());
     }
+    HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+int launch_diffuse_stencil(Sim& s, int t, int et, uint32_t field) {
+    AgentStore& A = s.a[t];
+    const StencilSpec& sp = s.e[et].sspec;
+    uint64_t ncells = 1;
+    for (uint32_t d = 0; d < sp.ndims; d++) ncells *= (uint64_t)sp.dims[d];
+    if (!A.nslots) return VH_OK;
+    hipLaunchKernelGGL(k_diffuse_stencil, dim3(grid_for(A.nslots, 256)), dim3(256), 0, s.stream, sp,
+                       (const uint8_t*)A.rstate(), (uint8_t*)A.wstate(), A.nslots, ncells, A.bytes, field);
     HIPCHECK(hipGetLastError());
     return VH_OK;
 }

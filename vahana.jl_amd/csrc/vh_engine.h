@@ -82,6 +82,7 @@ struct EdgeStore {
     // in-edges requested as a raster stencil during init (vh_add_edges_stencil)
     bool stencil = false;
     StencilSpec sspec;
+    uint64_t stencil_version = ~0ull;   // the CSR is the stencil of sspec while version equals it
     // in-edges requested as a device G(n, p) during init (vh_add_edges_gnp)
     bool gen = false;
     double gen_mean_deg = 0.0;
@@ -112,6 +113,9 @@ int launch_generic(int transition, const DCtx* d_ctx, int ci, int call_t, uint64
                    int with_edge, hipStream_t st);
 int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step);
 int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step);
+// diffuse over an unchanged device stencil: neighbours from the stencil
+// arithmetic instead of the CSR (same neighbour set; the body is an OR)
+int launch_diffuse_stencil(Sim& s, int t, int et, uint32_t field);
 // contact model: per-step flag bitmaps, one bit per slot (64 per word)
 struct ContactBits {
     uint64_t* sp;     // dead in the read state or dying in this step (padding set)

@@ -818,6 +818,7 @@ static int build_stencil(Sim& s, EdgeStore& E) {
     E.rows = rows;
     E.version++;
     E.stencil = false;
+    E.stencil_version = E.version;
     return VH_OK;
 }
 
@@ -1101,6 +1102,22 @@ static int check_errors(Sim& s) {
     return set_error(VH_ERR_ASSERT, "device assertion flags %u", e);
 }
 
+// diffuse whose cells' in-edges are still the device stencil built at
+// finish_init!, on an immortal, readable, written (not :Independent) cell
+// type called alone: the stencil kernel gives what k_generic<DIFFUSE> gives
+static bool diffuse_on_stencil(Sim& s, const vh_apply_desc* d) {
+    if (d->transition != VH_TR_DIFFUSE || d->n_call != 1 || d->with_edge >= 0) return false;
+    if (!d->params || d->params_bytes < sizeof(vh_raster_params)) return false;
+    const int t = d->call[0], et = d->role_edge[0];
+    if (et < 0 || et >= s.m->ne) return false;
+    const AgentStore& A = s.a[t];
+    const EdgeStore& E = s.e[et];
+    const vh_raster_params* rp = (const vh_raster_params*)d->params;
+    return !A.mortal && A.writeable && !A.indep && A.bytes && rp->field_offset < A.bytes &&
+           in_mask(d->read_agents, t) && in_mask(d->read_edges, et) && E.rows &&
+           E.stencil_version == E.version && E.target == t && (int)E.sspec.cell_type == t;
+}
+
 static int apply_one(Sim& s, const vh_apply_desc* d) {
     Model& m = *s.m;
     hipStream_t st = s.stream;
@@ -1171,7 +1188,10 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
     int contact_t = -1;
     uint64_t contact_deaths = 0;
 
-    if (kind == K_PLAIN) {
+    if (kind == K_PLAIN && diffuse_on_stencil(s, d)) {
+        const vh_raster_params* rp = (const vh_raster_params*)d->params;
+        RC(launch_diffuse_stencil(s, d->call[0], d->role_edge[0], rp->field_offset));
+    } else if (kind == K_PLAIN) {
         RC(upload_ctx(s, c));
         for (uint32_t i = 0; i < d->n_call; i++)
             RC(launch_generic(d->transition, s.d_ctx.as<DCtx>(), (int)i, d->call[i],

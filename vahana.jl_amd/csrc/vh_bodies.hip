@@ -559,10 +559,11 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
 // sit in one XCD's L2. Pass 1 (k_plan_gather) walks the plan in order,
 // y[j] = x[psrc[j]]: each XCD takes a contiguous range of it, so its gathers
 // hit the one partition its L2 holds, and y is written sequentially. Pass 2
-// (k_opinion with PLAN) reads y[pinv[k]] for CSR slot k: within a partition
-// the plan keeps CSR order, so contiguous agents read y as one sequential
-// stream per partition, which the XCD's L2 serves. The values are the same as
-// x[src[k]] and the sums keep their canonical order.
+// reads y: by y[pinv[k]] for CSR slot k (k_opinion MODE 1; within a
+// partition the plan keeps CSR order, so contiguous agents read y as one
+// sequential stream per partition), or from tile lists (MODE 2 and 3, below).
+// The values are the same as x[src[k]] and the sums keep their canonical
+// order.
 constexpr int GP_U = 8;   // plan entries per thread in pass 1
 
 // Pass 2 with tile lists: the light in-edges of a 256-agent block are
@@ -572,6 +573,7 @@ constexpr int GP_U = 8;   // plan entries per thread in pass 1
 // y[ty[e]], a run of consecutive plan entries per partition, and lands at
 // slot tslot[e] of the tile in LDS. Weights, terms and the sums in canonical
 // order are as in opinion_block.
+
 // LDS of a tile-list block; a heavy block gives each of its 4 waves a
 // quarter of `val` (OP_TILE / 4 = OPH_CHUNK slots)
 template <class T>

@@ -74,6 +74,7 @@ enum vh_transition {
     VH_TR_COUNT_EDGES = 7,   /* foo = num_edges(sim, id, E)                         */
     VH_TR_DIFFUSE = 8,       /* Bool field |= any in-neighbour's (test/raster.jl:38-41) */
     VH_TR_MOVE_TO = 9,       /* value_on_pos: move_to! cell (v,...,v) (test/raster.jl:279-285) */
+    VH_TR_REMOVE_EDGES = 10, /* remove_edges! bodies of test/mpi/test_edgetypes.jl:295-445 */
     VH_TR_SIR = 16,          /* SIR contagion (Stateless contact edges)            */
     VH_TR_OPINION_F64 = 17,  /* weighted bounded confidence, Float64 opinions      */
     VH_TR_OPINION_F32 = 18,  /* same, Float32 opinions                              */
@@ -93,6 +94,25 @@ typedef struct vh_test_params {      /* params of the test bodies (ids 0-7) */
     uint32_t pad;
     uint8_t edge_state[16];          /* edge state bytes for ADD_SELF_EDGE  */
 } vh_test_params;
+
+/* VH_TR_REMOVE_EDGES: vh_test_params.value selects the body; the agent's
+ * Int64 at offset 0 is its `idx`; role_edge[0] = ET, role_edge[1] = SendId.
+ * remove_edges!(sim, to, ET) src/EdgeMethods.jl:527-549 and
+ * remove_edges!(sim, from, to, ET) :552-599 act on the write container at
+ * the point of the call, in call order with add_edge!. */
+enum vh_remove_mode {
+    VH_RM_OWN = 0,          /* idx even: remove_edges!(id)                       (:315-317) */
+    VH_RM_NEIGHBOR = 1,     /* idx even: remove_edges!(first(neighborids(id)))   (:327-330) */
+    VH_RM_READD = 2,        /* f = first(neighborids(id)); remove_edges!(f, id);
+                               add_edge!(f, id, edge_state)                       (:340-347) */
+    VH_RM_FROM_TWO = 3,     /* n = neighborids(id); remove_edges!(n[1], id), and
+                               remove_edges!(n[2], id) unless :SingleEdge         (:387-393) */
+    VH_RM_TO_NEIGHBOR = 4,  /* idx even: remove_edges!(id, n) for a neighbour n
+                               (the reference draws n at random; the first here)  (:403-408) */
+    VH_RM_SEND_ID = 5,      /* add_edge!(id, neighborids(id, ET), SendId)          (:411-414) */
+    VH_RM_TO_SENDID = 6,    /* idx < 100: remove_edges!(id, neighborids(id, SendId), ET) (:415-421) */
+    VH_RM_ZERO_FROM = 7     /* remove_edges!(AgentID(0), id)                      (:428-430) */
+};
 
 typedef struct vh_raster_params {    /* raster bodies (ids 8, 9)                    */
     uint32_t field_offset;           /* DIFFUSE: byte offset of the Bool field       */

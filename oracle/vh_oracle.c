@@ -614,6 +614,31 @@ static uint64_t ctx_add_agent(ctx_t* c, int t, const uint8_t* st) {
     return add_agent_impl(c->s, t, st);
 }
 
+/* remove_edges! inside a transition (src/EdgeMethods.jl:527-599) with
+ * _can_remove_edges (:101-121); `all`: remove_edges!(sim, to, T) */
+static void ctx_remove_edges(ctx_t* c, int et, uint64_t from, uint64_t to, int all) {
+    if (et < 0 || et >= c->s->m->ne) { c->err = fail(VH_ERR_ARG, "unbound edge role"); return; }
+    edges_t* X = &c->s->e[et];
+    if (!X->writeable) {
+        c->err = fail(VH_ERR_ASSERT, "Edge of type %d can not removed, as the type is not in the "
+                                     "`write` argument of the transition function.", et);
+        return;
+    }
+    if (!X->add_existing) {
+        c->err = fail(VH_ERR_ASSERT, "Edge of type %d can not removed, as the type is not in the "
+                                     "`add_existing` argument of the transition function.", et);
+        return;
+    }
+    if (!valid_target(c->s, et, to)) { c->err = fail(VH_ERR_ASSERT, "invalid edge target"); return; }
+    if (all) { clear_target(c->s, et, to); return; }
+    if (HAS(ETH(c->s, et), VH_HINT_IGNOREFROM)) {
+        c->err = fail(VH_ERR_ASSERT, "remove_edges! with a source agent is not defined for "
+                                     "edgetypes with the :IgnoreFrom hint.");
+        return;
+    }
+    remove_from_to(c->s, et, from, to);
+}
+
 /* ------------------------------------------------------------ model bodies
  * Each returns 1 (state in `out`) or 0 (`nothing`: the agent dies). */
 typedef int (*body_fn)(ctx_t*, const uint8_t* st, uint64_t id, uint8_t* out);
@@ -967,6 +992,72 @@ static int body_move_to(ctx_t* c, const uint8_t* st, uint64_t id, uint8_t* out) 
     return 1;
 }
 
+/* the remove_edges! bodies of test/mpi/test_edgetypes.jl:295-445 (enum
+ * vh_remove_mode); idx = the Int64 at offset 0 of the caller's state */
+static int nth_from(ctx_t* c, int et, uint64_t id, uint32_t n, uint64_t* out) {
+    cont_t* k = rd_cont(c, et, id);
+    if (!k || !k->present || k->n <= n) return 0;
+    if (HAS(ETH(c->s, et), VH_HINT_IGNOREFROM)) {
+        c->err = fail(VH_ERR_ASSERT, "neighborids is not defined for the hint combination");
+        return 0;
+    }
+    *out = k->from[n];
+    return 1;
+}
+static int body_remove_edges(ctx_t* c, const uint8_t* st, uint64_t id, uint8_t* out) {
+    (void)out;
+    const vh_test_params* p = (const vh_test_params*)c->s->params;
+    int64_t idx = rd_i64(st);
+    int et = c->d->role_edge[0], es = c->d->role_edge[1];
+    uint64_t nb = 0, nb2 = 0;
+    switch ((int)p->value) {
+    case VH_RM_OWN:
+        if (idx % 2 == 0) ctx_remove_edges(c, et, 0, id, 1);
+        break;
+    case VH_RM_NEIGHBOR:
+        if (idx % 2 == 0) {
+            if (!nth_from(c, et, id, 0, &nb)) { if (!c->err) c->err = fail(VH_ERR_ASSERT, "no neighbour"); }
+            else ctx_remove_edges(c, et, 0, nb, 1);
+        }
+        break;
+    case VH_RM_READD:
+        if (!nth_from(c, et, id, 0, &nb)) { if (!c->err) c->err = fail(VH_ERR_ASSERT, "no neighbour"); break; }
+        ctx_remove_edges(c, et, nb, id, 0);
+        if (!c->err) ctx_add_edge(c, et, nb, id, p->edge_state);
+        break;
+    case VH_RM_FROM_TWO: {
+        int single = HAS(ETH(c->s, et), VH_HINT_SINGLEEDGE);
+        if (!nth_from(c, et, id, 0, &nb) || (!single && !nth_from(c, et, id, 1, &nb2))) {
+            if (!c->err) c->err = fail(VH_ERR_ASSERT, "no neighbour");
+            break;
+        }
+        ctx_remove_edges(c, et, nb, id, 0);
+        if (!single && !c->err) ctx_remove_edges(c, et, nb2, id, 0);
+        break;
+    }
+    case VH_RM_TO_NEIGHBOR:
+        if (idx % 2 == 0) {
+            if (!nth_from(c, et, id, 0, &nb)) { if (!c->err) c->err = fail(VH_ERR_ASSERT, "no neighbour"); }
+            else ctx_remove_edges(c, et, id, nb, 0);
+        }
+        break;
+    case VH_RM_SEND_ID:
+        if (!nth_from(c, et, id, 0, &nb)) { if (!c->err) c->err = fail(VH_ERR_ASSERT, "no neighbour"); }
+        else ctx_add_edge(c, es, id, nb, p->edge_state);
+        break;
+    case VH_RM_TO_SENDID:
+        if (idx < 100) {
+            if (!nth_from(c, es, id, 0, &nb)) { if (!c->err) c->err = fail(VH_ERR_ASSERT, "no neighbour"); }
+            else ctx_remove_edges(c, et, id, nb, 0);
+        }
+        break;
+    case VH_RM_ZERO_FROM:
+        ctx_remove_edges(c, et, 0, id, 0);
+        break;
+    }
+    return 1;
+}
+
 static body_fn body_of(uint32_t tr) {
     switch (tr) {
     case VH_TR_NOOP: return body_noop;
@@ -979,6 +1070,7 @@ static body_fn body_of(uint32_t tr) {
     case VH_TR_COUNT_EDGES: return body_count_edges;
     case VH_TR_DIFFUSE: return body_diffuse;
     case VH_TR_MOVE_TO: return body_move_to;
+    case VH_TR_REMOVE_EDGES: return body_remove_edges;
     case VH_TR_SIR: return body_sir;
     case VH_TR_OPINION_F64: return body_opinion_f64;
     case VH_TR_OPINION_F32: return body_opinion_f32;

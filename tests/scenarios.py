@@ -736,3 +736,110 @@ def snapshot_contact_restore(V, path):
             "states": not differ[~da].any() and bool(np.array_equal(differ, differ & da)),
             "edges": np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1]),
             "free": [int(i) + 1 for i in np.nonzero(db)[0]], "b": b}
+
+
+# ---------------------------------------------------------------- remove_edges!
+# test/mpi/test_edgetypes.jl:62-83 (the edge types the removal tests use)
+MPI_EDGES = {"MPIEdgeD": (), "MPIEdgeS": ("Stateless",), "MPIEdgeE": ("SingleEdge",),
+             "MPIEdgeT": ("SingleType",), "MPIEdgeI": ("IgnoreFrom",),
+             "MPIEdgeSE": ("Stateless", "SingleEdge"), "MPIEdgeST": ("Stateless", "SingleType"),
+             "MPIEdgeSI": ("Stateless", "IgnoreFrom"), "MPIEdgeEI": ("SingleEdge", "IgnoreFrom"),
+             "MPIEdgeSEI": ("Stateless", "SingleEdge", "IgnoreFrom"),
+             "MPIEdgeSTI": ("Stateless", "SingleType", "IgnoreFrom"),
+             "MPIEdgeSETI": ("Stateless", "SingleEdge", "SingleType", "IgnoreFrom")}
+REMOVE_TYPES = ["MPIEdgeD", "MPIEdgeS", "MPIEdgeE", "MPIEdgeI", "MPIEdgeSE", "MPIEdgeSI", "MPIEdgeEI",
+                "MPIEdgeSEI", "MPIEdgeSTI", "MPIEdgeSETI", "MPIEdgeT", "MPIEdgeST"]   # :457-460
+REMOVE_FROM_TYPES = ["MPIEdgeD", "MPIEdgeS", "MPIEdgeE", "MPIEdgeSE", "MPIEdgeT", "MPIEdgeST"]  # :466-468
+
+
+def mpi_edges_model(V):
+    """The model of test/mpi/test_edgetypes.jl:27-83 (AgentState1, the edge
+    types above, SendId)."""
+    A1 = V.struct("AgentState1", [("idx", "<i8")])
+    A2 = V.struct("AgentState2", [("idx", "<i8"), ("something", "?")])
+    mt = V.ModelTypes() | V.register_agenttype_(A1) | V.register_agenttype_(A2)
+    ets = {}
+    for name, hints in MPI_EDGES.items():
+        T = (V.struct(name) if "Stateless" in hints
+             else V.struct(name, [("fromid", "<u8"), ("toid", "<u8")]))
+        ets[name] = T
+        kw = {"target": A1} if "SingleType" in hints else {}
+        mt = mt | V.register_edgetype_(T, *hints, **kw)
+    SendId = V.struct("SendId")
+    mt = mt | V.register_edgetype_(SendId, "SingleEdge")
+    return mt | V.create_model("MPI EdgeTypes"), A1, ets, SendId
+
+
+def _graph_sim(V, model, A1, ET, pairs, n=100):
+    """add_graph!(sim, graph, i -> AgentState1(i), _ -> ET() / ET(0, 0))."""
+    sim = V.create_simulation(model)
+    st = np.zeros(n, dtype=A1.dtype)
+    st["idx"] = np.arange(1, n + 1)
+    ids = np.asarray(sim._add_agents_array(A1, st), dtype=np.uint64)
+    fr = ids[[a - 1 for a, _ in pairs]]
+    to = ids[[b - 1 for _, b in pairs]]
+    es = None if not ET.fields else np.zeros(len(pairs), dtype=ET.dtype)
+    sim._add_edges_array(ET, fr, to, es)
+    V.finish_init_(sim)
+    return sim
+
+
+def remove_edges(V):
+    """remove_edges! inside transitions, test/mpi/test_edgetypes.jl:295-445
+    (testforedgetype_remove and testforedgetype_remove_from at one process):
+    edge counts after each step and the two assertion cases."""
+    out = {}
+    cycle = [(i, i % 100 + 1) for i in range(1, 101)]                     # cycle_digraph(100)
+    complete = [(i, j) for i in range(1, 101) for j in range(1, 101) if i != j]   # complete_graph(100)
+    T = V.transitions
+    for name in REMOVE_TYPES:                                              # :295-355
+        model, A1, ets, SendId = mpi_edges_model(V)
+        ET = ets[name]
+        zero = None if not ET.fields else np.zeros(1, dtype=ET.dtype)[0]
+        sim = _graph_sim(V, model, A1, ET, cycle)
+        r = {}
+        removed = V.apply(sim, T.remove_edges("own", ET), [A1], [A1, ET], [ET], add_existing=[ET])
+        r["own"] = V.num_edges(removed, ET)                                # :320
+        V.finish_simulation_(removed)
+        copy = V.copy_simulation(sim)
+        if "IgnoreFrom" not in MPI_EDGES[name]:
+            removed = V.apply(sim, T.remove_edges("neighbor", ET), [A1], [A1, ET], [ET],
+                              add_existing=[ET])
+            r["neighbor"] = V.num_edges(removed, ET)                       # :332
+            V.finish_simulation_(removed)
+            V.apply_(copy, T.remove_edges("readd", ET, state=zero), [A1], [ET], [ET],
+                     add_existing=[ET])
+            r["readd"] = V.num_edges(copy, ET)                             # :349
+        V.finish_simulation_(sim)
+        V.finish_simulation_(copy)
+        out["remove/" + name] = r
+    for name in REMOVE_FROM_TYPES:                                         # :357-445
+        model, A1, ets, SendId = mpi_edges_model(V)
+        ET = ets[name]
+        single = "SingleEdge" in MPI_EDGES[name]
+        sim = _graph_sim(V, model, A1, ET, cycle if single else complete)
+        r = {"init": V.num_edges(sim, ET)}                                 # :377-381
+        sim2 = V.copy_simulation(sim)
+        sim3 = V.copy_simulation(sim)
+        V.apply_(sim, T.remove_edges("from_two", ET), [A1], [A1, ET], [ET], add_existing=[ET])
+        r["from_two"] = V.num_edges(sim, ET)                               # :392-397
+        if not single:
+            V.apply_(sim2, T.remove_edges("to_neighbor", ET), [A1], [A1, ET], [ET],
+                     add_existing=[ET])
+        else:
+            V.apply_(sim2, T.remove_edges("send_id", ET, SendId), [A1], [ET], [SendId])
+            V.apply_(sim2, T.remove_edges("to_sendid", ET, SendId), [A1], [A1, ET, SendId], [ET],
+                     add_existing=[ET])
+        r["sim2"] = V.num_edges(sim2, ET)                                  # :409, :422
+        V.apply_(sim3, T.remove_edges("zero_from", ET), [A1], [], [ET], add_existing=[ET])
+        r["zero_from"] = V.num_edges(sim3, ET)                             # :431-436
+        for key, write in (("assert_not_written", []), ("assert_not_add_existing", [ET])):
+            try:                                                           # :438-444
+                V.apply_(sim3, T.remove_edges("zero_from", ET), [A1], [], write)
+                r[key] = "no assertion"
+            except AssertionError:
+                r[key] = "AssertionError"
+        for x in (sim, sim2, sim3):
+            V.finish_simulation_(x)
+        out["remove_from/" + name] = r
+    return out

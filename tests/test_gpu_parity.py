@@ -22,7 +22,7 @@ def _eq(a, b):
                                   "edges_matrix", "edges_transition", "remove_agents",
                                   "independent", "addexisting", "graphs_k4",
                                   "edges_iter_removal", "raster_diffusion", "raster_nodeid",
-                                  "raster_move_to_dist", "raster_random"])
+                                  "raster_move_to_dist", "raster_random", "remove_edges"])
 def test_reference_scenarios(vh, ora, gpu_ready, name):
     fn = getattr(SC, name)
     print(f"[gpu] scenario {name}", flush=True)

@@ -154,6 +154,27 @@ def test_edges_iterator_and_removal(ora):
         assert r[2] == count, (name, r)
 
 
+def test_remove_edges_in_transitions(ora):
+    """test/mpi/test_edgetypes.jl:295-445 at one process: remove_edges!(to)
+    leaves 50 of the 100 cycle edges (own or the neighbour's); remove then
+    add_edge! of the same edge keeps all 100; remove_edges!(from, to) takes
+    2 x 100 of 9900 (or the single edge), 50 of the other direction (or 99 of
+    the cycle via SendId); an unknown source removes nothing; without the
+    edge type in `write`, or in `add_existing`, remove_edges! asserts."""
+    r = SC.remove_edges(ora)
+    for name in SC.REMOVE_TYPES:
+        want = {"own": 50}
+        if "IgnoreFrom" not in SC.MPI_EDGES[name]:
+            want.update(neighbor=50, readd=100)
+        assert r["remove/" + name] == want, name
+    for name in SC.REMOVE_FROM_TYPES:
+        single = "SingleEdge" in SC.MPI_EDGES[name]
+        want = {"init": 100 if single else 9900, "from_two": 0 if single else 9700,
+                "sim2": 1 if single else 9850, "zero_from": 100 if single else 9900,
+                "assert_not_written": "AssertionError", "assert_not_add_existing": "AssertionError"}
+        assert r["remove_from/" + name] == want, name
+
+
 def test_raster_diffusion(ora):
     out = SC.raster_diffusion(ora)
     for key in ("2d_10x8", "2d_7x9", "2d_17x6", "2d_6x7"):   # test/raster.jl:54-58, 81-85, ...

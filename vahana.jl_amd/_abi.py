@@ -39,6 +39,10 @@ EDGE_HINTS = {"Stateless": HINT_STATELESS, "IgnoreFrom": HINT_IGNOREFROM,
 TR_NOOP, TR_SUM_NEIGHBORS, TR_DIE_IF, TR_ADD_SELF_EDGE = 0, 1, 2, 3
 TR_READD_EDGES, TR_NEIGHBOR_FOO_EDGES, TR_SPAWN, TR_COUNT_EDGES = 4, 5, 6, 7
 TR_DIFFUSE, TR_MOVE_TO = 8, 9
+TR_REMOVE_EDGES = 10
+# enum vh_remove_mode
+RM = {"own": 0, "neighbor": 1, "readd": 2, "from_two": 3, "to_neighbor": 4, "send_id": 5,
+      "to_sendid": 6, "zero_from": 7}
 TR_SIR, TR_OPINION_F64, TR_OPINION_F32, TR_CONTACT = 16, 17, 18, 19
 
 CMP = {"always": 0, "never": 1, "lt": 2, "ge": 3, "eq": 4, "odd": 5, "even": 6,

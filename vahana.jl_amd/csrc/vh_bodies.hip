@@ -82,6 +82,33 @@ struct Emitter {
             E.to[p] = to;
             E.from[p] = from;
             for (uint32_t k = 0; k < E.bytes; k++) E.st[p * E.bytes + k] = ((const uint8_t*)st)[k];
+            if (E.kind) E.kind[p] = VH_LOG_ADD;
+        }
+        ne++;
+    }
+    // remove_edges!(sim, to, ET) (all) or remove_edges!(sim, from, to, ET)
+    // src/EdgeMethods.jl:527-599, logged in call order with the add_edge!s
+    // (_can_remove_edges :101-121: ET in `write` and in `add_existing`)
+    __device__ void remove(uint64_t from, uint64_t to, bool all) {
+        const DEmitEdge& E = c->ee[0];
+        if (E.et < 0 || !c->e[E.et].writeable) {
+            vh_flag(c, VH_DERR_WRITE);
+            return;
+        }
+        if (!c->e[E.et].add_existing) {
+            vh_flag(c, VH_DERR_ADDEX);
+            return;
+        }
+        if (!all && VH_HAS(c->e[E.et].hints, VH_HINT_IGNOREFROM)) {
+            vh_flag(c, VH_DERR_HINT);
+            return;
+        }
+        if (pass == 1) {
+            uint64_t p = E.offs[caller] + ne;
+            E.to[p] = to;
+            E.from[p] = from;
+            for (uint32_t k = 0; k < E.bytes; k++) E.st[p * E.bytes + k] = 0;
+            E.kind[p] = all ? VH_LOG_CLEAR : VH_LOG_REMOVE_FROM;
         }
         ne++;
     }
@@ -299,6 +326,79 @@ struct Body<VH_TR_MOVE_TO> {
         if (p->flags & 1u) em.edge(cell, id, p->edge_state);
         if (p->flags & 2u) em.edge(id, cell, p->edge_state);
         st_i64(out, v);
+        return true;
+    }
+};
+
+// remove_edges! bodies of test/mpi/test_edgetypes.jl:295-445 (enum
+// vh_remove_mode); `idx` is the Int64 at offset 0 of the caller's state
+template <>
+struct Body<VH_TR_REMOVE_EDGES> {
+    // first (or second) in-neighbour of `id` over `et`; false when it has fewer
+    __device__ static bool nth_neighbor(const DCtx* c, int et, uint64_t id, uint32_t n, uint64_t& out) {
+        uint32_t b, e;
+        if (!d_row(c, et, id, b, e) || e - b <= n) return false;
+        const DEdge& E = c->e[et];
+        if (VH_HAS(E.hints, VH_HINT_IGNOREFROM)) {
+            vh_flag(c, VH_DERR_HINT);
+            return false;
+        }
+        out = vh_src_id(E, b + n);
+        return true;
+    }
+    __device__ static bool run(const DCtx* c, const uint8_t* st, uint64_t id, Emitter& em, uint8_t*) {
+        const vh_test_params* p = (const vh_test_params*)c->params;
+        const int64_t idx = ld_i64(st);
+        const int et = c->role_e[0], es = c->role_e[1];
+        uint64_t nb = 0, nb2 = 0;
+        switch ((int)p->value) {
+        case VH_RM_OWN:
+            if (idx % 2 == 0) em.remove(0, id, true);
+            break;
+        case VH_RM_NEIGHBOR:
+            if (idx % 2 == 0) {
+                if (!nth_neighbor(c, et, id, 0, nb)) vh_flag(c, VH_DERR_TARGET);
+                else em.remove(0, nb, true);
+            }
+            break;
+        case VH_RM_READD:
+            if (!nth_neighbor(c, et, id, 0, nb)) {
+                vh_flag(c, VH_DERR_TARGET);
+                break;
+            }
+            em.remove(nb, id, false);
+            em.edge(nb, id, p->edge_state);
+            break;
+        case VH_RM_FROM_TWO: {
+            const bool single = VH_HAS(c->e[et].hints, VH_HINT_SINGLEEDGE);
+            if (!nth_neighbor(c, et, id, 0, nb) || (!single && !nth_neighbor(c, et, id, 1, nb2))) {
+                vh_flag(c, VH_DERR_TARGET);
+                break;
+            }
+            em.remove(nb, id, false);
+            if (!single) em.remove(nb2, id, false);
+            break;
+        }
+        case VH_RM_TO_NEIGHBOR:
+            if (idx % 2 == 0) {
+                if (!nth_neighbor(c, et, id, 0, nb)) vh_flag(c, VH_DERR_TARGET);
+                else em.remove(id, nb, false);
+            }
+            break;
+        case VH_RM_SEND_ID:   // the log is bound to SendId (role_edge[1])
+            if (!nth_neighbor(c, et, id, 0, nb)) vh_flag(c, VH_DERR_TARGET);
+            else em.edge(id, nb, p->edge_state);
+            break;
+        case VH_RM_TO_SENDID:
+            if (idx < 100) {
+                if (!nth_neighbor(c, es, id, 0, nb)) vh_flag(c, VH_DERR_TARGET);
+                else em.remove(id, nb, false);
+            }
+            break;
+        case VH_RM_ZERO_FROM:
+            em.remove(0, id, false);
+            break;
+        }
         return true;
     }
 };
@@ -1363,6 +1463,7 @@ int launch_generic(int tr, const DCtx* d_ctx, int ci, int call_t, uint64_t nslot
         CASE(VH_TR_SPAWN)
         CASE(VH_TR_COUNT_EDGES)
         CASE(VH_TR_DIFFUSE)
+        CASE(VH_TR_REMOVE_EDGES)
         CASE(VH_TR_MOVE_TO)
 #undef CASE
     default:

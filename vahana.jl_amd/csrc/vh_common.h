@@ -24,6 +24,7 @@
 #define VH_DERR_CAP         (1u << 5)   // fixed per-caller emission capacity exceeded
 #define VH_DERR_HINT        (1u << 6)   // accessor not defined for the hint combination
 #define VH_DERR_TARGET      (1u << 7)   // edge target of the wrong type / invalid id
+#define VH_DERR_ADDEX       (1u << 8)   // remove_edges! of a type not in `add_existing`
 
 // ------------------------------------------------------------------ codec
 __host__ __device__ __forceinline__ uint64_t vh_make_id(uint64_t type, uint64_t nr) {
@@ -100,7 +101,7 @@ struct DEdge {
     uint64_t rowbase[VH_MAXA + 1];
     uint64_t nrows[VH_MAXA + 1];
     uint32_t bytes, hints, src64, src_type;
-    uint32_t readable, writeable, pad0, pad1;
+    uint32_t readable, writeable, add_existing, pad1;
 };
 
 // Per-role emission logs of one apply! (add_edge! / add_agent! inside the body)
@@ -112,7 +113,12 @@ struct DEmitEdge {
     uint8_t* st;
     int32_t et;              // bound edge type (-1 none)
     uint32_t bytes;
+    uint8_t* kind;           // per log entry: VH_LOG_ADD / VH_LOG_CLEAR / VH_LOG_REMOVE_FROM
 };
+// log entry kinds: add_edge!, remove_edges!(to), remove_edges!(from, to)
+#define VH_LOG_ADD 0
+#define VH_LOG_CLEAR 1
+#define VH_LOG_REMOVE_FROM 2
 struct DEmitAgent {
     uint32_t* cnt;
     const uint64_t* offs;

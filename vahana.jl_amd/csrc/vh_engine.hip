@@ -402,6 +402,55 @@ __global__ void k_log_keys(const uint64_t* __restrict__ to, uint64_t n, RowLayou
     keys[i] = (uint32_t)(L.base[t] + nr - 1);
 }
 
+// removal entries of the log (remove_edges!): not edges (key past every
+// row), their target row goes to the event list
+__global__ void k_log_removals(const uint8_t* __restrict__ kind, uint64_t n, uint32_t* __restrict__ keys,
+                               uint32_t* __restrict__ ekeys) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool rm = kind[i] != VH_LOG_ADD;
+    ekeys[i] = rm ? keys[i] : 0xFFFFFFFFu;
+    if (rm) keys[i] = 0xFFFFFFFFu;
+}
+
+// sequence of every entry: 0 for the kept container, log position + 1
+__global__ void k_entry_seq(uint64_t nbase, uint64_t n, uint32_t* __restrict__ seq) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) seq[i] = i < nbase ? 0u : (uint32_t)(i - nbase + 1);
+}
+
+// one thread per row: an entry stays unless a removal of its row comes later
+// in call order (remove_edges!(to) any source, remove_edges!(from, to) its
+// source); with :SingleEdge only the row's last entry is a candidate (the
+// container holds the last add_edge!, which a later removal can clear).
+// Events: ekeys sorted by row (stable, so log order within a row), eidx the
+// log positions.
+__global__ void k_remove_filter(const uint32_t* __restrict__ off, const uint64_t* __restrict__ from,
+                                const uint32_t* __restrict__ seq, uint64_t rows, uint32_t single,
+                                const uint32_t* __restrict__ ekeys, const uint32_t* __restrict__ eidx,
+                                uint64_t nev, const uint8_t* __restrict__ kind,
+                                const uint64_t* __restrict__ lfrom, uint8_t* __restrict__ keep) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint32_t b = off[r], e = off[r + 1];
+    if (b == e) return;
+    uint64_t lo = 0, hi = nev;   // first event of row r
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)ekeys[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    for (uint32_t k = b; k < e; k++) {
+        bool kp = !single || k == e - 1;
+        for (uint64_t q = lo; kp && q < nev && ekeys[q] == (uint32_t)r; q++) {
+            const uint32_t pl = eidx[q];
+            if (pl + 1 <= seq[k]) continue;   // before the entry was added
+            if (kind[pl] == VH_LOG_CLEAR || (from && lfrom[pl] == from[k])) kp = false;
+        }
+        keep[k] = kp ? 1 : 0;
+    }
+}
+
 // explode an existing CSR into (key, source, state) entries in the new layout
 __global__ void k_base_expand(const uint32_t* __restrict__ off, const void* __restrict__ src,
                               uint32_t src64, uint32_t src_type, const uint8_t* __restrict__ st,
@@ -530,9 +579,12 @@ int vh::readback(Sim& s, int n) {
 // Build the CSR of E from (optionally) its current entries followed by `nlog`
 // log entries (to, from, states) in add_edge! order: a stable sort by target
 // row keeps, per target, the existing edges first and the new ones in call
-// order (src/EdgeMethods.jl:477-523); :SingleEdge keeps the last one.
+// order (src/EdgeMethods.jl:477-523); :SingleEdge keeps the last one. With
+// `d_kind`, log entries can be remove_edges! calls, applied to what precedes
+// them in call order (src/EdgeMethods.jl:527-599).
 static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
-                     const uint64_t* d_from, const uint8_t* d_st, uint64_t nlog) {
+                     const uint64_t* d_from, const uint8_t* d_st, uint64_t nlog,
+                     const uint8_t* d_kind = nullptr) {
     hipStream_t st = s.stream;
     Scratch& sc = s.scratch;
     RowLayout oldL = layout_of(E), newL;
@@ -559,9 +611,25 @@ static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
         hipLaunchKernelGGL(k_base_expand, dim3(grid_for(E.rows, 256)), dim3(256), 0, st,
                            E.off.as<uint32_t>(), E.src.p, (uint32_t)E.src64, E.src_type,
                            E.st.as<uint8_t>(), bytes, oldL, newL, E.rows, keys, from, sts, err);
+    uint32_t* ekeys = nullptr;   // removal events (remove_edges!)
+    uint32_t* eidx = nullptr;
+    uint32_t* seq = nullptr;
     if (nlog) {
         hipLaunchKernelGGL(k_log_keys, dim3(grid_for(nlog, 256)), dim3(256), 0, st, d_to, nlog, newL,
                            keys + nbase, err);
+        if (d_kind) {
+            ekeys = (uint32_t*)sc.take(nlog * 4 + 4);
+            eidx = (uint32_t*)sc.take(nlog * 4 + 4);
+            uint32_t* ka = (uint32_t*)sc.take(nlog * 4 + 4);
+            uint32_t* va = (uint32_t*)sc.take(nlog * 4 + 4);
+            seq = (uint32_t*)sc.take(n * 4 + 4);
+            if (!ekeys || !eidx || !ka || !va || !seq) return set_error(VH_ERR_OOM, "scratch");
+            hipLaunchKernelGGL(k_log_removals, dim3(grid_for(nlog, 256)), dim3(256), 0, st, d_kind, nlog,
+                               keys + nbase, ekeys);
+            if (radix_sort_pairs(&ekeys, &eidx, ka, va, nlog, 32, true, sc, st))
+                return set_error(VH_ERR_DEVICE, "radix sort failed");
+            hipLaunchKernelGGL(k_entry_seq, dim3(grid_for(n, 256)), dim3(256), 0, st, nbase, n, seq);
+        }
         if (from) HIPCHECK(hipMemcpyAsync(from + nbase, d_from, nlog * 8, hipMemcpyDeviceToDevice, st));
         if (bytes)
             HIPCHECK(hipMemcpyAsync(sts + nbase * bytes, d_st, nlog * bytes, hipMemcpyDeviceToDevice, st));
@@ -577,9 +645,11 @@ static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
     uint64_t* from_s = from;
     uint8_t* st_s = sts;
     uint32_t* skeys = keys;
+    uint32_t* seq_s = seq;
     if (!sorted) {
         uint32_t bits = 1;
         while (bits < 32 && (rows >> bits) != 0) bits++;
+        if (ekeys) bits = 32;   // removal entries sort last
         uint32_t* sv = vals;
         if (radix_sort_pairs(&skeys, &sv, kalt, valt, n, bits, true, sc, st))
             return set_error(VH_ERR_DEVICE, "radix sort failed");
@@ -593,6 +663,11 @@ static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
             if (!st_s) return set_error(VH_ERR_OOM, "scratch");
             gather_records(st_s, sts, sv, n, bytes, st);
         }
+        if (seq) {
+            seq_s = (uint32_t*)sc.take(n * 4 + 4);
+            if (!seq_s) return set_error(VH_ERR_OOM, "scratch");
+            gather_records(seq_s, seq, sv, n, 4, st);
+        }
     }
     // offsets
     uint32_t* off = (uint32_t*)sc.take((rows + 1) * 4);
@@ -600,8 +675,35 @@ static int build_csr(Sim& s, EdgeStore& E, bool keep_base, const uint64_t* d_to,
     hipLaunchKernelGGL(k_offsets_from_sorted, dim3(grid_for(rows + 1, 256)), dim3(256), 0, st, skeys, n,
                        rows, off);
     uint64_t nfinal = n;
-    // :SingleEdge keeps the last added edge per target
-    if (has(E.hints, VH_HINT_SINGLEEDGE) && n) {
+    if (ekeys && n) {
+        // remove_edges!: drop entries removed after they were added (and,
+        // with :SingleEdge, all but the last)
+        uint8_t* keep = (uint8_t*)sc.take(n + 8);
+        uint32_t* pos = (uint32_t*)sc.take((n + 1) * 4);
+        uint32_t* k32 = (uint32_t*)sc.take((n + 1) * 4);
+        uint32_t* off2 = (uint32_t*)sc.take((rows + 1) * 4);
+        uint64_t* from2 = from_s ? (uint64_t*)sc.take(n * 8 + 8) : nullptr;
+        uint8_t* st2 = bytes ? (uint8_t*)sc.take(n * bytes + 8) : nullptr;
+        if (!keep || !pos || !k32 || !off2 || (from_s && !from2) || (bytes && !st2))
+            return set_error(VH_ERR_OOM, "scratch");
+        fill_u8(keep, 0, n, st);   // (removal entries, past the last row, are dropped)
+        hipLaunchKernelGGL(k_remove_filter, dim3(grid_for(rows, 256)), dim3(256), 0, st, off, from_s, seq_s,
+                           rows, has(E.hints, VH_HINT_SINGLEEDGE) ? 1u : 0u, ekeys, eidx, nlog, d_kind, d_from,
+                           keep);
+        hipLaunchKernelGGL(k_u8_to_u32, dim3(grid_for(n, 256)), dim3(256), 0, st, keep, k32, n);
+        if (scan_excl_u32(k32, pos, n, sc, st)) return set_error(VH_ERR_DEVICE, "scan");
+        hipLaunchKernelGGL(k_compact_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, keep, pos, n,
+                           (const void*)from_s, (void*)from2, from_s ? 8u : 0u, st_s, st2, bytes);
+        hipLaunchKernelGGL(k_remap_offsets, dim3(grid_for(rows + 1, 256)), dim3(256), 0, st, off, pos,
+                           rows, off2);
+        off = off2;
+        from_s = from2;
+        st_s = st2;
+        HIPCHECK(hipMemcpyAsync(s.d_stats.p, off + rows, 4, hipMemcpyDeviceToDevice, st));
+        RC(readback(s, 1));
+        nfinal = (uint32_t)s.h_stats[0];
+    } else if (has(E.hints, VH_HINT_SINGLEEDGE) && n) {
+        // :SingleEdge keeps the last added edge per target
         uint8_t* keep = (uint8_t*)sc.take(n + 8);
         uint32_t* pos = (uint32_t*)sc.take((n + 1) * 4);
         uint32_t* k32 = (uint32_t*)sc.take((n + 1) * 4);
@@ -1009,7 +1111,7 @@ static int kind_of(uint32_t tr) {
     case VH_TR_DIFFUSE:
         return K_PLAIN;
     case VH_TR_ADD_SELF_EDGE: case VH_TR_READD_EDGES: case VH_TR_NEIGHBOR_FOO_EDGES: case VH_TR_SPAWN:
-    case VH_TR_MOVE_TO:
+    case VH_TR_MOVE_TO: case VH_TR_REMOVE_EDGES:
         return K_TWOPASS;
     case VH_TR_SIR: return K_SIR;
     case VH_TR_OPINION_F64: case VH_TR_OPINION_F32: return K_OPINION;
@@ -1053,6 +1155,7 @@ static void fill_ctx(Sim& s, const vh_apply_desc* d, DCtx& c) {
         D.src_type = E.src_type;
         D.readable = in_mask(d->read_edges, et);
         D.writeable = E.writeable;
+        D.add_existing = in_mask(d->add_existing_edges, et);
     }
     for (int i = 0; i < 4; i++) {
         c.role_a[i] = d->role_agent[i];
@@ -1099,6 +1202,9 @@ static int check_errors(Sim& s) {
         return set_error(VH_ERR_ASSERT, "the accessor is not defined for the hint combination");
     if (e & VH_DERR_TARGET)
         return set_error(VH_ERR_ASSERT, "invalid edge target (type or id)");
+    if (e & VH_DERR_ADDEX)
+        return set_error(VH_ERR_ASSERT, "Edges can not be removed, as the edge type is not in the "
+                                        "`add_existing` argument of the transition function.");
     return set_error(VH_ERR_ASSERT, "device assertion flags %u", e);
 }
 
@@ -1183,6 +1289,7 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
     // logs of the two-pass bodies
     uint64_t ne_tot[2] = {0, 0}, nb_tot = 0;
     uint64_t* e_to[2] = {nullptr, nullptr};
+    uint8_t* e_kind = nullptr;   // remove_edges! bodies: kind of each log entry
     uint64_t* e_from[2] = {nullptr, nullptr};
     uint8_t* e_st[2] = {nullptr, nullptr};
     int contact_t = -1;
@@ -1203,6 +1310,12 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
         // Edges go to role_edge[0] (role_edge[1] for NEIGHBOR_FOO_EDGES),
         // births to role_agent[0] (SPAWN).
         int erole = d->transition == VH_TR_NEIGHBOR_FOO_EDGES ? d->role_edge[1] : d->role_edge[0];
+        // remove_edges! bodies log removals beside the add_edge!s; the
+        // SendId body of that test adds to its second edge role
+        const bool removals = d->transition == VH_TR_REMOVE_EDGES;
+        if (removals && d->params && d->params_bytes >= sizeof(vh_test_params) &&
+            ((const vh_test_params*)d->params)->value == VH_RM_SEND_ID)
+            erole = d->role_edge[1];
         bool births = d->transition == VH_TR_SPAWN && d->role_agent[0] > 0;
         uint32_t* ecnt = nullptr;
         uint64_t* eoff = nullptr;
@@ -1227,7 +1340,7 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
                 nc += s.a[d->call[i]].nslots;
             }
             if (erole >= 0)
-                cc.ee[0] = DEmitEdge{ecnt, eoff, e_to[0], e_from[0], e_st[0], erole, m.e[erole].bytes};
+                cc.ee[0] = DEmitEdge{ecnt, eoff, e_to[0], e_from[0], e_st[0], erole, m.e[erole].bytes, e_kind};
             if (births) cc.ea[0] = DEmitAgent{bcnt, boff, d->role_agent[0], 0};
         };
         set_emit(c);
@@ -1255,6 +1368,10 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
             e_from[0] = (uint64_t*)s.scratch.take(ne_tot[0] * 8 + 8);
             e_st[0] = b ? (uint8_t*)s.scratch.take(ne_tot[0] * b + 8) : nullptr;
             if (!e_to[0] || !e_from[0] || (b && !e_st[0])) return set_error(VH_ERR_OOM, "scratch");
+            if (removals) {
+                e_kind = (uint8_t*)s.scratch.take(ne_tot[0] + 8);
+                if (!e_kind) return set_error(VH_ERR_OOM, "scratch");
+            }
         }
         if (nb_tot) {
             AgentStore& A = s.a[d->role_agent[0]];
@@ -1360,11 +1477,12 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
         const uint64_t* lfrom = nullptr;
         const uint8_t* lst = nullptr;
         uint64_t nlog = 0;
+        const uint8_t* lkind = nullptr;
         if (kind == K_TWOPASS && c.ee[0].cnt && c.ee[0].et == et) {
-            lto = e_to[0]; lfrom = e_from[0]; lst = e_st[0]; nlog = ne_tot[0];
+            lto = e_to[0]; lfrom = e_from[0]; lst = e_st[0]; nlog = ne_tot[0]; lkind = e_kind;
         }
         if (!keep_base || nlog || E.rows == 0)
-            RC(build_csr(s, E, keep_base, lto, lfrom, lst, nlog));
+            RC(build_csr(s, E, keep_base, lto, lfrom, lst, nlog, lkind));
     }
     // deaths remove edges of every edge type (src/AgentMethods.jl:314-345)
     if (any_death)

@@ -81,6 +81,17 @@ def spawn(agenttype, count, edgetype=None, both_ways=True):
                       params=p)
 
 
+def remove_edges(mode, edgetype, sendid=None, state=None):
+    """The remove_edges! bodies of test/mpi/test_edgetypes.jl:295-445
+    (src/EdgeMethods.jl:527-599), `mode` one of own, neighbor, readd,
+    from_two, to_neighbor, send_id, to_sendid, zero_from (enum vh_remove_mode);
+    `state`: the edge state of the edges added by readd / send_id."""
+    raw = b"" if state is None else bytes(state.tobytes() if hasattr(state, "tobytes") else state)
+    edges = [edgetype] if sendid is None else [edgetype, sendid]
+    return Transition(_abi.TR_REMOVE_EDGES, f"remove_edges_{mode}", role_edges=edges,
+                      params=_tp(_abi.RM[mode], edge_state=raw))
+
+
 def count_edges(edgetype):
     """foo = num_edges(sim, id, edgetype)."""
     return Transition(_abi.TR_COUNT_EDGES, "count_edges", role_edges=[edgetype], params=_tp())

@@ -43,7 +43,11 @@ def _bytes(st):
     return st.view(np.uint8).reshape(len(st), st.dtype.itemsize)
 
 
-def test_sir_parity(vh, ora, gpu_ready):
+@pytest.mark.parametrize("expose", ["1", "0"])
+def test_sir_parity(vh, ora, gpu_ready, expose, monkeypatch):
+    """SIR, with (default) and without the exposure pass (VH_SIR_EXPOSE)."""
+    monkeypatch.setenv("VH_SIR_EXPOSE", expose)
+    print(f"[gpu] SIR parity expose={expose}", flush=True)
     M = vh.models
     n = 3000
     edges = vh.inputs.graph_er(n, 8.0 / (n - 1), 21)

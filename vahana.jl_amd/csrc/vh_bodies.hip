@@ -447,17 +447,83 @@ __global__ __launch_bounds__(256) void k_generic(const DCtx* __restrict__ c, int
 
 // ----------------------------------------------------------------- SIR
 // state word: health byte 0, pad byte 1, days bytes 2-3
+// Exposure pass: every infected agent (read state) sets the bit of each of
+// its out-neighbours in `expo` (cleared before). A susceptible agent without
+// the bit has no infected in-neighbour, so "some in-edge with an infected
+// source and a draw below p" is false without drawing: k_sir skips its draws.
+// The pass runs while the infected are few: k_sir counts the infected agents
+// it writes into 64 counters per step parity (one atomic per block), and
+// k_sir_decide, at the start of the next step, sums them into the decision
+// word that the pass and k_sir both read, so both take the same decision.
+// Words of `ctl`: [0, 128) counters (parity * 64 + j), 128 + parity: the
+// counters of that parity are complete, 130: the decision.
+constexpr uint32_t SIR_CTL_WORDS = 131;
+
+__global__ void k_sir_decide(uint32_t* __restrict__ ctl, uint64_t step, uint64_t n) {
+    __shared__ uint32_t s_sum[64];
+    const uint32_t j = threadIdx.x, cur = (uint32_t)(step & 1), nxt = cur ^ 1u;
+    s_sum[j] = ctl[cur * 64 + j];
+    __syncthreads();
+    if (j == 0) {
+        uint64_t total = 0;
+        for (int q = 0; q < 64; q++) total += s_sum[q];
+        ctl[130] = ctl[128 + cur] && total * 32 <= n ? 1u : 0u;
+        ctl[128 + nxt] = 1u;   // this step's k_sir fills the other set
+    }
+    ctl[nxt * 64 + j] = 0u;
+}
+__device__ __forceinline__ bool sir_expose_on(const uint32_t* ctl) { return ctl && ctl[130] != 0u; }
+__global__ __launch_bounds__(256) void k_sir_expose(const uint32_t* __restrict__ rs,
+                                                    const uint32_t* __restrict__ out_off,
+                                                    const uint32_t* __restrict__ out_dst, uint64_t n,
+                                                    const uint32_t* __restrict__ ctl,
+                                                    uint32_t* __restrict__ expo) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !sir_expose_on(ctl) || (rs[i] & 0xFFu) != VH_I) return;
+    for (uint32_t k = out_off[i]; k < out_off[i + 1]; k++) {
+        const uint32_t v = out_dst[k];
+        atomicOr(expo + (v >> 5), 1u << (v & 31u));
+    }
+}
+
+// row of every in-edge (transpose input) and its source index as key
+__global__ void k_edge_rows(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t rows,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (uint32_t k = off[r]; k < off[r + 1]; k++) {
+        keys[k] = src[k] - 1u;
+        vals[k] = (uint32_t)r;
+    }
+}
+// off[v] = first index whose key >= v, v in [0, n]
+__global__ void k_sorted_offsets(const uint32_t* __restrict__ keys, uint64_t ne, uint64_t n,
+                                 uint32_t* __restrict__ off) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n) return;
+    uint64_t lo = 0, hi = ne;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    off[v] = (uint32_t)lo;
+}
+
 __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
                                              const uint32_t* __restrict__ src,
                                              const uint32_t* __restrict__ rs,
                                              uint32_t* __restrict__ ws, uint64_t n, uint64_t nrows,
                                              uint64_t rowbase, uint32_t t, double p, uint64_t seed,
-                                             uint32_t rec, uint64_t step) {
+                                             uint32_t rec, uint64_t step, const uint32_t* __restrict__ expo,
+                                             uint32_t* __restrict__ ctl) {
+    __shared__ uint32_t s_inf[4];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = rs[i];
+    const bool in = i < n;
+    uint32_t s = in ? rs[i] : 0u;
     uint32_t h = s & 0xFFu;
-    if (h == VH_S) {
+    const bool skip = in && h == VH_S && sir_expose_on(ctl) && !((expo[i >> 5] >> (i & 31u)) & 1u);
+    if (in && h == VH_S && !skip) {
         if (i < nrows) {
             const uint64_t pre = vh_hash_pre(seed, vh_make_id(t, i + 1), step);
             uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
@@ -472,11 +538,20 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
                 }
             }
         }
-    } else if (h == VH_I) {
+    } else if (in && h == VH_I) {
         uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
         s = (s & 0xFF00u) | (days << 16) | (days >= rec ? VH_R : VH_I);
     }
-    ws[i] = s;
+    if (in) ws[i] = s;
+    if (ctl) {   // infected agents of the written state, for the next step
+        const uint64_t m = __ballot(in && (s & 0xFFu) == VH_I);
+        if ((threadIdx.x & 63u) == 0) s_inf[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t c = s_inf[0] + s_inf[1] + s_inf[2] + s_inf[3];
+            if (c) atomicAdd(ctl + ((step + 1) & 1) * 64 + (blockIdx.x & 63u), c);
+        }
+    }
 }
 
 // ----------------------------------------------------------------- opinion
@@ -1486,16 +1561,65 @@ static int check_model_layout(Sim& s, int t, int et, uint32_t abytes, uint32_t e
     return VH_OK;
 }
 
+static long long env_ll(const char* name, long long dflt);
+
+// Out-edges of E's rows (one agent type, rows from 0, 1-based sources up to
+// nsrc): the transpose, by a stable radix sort of the in-edges by source;
+// stamps array of nsrc entries, cleared once
+static int build_out_csr(Sim& s, EdgeStore& E, uint64_t nsrc) {
+    if (E.out_version == E.version && E.expo.p && E.expo.cap >= ((nsrc + 31) / 32) * 4 + 16) return VH_OK;
+    hipStream_t st = s.stream;
+    const uint64_t ne = E.nedges, rows = E.rows;
+    uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4 + 4);
+    uint32_t* vals = (uint32_t*)s.scratch.take(ne * 4 + 4);
+    uint32_t* ka = (uint32_t*)s.scratch.take(ne * 4 + 4);
+    uint32_t* va = (uint32_t*)s.scratch.take(ne * 4 + 4);
+    if (!keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
+    hipLaunchKernelGGL(k_edge_rows, dim3(grid_for(rows, 256)), dim3(256), 0, st, E.off.as<uint32_t>(),
+                       E.src.as<uint32_t>(), rows, keys, vals);
+    uint32_t bits = 1;
+    while (bits < 32 && (nsrc >> bits) != 0) bits++;
+    if (radix_sort_pairs(&keys, &vals, ka, va, ne, bits, false, s.scratch, st))
+        return set_error(VH_ERR_DEVICE, "radix sort failed");
+    RC(E.out_off.ensure((nsrc + 1) * 4 + 8, st));
+    RC(E.out_dst.ensure(ne * 4 + 8, st));
+    hipLaunchKernelGGL(k_sorted_offsets, dim3(grid_for(nsrc + 1, 256)), dim3(256), 0, st, keys, ne, nsrc,
+                       E.out_off.as<uint32_t>());
+    HIPCHECK(hipMemcpyAsync(E.out_dst.p, vals, ne * 4, hipMemcpyDeviceToDevice, st));
+    RC(E.expo.ensure(((nsrc + 31) / 32) * 4 + 16, st));
+    RC(E.expo_cnt.ensure(SIR_CTL_WORDS * 4 + 8, st));
+    fill_u32(E.expo_cnt.as<uint32_t>(), 0u, SIR_CTL_WORDS, st);   // no complete counts yet
+    HIPCHECK(hipGetLastError());
+    E.out_version = E.version;
+    return VH_OK;
+}
+
 int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step) {
     RC(check_model_layout(s, t, et, 4, 0, "SIR"));
     AgentStore& A = s.a[t];
     EdgeStore& E = s.e[et];
     if (!A.writeable) return set_error(VH_ERR_ASSERT, "SIR: the agent type must be in `write`");
     if (!A.nslots) return VH_OK;
+    // the exposure pass needs the in-edges of `t` only (rows from 0, sources
+    // of `t`) and a step number below 2^32; VH_SIR_EXPOSE=0 turns it off
+    uint32_t* expo = nullptr;
+    uint32_t* cnt = nullptr;
+    const uint64_t nrows = E.rows ? E.nrows[t] : 0ull;
+    if (nrows && E.nedges && !E.src64 && E.src_type == (uint32_t)t && E.rowbase[t] == 0 && nrows == E.rows &&
+        A.nslots < 0xFFFFFFFFull && env_ll("VH_SIR_EXPOSE", 1) != 0) {
+        RC(build_out_csr(s, E, A.nslots));
+        expo = E.expo.as<uint32_t>();
+        cnt = E.expo_cnt.as<uint32_t>();
+        fill_u32(expo, 0, (A.nslots + 31) / 32, s.stream);
+        hipLaunchKernelGGL(k_sir_decide, dim3(1), dim3(64), 0, s.stream, cnt, step, A.nslots);
+        hipLaunchKernelGGL(k_sir_expose, dim3(grid_for(A.nslots, 256)), dim3(256), 0, s.stream,
+                           (const uint32_t*)A.rstate(), E.out_off.as<uint32_t>(), E.out_dst.as<uint32_t>(),
+                           A.nslots, (const uint32_t*)cnt, expo);
+    }
     hipLaunchKernelGGL(k_sir, dim3(grid_for(A.nslots, 256)), dim3(256), 0, s.stream,
                        E.off.as<uint32_t>(), E.src.as<uint32_t>(), (const uint32_t*)A.rstate(),
-                       (uint32_t*)A.wstate(), A.nslots, E.rows ? E.nrows[t] : 0ull, E.rowbase[t],
-                       (uint32_t)t, p.p_inf, p.seed, p.recovery, step);
+                       (uint32_t*)A.wstate(), A.nslots, nrows, E.rowbase[t],
+                       (uint32_t)t, p.p_inf, p.seed, p.recovery, step, (const uint32_t*)expo, cnt);
     HIPCHECK(hipGetLastError());
     return VH_OK;
 }
@@ -1647,7 +1771,7 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc) {
     return VH_OK;
 }
 
-static long long env_ll(const char* name, long long dflt) {
+long long env_ll(const char* name, long long dflt) {
     const char* v = getenv(name);
     return v ? atoll(v) : dflt;
 }

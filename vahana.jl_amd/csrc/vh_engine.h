@@ -78,6 +78,18 @@ struct EdgeStore {
     bool plan_tiles = false, plan_htiles = false, plan_packed = false;
     uint64_t plan_version = ~0ull;
     uint32_t plan_psize = 0;       // sources per partition
+    // out-edges of the SIR exposure pass (build_out_csr), valid for
+    // `out_version`; `expo`: one bit per agent, set when it has an infected
+    // in-neighbour in this step; `expo_cnt`: infected counts by step parity
+    DevBuf out_off, out_dst, expo, expo_cnt;
+    uint64_t out_version = ~0ull;
+    // every device buffer of the store (release, accounting)
+    template <class F>
+    void each_buf(F f) {
+        DevBuf* all[] = {&off, &src, &st, &heavy, &psrc, &pinv, &pval, &lbase, &ty, &tslot, &hbase, &hty,
+                         &hslot, &prun, &pfirst, &tpk, &out_off, &out_dst, &expo, &expo_cnt};
+        for (DevBuf* b : all) f(*b);
+    }
     bool writeable = false;
     // in-edges requested as a raster stencil during init (vh_add_edges_stencil)
     bool stencil = false;

@@ -198,8 +198,7 @@ static void sim_release(Sim& s) {
         A.died.release(); A.died_now.release(); A.reuse.release();
     }
     for (int et = 0; et < VH_MAXE; et++) {
-        EdgeStore& E = s.e[et];
-        E.off.release(); E.src.release(); E.st.release(); E.heavy.release();
+        s.e[et].each_buf([](DevBuf& b) { b.release(); });
     }
     s.d_ctx.release(); s.d_err.release(); s.d_stats.release();
     s.scratch.release();

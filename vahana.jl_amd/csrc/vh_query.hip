@@ -515,8 +515,7 @@ extern "C" int vh_device_bytes(vh_sim_t h, uint64_t* out) {
         b += A.state[0].cap + A.state[1].cap + A.died.cap + A.died_now.cap + A.reuse.cap;
     }
     for (int et = 0; et < VH_MAXE; et++) {
-        EdgeStore& E = s.e[et];
-        b += E.off.cap + E.src.cap + E.st.cap + E.heavy.cap;
+        s.e[et].each_buf([&b](DevBuf& d) { b += d.cap; });
     }
     *out = b;
     return VH_OK;

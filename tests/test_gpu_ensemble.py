@@ -24,7 +24,13 @@ def _solo(V, n, md, p, seed, gseed, steps, n_init):
     return np.ascontiguousarray(st).view(np.uint8)
 
 
-def test_ensemble_members_match_solo(vh, ora, gpu_ready):
+@pytest.mark.parametrize("expose", ["1", "0"])
+def test_ensemble_members_match_solo(vh, ora, gpu_ready, expose, monkeypatch):
+    """With (default) and without the exposure pass (VH_SIR_EXPOSE), in the
+    ensemble and in the solo runs; 2% initially infected, so the pass is on
+    at first and turns off where the epidemic grows."""
+    monkeypatch.setenv("VH_SIR_EXPOSE", expose)
+    print(f"[gpu] ensemble vs solo expose={expose}", flush=True)
     E = vh.ensemble.Ensemble()
     ids = []
     for n, md, p, seed, gseed in SPECS:

@@ -82,6 +82,30 @@ __global__ void k_sir_init(uint32_t* st, uint64_t n, uint64_t seed, double frac,
     st[i] = vh_u01(vh_hash(seed, id, 0, VH_KEY_INIT)) < frac ? (uint32_t)VH_I : (uint32_t)VH_S;
 }
 
+// in-edge rows as transpose input: key = source index, value = row
+__global__ void k_member_edge_rows(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                   uint64_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (uint32_t k = off[r]; k < off[r + 1]; k++) {
+        keys[k] = src[k] - 1u;
+        vals[k] = (uint32_t)r;
+    }
+}
+// out_off[v] = first index whose key >= v (keys sorted), v in [0, n]
+__global__ void k_member_offsets(const uint32_t* __restrict__ keys, uint64_t ne, uint64_t n,
+                                 uint32_t* __restrict__ out_off) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n) return;
+    uint64_t lo = 0, hi = ne;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    out_off[v] = (uint32_t)lo;
+}
+
 struct MemberDesc {
     const uint32_t* off;
     const uint32_t* src;
@@ -92,12 +116,56 @@ struct MemberDesc {
     uint64_t seed;
     double p;
     uint32_t rec, type;
+    const uint32_t* out_off;   // out-edges (exposure pass)
+    const uint32_t* out_dst;
+    uint32_t* bits;            // two exposure bitmaps of nw words (launch parity)
+    uint64_t nw;
 };
 
-// one launch for every member: logical block b belongs to member
-// bmember[b] and covers agents [bstart[b], bstart[b] + block) of it
-__global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
-                               const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap) {
+// Exposure (as in the solo SIR step): per launch, k_ens_decide turns the
+// member's infected count of the previous launch (per-block counts stored by
+// k_sir_ensemble, all present) into dec[member]: on while fewer than 1/32 of
+// its agents are infected. With dec on, k_ens_expose sets the bit of every
+// out-neighbour of an infected agent in the launch's bitmap, and
+// k_sir_ensemble lets a susceptible agent without its bit skip the draws
+// (no infected in-neighbour: no infection). Both read the same dec word; a
+// stale bit or count can only make an agent draw, never skip wrongly.
+// k_sir_ensemble clears the other bitmap's words of its agents for the
+// next launch.
+__global__ __launch_bounds__(256) void k_ens_decide(const MemberDesc* __restrict__ md,
+                                                    const uint32_t* __restrict__ mblk,
+                                                    const uint32_t* __restrict__ blkcnt, uint32_t nblocks,
+                                                    uint32_t parity, uint32_t* __restrict__ dec) {
+    __shared__ uint64_t s_tot[4];
+    __shared__ uint32_t s_bad[4];
+    const uint32_t mi = blockIdx.x, tid = threadIdx.x;
+    const uint32_t b0 = mblk[2 * mi], nb = mblk[2 * mi + 1];
+    uint64_t tot = 0;
+    uint32_t bad = 0;
+    for (uint32_t q = tid; q < nb; q += 256) {
+        const uint32_t c = blkcnt[(uint64_t)parity * nblocks + b0 + q];
+        bad |= c == 0xFFFFFFFFu ? 1u : 0u;
+        tot += c == 0xFFFFFFFFu ? 0u : c;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        tot += __shfl_xor(tot, o, 64);
+        bad |= __shfl_xor(bad, o, 64);
+    }
+    if ((tid & 63u) == 0) {
+        s_tot[tid >> 6] = tot;
+        s_bad[tid >> 6] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint64_t t = s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3];
+        const uint32_t bd = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
+        const uint64_t n = md[mi].n;
+        dec[mi] = n && !bd && t * 32 <= n ? 1u : 0u;
+    }
+}
+
+__device__ __forceinline__ uint32_t ens_block(uint32_t nblocks, uint32_t remap) {
     uint32_t b = blockIdx.x;
     if (remap) {
         // XCD-aware order: consecutive blocks of one member land on one XCD
@@ -105,14 +173,49 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
         uint32_t per = (nblocks + 7) / 8;
         uint32_t x = b % 8, q = b / 8;
         b = x * per + q;
-        if (b >= nblocks) return;
     }
-    const MemberDesc& m = md[bmember[b]];
-    uint64_t i = (uint64_t)bstart[b] + threadIdx.x;
-    if (i >= m.n) return;
-    uint32_t s = m.rs[i];
+    return b;   // >= nblocks: no block
+}
+
+__global__ void k_ens_expose(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
+                             const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap,
+                             const uint32_t* __restrict__ dec, uint32_t parity) {
+    const uint32_t b = ens_block(nblocks, remap);
+    if (b >= nblocks) return;
+    const uint32_t mi = bmember[b];
+    if (!dec[mi]) return;
+    const MemberDesc& m = md[mi];
+    const uint64_t i = (uint64_t)bstart[b] + threadIdx.x;
+    if (i >= m.n || (m.rs[i] & 0xFFu) != VH_I) return;
+    uint32_t* bits = m.bits + parity * m.nw;
+    for (uint32_t k = m.out_off[i]; k < m.out_off[i + 1]; k++) {
+        const uint32_t v = m.out_dst[k];
+        atomicOr(bits + (v >> 5), 1u << (v & 31u));
+    }
+}
+
+// one launch for every member: logical block b belongs to member
+// bmember[b] and covers agents [bstart[b], bstart[b] + block) of it
+// (dec null: no exposure; else per-block infected counts of the written
+// state go to blkcnt[(parity ^ 1) * nblocks + b])
+__global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
+                               const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap,
+                               const uint32_t* __restrict__ dec, uint32_t parity,
+                               uint32_t* __restrict__ blkcnt) {
+    __shared__ uint32_t s_inf[16];
+    const uint32_t b = ens_block(nblocks, remap);
+    if (b >= nblocks) return;   // whole block
+    const uint32_t mi = bmember[b];
+    const MemberDesc& m = md[mi];
+    const uint64_t a0 = bstart[b];
+    if (a0 >= m.n) return;      // whole block (a member past its horizon has n = 0)
+    const uint64_t i = a0 + threadIdx.x;
+    const bool in = i < m.n;
+    const bool ex = dec && dec[mi];
+    uint32_t s = in ? m.rs[i] : 0u;
     uint32_t h = s & 0xFFu;
-    if (h == VH_S) {
+    const bool skip = in && h == VH_S && ex && !((m.bits[parity * m.nw + (i >> 5)] >> (i & 31u)) & 1u);
+    if (in && h == VH_S && !skip) {
         const uint64_t pre = vh_hash_pre(m.seed, vh_make_id(m.type, i + 1), m.step);
         uint32_t bb = m.off[i], e = m.off[i + 1];
         // the draw first, the source's state only below p (as in k_sir)
@@ -123,11 +226,25 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
                 break;
             }
         }
-    } else if (h == VH_I) {
+    } else if (in && h == VH_I) {
         uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
         s = (s & 0xFF00u) | (days << 16) | (days >= m.rec ? VH_R : VH_I);
     }
-    m.ws[i] = s;
+    if (in) m.ws[i] = s;
+    if (dec) {
+        // this block's words of the other bitmap, for the next launch
+        const uint64_t w0 = a0 >> 5;
+        if (threadIdx.x < (blockDim.x >> 5) && w0 + threadIdx.x < m.nw)
+            m.bits[(parity ^ 1u) * m.nw + w0 + threadIdx.x] = 0u;
+        const uint64_t msk = __ballot(in && (s & 0xFFu) == VH_I);
+        if ((threadIdx.x & 63u) == 0) s_inf[threadIdx.x >> 6] = (uint32_t)__popcll(msk);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (uint32_t q = 0; q < (blockDim.x >> 6); q++) c += s_inf[q];
+            blkcnt[(uint64_t)(parity ^ 1u) * nblocks + b] = c;
+        }
+    }
 }
 
 }  // namespace
@@ -191,8 +308,14 @@ struct Member {
     double p;
     uint32_t rec;
     DevBuf off, src, st[2];
+    DevBuf out_off, out_dst, bits;   // out-edges and the two exposure bitmaps
+    uint64_t nw = 0;                 // words per bitmap
     int cur = 0;
     std::shared_ptr<StepDone> done;   // the last launch that wrote this member
+    void release() {
+        off.release(); src.release(); st[0].release(); st[1].release();
+        out_off.release(); out_dst.release(); bits.release();
+    }
 };
 
 struct vh_ensemble {
@@ -211,6 +334,8 @@ struct vh_ensemble {
     size_t h_descs_cap = 0;
     uint32_t block = 256, remap = 1;
     DevBuf d_desc, d_bmember, d_bstart;
+    DevBuf d_mblk, d_blkcnt, d_dec;   // exposure: member block ranges, per-block counts, decisions
+    uint64_t launches = 0;            // step launches so far (bitmap parity)
     std::vector<MemberDesc> h_desc;
     std::vector<uint32_t> h_bmember, h_bstart;
     uint32_t nblocks = 0;
@@ -236,15 +361,26 @@ static int build_table(vh_ensemble* e) {
     e->h_bstart.clear();
     e->h_desc.clear();
     uint32_t mi = 0;
+    std::vector<uint32_t> mblk;
     for (auto& kv : e->members) {
         Member* m = kv.second;
+        mblk.push_back((uint32_t)e->h_bmember.size());
         for (uint64_t a = 0; a < m->n; a += e->block) {
             e->h_bmember.push_back(mi);
             e->h_bstart.push_back((uint32_t)a);
         }
+        mblk.push_back((uint32_t)e->h_bmember.size() - mblk.back());
         mi++;
     }
     e->nblocks = (uint32_t)e->h_bmember.size();
+    // exposure: member block ranges; block counts unknown until a launch
+    // with this table has written them
+    RC(e->d_mblk.ensure(mblk.size() * 4 + 8, e->stream));
+    RC(e->d_blkcnt.ensure((size_t)e->nblocks * 8 + 8, e->stream));
+    RC(e->d_dec.ensure(e->members.size() * 4 + 8, e->stream));
+    if (!mblk.empty())
+        HIPCHECK(hipMemcpy(e->d_mblk.p, mblk.data(), mblk.size() * 4, hipMemcpyHostToDevice));
+    fill_u32(e->d_blkcnt.as<uint32_t>(), 0xFFFFFFFFu, (uint64_t)e->nblocks * 2, e->stream);
     RC(e->d_bmember.ensure(e->h_bmember.size() * 4 + 8, e->stream));
     RC(e->d_bstart.ensure(e->h_bstart.size() * 4 + 8, e->stream));
     RC(e->d_desc.ensure((e->members.size() + 1) * sizeof(MemberDesc), e->stream));
@@ -279,6 +415,10 @@ static int upload_desc(vh_ensemble* e) {
         d.p = m->p;
         d.rec = m->rec;
         d.type = 1;
+        d.out_off = m->out_off.as<uint32_t>();
+        d.out_dst = m->out_dst.as<uint32_t>();
+        d.bits = m->bits.as<uint32_t>();
+        d.nw = m->nw;
         e->h_desc.push_back(d);
     }
     return VH_OK;
@@ -306,10 +446,11 @@ extern "C" int vh_ensemble_destroy(vh_ensemble_t e) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     for (auto& kv : e->members) {
         Member* m = kv.second;
-        m->off.release(); m->src.release(); m->st[0].release(); m->st[1].release();
+        m->release();
         delete m;
     }
     e->d_desc.release(); e->d_bmember.release(); e->d_bstart.release(); e->d_cdf.release();
+    e->d_mblk.release(); e->d_blkcnt.release(); e->d_dec.release();
     e->scratch.release();
     if (e->h_table) (void)hipHostFree(e->h_table);
     if (e->h_descs) (void)hipHostFree(e->h_descs);
@@ -318,6 +459,35 @@ extern "C" int vh_ensemble_destroy(vh_ensemble_t e) {
         (void)hipStreamDestroy(e->stream);
     }
     delete e;
+    return VH_OK;
+}
+
+// the member's out-edges (the transpose of its in-edge CSR, by a stable radix
+// sort on the source) and its two exposure bitmaps, cleared
+static int member_out_edges(Member* m, uint64_t ne, Scratch& sc, hipStream_t st) {
+    const uint64_t n = m->n;
+    uint32_t* keys = (uint32_t*)sc.take(ne * 4 + 4);
+    uint32_t* vals = (uint32_t*)sc.take(ne * 4 + 4);
+    uint32_t* ka = (uint32_t*)sc.take(ne * 4 + 4);
+    uint32_t* va = (uint32_t*)sc.take(ne * 4 + 4);
+    if (!keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
+    RC(m->out_off.ensure((n + 1) * 4 + 8, st));
+    RC(m->out_dst.ensure(ne * 4 + 8, st));
+    m->nw = (n + 31) / 32;
+    RC(m->bits.ensure(2 * m->nw * 4 + 8, st));
+    fill_u32(m->bits.as<uint32_t>(), 0u, 2 * m->nw, st);
+    if (ne) {
+        hipLaunchKernelGGL(k_member_edge_rows, dim3(grid_for(n, 256)), dim3(256), 0, st, m->off.as<uint32_t>(),
+                           m->src.as<uint32_t>(), n, keys, vals);
+        uint32_t bits = 1;
+        while (bits < 32 && (n >> bits) != 0) bits++;
+        if (radix_sort_pairs(&keys, &vals, ka, va, ne, bits, false, sc, st))
+            return set_error(VH_ERR_DEVICE, "radix sort failed");
+        HIPCHECK(hipMemcpyAsync(m->out_dst.p, vals, ne * 4, hipMemcpyDeviceToDevice, st));
+    }
+    hipLaunchKernelGGL(k_member_offsets, dim3(grid_for(n + 1, 256)), dim3(256), 0, st, keys, ne, n,
+                       m->out_off.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
     return VH_OK;
 }
 
@@ -338,7 +508,8 @@ extern "C" int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* sm, uin
     uint64_t ne = 0;
     e->scratch.reset(st);
     if (!rc) rc = gnp_device(sm->n, sm->mean_deg, sm->graph_seed, m->off, m->src, e->d_cdf, e->scratch, st, &ne);
-    if (rc) { delete m; return rc; }
+    if (!rc) rc = member_out_edges(m, ne, e->scratch, st);
+    if (rc) { m->release(); delete m; return rc; }
     m->nedges = ne;
     double frac = (double)sm->n_init / (double)sm->n;
     hipLaunchKernelGGL(k_sir_init, dim3(grid_for(sm->n, 256)), dim3(256), 0, st, m->st[0].as<uint32_t>(),
@@ -358,7 +529,7 @@ extern "C" int vh_ensemble_remove(vh_ensemble_t e, uint32_t id) {
     // launches after the member's last step only read its (zero) size from
     // their descriptor tables, so its buffers are free once that step is done
     if (m->done) HIPCHECK(hipEventSynchronize(m->done->ev));
-    m->off.release(); m->src.release(); m->st[0].release(); m->st[1].release();
+    m->release();
     delete m;
     e->members.erase(it);
     e->table_valid = false;
@@ -425,21 +596,38 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
     // every (xcd, slot) pair, and with it every logical block, exists
     uint32_t grid = e->remap ? 8u * ((e->nblocks + 7u) / 8u) : e->nblocks;
     dim3 g(grid), b(e->block);
+    // exposure pass unless VH_SIR_EXPOSE=0 (read per call)
+    const char* ev = getenv("VH_SIR_EXPOSE");
+    const bool expose = !(ev && atoi(ev) == 0);
+    const uint64_t launch0 = e->launches;
+    auto launch = [&](uint32_t s) {
+        const MemberDesc* md = e->d_desc.as<MemberDesc>() + (size_t)s * nm;
+        const uint32_t parity = (uint32_t)((launch0 + s) & 1u);
+        if (expose) {
+            hipLaunchKernelGGL(k_ens_decide, dim3((uint32_t)nm), dim3(256), 0, e->stream, md,
+                               e->d_mblk.as<uint32_t>(), e->d_blkcnt.as<uint32_t>(), e->nblocks, parity,
+                               e->d_dec.as<uint32_t>());
+            hipLaunchKernelGGL(k_ens_expose, g, b, 0, e->stream, md, e->d_bmember.as<uint32_t>(),
+                               e->d_bstart.as<uint32_t>(), e->nblocks, e->remap,
+                               (const uint32_t*)e->d_dec.as<uint32_t>(), parity);
+        }
+        hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, md, e->d_bmember.as<uint32_t>(),
+                           e->d_bstart.as<uint32_t>(), e->nblocks, e->remap,
+                           expose ? (const uint32_t*)e->d_dec.as<uint32_t>() : nullptr, parity,
+                           e->d_blkcnt.as<uint32_t>());
+    };
+    e->launches += steps;
     if (use_graph && steps > 1) {
         hipGraph_t graph;
         HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        for (uint32_t s = 0; s < steps; s++)
-            hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, e->d_desc.as<MemberDesc>() + (size_t)s * nm,
-                               e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(), e->nblocks, e->remap);
+        for (uint32_t s = 0; s < steps; s++) launch(s);
         HIPCHECK(hipStreamEndCapture(e->stream, &graph));
         hipError_t r = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (r != hipSuccess) return hip_fail(r, "hipGraphInstantiate");
         HIPCHECK(hipGraphLaunch(e->exec, e->stream));   // kept until the next sync
     } else {
-        for (uint32_t s = 0; s < steps; s++)
-            hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, e->d_desc.as<MemberDesc>() + (size_t)s * nm,
-                               e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(), e->nblocks, e->remap);
+        for (uint32_t s = 0; s < steps; s++) launch(s);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipEventRecord(done->ev, e->stream));

@@ -51,7 +51,9 @@ def footprint(req):
     """Device bytes a request holds while active (state, CSR, build scratch)."""
     e = req.n * req.mean_deg
     if req.kind == "sir":
-        return int(req.n * 12 + e * 4 + req.n * 8)
+        # states, offsets, sources, generator scratch; out-edges and the
+        # exposure bitmaps
+        return int(req.n * 12 + e * 4 + req.n * 8 + e * 4 + req.n * 5)
     # two Float64 state buffers, offsets, sources + weights, generator scratch
     fp = req.n * 16 + req.n * 4 + e * 8 + req.n * 8
     if req.n * 8 >= OPINION_PLAN_MIN_BYTES:

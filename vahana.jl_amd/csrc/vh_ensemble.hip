@@ -177,20 +177,27 @@ __device__ __forceinline__ uint32_t ens_block(uint32_t nblocks, uint32_t remap) 
     return b;   // >= nblocks: no block
 }
 
-__global__ void k_ens_expose(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
-                             const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap,
-                             const uint32_t* __restrict__ dec, uint32_t parity) {
-    const uint32_t b = ens_block(nblocks, remap);
-    if (b >= nblocks) return;
-    const uint32_t mi = bmember[b];
-    if (!dec[mi]) return;
-    const MemberDesc& m = md[mi];
-    const uint64_t i = (uint64_t)bstart[b] + threadIdx.x;
-    if (i >= m.n || (m.rs[i] & 0xFFu) != VH_I) return;
-    uint32_t* bits = m.bits + parity * m.nw;
-    for (uint32_t k = m.out_off[i]; k < m.out_off[i + 1]; k++) {
-        const uint32_t v = m.out_dst[k];
-        atomicOr(bits + (v >> 5), 1u << (v & 31u));
+// grid-stride over the logical blocks (of `bsize` agents): members whose
+// pass is off cost one read per logical block, not a block launch
+__global__ __launch_bounds__(256) void k_ens_expose(const MemberDesc* __restrict__ md,
+                                                    const uint32_t* __restrict__ bmember,
+                                                    const uint32_t* __restrict__ bstart, uint32_t nblocks,
+                                                    uint32_t bsize, const uint32_t* __restrict__ dec,
+                                                    uint32_t parity) {
+    for (uint32_t lb = blockIdx.x; lb < nblocks; lb += gridDim.x) {
+        const uint32_t mi = bmember[lb];
+        if (!dec[mi]) continue;   // uniform per block
+        const MemberDesc& m = md[mi];
+        uint32_t* bits = m.bits + parity * m.nw;
+        const uint64_t a0 = bstart[lb];
+        for (uint32_t o = threadIdx.x; o < bsize; o += 256) {
+            const uint64_t i = a0 + o;
+            if (i >= m.n || (m.rs[i] & 0xFFu) != VH_I) continue;
+            for (uint32_t k = m.out_off[i]; k < m.out_off[i + 1]; k++) {
+                const uint32_t v = m.out_dst[k];
+                atomicOr(bits + (v >> 5), 1u << (v & 31u));
+            }
+        }
     }
 }
 
@@ -607,9 +614,9 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
             hipLaunchKernelGGL(k_ens_decide, dim3((uint32_t)nm), dim3(256), 0, e->stream, md,
                                e->d_mblk.as<uint32_t>(), e->d_blkcnt.as<uint32_t>(), e->nblocks, parity,
                                e->d_dec.as<uint32_t>());
-            hipLaunchKernelGGL(k_ens_expose, g, b, 0, e->stream, md, e->d_bmember.as<uint32_t>(),
-                               e->d_bstart.as<uint32_t>(), e->nblocks, e->remap,
-                               (const uint32_t*)e->d_dec.as<uint32_t>(), parity);
+            hipLaunchKernelGGL(k_ens_expose, dim3(std::min<uint32_t>(e->nblocks, 8192u)), dim3(256), 0,
+                               e->stream, md, e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(),
+                               e->nblocks, e->block, (const uint32_t*)e->d_dec.as<uint32_t>(), parity);
         }
         hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, md, e->d_bmember.as<uint32_t>(),
                            e->d_bstart.as<uint32_t>(), e->nblocks, e->remap,

@@ -30,6 +30,7 @@ def test_ensemble_members_match_solo(vh, ora, gpu_ready, expose, monkeypatch):
     ensemble and in the solo runs; 2% initially infected, so the pass is on
     at first and turns off where the epidemic grows."""
     monkeypatch.setenv("VH_SIR_EXPOSE", expose)
+    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", "0")   # the solo runs of small members too
     print(f"[gpu] ensemble vs solo expose={expose}", flush=True)
     E = vh.ensemble.Ensemble()
     ids = []

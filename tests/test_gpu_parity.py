@@ -47,6 +47,7 @@ def _bytes(st):
 def test_sir_parity(vh, ora, gpu_ready, expose, monkeypatch):
     """SIR, with (default) and without the exposure pass (VH_SIR_EXPOSE)."""
     monkeypatch.setenv("VH_SIR_EXPOSE", expose)
+    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", "0")   # (3000 agents)
     print(f"[gpu] SIR parity expose={expose}", flush=True)
     M = vh.models
     n = 3000

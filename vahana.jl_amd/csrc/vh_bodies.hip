@@ -1601,12 +1601,15 @@ int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step) {
     if (!A.writeable) return set_error(VH_ERR_ASSERT, "SIR: the agent type must be in `write`");
     if (!A.nslots) return VH_OK;
     // the exposure pass needs the in-edges of `t` only (rows from 0, sources
-    // of `t`) and a step number below 2^32; VH_SIR_EXPOSE=0 turns it off
+    // of `t`); it runs for at least VH_SIR_EXPOSE_MIN agents (default 65536:
+    // below, its three extra launches per step cost more than the draws it
+    // saves); VH_SIR_EXPOSE=0 turns it off
     uint32_t* expo = nullptr;
     uint32_t* cnt = nullptr;
     const uint64_t nrows = E.rows ? E.nrows[t] : 0ull;
     if (nrows && E.nedges && !E.src64 && E.src_type == (uint32_t)t && E.rowbase[t] == 0 && nrows == E.rows &&
-        A.nslots < 0xFFFFFFFFull && env_ll("VH_SIR_EXPOSE", 1) != 0) {
+        A.nslots < 0xFFFFFFFFull && env_ll("VH_SIR_EXPOSE", 1) != 0 &&
+        (long long)A.nslots >= env_ll("VH_SIR_EXPOSE_MIN", 65536)) {
         RC(build_out_csr(s, E, A.nslots));
         expo = E.expo.as<uint32_t>();
         cnt = E.expo_cnt.as<uint32_t>();

@@ -317,6 +317,7 @@ struct Member {
     DevBuf off, src, st[2];
     DevBuf out_off, out_dst, bits;   // out-edges and the two exposure bitmaps
     uint64_t nw = 0;                 // words per bitmap
+    bool out_ready = false;          // built (on the first step with the exposure pass)
     int cur = 0;
     std::shared_ptr<StepDone> done;   // the last launch that wrote this member
     void release() {
@@ -342,6 +343,7 @@ struct vh_ensemble {
     uint32_t block = 256, remap = 1;
     DevBuf d_desc, d_bmember, d_bstart;
     DevBuf d_mblk, d_blkcnt, d_dec;   // exposure: member block ranges, per-block counts, decisions
+    Scratch oscr;                     // scratch of the members' out-edge builds (step stream)
     uint64_t launches = 0;            // step launches so far (bitmap parity)
     std::vector<MemberDesc> h_desc;
     std::vector<uint32_t> h_bmember, h_bstart;
@@ -459,6 +461,7 @@ extern "C" int vh_ensemble_destroy(vh_ensemble_t e) {
     e->d_desc.release(); e->d_bmember.release(); e->d_bstart.release(); e->d_cdf.release();
     e->d_mblk.release(); e->d_blkcnt.release(); e->d_dec.release();
     e->scratch.release();
+    e->oscr.release();
     if (e->h_table) (void)hipHostFree(e->h_table);
     if (e->h_descs) (void)hipHostFree(e->h_descs);
     if (e->own_streams) {
@@ -495,6 +498,7 @@ static int member_out_edges(Member* m, uint64_t ne, Scratch& sc, hipStream_t st)
     hipLaunchKernelGGL(k_member_offsets, dim3(grid_for(n + 1, 256)), dim3(256), 0, st, keys, ne, n,
                        m->out_off.as<uint32_t>());
     HIPCHECK(hipGetLastError());
+    m->out_ready = true;
     return VH_OK;
 }
 
@@ -515,7 +519,6 @@ extern "C" int vh_ensemble_add_sir(vh_ensemble_t e, const vh_sir_member* sm, uin
     uint64_t ne = 0;
     e->scratch.reset(st);
     if (!rc) rc = gnp_device(sm->n, sm->mean_deg, sm->graph_seed, m->off, m->src, e->d_cdf, e->scratch, st, &ne);
-    if (!rc) rc = member_out_edges(m, ne, e->scratch, st);
     if (rc) { m->release(); delete m; return rc; }
     m->nedges = ne;
     double frac = (double)sm->n_init / (double)sm->n;
@@ -573,6 +576,23 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
     }
     if (!e->table_valid || e->table_block != e->block) RC(build_table(e));
     size_t nm = e->members.size();
+    // exposure pass unless VH_SIR_EXPOSE=0, for ensembles of at least
+    // VH_SIR_EXPOSE_MIN agents (default 65536: below, its two extra launches
+    // per step cost more than the draws it saves); the members' out-edges
+    // are built on the first such step, on the step stream
+    const char* ev = getenv("VH_SIR_EXPOSE");
+    const char* evm = getenv("VH_SIR_EXPOSE_MIN");
+    uint64_t total = 0;
+    for (auto& kv : e->members) total += kv.second->n;
+    const bool expose = !(ev && atoi(ev) == 0) && (long long)total >= (evm ? atoll(evm) : 65536ll);
+    if (expose) {
+        for (auto& kv : e->members) {
+            Member* m = kv.second;
+            if (m->out_ready) continue;
+            e->oscr.reset(e->stream);   // (waits for the previous build when it spilled)
+            RC(member_out_edges(m, m->nedges, e->oscr, e->stream));
+        }
+    }
     // one descriptor table per step (buffers alternate, step numbers grow)
     RC(e->d_desc.ensure((size_t)steps * nm * sizeof(MemberDesc), e->stream));
     auto done = std::make_shared<StepDone>();
@@ -603,9 +623,6 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
     // every (xcd, slot) pair, and with it every logical block, exists
     uint32_t grid = e->remap ? 8u * ((e->nblocks + 7u) / 8u) : e->nblocks;
     dim3 g(grid), b(e->block);
-    // exposure pass unless VH_SIR_EXPOSE=0 (read per call)
-    const char* ev = getenv("VH_SIR_EXPOSE");
-    const bool expose = !(ev && atoi(ev) == 0);
     const uint64_t launch0 = e->launches;
     auto launch = [&](uint32_t s) {
         const MemberDesc* md = e->d_desc.as<MemberDesc>() + (size_t)s * nm;

@@ -743,27 +743,35 @@ constexpr int GP_U = 8;   // plan entries per thread in pass 1
 
 // Pass 2 with tile lists: the light in-edges of a 256-agent block are
 // numbered in agent order from lbase[block] (the block's "light edges"), in
-// tiles of OP_TILE. At plan time every tile's entries were sorted by plan
+// tiles of TileCfg<T>::TILE slots. At plan time every tile's entries were sorted by plan
 // position, i.e. by (source partition, CSR slot): entry e of a tile reads
 // y[ty[e]], a run of consecutive plan entries per partition, and lands at
 // slot tslot[e] of the tile in LDS. Weights, terms and the sums in canonical
 // order are as in opinion_block.
 
+// Tile-list sizes by state type (measured at 10^7 agents: 1024 slots for
+// Float64 and 1536 for Float32 give the most blocks per CU for their step)
+template <class T>
+struct TileCfg {
+    static constexpr int TILE = sizeof(T) == 8 ? 1024 : 1536;   // light slots per tile
+    static constexpr int CHUNK = TILE / 4;                       // heavy-row slots staged per wave
+    static constexpr int TILES = (64 * 256 + TILE - 1) / TILE;   // tiles of a light block at most
+};
+constexpr int OP_TILE_BITS = 4;   // bits of a tile number within a light block (list sort keys)
+
 // LDS of a tile-list block; a heavy block gives each of its 4 waves a
-// quarter of `val` (OP_TILE / 4 = OPH_CHUNK slots)
+// quarter of `val` (CHUNK slots)
 template <class T>
 struct TileLDS {
-    T val[OP_TILE];   // x_j by slot, then the numerator term
-    float den[OP_TILE];
-    uint8_t own[OP_TILE];
+    T val[TileCfg<T>::TILE];   // x_j by slot, then the numerator term
+    float den[TileCfg<T>::TILE];
+    uint8_t own[TileCfg<T>::TILE];
     uint32_t b[256], p[256], wsum[4];
     T xi[256];
     uint32_t prun[32];
     uint16_t pfirst[32];
 };
-constexpr int OPH_CHUNK = OP_TILE / 4;   // heavy-row slots staged per wave
 constexpr int OPP_RUNS = 32;              // partitions of a packed tile list (5 bits beside 11)
-constexpr int OPP_TILES = 64 * 256 / OP_TILE;   // tiles of a light block at most
 
 // PACKED: the list entry is slot | partition << 11, and its plan position is
 // prun[b][t][p] + (entry - pfirst[b][t][p]) for tile t of block b; ty is not
@@ -779,6 +787,7 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
                                                     const uint16_t* __restrict__ tslot, uint32_t nblkp,
                                                     const uint32_t* __restrict__ prun,
                                                     const uint16_t* __restrict__ pfirst) {
+    constexpr uint32_t TL = TileCfg<T>::TILE, NT = TileCfg<T>::TILES;
     T* s_val = sm.val;
     float* s_den = sm.den;
     uint8_t* s_own = sm.own;
@@ -799,8 +808,8 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
     const uint32_t lb = blk < nblkp ? lbase[blk] : 0u;
     uint32_t nx_run = 0, nx_first = 0;   // run table entry `tid` of the next tile
     if (PACKED && tid < OPP_RUNS && blk < nblkp) {
-        nx_run = prun[(uint64_t)blk * OPP_TILES * OPP_RUNS + tid];
-        nx_first = pfirst[(uint64_t)blk * OPP_TILES * OPP_RUNS + tid];
+        nx_run = prun[(uint64_t)blk * NT * OPP_RUNS + tid];
+        nx_first = pfirst[(uint64_t)blk * NT * OPP_RUNS + tid];
     }
     const bool light = valid && d <= 64u;
     const uint32_t ld = light ? d : 0u;
@@ -824,17 +833,17 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
     s_p[tid] = P;
     s_xi[tid] = xi;
     T num = (T)0, den = (T)0;
-    constexpr int QU = OP_TILE / 256;   // slots of a tile per thread
-    for (uint32_t t0 = 0; t0 < total; t0 += OP_TILE) {
+    constexpr int QU = TL / 256;   // slots of a tile per thread
+    for (uint32_t t0 = 0; t0 < total; t0 += TL) {
         const uint32_t lo = P > t0 ? P : t0;
-        const uint32_t hi = (P + ld) < (t0 + OP_TILE) ? (P + ld) : (t0 + OP_TILE);
+        const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
         for (uint32_t q = lo; q < hi; q++) s_own[q - t0] = (uint8_t)tid;
-        const uint32_t cnt = (total - t0) < (uint32_t)OP_TILE ? (total - t0) : (uint32_t)OP_TILE;
+        const uint32_t cnt = (total - t0) < TL ? (total - t0) : TL;
         if (PACKED && tid < OPP_RUNS) {
             sm.prun[tid] = nx_run;
             sm.pfirst[tid] = (uint16_t)nx_first;
-            if (t0 + OP_TILE < total) {
-                const uint64_t g = (uint64_t)blk * OPP_TILES + t0 / OP_TILE + 1;
+            if (t0 + TL < total) {
+                const uint64_t g = (uint64_t)blk * NT + t0 / TL + 1;
                 nx_run = prun[g * OPP_RUNS + tid];
                 nx_first = pfirst[g * OPP_RUNS + tid];
             }
@@ -907,7 +916,7 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
 }
 
 // A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
-// h are numbered from hbase[h] in CSR order, in chunks of OPH_CHUNK slots,
+// h are numbered from hbase[h] in CSR order, in chunks of CHUNK slots,
 // each chunk listed in plan order (hty: plan position, hslot: slot in the
 // chunk). The wave copies a chunk's runs of y into its LDS quarter `buf`,
 // then lane l adds slots l, l + 64, ... in CSR order: the canonical
@@ -918,13 +927,14 @@ __device__ __forceinline__ void opinion_heavy_tiles(const uint32_t* __restrict__
                                                     T* __restrict__ xo, uint64_t r, uint32_t hb, T eps,
                                                     const uint32_t* __restrict__ hty,
                                                     const uint16_t* __restrict__ hslot, T* buf) {
-    constexpr int HU = OPH_CHUNK / 64;
+    constexpr uint32_t CH = TileCfg<T>::CHUNK;
+    constexpr int HU = CH / 64;
     const uint32_t lane = threadIdx.x & 63;
     const T xi = x[r];
     const uint32_t b = off[r], d = off[r + 1] - b;
     T num = (T)0, den = (T)0;
-    for (uint32_t c0 = 0; c0 < d; c0 += OPH_CHUNK) {
-        const uint32_t cnt = d - c0 < (uint32_t)OPH_CHUNK ? d - c0 : (uint32_t)OPH_CHUNK;
+    for (uint32_t c0 = 0; c0 < d; c0 += CH) {
+        const uint32_t cnt = d - c0 < CH ? d - c0 : CH;
         float wq[HU];
         uint32_t jj[HU], sl[HU];
         T v[HU];
@@ -1020,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
             if (hbase) {   // one quarter of the block's LDS per wave
                 if (wv < nheavy)
                     opinion_heavy_tiles<T>(off, x, y, w, xo, heavy[wv], hbase[wv], eps, hty, hslot,
-                                           sm.val + (threadIdx.x >> 6) * OPH_CHUNK);
+                                           sm.val + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
                 return;
             }
         }
@@ -1068,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_light_totals(const uint32_t* __restrict
 __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ src, uint64_t nrows,
                                                     const uint32_t* __restrict__ lbase, uint32_t psize,
-                                                    uint32_t pbits, uint32_t* __restrict__ keys,
+                                                    uint32_t pbits, uint32_t tile, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
     __shared__ uint32_t s_w[4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1091,12 +1101,12 @@ __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__
 #pragma unroll
     for (int q = 0; q < 4; q++) pre += q < wv ? s_w[q] : 0u;
     const uint32_t P = pre + incl - ld, lb = lbase[blockIdx.x];
-    const uint32_t hi = blockIdx.x << (3 + pbits);
+    const uint32_t hi = blockIdx.x << (OP_TILE_BITS + pbits);
     for (uint32_t e = 0; e < ld; e++) {
         const uint32_t q = P + e, idx = lb + q;
-        keys[idx] = hi | ((q / (uint32_t)OP_TILE) << pbits) | ((src[b + e] - 1u) / psize);
+        keys[idx] = hi | ((q / tile) << pbits) | ((src[b + e] - 1u) / psize);
         kidx[idx] = b + e;
-        tsl[idx] = (uint16_t)(q % (uint32_t)OP_TILE);
+        tsl[idx] = (uint16_t)(q % tile);
     }
 }
 
@@ -1113,36 +1123,37 @@ __global__ void k_light_fill(const uint32_t* __restrict__ order, const uint32_t*
 // packed tile lists (at most OPP_RUNS partitions): per entry of the sorted
 // light lists, slot | partition << 11; for tile t of block b and each
 // partition p present, the plan position of its run's first entry
-// (prun[(b * OPP_TILES + t) * OPP_RUNS + p]) and that entry's index in the
+// (prun[(b * ntiles + t) * OPP_RUNS + p]) and that entry's index in the
 // tile (pfirst[...])
 __global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
                             const uint32_t* __restrict__ kidx, const uint16_t* __restrict__ tsl,
                             const uint32_t* __restrict__ pinv, const uint32_t* __restrict__ lbase,
-                            uint32_t pbits, uint64_t nl, uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
+                            uint32_t pbits, uint32_t tile, uint32_t ntiles, uint64_t nl,
+                            uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
                             uint16_t* __restrict__ pfirst) {
     uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nl) return;
     const uint32_t key = keys[e], p = key & ((1u << pbits) - 1u);
-    const uint32_t blk = key >> (3 + pbits), t = (key >> pbits) & 7u;
-    const uint32_t t0 = lbase[blk] + t * (uint32_t)OP_TILE;
+    const uint32_t blk = key >> (OP_TILE_BITS + pbits), t = (key >> pbits) & ((1u << OP_TILE_BITS) - 1u);
+    const uint32_t t0 = lbase[blk] + t * tile;
     const uint32_t idx = order[e];
     tpk[e] = (uint16_t)(tsl[idx] | (p << 11));
     if (e == t0 || keys[e - 1] != key) {
-        const uint64_t g = (uint64_t)blk * OPP_TILES + t;
+        const uint64_t g = (uint64_t)blk * ntiles + t;
         prun[g * OPP_RUNS + p] = pinv[kidx[idx]];
         pfirst[g * OPP_RUNS + p] = (uint16_t)(e - t0);
     }
 }
 
 
-// in-edge count and OPH_CHUNK-chunk count of each heavy row
+// in-edge count and chunk count of each heavy row
 __global__ void k_heavy_sizes(const uint32_t* __restrict__ off, const uint32_t* __restrict__ heavy, uint64_t nheavy,
-                              uint32_t* __restrict__ deg, uint32_t* __restrict__ chunks) {
+                              uint32_t chunk, uint32_t* __restrict__ deg, uint32_t* __restrict__ chunks) {
     uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nheavy) return;
     const uint32_t r = heavy[h], d = off[r + 1] - off[r];
     deg[h] = d;
-    chunks[h] = (d + OPH_CHUNK - 1) / OPH_CHUNK;
+    chunks[h] = (d + chunk - 1) / chunk;
 }
 
 // one wave per heavy row: sort key (chunk, source partition), CSR slot and
@@ -1152,16 +1163,16 @@ __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ heavy, uint64_t nheavy,
                                                     const uint32_t* __restrict__ hbase,
                                                     const uint32_t* __restrict__ cbase, uint32_t psize,
-                                                    uint32_t pbits, uint32_t* __restrict__ keys,
+                                                    uint32_t pbits, uint32_t chunk, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
     const uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (h >= nheavy) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t r = heavy[h], b = off[r], d = off[r + 1] - b, hb = hbase[h], cb = cbase[h];
     for (uint32_t t = lane; t < d; t += 64) {
-        keys[hb + t] = ((cb + t / OPH_CHUNK) << pbits) | ((src[b + t] - 1u) / psize);
+        keys[hb + t] = ((cb + t / chunk) << pbits) | ((src[b + t] - 1u) / psize);
         kidx[hb + t] = b + t;
-        tsl[hb + t] = (uint16_t)(t % OPH_CHUNK);
+        tsl[hb + t] = (uint16_t)(t % chunk);
     }
 }
 
@@ -1649,7 +1660,7 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
 // partition) with a stable radix sort, so each tile lists its entries in
 // plan order. Left out (E.plan_tiles false) when the key needs more than 32
 // bits.
-int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
+int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts, uint32_t tile) {
     hipStream_t st = s.stream;
     E.plan_tiles = false;
     E.plan_packed = false;
@@ -1658,8 +1669,9 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
     uint32_t bbits = 1, pbits = 1;
     while ((1ull << bbits) < nblk) bbits++;
     while ((1ull << pbits) < nparts) pbits++;
-    static_assert(64 * 256 / OP_TILE <= 8, "tiles per block fit 3 key bits");
-    if (bbits + 3 + pbits > 32) return VH_OK;
+    const uint32_t ntiles = (64 * 256 + tile - 1) / tile, chunk = tile / 4;
+    if (ntiles > (1u << OP_TILE_BITS)) return VH_OK;
+    if (bbits + OP_TILE_BITS + pbits > 32) return VH_OK;
     RC(E.lbase.ensure(((uint64_t)nblk + 1) * 4 + 8, st));
     uint32_t* tot = (uint32_t*)s.scratch.take((uint64_t)nblk * 4 + 4);
     if (!tot) return set_error(VH_ERR_OOM, "scratch");
@@ -1680,21 +1692,21 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
         uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2);
         if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
         hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, E.src.as<uint32_t>(), nrows,
-                           E.lbase.as<uint32_t>(), psize, pbits, keys, kidx, tsl);
-        if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + 3 + pbits, true, s.scratch, st))
+                           E.lbase.as<uint32_t>(), psize, pbits, tile, keys, kidx, tsl);
+        if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + OP_TILE_BITS + pbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "tile sort");
         hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, vals, kidx, tsl,
                            E.pinv.as<uint32_t>(), (uint64_t)nl, E.ty.as<uint32_t>(), E.tslot.as<uint16_t>());
         HIPCHECK(hipGetLastError());
         if (nparts <= (uint32_t)OPP_RUNS) {
             // packed lists: 16-bit entries, a run table per tile (a fixed
-            // OPP_TILES x OPP_RUNS slot per block)
-            const uint64_t nt = (uint64_t)nblk * OPP_TILES * OPP_RUNS;
+            // ntiles x OPP_RUNS slot per block)
+            const uint64_t nt = (uint64_t)nblk * ntiles * OPP_RUNS;
             RC(E.prun.ensure(nt * 4 + 8, st));
             RC(E.pfirst.ensure(nt * 2 + 8, st));
             RC(E.tpk.ensure((uint64_t)nl * 2 + 8, st));
             hipLaunchKernelGGL(k_pack_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, tsl,
-                               E.pinv.as<uint32_t>(), E.lbase.as<uint32_t>(), pbits, (uint64_t)nl,
+                               E.pinv.as<uint32_t>(), E.lbase.as<uint32_t>(), pbits, tile, ntiles, (uint64_t)nl,
                                E.tpk.as<uint16_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>());
             HIPCHECK(hipGetLastError());
             E.plan_packed = true;
@@ -1711,7 +1723,7 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
     if (!hd || !hc || !cbase) return set_error(VH_ERR_OOM, "scratch");
     RC(E.hbase.ensure(nheavy * 4 + 8, st));
     hipLaunchKernelGGL(k_heavy_sizes, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off, E.heavy.as<uint32_t>(),
-                       nheavy, hd, hc);
+                       nheavy, chunk, hd, hc);
     if (scan_excl_u32(hd, E.hbase.as<uint32_t>(), nheavy, s.scratch, st) ||
         scan_excl_u32(hc, cbase, nheavy, s.scratch, st))
         return set_error(VH_ERR_DEVICE, "scan");
@@ -1733,7 +1745,7 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
     uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nhe * 2);
     if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
     hipLaunchKernelGGL(k_heavy_keys, dim3(grid_for(nheavy * 64, 256)), dim3(256), 0, st, off, E.src.as<uint32_t>(),
-                       E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, psize, pbits, keys, kidx, tsl);
+                       E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, psize, pbits, chunk, keys, kidx, tsl);
     if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
         return set_error(VH_ERR_DEVICE, "chunk sort");
     hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl,
@@ -1745,8 +1757,8 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts) {
 
 // The plan of the in-edges of E (one target type at row 0; sources 1-based
 // numbers <= nsrc < 2^32).
-int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc) {
-    if (E.plan_version == E.version && E.plan_psize == psize) return VH_OK;
+int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint32_t tile) {
+    if (E.plan_version == E.version && E.plan_psize == psize && E.plan_tile == tile) return VH_OK;
     hipStream_t st = s.stream;
     const uint64_t ne = E.nedges;
     RC(E.psrc.ensure(ne * 4 + 8, st));
@@ -1767,10 +1779,11 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc) {
         hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), vals, ne,
                            E.psrc.as<uint32_t>(), E.pinv.as<uint32_t>());
         HIPCHECK(hipGetLastError());
-        RC(build_tile_lists(s, E, psize, (uint32_t)nparts));
+        RC(build_tile_lists(s, E, psize, (uint32_t)nparts, tile));
     }
     E.plan_version = E.version;
     E.plan_psize = psize;
+    E.plan_tile = tile;
     return VH_OK;
 }
 
@@ -1813,7 +1826,8 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
         const uint64_t np = (A.nslots + psize - 1) / psize;
         if (!part_set && np > (uint64_t)OPP_RUNS && np * 2 <= (uint64_t)OPP_RUNS * 3)
             psize = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;
-        RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots));
+        RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots,
+                             f32 ? (uint32_t)TileCfg<float>::TILE : (uint32_t)TileCfg<double>::TILE));
         RC(E.pval.ensure(E.nedges * tb + 8, s.stream));
         uint64_t nb = (E.nedges + 256 * GP_U - 1) / (256 * GP_U);
         dim3 gg((uint32_t)((nb + 7) / 8 * 8)), bb(256);

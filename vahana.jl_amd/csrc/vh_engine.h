@@ -78,6 +78,7 @@ struct EdgeStore {
     bool plan_tiles = false, plan_htiles = false, plan_packed = false;
     uint64_t plan_version = ~0ull;
     uint32_t plan_psize = 0;       // sources per partition
+    uint32_t plan_tile = 0;        // light slots per tile of the tile lists
     // out-edges of the SIR exposure pass (build_out_csr), valid for
     // `out_version`; `expo`: one bit per agent, set when it has an infected
     // in-neighbour in this step; `expo_cnt`: infected counts by step parity

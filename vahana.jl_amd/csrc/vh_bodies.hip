@@ -786,7 +786,8 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
                                                     const uint32_t* __restrict__ ty,
                                                     const uint16_t* __restrict__ tslot, uint32_t nblkp,
                                                     const uint32_t* __restrict__ prun,
-                                                    const uint16_t* __restrict__ pfirst) {
+                                                    const uint16_t* __restrict__ pfirst,
+                                                    const float* __restrict__ twt) {
     constexpr uint32_t TL = TileCfg<T>::TILE, NT = TileCfg<T>::TILES;
     T* s_val = sm.val;
     float* s_den = sm.den;
@@ -806,6 +807,24 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
         }
     }
     const uint32_t lb = blk < nblkp ? lbase[blk] : 0u;
+    const uint32_t le = blk < nblkp ? lbase[blk + 1] : 0u;
+    constexpr int QU = TL / 256;   // slots of a tile per thread
+    // list entries e = tid + 256 u of the next tile, fetched one tile ahead
+    // (the first tile's during the block scan); 0xFFFFFFFF past the tile
+    uint32_t nsl[QU], njj[QU];
+    {
+        const uint32_t c0 = (le - lb) < TL ? (le - lb) : TL;
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            const uint32_t q = (uint32_t)(tid + 256 * u);
+            nsl[u] = 0xFFFFFFFFu;
+            njj[u] = 0u;
+            if (q < c0) {
+                nsl[u] = __builtin_nontemporal_load(tslot + lb + q);
+                if constexpr (!PACKED) njj[u] = __builtin_nontemporal_load(ty + lb + q);
+            }
+        }
+    }
     uint32_t nx_run = 0, nx_first = 0;   // run table entry `tid` of the next tile
     if (PACKED && tid < OPP_RUNS && blk < nblkp) {
         nx_run = prun[(uint64_t)blk * NT * OPP_RUNS + tid];
@@ -833,12 +852,42 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
     s_p[tid] = P;
     s_xi[tid] = xi;
     T num = (T)0, den = (T)0;
-    constexpr int QU = TL / 256;   // slots of a tile per thread
     for (uint32_t t0 = 0; t0 < total; t0 += TL) {
         const uint32_t lo = P > t0 ? P : t0;
         const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
-        for (uint32_t q = lo; q < hi; q++) s_own[q - t0] = (uint8_t)tid;
         const uint32_t cnt = (total - t0) < TL ? (total - t0) : TL;
+        float wq[QU];
+        uint32_t jj[QU], sl[QU], wi[QU];
+        T v[QU];
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            sl[u] = nsl[u];
+            jj[u] = njj[u];
+        }
+        if constexpr (PACKED) {
+            // the weights of this tile's entries (list order) and the next
+            // tile's entries, in flight before the tile's LDS set-up
+#pragma unroll
+            for (int u = 0; u < QU; u++) {
+                const uint32_t q = (uint32_t)(tid + 256 * u);
+                wq[u] = q < cnt ? __builtin_nontemporal_load(twt + lb + t0 + q) : 0.0f;
+            }
+        }
+        {
+            const uint32_t t1 = t0 + TL;
+            const uint32_t c1 = total > t1 ? ((total - t1) < TL ? (total - t1) : TL) : 0u;
+#pragma unroll
+            for (int u = 0; u < QU; u++) {
+                const uint32_t q = (uint32_t)(tid + 256 * u);
+                nsl[u] = 0xFFFFFFFFu;
+                njj[u] = 0u;
+                if (q < c1) {
+                    nsl[u] = __builtin_nontemporal_load(tslot + lb + t1 + q);
+                    if constexpr (!PACKED) njj[u] = __builtin_nontemporal_load(ty + lb + t1 + q);
+                }
+            }
+        }
+        for (uint32_t q = lo; q < hi; q++) s_own[q - t0] = (uint8_t)tid;
         if (PACKED && tid < OPP_RUNS) {
             sm.prun[tid] = nx_run;
             sm.pfirst[tid] = (uint16_t)nx_first;
@@ -849,36 +898,26 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
             }
         }
         __syncthreads();
-        // weights of this thread's slots q = tid + 256 u (slot order) and
-        // the tile's list entries e = tid + 256 u (plan order), all in flight
-        float wq[QU];
-        uint32_t jj[QU], sl[QU], wi[QU];
-        T v[QU];
-        // the weight index of every slot first (LDS only), then all global
-        // loads at once
+        if constexpr (!PACKED) {
+            // weights of this thread's slots q = tid + 256 u (slot order):
+            // the weight index of every slot first (LDS only), then the loads
 #pragma unroll
-        for (int u = 0; u < QU; u++) {
-            const uint32_t q = (uint32_t)(tid + 256 * u);
-            wi[u] = 0xFFFFFFFFu;
-            if (q < cnt) {
-                const uint32_t a = s_own[q];
-                wi[u] = s_b[a] + (t0 + q - s_p[a]);
+            for (int u = 0; u < QU; u++) {
+                const uint32_t q = (uint32_t)(tid + 256 * u);
+                wi[u] = 0xFFFFFFFFu;
+                if (q < cnt) {
+                    const uint32_t a = s_own[q];
+                    wi[u] = s_b[a] + (t0 + q - s_p[a]);
+                }
             }
-        }
 #pragma unroll
-        for (int u = 0; u < QU; u++) {
-            const uint32_t q = (uint32_t)(tid + 256 * u);
-            wq[u] = 0.0f;
-            jj[u] = 0u;
-            sl[u] = 0xFFFFFFFFu;
-            if (q < cnt) {
-                wq[u] = __builtin_nontemporal_load(w + wi[u]);
-                if constexpr (!PACKED) jj[u] = __builtin_nontemporal_load(ty + lb + t0 + q);
-                sl[u] = __builtin_nontemporal_load(tslot + lb + t0 + q);
+            for (int u = 0; u < QU; u++) {
+                const uint32_t q = (uint32_t)(tid + 256 * u);
+                wq[u] = q < cnt ? __builtin_nontemporal_load(w + wi[u]) : 0.0f;
             }
         }
         if constexpr (PACKED) {
-            // (after all loads are issued) entry -> slot and plan position
+            // entry -> slot and plan position
 #pragma unroll
             for (int u = 0; u < QU; u++) {
                 const uint32_t q = (uint32_t)(tid + 256 * u);
@@ -891,22 +930,51 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
         }
 #pragma unroll
         for (int u = 0; u < QU; u++) v[u] = sl[u] < cnt ? y[jj[u]] : (T)0;
+        if constexpr (PACKED) {
+            // entry e's term lands at its slot
 #pragma unroll
-        for (int u = 0; u < QU; u++)
-            if (sl[u] < cnt) s_val[sl[u]] = v[u];
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < QU; u++) {
-            const uint32_t q = (uint32_t)(tid + 256 * u);
-            if (q < cnt) {
-                const T xj = s_val[q];
-                const bool acc = fabs(xj - s_xi[s_own[q]]) <= eps;
-                s_val[q] = acc ? (T)wq[u] * xj : (T)0;
-                s_den[q] = acc ? wq[u] : 0.0f;
+            for (int u = 0; u < QU; u++) {
+                if (sl[u] < cnt) {
+                    const T xj = v[u];
+                    const bool acc = fabs(xj - s_xi[s_own[sl[u]]]) <= eps;
+                    s_val[sl[u]] = acc ? (T)wq[u] * xj : (T)0;
+                    s_den[sl[u]] = acc ? wq[u] : 0.0f;
+                }
             }
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int u = 0; u < QU; u++)
+                if (sl[u] < cnt) s_val[sl[u]] = v[u];
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < QU; u++) {
+                const uint32_t q = (uint32_t)(tid + 256 * u);
+                if (q < cnt) {
+                    const T xj = s_val[q];
+                    const bool acc = fabs(xj - s_xi[s_own[q]]) <= eps;
+                    s_val[q] = acc ? (T)wq[u] * xj : (T)0;
+                    s_den[q] = acc ? wq[u] : 0.0f;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (uint32_t q = lo; q < hi; q++) {
+        // in slot order; four slots' LDS reads issued together
+        uint32_t q = lo;
+        for (; q + 4 <= hi; q += 4) {
+            const T a0 = s_val[q - t0], a1 = s_val[q + 1 - t0], a2 = s_val[q + 2 - t0], a3 = s_val[q + 3 - t0];
+            const float d0 = s_den[q - t0], d1 = s_den[q + 1 - t0], d2 = s_den[q + 2 - t0],
+                        d3 = s_den[q + 3 - t0];
+            num = num + a0;
+            num = num + a1;
+            num = num + a2;
+            num = num + a3;
+            den = den + (T)d0;
+            den = den + (T)d1;
+            den = den + (T)d2;
+            den = den + (T)d3;
+        }
+        for (; q < hi; q++) {
             num = num + s_val[q - t0];
             den = den + (T)s_den[q - t0];
         }
@@ -1005,7 +1073,8 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
                                                  const uint32_t* __restrict__ hty,
                                                  const uint16_t* __restrict__ hslot,
                                                  const uint32_t* __restrict__ prun,
-                                                 const uint16_t* __restrict__ pfirst) {
+                                                 const uint16_t* __restrict__ pfirst,
+                                                 const float* __restrict__ twt) {
     constexpr bool PLAN = MODE != 0;
     constexpr bool TILES = MODE >= 2;
     // tile modes: one LDS area for the light blocks and the heavy waves
@@ -1020,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
         }
         if constexpr (TILES) {
             opinion_block_tiles<T, MODE == 3>(sm, off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp,
-                                              prun, pfirst);
+                                              prun, pfirst, twt);
         } else {
             opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
         }
@@ -1121,8 +1190,9 @@ __global__ void k_light_fill(const uint32_t* __restrict__ order, const uint32_t*
 }
 
 // packed tile lists (at most OPP_RUNS partitions): per entry of the sorted
-// light lists, slot | partition << 11; for tile t of block b and each
-// partition p present, the plan position of its run's first entry
+// light lists, slot | partition << 11 and the weight of its CSR slot (0
+// without 4-byte edge states); for tile t of block b and each partition p
+// present, the plan position of its run's first entry
 // (prun[(b * ntiles + t) * OPP_RUNS + p]) and that entry's index in the
 // tile (pfirst[...])
 __global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
@@ -1130,7 +1200,8 @@ __global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* _
                             const uint32_t* __restrict__ pinv, const uint32_t* __restrict__ lbase,
                             uint32_t pbits, uint32_t tile, uint32_t ntiles, uint64_t nl,
                             uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
-                            uint16_t* __restrict__ pfirst) {
+                            uint16_t* __restrict__ pfirst, const float* __restrict__ w,
+                            float* __restrict__ twt) {
     uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nl) return;
     const uint32_t key = keys[e], p = key & ((1u << pbits) - 1u);
@@ -1138,6 +1209,7 @@ __global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* _
     const uint32_t t0 = lbase[blk] + t * tile;
     const uint32_t idx = order[e];
     tpk[e] = (uint16_t)(tsl[idx] | (p << 11));
+    twt[e] = w ? w[kidx[idx]] : 0.0f;
     if (e == t0 || keys[e - 1] != key) {
         const uint64_t g = (uint64_t)blk * ntiles + t;
         prun[g * OPP_RUNS + p] = pinv[kidx[idx]];
@@ -1705,9 +1777,11 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts, uint
             RC(E.prun.ensure(nt * 4 + 8, st));
             RC(E.pfirst.ensure(nt * 2 + 8, st));
             RC(E.tpk.ensure((uint64_t)nl * 2 + 8, st));
+            RC(E.twt.ensure((uint64_t)nl * 4 + 8, st));
             hipLaunchKernelGGL(k_pack_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, tsl,
                                E.pinv.as<uint32_t>(), E.lbase.as<uint32_t>(), pbits, tile, ntiles, (uint64_t)nl,
-                               E.tpk.as<uint16_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>());
+                               E.tpk.as<uint16_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
+                               E.bytes == 4 ? E.st.as<float>() : nullptr, E.twt.as<float>());
             HIPCHECK(hipGetLastError());
             E.plan_packed = true;
         }
@@ -1861,7 +1935,7 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
                            E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot,
                            (uint32_t)plan, E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), tsl_arg, nblkp, hb,
                            E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
-                           E.pfirst.as<uint16_t>());
+                           E.pfirst.as<uint16_t>(), E.twt.as<float>());
     } else {
         auto k = mode == 3   ? k_opinion<double, 3>
                  : mode == 2 ? k_opinion<double, 2>
@@ -1872,7 +1946,7 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
                            E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan,
                            E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), tsl_arg, nblkp, hb,
                            E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
-                           E.pfirst.as<uint16_t>());
+                           E.pfirst.as<uint16_t>(), E.twt.as<float>());
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

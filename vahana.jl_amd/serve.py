@@ -58,8 +58,9 @@ def footprint(req):
     fp = req.n * 16 + req.n * 4 + e * 8 + req.n * 8
     if req.n * 8 >= OPINION_PLAN_MIN_BYTES:
         # the two-pass gather: plan sources, positions and states, tile lists
-        # and run tables; the scratch of their one-time sorts
-        fp += e * 24 + req.n * 8 + e * 40
+        # with their weights and run tables; the scratch of their one-time
+        # sorts
+        fp += e * 28 + req.n * 8 + e * 40
     return int(fp)
 
 

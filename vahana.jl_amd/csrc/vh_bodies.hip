@@ -739,7 +739,7 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
 // sequential stream per partition), or from tile lists (MODE 2 and 3, below).
 // The values are the same as x[src[k]] and the sums keep their canonical
 // order.
-constexpr int GP_U = 8;   // plan entries per thread in pass 1
+constexpr int GP_U = 4;   // plan entries per thread in pass 1 (measured: 4 beats 1, 2, 8, 16)
 
 // Pass 2 with tile lists: the light in-edges of a 256-agent block are
 // numbered in agent order from lbase[block] (the block's "light edges"), in

@@ -103,6 +103,36 @@ def test_opinion_parity(vh, ora, gpu_ready, f32, mode, monkeypatch):
         assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
 
 
+@pytest.mark.parametrize("f32", [False, True])
+def test_opinion_plan_after_edge_change(vh, ora, gpu_ready, f32, monkeypatch):
+    """The two-pass plan (its lists and the edge weights copied into list
+    order) follows a change of the edge store: after three steps every agent
+    adds a self-loop of weight 0.75 (add_existing), and the next steps must
+    match the checker's."""
+    monkeypatch.setenv("VH_OPINION_PLAN_BYTES", "0")
+    monkeypatch.setenv("VH_OPINION_PART_BYTES", "2048")
+    print(f"[gpu] opinion plan after edge change f32={f32}", flush=True)
+    M = vh.models
+    n = 4000
+    a, b = vh.inputs.graph_ba(n, 5, 8)
+    edges = vh.inputs.symmetric_weighted(a, b, 9)
+    g = M.build_opinion(vh, n, seed=8, f32=f32, edges=edges)
+    c = M.build_opinion(ora, n, seed=8, f32=f32, edges=edges)
+    ne = vh.num_edges(g.sim, g.E)
+    for step in range(6):
+        if step == 3:
+            for V, m in ((vh, g), (ora, c)):
+                st = np.zeros(1, dtype=m.E.dtype)
+                st["w"] = 0.75
+                V.apply_(m.sim, V.transitions.add_self_edge(m.E, st), [m.T], [], [m.E], add_existing=[m.E])
+            assert vh.num_edges(g.sim, g.E) == ne + n == ora.num_edges(c.sim, c.E)
+        M.step_opinion(vh, g, eps=0.2)
+        M.step_opinion(ora, c, eps=0.2)
+        x, _ = _agents(g)
+        y, _ = _agents(c)
+        assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
+
+
 # (contacts, redraws, beta, p_death): the default model; more contacts than
 # one LDS row holds (the unstaged path); no contacts; a single draw per
 # contact under heavy deaths (draws exhausted on dead partners); certain

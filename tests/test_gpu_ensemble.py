@@ -55,6 +55,39 @@ def test_ensemble_members_match_solo(vh, ora, gpu_ready, expose, monkeypatch):
     assert info["members"] == len(SPECS) and info["agents"] == sum(s[0] for s in SPECS)
 
 
+def test_ensemble_incremental_exposure(vh, ora, gpu_ready, monkeypatch):
+    """Slow epidemics (0.1% initially infected, few infections per step) keep
+    the exposure pass on for every launch, so the bitmaps are updated from the
+    newly infected agents between full launches (every 8th); 30 steps in
+    graph and plain launches of several lengths, each member against its solo
+    run without the pass, one against the checker."""
+    monkeypatch.setenv("VH_SIR_EXPOSE", "1")
+    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", "0")
+    print("[gpu] ensemble incremental exposure", flush=True)
+    specs = [(20000, 8.0, 0.02, 31, 301), (50000, 6.0, 0.03, 32, 302), (5000, 10.0, 0.015, 33, 303)]
+    E = vh.ensemble.Ensemble()
+    ids = [E.add_sir(n, md, p, seed, gseed, recovery=7, n_init=max(1, n // 1000))
+           for n, md, p, seed, gseed in specs]
+    E.step(11, use_graph=True)
+    E.step(3, use_graph=False)
+    E.step(16, use_graph=True)
+    steps = 30
+    monkeypatch.setenv("VH_SIR_EXPOSE", "0")
+    for mid, (n, md, p, seed, gseed) in zip(ids, specs):
+        got, next_step = E.read(mid)
+        assert next_step == steps + 1
+        g = np.ascontiguousarray(got)
+        inf = int((g.view(np.uint8).reshape(n, -1)[:, 0] == vh.I).sum())
+        rec = int((g.view(np.uint8).reshape(n, -1)[:, 0] == vh.R).sum())
+        assert rec > 0 and inf * 32 <= n, f"member n={n}: {inf} infected, {rec} recovered"
+        want = _solo(vh, n, md, p, seed, gseed, steps, max(1, n // 1000))
+        assert np.array_equal(g.view(np.uint8), want), f"member n={n}"
+    n, md, p, seed, gseed = specs[2]
+    got, _ = E.read(ids[2])
+    assert np.array_equal(np.ascontiguousarray(got).view(np.uint8),
+                          _solo(ora, n, md, p, seed, gseed, steps, max(1, n // 1000)))
+
+
 def test_ensemble_join_and_leave(vh, gpu_ready):
     """Members added and removed between steps keep their own step counters."""
     E = vh.ensemble.Ensemble()

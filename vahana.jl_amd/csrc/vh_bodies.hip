@@ -526,17 +526,13 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
     if (in && h == VH_S && !skip) {
         if (i < nrows) {
             const uint64_t pre = vh_hash_pre(seed, vh_make_id(t, i + 1), step);
+            const uint64_t pt = vh_u01_below(p);
             uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
             // "some in-edge k has an infected source and u_k < p" is
             // order-free: the draw (ALU) first, the source's state (a random
             // gather) only for the few draws below p
-            for (uint32_t k = b; k < e; k++) {
-                if (vh_u01(vh_hash_fin(pre, k - b + 1)) >= p) continue;
-                if ((rs[src[k] - 1] & 0xFFu) == VH_I) {
-                    s = (s & 0xFF00u) | VH_I;
-                    break;
-                }
-            }
+            if (vh_any_infecting(pre, pt, b, e, [&](uint32_t k) { return (rs[src[k] - 1] & 0xFFu) == VH_I; }))
+                s = (s & 0xFF00u) | VH_I;
         }
     } else if (in && h == VH_I) {
         uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
@@ -1344,17 +1340,13 @@ __global__ __launch_bounds__(256) void k_contact_main(
             if (off && i < nrows) {
                 float susc = __uint_as_float((uint32_t)(s >> 32));
                 double pinf = p.beta * (double)susc;
+                const uint64_t pt = vh_u01_below(pinf);
                 uint32_t b = off[rowbase + i], e = off[rowbase + i + 1];
                 // "some in-edge k has an infected source and u_k < pinf" is
                 // order-free: test the draw first (ALU) and look up the
                 // source's flag only for the few draws below pinf
-                for (uint32_t k = b; k < e; k++) {
-                    if (vh_u01(vh_hash_fin(pre, k - b + 1)) >= pinf) continue;
-                    if (cbit(B.inf, src[k] - 1)) {
-                        h = VH_I;
-                        break;
-                    }
-                }
+                if (vh_any_infecting(pre, pt, b, e, [&](uint32_t k) { return cbit(B.inf, src[k] - 1); }))
+                    h = VH_I;
             }
         } else if (h == VH_I) {
             if (vh_u01(vh_hash_fin(pre, VH_KEY_RECOVER)) < p.gamma) h = VH_R;

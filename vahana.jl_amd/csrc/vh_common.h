@@ -6,6 +6,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "../../include/vahana_hip.h"
@@ -61,6 +62,38 @@ __host__ __device__ __forceinline__ uint64_t vh_hash(uint64_t s, uint64_t a, uin
 // uniform double in [0,1) from the top 53 bits; float from the top 24
 __host__ __device__ __forceinline__ double vh_u01(uint64_t h) {
     return (double)(h >> 11) * 0x1.0p-53;
+}
+// u01(h) < p exactly when (h >> 11) < vh_u01_below(p): u01's value is
+// (h >> 11) * 2^-53 without rounding, so the test is (h >> 11) < p * 2^53,
+// i.e. below the ceiling of p * 2^53; p <= 0 never passes, p >= 1 and NaN
+// always do (`u01(h) >= p` is false for them). An integer compare per draw
+// instead of a conversion to double.
+__host__ __device__ __forceinline__ uint64_t vh_u01_below(double p) {
+    if (!(p > 0.0)) return p != p ? ~0ull : 0ull;
+    if (p >= 1.0) return ~0ull;
+    return (uint64_t)ceil(p * 0x1.0p53);
+}
+// Whether some in-edge k in [b, e) has u01(vh_hash_fin(pre, k - b + 1)) < p
+// (pt = vh_u01_below(p)) and an infected source (infected(k)). The answer
+// does not depend on the order of the in-edges, so the draws are computed
+// four at a time and a source is looked up only for a draw below p.
+template <class F>
+__device__ __forceinline__ bool vh_any_infecting(uint64_t pre, uint64_t pt, uint32_t b, uint32_t e,
+                                                 F infected) {
+    uint32_t k = b;
+    for (; k + 4 <= e; k += 4) {
+        const uint64_t c = k - b + 1;
+        const bool d0 = (vh_hash_fin(pre, c) >> 11) < pt, d1 = (vh_hash_fin(pre, c + 1) >> 11) < pt;
+        const bool d2 = (vh_hash_fin(pre, c + 2) >> 11) < pt, d3 = (vh_hash_fin(pre, c + 3) >> 11) < pt;
+        if (d0 | d1 | d2 | d3) {
+            if ((d0 && infected(k)) || (d1 && infected(k + 1)) || (d2 && infected(k + 2)) ||
+                (d3 && infected(k + 3)))
+                return true;
+        }
+    }
+    for (; k < e; k++)
+        if ((vh_hash_fin(pre, k - b + 1) >> 11) < pt && infected(k)) return true;
+    return false;
 }
 __host__ __device__ __forceinline__ float vh_u01f(uint64_t h) {
     return (float)(h >> 40) * 0x1.0p-24f;

@@ -124,18 +124,24 @@ struct MemberDesc {
 
 // Exposure (as in the solo SIR step): per launch, k_ens_decide turns the
 // member's infected count of the previous launch (per-block counts stored by
-// k_sir_ensemble, all present) into dec[member]: on while fewer than 1/32 of
-// its agents are infected. With dec on, k_ens_expose sets the bit of every
-// out-neighbour of an infected agent in the launch's bitmap, and
-// k_sir_ensemble lets a susceptible agent without its bit skip the draws
-// (no infected in-neighbour: no infection). Both read the same dec word; a
-// stale bit or count can only make an agent draw, never skip wrongly.
-// k_sir_ensemble clears the other bitmap's words of its agents for the
-// next launch.
+// k_sir_ensemble, all present) into dec[member]: bit 0 on while fewer than
+// 1/32 of its agents are infected; bit 1 "full" when the previous launch was
+// off or the launch number is a multiple of EXPO_FULL. With dec on,
+// k_ens_expose sets the bit of every out-neighbour of an infected agent in
+// the launch's bitmap (full), or only of the agents infected in the previous
+// launch (days 0) on top of the previous launch's bitmap, and k_sir_ensemble
+// lets a susceptible agent without its bit skip the draws (no infected
+// in-neighbour: no infection). Every infected agent was either infected since
+// the last full launch or marked in it, so the bitmap covers all of them; bits
+// of agents that recovered since stay until the next full launch, and a stale
+// bit or count can only make an agent draw, never skip wrongly.
+// k_sir_ensemble prepares the other bitmap's words of its agents for the
+// next launch: cleared before a full launch, else a copy of this launch's.
+constexpr uint64_t EXPO_FULL = 8;   // a full exposure launch at least every 8 launches
 __global__ __launch_bounds__(256) void k_ens_decide(const MemberDesc* __restrict__ md,
                                                     const uint32_t* __restrict__ mblk,
                                                     const uint32_t* __restrict__ blkcnt, uint32_t nblocks,
-                                                    uint32_t parity, uint32_t* __restrict__ dec) {
+                                                    uint32_t parity, uint64_t lidx, uint32_t* __restrict__ dec) {
     __shared__ uint64_t s_tot[4];
     __shared__ uint32_t s_bad[4];
     const uint32_t mi = blockIdx.x, tid = threadIdx.x;
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(256) void k_ens_decide(const MemberDesc* __restrict
     for (uint32_t q = tid; q < nb; q += 256) {
         const uint32_t c = blkcnt[(uint64_t)parity * nblocks + b0 + q];
         bad |= c == 0xFFFFFFFFu ? 1u : 0u;
-        tot += c == 0xFFFFFFFFu ? 0u : c;
+        tot += c == 0xFFFFFFFFu ? 0u : (c & 0xFFFFu);
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -161,7 +167,9 @@ __global__ __launch_bounds__(256) void k_ens_decide(const MemberDesc* __restrict
         const uint64_t t = s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3];
         const uint32_t bd = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
         const uint64_t n = md[mi].n;
-        dec[mi] = n && !bd && t * 32 <= n ? 1u : 0u;
+        const bool on = n && !bd && t * 32 <= n;
+        const bool full = !(dec[mi] & 1u) || lidx % EXPO_FULL == 0;
+        dec[mi] = on ? (full ? 3u : 1u) : 0u;
     }
 }
 
@@ -178,21 +186,31 @@ __device__ __forceinline__ uint32_t ens_block(uint32_t nblocks, uint32_t remap) 
 }
 
 // grid-stride over the logical blocks (of `bsize` agents): members whose
-// pass is off cost one read per logical block, not a block launch
+// pass is off, and blocks without an agent to mark (by the block's counts of
+// the previous launch: infected, and infected in it), cost a read or two
+// per logical block, not a block launch
 __global__ __launch_bounds__(256) void k_ens_expose(const MemberDesc* __restrict__ md,
                                                     const uint32_t* __restrict__ bmember,
                                                     const uint32_t* __restrict__ bstart, uint32_t nblocks,
                                                     uint32_t bsize, const uint32_t* __restrict__ dec,
-                                                    uint32_t parity) {
+                                                    uint32_t parity, const uint32_t* __restrict__ blkcnt) {
     for (uint32_t lb = blockIdx.x; lb < nblocks; lb += gridDim.x) {
         const uint32_t mi = bmember[lb];
-        if (!dec[mi]) continue;   // uniform per block
+        const uint32_t dm = dec[mi];
+        if (!(dm & 1u)) continue;   // uniform per block
+        const bool full = (dm & 2u) != 0u;
+        // (on: every count of the member is present)
+        const uint32_t bc = blkcnt[(uint64_t)parity * nblocks + lb];
+        if ((full ? (bc & 0xFFFFu) : (bc >> 16)) == 0u) continue;
         const MemberDesc& m = md[mi];
         uint32_t* bits = m.bits + parity * m.nw;
         const uint64_t a0 = bstart[lb];
         for (uint32_t o = threadIdx.x; o < bsize; o += 256) {
             const uint64_t i = a0 + o;
-            if (i >= m.n || (m.rs[i] & 0xFFu) != VH_I) continue;
+            if (i >= m.n) continue;
+            const uint32_t st = m.rs[i];
+            // infected, and (unless full) infected in the previous launch
+            if ((st & 0xFFu) != VH_I || (!full && (st >> 16) != 0u)) continue;
             for (uint32_t k = m.out_off[i]; k < m.out_off[i + 1]; k++) {
                 const uint32_t v = m.out_dst[k];
                 atomicOr(bits + (v >> 5), 1u << (v & 31u));
@@ -207,9 +225,9 @@ __global__ __launch_bounds__(256) void k_ens_expose(const MemberDesc* __restrict
 // state go to blkcnt[(parity ^ 1) * nblocks + b])
 __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
                                const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap,
-                               const uint32_t* __restrict__ dec, uint32_t parity,
+                               const uint32_t* __restrict__ dec, uint32_t parity, uint64_t lidx,
                                uint32_t* __restrict__ blkcnt) {
-    __shared__ uint32_t s_inf[16];
+    __shared__ uint32_t s_inf[16], s_new[16];
     const uint32_t b = ens_block(nblocks, remap);
     if (b >= nblocks) return;   // whole block
     const uint32_t mi = bmember[b];
@@ -218,38 +236,48 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
     if (a0 >= m.n) return;      // whole block (a member past its horizon has n = 0)
     const uint64_t i = a0 + threadIdx.x;
     const bool in = i < m.n;
-    const bool ex = dec && dec[mi];
+    const uint32_t dm = dec ? dec[mi] : 0u;
+    const bool ex = (dm & 1u) != 0u;
     uint32_t s = in ? m.rs[i] : 0u;
     uint32_t h = s & 0xFFu;
     const bool skip = in && h == VH_S && ex && !((m.bits[parity * m.nw + (i >> 5)] >> (i & 31u)) & 1u);
     if (in && h == VH_S && !skip) {
         const uint64_t pre = vh_hash_pre(m.seed, vh_make_id(m.type, i + 1), m.step);
+        const uint64_t pt = vh_u01_below(m.p);
         uint32_t bb = m.off[i], e = m.off[i + 1];
         // the draw first, the source's state only below p (as in k_sir)
-        for (uint32_t k = bb; k < e; k++) {
-            if (vh_u01(vh_hash_fin(pre, k - bb + 1)) >= m.p) continue;
-            if ((m.rs[m.src[k] - 1] & 0xFFu) == VH_I) {
-                s = (s & 0xFF00u) | VH_I;
-                break;
-            }
-        }
+        if (vh_any_infecting(pre, pt, bb, e, [&](uint32_t k) { return (m.rs[m.src[k] - 1] & 0xFFu) == VH_I; }))
+            s = (s & 0xFF00u) | VH_I;
     } else if (in && h == VH_I) {
         uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
         s = (s & 0xFF00u) | (days << 16) | (days >= m.rec ? VH_R : VH_I);
     }
     if (in) m.ws[i] = s;
     if (dec) {
-        // this block's words of the other bitmap, for the next launch
+        // this block's words of the other bitmap, for the next launch:
+        // cleared before a full launch (this one off, or the next launch
+        // number a multiple of EXPO_FULL), else this launch's words
         const uint64_t w0 = a0 >> 5;
+        const bool next_full = !ex || (lidx + 1) % EXPO_FULL == 0;
         if (threadIdx.x < (blockDim.x >> 5) && w0 + threadIdx.x < m.nw)
-            m.bits[(parity ^ 1u) * m.nw + w0 + threadIdx.x] = 0u;
+            m.bits[(parity ^ 1u) * m.nw + w0 + threadIdx.x] =
+                next_full ? 0u : m.bits[parity * m.nw + w0 + threadIdx.x];
+        // infected agents of the written state, and those infected in this
+        // launch (days 0): count | new count << 16
         const uint64_t msk = __ballot(in && (s & 0xFFu) == VH_I);
-        if ((threadIdx.x & 63u) == 0) s_inf[threadIdx.x >> 6] = (uint32_t)__popcll(msk);
+        const uint64_t nmsk = __ballot(in && (s & 0xFFu) == VH_I && (s >> 16) == 0u);
+        if ((threadIdx.x & 63u) == 0) {
+            s_inf[threadIdx.x >> 6] = (uint32_t)__popcll(msk);
+            s_new[threadIdx.x >> 6] = (uint32_t)__popcll(nmsk);
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t c = 0;
-            for (uint32_t q = 0; q < (blockDim.x >> 6); q++) c += s_inf[q];
-            blkcnt[(uint64_t)(parity ^ 1u) * nblocks + b] = c;
+            uint32_t c = 0, cn = 0;
+            for (uint32_t q = 0; q < (blockDim.x >> 6); q++) {
+                c += s_inf[q];
+                cn += s_new[q];
+            }
+            blkcnt[(uint64_t)(parity ^ 1u) * nblocks + b] = c | cn << 16;
         }
     }
 }
@@ -345,6 +373,7 @@ struct vh_ensemble {
     DevBuf d_mblk, d_blkcnt, d_dec;   // exposure: member block ranges, per-block counts, decisions
     Scratch oscr;                     // scratch of the members' out-edge builds (step stream)
     uint64_t launches = 0;            // step launches so far (bitmap parity)
+    bool expose_last = false;         // the exposure pass ran in the last step call
     std::vector<MemberDesc> h_desc;
     std::vector<uint32_t> h_bmember, h_bstart;
     uint32_t nblocks = 0;
@@ -592,7 +621,11 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
             e->oscr.reset(e->stream);   // (waits for the previous build when it spilled)
             RC(member_out_edges(m, m->nedges, e->oscr, e->stream));
         }
+        // the bitmaps were not kept while the pass was off: every member's
+        // next exposure launch is a full one
+        if (!e->expose_last) fill_u32(e->d_dec.as<uint32_t>(), 0u, nm, e->stream);
     }
+    e->expose_last = expose;
     // one descriptor table per step (buffers alternate, step numbers grow)
     RC(e->d_desc.ensure((size_t)steps * nm * sizeof(MemberDesc), e->stream));
     auto done = std::make_shared<StepDone>();
@@ -630,15 +663,16 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
         if (expose) {
             hipLaunchKernelGGL(k_ens_decide, dim3((uint32_t)nm), dim3(256), 0, e->stream, md,
                                e->d_mblk.as<uint32_t>(), e->d_blkcnt.as<uint32_t>(), e->nblocks, parity,
-                               e->d_dec.as<uint32_t>());
+                               (uint64_t)(launch0 + s), e->d_dec.as<uint32_t>());
             hipLaunchKernelGGL(k_ens_expose, dim3(std::min<uint32_t>(e->nblocks, 8192u)), dim3(256), 0,
                                e->stream, md, e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(),
-                               e->nblocks, e->block, (const uint32_t*)e->d_dec.as<uint32_t>(), parity);
+                               e->nblocks, e->block, (const uint32_t*)e->d_dec.as<uint32_t>(), parity,
+                               (const uint32_t*)e->d_blkcnt.as<uint32_t>());
         }
         hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, md, e->d_bmember.as<uint32_t>(),
                            e->d_bstart.as<uint32_t>(), e->nblocks, e->remap,
                            expose ? (const uint32_t*)e->d_dec.as<uint32_t>() : nullptr, parity,
-                           e->d_blkcnt.as<uint32_t>());
+                           (uint64_t)(launch0 + s), e->d_blkcnt.as<uint32_t>());
     };
     e->launches += steps;
     if (use_graph && steps > 1) {

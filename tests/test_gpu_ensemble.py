@@ -88,8 +88,13 @@ def test_ensemble_incremental_exposure(vh, ora, gpu_ready, monkeypatch):
                           _solo(ora, n, md, p, seed, gseed, steps, max(1, n // 1000)))
 
 
-def test_ensemble_join_and_leave(vh, gpu_ready):
-    """Members added and removed between steps keep their own step counters."""
+@pytest.mark.parametrize("expose_min", ["65536", "0"])
+def test_ensemble_join_and_leave(vh, gpu_ready, expose_min, monkeypatch):
+    """Members added and removed between steps keep their own step counters;
+    with VH_SIR_EXPOSE_MIN=0 the exposure pass runs across the rebuilt block
+    tables (every member's first launch after a change has the pass off)."""
+    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", expose_min)
+    print(f"[gpu] ensemble join and leave expose_min={expose_min}", flush=True)
     E = vh.ensemble.Ensemble()
     a = E.add_sir(5000, 10.0, 0.08, 21, 201, n_init=50)
     E.step(5)

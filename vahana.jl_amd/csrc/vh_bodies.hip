@@ -1244,28 +1244,53 @@ __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__
     }
 }
 
-// grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U))
+// grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U)); logical
+// block l takes plan entries [l * 1024, (l + 1) * 1024). Float32: thread t
+// takes the 4 consecutive entries from l * 1024 + 4t (one 16-byte load of
+// their sources, four gathers, one 16-byte store; 0.731 against 0.739 ms per
+// step with the Float64 layout below). The buffers are 256-byte aligned; a
+// partial group at the end goes entry by entry.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t pu32x4 __attribute__((ext_vector_type(4)));
+static_assert(GP_U == 4, "k_plan_gather takes four plan entries per thread");
 template <class T>
 __global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc, const T* __restrict__ x,
                                                      T* __restrict__ y, uint64_t ne) {
     const uint32_t per = gridDim.x / 8;
     const uint32_t l = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    const uint64_t j0 = (uint64_t)l * (256 * GP_U) + threadIdx.x;
-    uint32_t sj[GP_U];
-    T v[GP_U];
+    const uint64_t base = (uint64_t)l * (256 * GP_U);
+    if constexpr (sizeof(T) == 4) {
+        const uint64_t j0 = base + (uint64_t)threadIdx.x * 4;
+        if (j0 >= ne) return;
+        if (j0 + 4 <= ne) {
+            const pu32x4 sv = __builtin_nontemporal_load((const pu32x4*)(psrc + j0));
+            const f32x4 v = {(float)x[sv.x - 1], (float)x[sv.y - 1], (float)x[sv.z - 1], (float)x[sv.w - 1]};
+            __builtin_nontemporal_store(v, (f32x4*)(y + j0));
+        } else {
+            for (uint64_t j = j0; j < ne; j++) y[j] = x[psrc[j] - 1];
+        }
+    } else {
+        // Float64: entries base + t + 256u, u < 4 (four 4-byte source loads,
+        // four gathers, four 8-byte stores; measured faster for Float64 than
+        // the contiguous 2 x 2 layout: 470 against 536 us at 10^8 entries)
+        const uint64_t j0 = base + threadIdx.x;
+        uint32_t sj[GP_U];
+        T v[GP_U];
 #pragma unroll
-    for (int u = 0; u < GP_U; u++) {
-        const uint64_t j = j0 + (uint64_t)u * 256;
-        sj[u] = j < ne ? __builtin_nontemporal_load(psrc + j) : 0u;
-    }
+        for (int u = 0; u < GP_U; u++) {
+            const uint64_t j = j0 + (uint64_t)u * 256;
+            sj[u] = j < ne ? __builtin_nontemporal_load(psrc + j) : 0u;
+        }
 #pragma unroll
-    for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
+        for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
 #pragma unroll
-    for (int u = 0; u < GP_U; u++) {
-        const uint64_t j = j0 + (uint64_t)u * 256;
-        if (j < ne) __builtin_nontemporal_store(v[u], y + j);
+        for (int u = 0; u < GP_U; u++) {
+            const uint64_t j = j0 + (uint64_t)u * 256;
+            if (j < ne) __builtin_nontemporal_store(v[u], y + j);
+        }
     }
 }
+
 
 __global__ void k_heavy_rows(const uint32_t* __restrict__ off, uint64_t rows, uint64_t rowbase,
                              uint64_t nrows, uint32_t* __restrict__ list, uint32_t* __restrict__ cnt) {

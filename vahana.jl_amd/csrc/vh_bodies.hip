@@ -986,16 +986,17 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
 // then lane l adds slots l, l + 64, ... in CSR order: the canonical
 // lane-strided partials of opinion_heavy, then the same xor butterfly.
 template <class T>
-__device__ __forceinline__ void opinion_heavy_tiles(const uint32_t* __restrict__ off, const T* __restrict__ x,
-                                                    const T* __restrict__ y, const float* __restrict__ w,
-                                                    T* __restrict__ xo, uint64_t r, uint32_t hb, T eps,
-                                                    const uint32_t* __restrict__ hty,
+__device__ __forceinline__ void opinion_heavy_tiles(const T* __restrict__ x, const T* __restrict__ y,
+                                                    const float* __restrict__ w, T* __restrict__ xo,
+                                                    const uint4 hi, T eps, const uint32_t* __restrict__ hty,
                                                     const uint16_t* __restrict__ hslot, T* buf) {
     constexpr uint32_t CH = TileCfg<T>::CHUNK;
     constexpr int HU = CH / 64;
     const uint32_t lane = threadIdx.x & 63;
+    // hi: row, CSR start, degree, first chunk-list entry (built with the lists)
+    const uint64_t r = hi.x;
+    const uint32_t b = hi.y, d = hi.z, hb = hi.w;
     const T xi = x[r];
-    const uint32_t b = off[r], d = off[r + 1] - b;
     T num = (T)0, den = (T)0;
     for (uint32_t c0 = 0; c0 < d; c0 += CH) {
         const uint32_t cnt = d - c0 < CH ? d - c0 : CH;
@@ -1065,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
                                                  const uint32_t* __restrict__ lbase,
                                                  const uint32_t* __restrict__ ty,
                                                  const uint16_t* __restrict__ tslot, uint32_t nblkp,
-                                                 const uint32_t* __restrict__ hbase,
+                                                 const uint4* __restrict__ hinfo,
                                                  const uint32_t* __restrict__ hty,
                                                  const uint16_t* __restrict__ hslot,
                                                  const uint32_t* __restrict__ prun,
@@ -1092,9 +1093,9 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
     } else {
         uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
         if constexpr (TILES) {
-            if (hbase) {   // one quarter of the block's LDS per wave
+            if (hinfo) {   // one quarter of the block's LDS per wave
                 if (wv < nheavy)
-                    opinion_heavy_tiles<T>(off, x, y, w, xo, heavy[wv], hbase[wv], eps, hty, hslot,
+                    opinion_heavy_tiles<T>(x, y, w, xo, hinfo[wv], eps, hty, hslot,
                                            sm.val + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
                 return;
             }
@@ -1242,6 +1243,15 @@ __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__
         kidx[hb + t] = b + t;
         tsl[hb + t] = (uint16_t)(t % chunk);
     }
+}
+
+// row, CSR start, degree and first chunk-list entry of each heavy row
+__global__ void k_heavy_info(const uint32_t* __restrict__ off, const uint32_t* __restrict__ heavy, uint64_t nheavy,
+                             const uint32_t* __restrict__ hbase, uint4* __restrict__ hinfo) {
+    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nheavy) return;
+    const uint32_t r = heavy[h], b = off[r];
+    hinfo[h] = make_uint4(r, b, off[r + 1] - b, hbase[h]);
 }
 
 // grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U)); logical
@@ -1841,6 +1851,9 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts, uint
         return set_error(VH_ERR_DEVICE, "chunk sort");
     hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl,
                        E.pinv.as<uint32_t>(), (uint64_t)nhe, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
+    RC(E.hinfo.ensure(nheavy * 16 + 16, st));
+    hipLaunchKernelGGL(k_heavy_info, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off, E.heavy.as<uint32_t>(),
+                       nheavy, E.hbase.as<uint32_t>(), E.hinfo.as<uint4>());
     HIPCHECK(hipGetLastError());
     E.plan_htiles = true;
     return VH_OK;
@@ -1940,8 +1953,8 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     const uint16_t* tsl_arg = mode == 3 ? E.tpk.as<uint16_t>() : E.tslot.as<uint16_t>();
     const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
     // heavy rows from chunk lists unless VH_OPINION_HEAVY_TILES=0
-    const uint32_t* hb = mode >= 2 && E.plan_htiles && env_ll("VH_OPINION_HEAVY_TILES", 1) != 0
-                             ? E.hbase.as<uint32_t>() : nullptr;
+    const uint4* hb = mode >= 2 && E.plan_htiles && env_ll("VH_OPINION_HEAVY_TILES", 1) != 0
+                          ? E.hinfo.as<uint4>() : nullptr;
     if (f32) {
         auto k = mode == 3   ? k_opinion<float, 3>
                  : mode == 2 ? k_opinion<float, 2>

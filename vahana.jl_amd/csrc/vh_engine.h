@@ -74,6 +74,7 @@ struct EdgeStore {
     DevBuf psrc, pinv, pval;       // uint32 sources in plan order, plan position per CSR slot, states
     DevBuf lbase, ty, tslot;       // light-edge base per block, tile lists (plan position, slot)
     DevBuf hbase, hty, hslot;      // the same for the heavy rows (base per heavy row, chunk lists)
+    DevBuf hinfo;                  // per heavy row: row, CSR start, degree, hbase (one 16-byte load)
     DevBuf prun, pfirst, tpk;      // packed tile lists: run tables per block and tile, entries
     DevBuf twt;                    // edge weight of each packed tile-list entry, in list order
     bool plan_tiles = false, plan_htiles = false, plan_packed = false;
@@ -89,7 +90,7 @@ struct EdgeStore {
     template <class F>
     void each_buf(F f) {
         DevBuf* all[] = {&off, &src, &st, &heavy, &psrc, &pinv, &pval, &lbase, &ty, &tslot, &hbase, &hty,
-                         &hslot, &prun, &pfirst, &tpk, &twt, &out_off, &out_dst, &expo, &expo_cnt};
+                         &hslot, &hinfo, &prun, &pfirst, &tpk, &twt, &out_off, &out_dst, &expo, &expo_cnt};
         for (DevBuf* b : all) f(*b);
     }
     bool writeable = false;

@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--workload", default="opinion",
                     choices=["opinion", "opinion32", "sir", "contact", "raster", "ensemble",
-                             "serve", "gather_floor"])
+                             "serve"])
     ap.add_argument("--replicas", type=int, default=1024)
     ap.add_argument("--requests", type=int, default=60)
     ap.add_argument("--rate", type=float, default=0.0,
@@ -171,8 +171,6 @@ def main():
         return bench_ensemble(vh, args)
     if args.workload == "serve":
         return bench_serve(vh, args)
-    if args.workload == "gather_floor":
-        return bench_gather_floor(vh)
     dev = torch.device("cuda:0")
     n = args.n
     f32 = args.workload == "opinion32"
@@ -266,8 +264,9 @@ def main():
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # runs of this exact workload (scaled by the copy-kernel calibration)
     # (the Float64 state of 10^7 agents is gathered in two passes unless
-    # VH_OPINION_PLAN_BYTES turns that off)
-    two_pass = 0 <= int(os.environ.get("VH_OPINION_PLAN_BYTES", 16 << 20)) <= n * 8
+    # the option opinion_plan_bytes turns that off)
+    pb = vh.get_options()["opinion_plan_bytes"]
+    two_pass = 0 <= pb <= n * 8
     pmc = {("opinion", 10_000_000, False, False): "k_opinion_f64_ba10M_traffic.json",
            ("opinion", 10_000_000, True, False): "k_opinion_f64_ba10M_bfs_traffic.json",
            ("opinion", 10_000_000, True, True): "k_opinion_f64_ba10M_bfs_plan_traffic.json"}.get(
@@ -327,33 +326,6 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def bench_gather_floor(vh):
-    """Random 8-byte gathers versus footprint, and a streaming read: the
-    memory-system bounds of the neighbour-state gather."""
-    import ctypes as C
-    L = vh._abi.lib()
-    n = 100_000_000
-    rows = []
-    for mb in (2, 4, 16, 40, 80, 160, 400, 1024):
-        ms = C.c_float()
-        L.probe_memory(0, mb << 20, n, 10, C.byref(ms))
-        rows.append({"footprint_MiB": mb, "ms": ms.value, "gathers_per_s": n / (ms.value * 1e-3)})
-        log(f"gather footprint {mb:5d} MiB: {ms.value:.3f} ms  {n / ms.value / 1e6:.1f} G/s")
-    hc = {}
-    for mode, name in ((2, "hot22_cold_normal"), (3, "hot22_cold_nontemporal"),
-                       (4, "uncached_memory"), (5, "finegrained_memory")):
-        ms = C.c_float()
-        L.probe_memory(mode, 80 << 20, n, 10, C.byref(ms))
-        hc[name] = ms.value
-        log(f"{name} (80 MiB): {ms.value:.3f} ms  {n / ms.value / 1e6:.1f} G/s")
-    ms = C.c_float()
-    L.probe_memory(1, 800 << 20, 0, 10, C.byref(ms))
-    stream = (800 << 20) / (ms.value * 1e-3) / 1e9
-    log(f"stream read 800 MiB: {ms.value:.3f} ms  {stream:.0f} GB/s")
-    print(json.dumps({"metric": "random 8-byte gathers/s by footprint", "gathers": n,
-                      "rows": rows, "hot_cold_ms": hc, "stream_read_GBps": stream}), flush=True)
-
-
 def bench_ensemble(vh, args):
     """1024 SIR replicas (sizes log-uniform in [1e3, 1e6], mean in-degree
     U[4, 20], p_inf U[0.01, 0.1], distinct seeds), 200 steps in one
@@ -381,17 +353,17 @@ def bench_ensemble(vh, args):
     # one at a time: a sample of the replicas, each alone (same kernels, same
     # graph-captured step sequence), scaled to all replicas by agent count.
     # Each replica runs twice with and twice without the exposure pass
-    # (VH_SIR_EXPOSE) and its fastest run counts: alone, a small replica pays
+    # (option sir_expose) and its fastest run counts: alone, a small replica pays
     # the pass's two extra launches per step without gaining from it.
     sample = list(range(0, R, max(1, R // 64)))
     solo_time, solo_agents = 0.0, 0
     solo_by_mode = {"1": 0.0, "0": 0.0}
-    saved_env = os.environ.get("VH_SIR_EXPOSE")
+    saved_opts = vh.get_options()
     for i in sample:
         n, md, p, seed = specs[i]
         best = None
         for mode in ("1", "0"):
-            os.environ["VH_SIR_EXPOSE"] = mode
+            vh.set_options(sir_expose=int(mode))
             mode_best = None
             for _rep in range(2):   # the same steps from the same start each time
                 S = vh.ensemble.Ensemble()
@@ -408,10 +380,7 @@ def bench_ensemble(vh, args):
             best = mode_best if best is None else min(best, mode_best)
         solo_time += best
         solo_agents += n
-    if saved_env is None:
-        os.environ.pop("VH_SIR_EXPOSE", None)
-    else:
-        os.environ["VH_SIR_EXPOSE"] = saved_env
+    vh.set_options(**saved_opts)
     solo_rate = solo_agents * steps / solo_time
     solo_total = len(specs) * (solo_time / len(sample))
     out = {"metric": "ensemble agent-updates/s (1024 SIR replicas, one launch per step)",

@@ -308,6 +308,26 @@ int vh_agentstate(vh_sim_t s, uint64_t id, uint8_t type_id, void* out);
 int vh_device_bytes(vh_sim_t s, uint64_t* out);
 /* last error message on this thread */
 const char* vh_last_error(void);
+/* ------------------------------------------------------------- options
+ * Process-wide tuning options of the transition kernels (no reference
+ * counterpart: the reference has no device kernels). Every apply! and
+ * ensemble step reads the values set last; set them between calls, not while
+ * another thread applies. Results do not depend on them (every setting gives
+ * bit-identical states), only the kernels chosen do. */
+typedef struct vh_options {
+    int64_t opinion_plan_bytes;   /* opinion states gathered in two passes through a plan from a
+                                     state vector of at least this many bytes (< 0: never);
+                                     default 16 MiB */
+    int64_t opinion_part_bytes;   /* source partition of the plan in bytes (0: 2 MiB, widened up
+                                     to 1.5x to make at most 32 partitions) */
+    int32_t sir_expose;           /* SIR exposure pass: 1 on (default), 0 off */
+    uint32_t pad;
+    int64_t sir_expose_min;       /* agents from which a SIR simulation or ensemble uses the
+                                     exposure pass; default 65536 */
+} vh_options;
+int vh_get_options(vh_options* out);
+int vh_set_options(const vh_options* in);
+
 /* enable_asserts src/Vahana.jl:42-48: with asserts on (the default) every
  * apply! waits for its kernels and reports device-side assertion flags
  * (permissions, removed agents, capacity) as VH_ERR_ASSERT / VH_ERR_CAPACITY;
@@ -399,15 +419,6 @@ int vh_fill_agent_u01(uint64_t seed, uint32_t type, uint64_t b, uint64_t c, uint
  * lowest-numbered vertex; new_nr[v - 1] = 1-based position of v (the order
  * vh_finish_init_relabel applies) */
 int vh_bfs_order(uint64_t n, uint64_t m, const uint32_t* a, const uint32_t* b, uint64_t* new_nr);
-
-/* ------------------------------------------------------ diagnostics
- * Memory-system probe (no reference counterpart): mode 0 times n random
- * 8-byte gathers over `footprint` bytes, mode 1 a streaming read of
- * `footprint` bytes, modes 2/3 gathers with a 2 MiB hot prefix (normal or
- * non-temporal cold loads), modes 4/5 mode 0 over uncached / fine-grained
- * device memory (allocated once, kept until the process ends); *ms is the
- * mean launch time over `reps` launches. */
-int vh_probe_memory(int mode, uint64_t footprint, uint64_t n, uint32_t reps, float* ms);
 
 /* ------------------------------------------------------ counter-hash RNG
  * SplitMix64 finaliser chained over (seed, a, b, c); the same function is

@@ -34,3 +34,12 @@ def gpu_ready():
         pytest.skip("no GPU visible")
     torch.cuda.init()
     return True
+
+
+@pytest.fixture
+def opts(vh):
+    """Kernel options for one test (vh_set_options), restored afterwards:
+    ``opts(sir_expose=0)``."""
+    saved = vh.get_options()
+    yield lambda **kw: vh.set_options(**kw)
+    vh.set_options(**saved)

@@ -1,0 +1,74 @@
+"""Init-phase edge checks of the C ABI on a simulation (GPU: a simulation
+owns a HIP stream from vh_sim_create on). The checker runs the same calls."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _single_edge_sim(V, stateless=False):
+    A = V.struct("A", [("v", "<i8")])
+    E = V.struct("E") if stateless else V.struct("E", [("w", "<i8")])
+    hints = ("SingleEdge", "Stateless") if stateless else ("SingleEdge",)
+    model = (V.ModelTypes() | V.register_agenttype_(A) | V.register_edgetype_(E, *hints)
+             | V.create_model("single_edge_bulk"))
+    sim = V.create_simulation(model)
+    ids = sim._add_agents_array(A, np.zeros(4, dtype=A.dtype))
+    return sim, A, E, ids
+
+
+def _st(E, vals):
+    s = np.zeros(len(vals), dtype=E.dtype)
+    s["w"] = vals
+    return s
+
+
+def _bulk_identical(V):
+    sim, A, E, ids = _single_edge_sim(V)
+    sim._add_edges_array(E, [ids[0], ids[0]], [ids[1], ids[1]], _st(E, [5, 5]))
+    sim._add_edges_array(E, [ids[0]], [ids[1]], _st(E, [5]))
+    sim._add_edges_array(E, [ids[2], ids[3], ids[2]], [ids[3], ids[2], ids[3]], _st(E, [1, 2, 1]))
+    V.finish_init_(sim)
+    return [repr(V.edges(sim, i, E)) for i in ids]
+
+
+def test_single_edge_bulk_identical_accepted(vh, ora, gpu_ready):
+    """_can_add (src/EdgeMethods.jl:257-294): equal edges to one :SingleEdge
+    target are accepted, inside one bulk call and across calls."""
+    print("[gpu] single-edge bulk, identical edges", flush=True)
+    got, want = _bulk_identical(vh), _bulk_identical(ora)
+    assert got == want
+    assert got[0] == "None" and "0x100000000000001" in got[1]
+
+
+@pytest.mark.parametrize("case", ["source", "state", "across_calls", "later_in_call"])
+def test_single_edge_bulk_different_asserts(vh, ora, gpu_ready, case):
+    """A second, different edge to the same :SingleEdge target asserts,
+    whether its first edge came in the same bulk call or an earlier one."""
+    print(f"[gpu] single-edge bulk, different edges ({case})", flush=True)
+    for V in (vh, ora):
+        sim, A, E, ids = _single_edge_sim(V)
+        with pytest.raises(AssertionError):
+            if case == "source":
+                sim._add_edges_array(E, [ids[0], ids[2]], [ids[1], ids[1]], _st(E, [5, 5]))
+            elif case == "state":
+                sim._add_edges_array(E, [ids[0], ids[0]], [ids[1], ids[1]], _st(E, [5, 6]))
+            elif case == "across_calls":
+                sim._add_edges_array(E, [ids[0], ids[3]], [ids[1], ids[2]], _st(E, [5, 5]))
+                sim._add_edges_array(E, [ids[0], ids[0]], [ids[2], ids[1]], _st(E, [7, 7]))
+            else:   # first edge of the target in this call, after other entries
+                sim._add_edges_array(E, [ids[0]], [ids[3]], _st(E, [1]))
+                sim._add_edges_array(E, [ids[0], ids[1], ids[2], ids[1]],
+                                     [ids[3], ids[2], ids[0], ids[2]], _st(E, [1, 4, 4, 8]))
+
+
+def test_stateful_edges_need_states(vh, gpu_ready):
+    print("[gpu] stateful edge type without states", flush=True)
+    sim, A, E, ids = _single_edge_sim(vh)
+    with pytest.raises(vh.VahanaError):
+        sim._add_edges_array(E, [ids[0]], [ids[1]], None)
+    # a :Stateless type needs none (equal sources: accepted)
+    sim2, A2, E2, ids2 = _single_edge_sim(vh, stateless=True)
+    sim2._add_edges_array(E2, [ids2[0], ids2[0]], [ids2[1], ids2[1]], None)
+    vh.finish_init_(sim2)
+    assert vh.neighborids(sim2, ids2[1], E2) == ids2[0]

@@ -25,12 +25,11 @@ def _solo(V, n, md, p, seed, gseed, steps, n_init):
 
 
 @pytest.mark.parametrize("expose", ["1", "0"])
-def test_ensemble_members_match_solo(vh, ora, gpu_ready, expose, monkeypatch):
-    """With (default) and without the exposure pass (VH_SIR_EXPOSE), in the
+def test_ensemble_members_match_solo(vh, ora, gpu_ready, expose, opts):
+    """With (default) and without the exposure pass (option sir_expose), in the
     ensemble and in the solo runs; 2% initially infected, so the pass is on
     at first and turns off where the epidemic grows."""
-    monkeypatch.setenv("VH_SIR_EXPOSE", expose)
-    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", "0")   # the solo runs of small members too
+    opts(sir_expose=int(expose), sir_expose_min=0)   # the solo runs of small members too
     print(f"[gpu] ensemble vs solo expose={expose}", flush=True)
     E = vh.ensemble.Ensemble()
     ids = []
@@ -55,14 +54,13 @@ def test_ensemble_members_match_solo(vh, ora, gpu_ready, expose, monkeypatch):
     assert info["members"] == len(SPECS) and info["agents"] == sum(s[0] for s in SPECS)
 
 
-def test_ensemble_incremental_exposure(vh, ora, gpu_ready, monkeypatch):
+def test_ensemble_incremental_exposure(vh, ora, gpu_ready, opts):
     """Slow epidemics (0.1% initially infected, few infections per step) keep
     the exposure pass on for every launch, so the bitmaps are updated from the
     newly infected agents between full launches (every 8th); 30 steps in
     graph and plain launches of several lengths, each member against its solo
     run without the pass, one against the checker."""
-    monkeypatch.setenv("VH_SIR_EXPOSE", "1")
-    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", "0")
+    opts(sir_expose=1, sir_expose_min=0)
     print("[gpu] ensemble incremental exposure", flush=True)
     specs = [(20000, 8.0, 0.02, 31, 301), (50000, 6.0, 0.03, 32, 302), (5000, 10.0, 0.015, 33, 303)]
     E = vh.ensemble.Ensemble()
@@ -72,7 +70,7 @@ def test_ensemble_incremental_exposure(vh, ora, gpu_ready, monkeypatch):
     E.step(3, use_graph=False)
     E.step(16, use_graph=True)
     steps = 30
-    monkeypatch.setenv("VH_SIR_EXPOSE", "0")
+    opts(sir_expose=0)
     for mid, (n, md, p, seed, gseed) in zip(ids, specs):
         got, next_step = E.read(mid)
         assert next_step == steps + 1
@@ -89,11 +87,11 @@ def test_ensemble_incremental_exposure(vh, ora, gpu_ready, monkeypatch):
 
 
 @pytest.mark.parametrize("expose_min", ["65536", "0"])
-def test_ensemble_join_and_leave(vh, gpu_ready, expose_min, monkeypatch):
+def test_ensemble_join_and_leave(vh, gpu_ready, expose_min, opts):
     """Members added and removed between steps keep their own step counters;
-    with VH_SIR_EXPOSE_MIN=0 the exposure pass runs across the rebuilt block
+    with sir_expose_min=0 the exposure pass runs across the rebuilt block
     tables (every member's first launch after a change has the pass off)."""
-    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", expose_min)
+    opts(sir_expose_min=int(expose_min))
     print(f"[gpu] ensemble join and leave expose_min={expose_min}", flush=True)
     E = vh.ensemble.Ensemble()
     a = E.add_sir(5000, 10.0, 0.08, 21, 201, n_init=50)

@@ -44,10 +44,9 @@ def _bytes(st):
 
 
 @pytest.mark.parametrize("expose", ["1", "0"])
-def test_sir_parity(vh, ora, gpu_ready, expose, monkeypatch):
-    """SIR, with (default) and without the exposure pass (VH_SIR_EXPOSE)."""
-    monkeypatch.setenv("VH_SIR_EXPOSE", expose)
-    monkeypatch.setenv("VH_SIR_EXPOSE_MIN", "0")   # (3000 agents)
+def test_sir_parity(vh, ora, gpu_ready, expose, opts):
+    """SIR, with (default) and without the exposure pass (option sir_expose)."""
+    opts(sir_expose=int(expose), sir_expose_min=0)   # (3000 agents)
     print(f"[gpu] SIR parity expose={expose}", flush=True)
     M = vh.models
     n = 3000
@@ -63,29 +62,20 @@ def test_sir_parity(vh, ora, gpu_ready, expose, monkeypatch):
     assert (a["health"] == vh.R).sum() > 0
 
 
-# neighbour-state gather (VH_OPINION_PLAN_BYTES, VH_OPINION_PART_BYTES,
-# VH_OPINION_TILES, VH_OPINION_HEAVY_TILES, VH_OPINION_PACKED): direct;
-# two-pass plan with 16 (f64) or 8 (f32) source partitions and pass 2 from
-# packed tile lists and chunk lists, from packed tile lists with the heavy
-# agents by y[pinv[k]], from unpacked tile lists, or all by y[pinv[k]];
-# two-pass plan with one source per partition (4000 partitions, a two-digit
-# plan sort, unpacked tile lists)
-GATHER_MODES = {"direct": ("-1", None, "1", "1", "1"), "plan": ("0", "2048", "1", "1", "1"),
-                "plan_heavy_pinv": ("0", "2048", "1", "0", "1"),
-                "plan_unpacked": ("0", "2048", "1", "1", "0"), "plan_pinv": ("0", "2048", "0", "1", "1"),
-                "plan_1src": ("0", "1", "1", "1", "1")}
+# neighbour-state gather (options opinion_plan_bytes, opinion_part_bytes):
+# one pass; two passes through a plan with 2 KiB source partitions (16 for
+# Float64, 8 for Float32 states), with 4 KiB partitions, with the default
+# 2 MiB (one partition at this size), and with one source per partition
+# (4000 partitions: a two-digit plan sort, one-entry runs)
+GATHER_MODES = {"direct": (-1, 0), "plan": (0, 2048), "plan_4k": (0, 4096), "plan_auto": (0, 0),
+                "plan_1src": (0, 1)}
 
 
 @pytest.mark.parametrize("mode", list(GATHER_MODES))
 @pytest.mark.parametrize("f32", [False, True])
-def test_opinion_parity(vh, ora, gpu_ready, f32, mode, monkeypatch):
-    plan_bytes, part_bytes, tiles, heavy_tiles, packed = GATHER_MODES[mode]
-    monkeypatch.setenv("VH_OPINION_PLAN_BYTES", plan_bytes)
-    monkeypatch.setenv("VH_OPINION_TILES", tiles)
-    monkeypatch.setenv("VH_OPINION_HEAVY_TILES", heavy_tiles)
-    monkeypatch.setenv("VH_OPINION_PACKED", packed)
-    if part_bytes:
-        monkeypatch.setenv("VH_OPINION_PART_BYTES", part_bytes)
+def test_opinion_parity(vh, ora, gpu_ready, f32, mode, opts):
+    plan_bytes, part_bytes = GATHER_MODES[mode]
+    opts(opinion_plan_bytes=plan_bytes, opinion_part_bytes=part_bytes)
     print(f"[gpu] opinion parity f32={f32} gather={mode}", flush=True)
     M = vh.models
     n = 4000
@@ -104,13 +94,12 @@ def test_opinion_parity(vh, ora, gpu_ready, f32, mode, monkeypatch):
 
 
 @pytest.mark.parametrize("f32", [False, True])
-def test_opinion_plan_after_edge_change(vh, ora, gpu_ready, f32, monkeypatch):
+def test_opinion_plan_after_edge_change(vh, ora, gpu_ready, f32, opts):
     """The two-pass plan (its lists and the edge weights copied into list
     order) follows a change of the edge store: after three steps every agent
     adds a self-loop of weight 0.75 (add_existing), and the next steps must
     match the checker's."""
-    monkeypatch.setenv("VH_OPINION_PLAN_BYTES", "0")
-    monkeypatch.setenv("VH_OPINION_PART_BYTES", "2048")
+    opts(opinion_plan_bytes=0, opinion_part_bytes=2048)
     print(f"[gpu] opinion plan after edge change f32={f32}", flush=True)
     M = vh.models
     n = 4000

@@ -763,6 +763,47 @@ def has_edge(sim, id_, T):
     return len(fr) > 0
 
 
+def get_options():
+    """The kernel options (vh_get_options) as a dict."""
+    o = _abi.Options()
+    L = _abi.lib()
+    _check(L.get_options(C.byref(o)), L)
+    return {k: getattr(o, k) for k, _ in _abi.Options._fields_ if k != "pad"}
+
+
+def set_options(**kw):
+    """Set kernel options (vh_set_options): opinion_plan_bytes,
+    opinion_part_bytes, sir_expose, sir_expose_min. Returns the previous
+    values. Results are bit-identical under every setting."""
+    prev = get_options()
+    cur = dict(prev)
+    for k, v in kw.items():
+        if k not in cur:
+            raise KeyError(f"unknown option {k}")
+        cur[k] = int(v)
+    o = _abi.Options(**cur)
+    L = _abi.lib()
+    _check(L.set_options(C.byref(o)), L)
+    return prev
+
+
+class options:
+    """Context manager: ``with options(sir_expose=0): ...`` restores the
+    previous options on exit."""
+
+    def __init__(self, **kw):
+        self.kw = kw
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = set_options(**self.kw)
+        return self
+
+    def __exit__(self, *exc):
+        set_options(**self.prev)
+        return False
+
+
 def enable_asserts(enable=True):
     """enable_asserts src/Vahana.jl:42-48."""
     _abi.lib().enable_asserts(1 if enable else 0)

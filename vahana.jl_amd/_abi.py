@@ -66,6 +66,11 @@ class ApplyDesc(C.Structure):
                 ("params", C.c_void_p), ("params_bytes", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class Options(C.Structure):
+    _fields_ = [("opinion_plan_bytes", C.c_int64), ("opinion_part_bytes", C.c_int64),
+                ("sir_expose", C.c_int32), ("pad", C.c_uint32), ("sir_expose_min", C.c_int64)]
+
+
 class MapDesc(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("offset", C.c_uint32), ("field_dtype", C.c_uint32),
                 ("pad", C.c_uint32), ("value", C.c_int64)]
@@ -145,6 +150,8 @@ DEVICE_ONLY = [
     ("stream_destroy", C.c_int, [_vp]),
     ("device_bytes", C.c_int, [_vp, _u64p]),
     ("enable_asserts", C.c_int, [C.c_int]),
+    ("get_options", C.c_int, [C.POINTER(Options)]),
+    ("set_options", C.c_int, [C.POINTER(Options)]),
     ("debug_fail_allocs", C.c_int, [C.c_uint32]),
     ("debug_set_device_error", C.c_int, [_vp, C.c_uint32]),
     ("graph_er", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
@@ -155,8 +162,6 @@ DEVICE_ONLY = [
     ("bfs_order", C.c_int, [C.c_uint64, C.c_uint64, _u32p, _u32p, _u64p]),
     ("fill_agent_u01", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64,
                                  C.POINTER(C.c_double)]),
-    ("probe_memory", C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32,
-                               C.POINTER(C.c_float)]),
     ("graph_gnp", C.c_int, [C.c_uint64, C.c_double, C.c_uint64, _u32p, _u32p, C.c_uint64, _u64p]),
     ("ensemble_create", C.c_int, [C.POINTER(_vp)]),
     ("ensemble_destroy", C.c_int, [_vp]),

@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "vh_engine.h"
 
 using namespace vh;
@@ -551,83 +553,78 @@ __global__ __launch_bounds__(256) void k_sir(const uint32_t* __restrict__ off,
 }
 
 // ----------------------------------------------------------------- opinion
-// Canonical order: degree <= 64 sequential; degree > 64 lane-strided
-// partials (lane l: k = l, l+64, ...) and an xor butterfly.  One launch per
-// step: blocks [0, nlight) run thread-per-agent over all agents (skipping the
-// heavy ones), the remaining blocks run one wave per heavy agent.
-// Light agents (degree <= 64) of a 256-agent block: their in-edges are
-// concatenated in agent order and processed in tiles of OP_TILE edge slots.
-// Phase 1 (all threads, coalesced src/w loads, parallel gathers) writes each
-// edge's two terms to LDS; phase 2 lets every thread add its own agent's terms
-// in edge order, so each agent's sums are the sequential canonical sums.
-constexpr int OP_TILE = 2048;
-
-// neighbour state load: sources up to `hot` (1-based) with normal loads, the
-// others non-temporal, so that the hot prefix can stay in L2 while the cold
-// gathers stream past it
+// Weighted bounded confidence: x_i <- sum_{j: |x_j - x_i| <= eps} w x_j /
+// sum w over the in-edges; x_i when none is accepted. Canonical order of both
+// sums: degree <= 64 sequential in CSR (insertion) order; degree > 64 lane l
+// of 64 adds k = l, l + 64, ... in order, then the xor butterfly
+// p[l] + p[l ^ o], o = 32..1. A rejected neighbour adds +0.0, which is
+// bit-identical to skipping it (the sums start at +0.0 and every term is
+// >= +0.0). Agents of degree <= 64 are light (thread per agent in a 256-agent
+// block), the others heavy (one wave each).
 template <class T>
-__device__ __forceinline__ T ld_state(const T* __restrict__ x, uint32_t s, uint32_t hot) {
-    return s <= hot ? x[s - 1] : __builtin_nontemporal_load(x + (s - 1));
+__device__ __forceinline__ void op_accum(T xj, float w, T xi, T eps, T& num, T& den) {
+    const bool acc = fabs(xj - xi) <= eps;
+    num = num + (acc ? (T)w * xj : (T)0);
+    den = den + (acc ? (T)w : (T)0);
 }
+
+// block exclusive scan of the light degrees: the thread's first slot in the
+// block's light edges and their total
+__device__ __forceinline__ uint32_t op_light_scan(uint32_t ld, uint32_t* s_wsum, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = ld;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y1 = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y1;
+    }
+    if (lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t v = s_wsum[q];
+        pre += q < wv ? v : 0u;
+        total += v;
+    }
+    return pre + incl - ld;
+}
+
+// -- one pass (state vectors below the plan threshold). The light edges of a
+// block are processed in tiles of OP_TILE slots: all threads load sources and
+// weights coalesced and gather OP_U neighbour states each, staging state and
+// weight by slot in LDS; then every thread adds its own agent's slots.
+constexpr int OP_TILE = 2048;
 constexpr int OP_U = 8;   // gathers in flight per thread
 
-// neighbour state of in-edge slot `s`: with PLAN, `s` is the edge's position
-// in the gathered plan vector y (k_plan_gather); otherwise the source's
-// 1-based number in x
-template <class T, bool PLAN>
-__device__ __forceinline__ T ld_nb(const T* __restrict__ x, const T* __restrict__ y, uint32_t s, uint32_t hot) {
-    if constexpr (PLAN)
-        return y[s];
-    else
-        return ld_state(x, s, hot);
-}
-
-template <class T, bool PLAN>
+template <class T>
 __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
-                                              const T* __restrict__ x, const T* __restrict__ y,
-                                              const float* __restrict__ w,
+                                              const T* __restrict__ x, const float* __restrict__ w,
                                               T* __restrict__ xo, uint64_t n, uint64_t nrows,
-                                              uint64_t rowbase, T eps, uint32_t blk, uint32_t hot) {
-    __shared__ T s_num[OP_TILE];
-    __shared__ float s_den[OP_TILE];
+                                              uint64_t rowbase, T eps, uint32_t blk) {
+    __shared__ T s_x[OP_TILE];
+    __shared__ float s_w[OP_TILE];
     __shared__ uint8_t s_own[OP_TILE];
     __shared__ uint32_t s_b[256], s_p[256], s_wsum[4];
-    __shared__ T s_xi[256];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x;
     const uint64_t i = (uint64_t)blk * 256 + tid;
-    const bool valid = i < n;
     uint32_t b = 0, d = 0;
     T xi = (T)0;
-    if (valid) {
+    if (i < n) {
         xi = __builtin_nontemporal_load(x + i);
         if (i < nrows) {
             b = __builtin_nontemporal_load(off + rowbase + i);
             d = __builtin_nontemporal_load(off + rowbase + i + 1) - b;
         }
     }
-    const bool light = valid && d <= 64u;
+    const bool light = i < n && d <= 64u;
     const uint32_t ld = light ? d : 0u;
-    // block exclusive scan of the light degrees
-    uint32_t incl = ld;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
-    if (lane == 63) s_wsum[wv] = incl;
-    __syncthreads();
-    uint32_t pre = 0, total = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        uint32_t v = s_wsum[q];
-        pre += q < wv ? v : 0u;
-        total += v;
-    }
-    const uint32_t P = pre + incl - ld;
+    uint32_t total;
+    const uint32_t P = op_light_scan(ld, s_wsum, total);
     s_b[tid] = b;
     s_p[tid] = P;
-    s_xi[tid] = xi;
     T num = (T)0, den = (T)0;
     for (uint32_t t0 = 0; t0 < total; t0 += OP_TILE) {
         const uint32_t lo = P > t0 ? P : t0;
@@ -635,17 +632,15 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
         for (uint32_t p = lo; p < hi; p++) s_own[p - t0] = (uint8_t)tid;
         __syncthreads();
         const uint32_t cnt = (total - t0) < (uint32_t)OP_TILE ? (total - t0) : (uint32_t)OP_TILE;
-        // batches of OP_U slots per thread: all index loads, then all
-        // gathers, then the terms (keeps OP_U gathers in flight per lane)
+        // batches of OP_U slots per thread: all index loads, then all gathers
         for (uint32_t q0 = 0; q0 < cnt; q0 += 256u * OP_U) {
-            uint32_t kk[OP_U], sj[OP_U], ag[OP_U];
+            uint32_t kk[OP_U], sj[OP_U];
             float wk[OP_U];
             T xj[OP_U];
 #pragma unroll
             for (int u = 0; u < OP_U; u++) {
                 const uint32_t q = q0 + (uint32_t)u * 256u + tid;
                 const uint32_t a = q < cnt ? s_own[q] : 0u;
-                ag[u] = a;
                 kk[u] = q < cnt ? s_b[a] + (t0 + q - s_p[a]) : 0xFFFFFFFFu;
             }
 #pragma unroll
@@ -654,66 +649,45 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
                 wk[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(w + kk[u]) : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < OP_U; u++)
-                xj[u] = kk[u] != 0xFFFFFFFFu ? ld_nb<T, PLAN>(x, y, sj[u], hot) : (T)0;
+            for (int u = 0; u < OP_U; u++) xj[u] = kk[u] != 0xFFFFFFFFu ? x[sj[u] - 1] : (T)0;
 #pragma unroll
             for (int u = 0; u < OP_U; u++) {
                 const uint32_t q = q0 + (uint32_t)u * 256u + tid;
                 if (q < cnt) {
-                    const bool acc = fabs(xj[u] - s_xi[ag[u]]) <= eps;
-                    s_num[q] = acc ? (T)wk[u] * xj[u] : (T)0;
-                    s_den[q] = acc ? wk[u] : 0.0f;
+                    s_x[q] = xj[u];
+                    s_w[q] = wk[u];
                 }
             }
         }
         __syncthreads();
-        for (uint32_t p = lo; p < hi; p++) {
-            num = num + s_num[p - t0];
-            den = den + (T)s_den[p - t0];
-        }
+        for (uint32_t p = lo; p < hi; p++) op_accum(s_x[p - t0], s_w[p - t0], xi, eps, num, den);
         __syncthreads();
     }
     if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
 }
 
-template <class T, bool PLAN>
+template <class T>
 __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
                                               const uint32_t* __restrict__ src,
-                                              const T* __restrict__ x, const T* __restrict__ y,
-                                              const float* __restrict__ w,
-                                              T* __restrict__ xo, uint64_t r, uint64_t rowbase, T eps,
-                                              uint32_t hot) {
+                                              const T* __restrict__ x, const float* __restrict__ w,
+                                              T* __restrict__ xo, uint64_t r, uint64_t rowbase, T eps) {
     const int lane = threadIdx.x & 63;
-    uint64_t i = r - rowbase;
+    const uint64_t i = r - rowbase;
     const T xi = x[i];
-    uint32_t b = off[r], e = off[r + 1];
+    const uint32_t b = off[r], e = off[r + 1];
     T num = (T)0, den = (T)0;
     // lane l adds k = l, l+64, ... in order; four strides in flight per lane
     uint32_t k = b + lane;
     for (; k + 192 < e; k += 256) {
-        uint32_t s0 = src[k], s1 = src[k + 64], s2 = src[k + 128], s3 = src[k + 192];
-        float w0 = w[k], w1 = w[k + 64], w2 = w[k + 128], w3 = w[k + 192];
-        T x0 = ld_nb<T, PLAN>(x, y, s0, hot), x1 = ld_nb<T, PLAN>(x, y, s1, hot);
-        T x2 = ld_nb<T, PLAN>(x, y, s2, hot), x3 = ld_nb<T, PLAN>(x, y, s3, hot);
-        bool a0 = fabs(x0 - xi) <= eps, a1 = fabs(x1 - xi) <= eps;
-        bool a2 = fabs(x2 - xi) <= eps, a3 = fabs(x3 - xi) <= eps;
-        num = num + (a0 ? (T)w0 * x0 : (T)0);
-        den = den + (a0 ? (T)w0 : (T)0);
-        num = num + (a1 ? (T)w1 * x1 : (T)0);
-        den = den + (a1 ? (T)w1 : (T)0);
-        num = num + (a2 ? (T)w2 * x2 : (T)0);
-        den = den + (a2 ? (T)w2 : (T)0);
-        num = num + (a3 ? (T)w3 * x3 : (T)0);
-        den = den + (a3 ? (T)w3 : (T)0);
+        const uint32_t s0 = src[k], s1 = src[k + 64], s2 = src[k + 128], s3 = src[k + 192];
+        const float w0 = w[k], w1 = w[k + 64], w2 = w[k + 128], w3 = w[k + 192];
+        const T x0 = x[s0 - 1], x1 = x[s1 - 1], x2 = x[s2 - 1], x3 = x[s3 - 1];
+        op_accum(x0, w0, xi, eps, num, den);
+        op_accum(x1, w1, xi, eps, num, den);
+        op_accum(x2, w2, xi, eps, num, den);
+        op_accum(x3, w3, xi, eps, num, den);
     }
-    for (; k < e; k += 64) {
-        T xj = ld_nb<T, PLAN>(x, y, src[k], hot);
-        T wk = (T)w[k];
-        if (fabs(xj - xi) <= eps) {
-            num = num + wk * xj;
-            den = den + wk;
-        }
-    }
+    for (; k < e; k += 64) op_accum(x[src[k] - 1], w[k], xi, eps, num, den);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         num = num + __shfl_xor(num, o, 64);
@@ -722,31 +696,42 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
     if (lane == 0) xo[i] = den > (T)0 ? num / den : xi;
 }
 
+// blocks [0, nh) run one wave per heavy agent (the longest work starts
+// first), the others are light blocks
+template <class T>
+__global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ src,
+                                                 const T* __restrict__ x, const float* __restrict__ w,
+                                                 T* __restrict__ xo, uint64_t n, uint64_t nrows,
+                                                 uint64_t rowbase, T eps, const uint32_t* __restrict__ heavy,
+                                                 uint64_t nheavy, uint32_t nh) {
+    if (blockIdx.x >= nh) {
+        opinion_block<T>(off, src, x, w, xo, n, nrows, rowbase, eps, blockIdx.x - nh);
+    } else {
+        const uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        if (wv < nheavy) opinion_heavy<T>(off, src, x, w, xo, heavy[wv], rowbase, eps);
+    }
+}
+
 // ----------------------------------------------------------------- gather plan
 // For a large state vector the neighbour states of a static edge store are
-// gathered in two passes through a plan: the CSR slots sorted (stably) by
-// source partition and, within one, light rows' slots before heavy rows'
-// (degree > 64), a partition being `psize` consecutive sources sized to
-// sit in one XCD's L2. Pass 1 (k_plan_gather) walks the plan in order,
-// y[j] = x[psrc[j]]: each XCD takes a contiguous range of it, so its gathers
-// hit the one partition its L2 holds, and y is written sequentially. Pass 2
-// reads y: by y[pinv[k]] for CSR slot k (k_opinion MODE 1; within a
-// partition the plan keeps CSR order, so contiguous agents read y as one
-// sequential stream per partition), or from tile lists (MODE 2 and 3, below).
-// The values are the same as x[src[k]] and the sums keep their canonical
-// order.
+// gathered in two passes. The plan lists the CSR slots sorted stably by
+// source partition (`psize` consecutive sources, sized so that one XCD's L2
+// holds a partition's states), light rows' slots before heavy rows' within a
+// partition. Pass 1 (k_plan_gather) walks the plan, each XCD one contiguous
+// eighth of it (so its gathers hit the partitions its L2 holds), and stores
+// y[dst[j]] = x[psrc[j] - 1]. dst puts the values in list order: the light
+// in-edges of a 256-agent block are numbered from lbase[block] in agent order,
+// in tiles of TILE slots, and the entries of a tile are listed by (source
+// partition, slot), so pass 1 writes a tile's values as one run per
+// partition; the heavy rows' chunk lists follow from position `hy` on. Pass 2
+// (k_opinion_plan) reads y, each entry's slot in its tile (tslot) and weight
+// (twt) as three sequential streams, one tile ahead, stages value and weight
+// by slot in LDS, and every thread adds its own agent's slots. The values are
+// exactly x[src[k]] and the sums keep their canonical order, so the results
+// are the one-pass kernel's.
 constexpr int GP_U = 4;   // plan entries per thread in pass 1 (measured: 4 beats 1, 2, 8, 16)
 
-// Pass 2 with tile lists: the light in-edges of a 256-agent block are
-// numbered in agent order from lbase[block] (the block's "light edges"), in
-// tiles of TileCfg<T>::TILE slots. At plan time every tile's entries were sorted by plan
-// position, i.e. by (source partition, CSR slot): entry e of a tile reads
-// y[ty[e]], a run of consecutive plan entries per partition, and lands at
-// slot tslot[e] of the tile in LDS. Weights, terms and the sums in canonical
-// order are as in opinion_block.
-
-// Tile-list sizes by state type (measured at 10^7 agents: 1024 slots for
-// Float64 and 1536 for Float32 give the most blocks per CU for their step)
 template <class T>
 struct TileCfg {
     static constexpr int TILE = sizeof(T) == 8 ? 1024 : 1536;   // light slots per tile
@@ -755,119 +740,68 @@ struct TileCfg {
 };
 constexpr int OP_TILE_BITS = 4;   // bits of a tile number within a light block (list sort keys)
 
-// LDS of a tile-list block; a heavy block gives each of its 4 waves a
-// quarter of `val` (CHUNK slots)
+// LDS of a pass-2 block; a heavy block gives each of its 4 waves a quarter of x
 template <class T>
-struct TileLDS {
-    T val[TileCfg<T>::TILE];   // x_j by slot, then the numerator term
-    float den[TileCfg<T>::TILE];
-    uint8_t own[TileCfg<T>::TILE];
-    uint32_t b[256], p[256], wsum[4];
-    T xi[256];
-    uint32_t prun[32];
-    uint16_t pfirst[32];
+struct PlanLDS {
+    T x[TileCfg<T>::TILE];       // neighbour state by slot
+    float w[TileCfg<T>::TILE];   // edge weight by slot
+    uint32_t wsum[4];
 };
-constexpr int OPP_RUNS = 32;              // partitions of a packed tile list (5 bits beside 11)
 
-// PACKED: the list entry is slot | partition << 11, and its plan position is
-// prun[b][t][p] + (entry - pfirst[b][t][p]) for tile t of block b; ty is not
-// read. Threads 0-31 fetch the first tile's run table with the agents' data
-// and each next tile's while the current one is processed.
-template <class T, bool PACKED>
-__device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32_t* __restrict__ off,
-                                                    const T* __restrict__ x, const T* __restrict__ y,
-                                                    const float* __restrict__ w, T* __restrict__ xo, uint64_t n,
-                                                    uint64_t nrows, T eps, uint32_t blk,
-                                                    const uint32_t* __restrict__ lbase,
-                                                    const uint32_t* __restrict__ ty,
-                                                    const uint16_t* __restrict__ tslot, uint32_t nblkp,
-                                                    const uint32_t* __restrict__ prun,
-                                                    const uint16_t* __restrict__ pfirst,
-                                                    const float* __restrict__ twt) {
-    constexpr uint32_t TL = TileCfg<T>::TILE, NT = TileCfg<T>::TILES;
-    T* s_val = sm.val;
-    float* s_den = sm.den;
-    uint8_t* s_own = sm.own;
-    uint32_t *s_b = sm.b, *s_p = sm.p, *s_wsum = sm.wsum;
-    T* s_xi = sm.xi;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+template <class T>
+__device__ __forceinline__ void opinion_block_plan(PlanLDS<T>& sm, const uint32_t* __restrict__ off,
+                                                   const T* __restrict__ x, const T* __restrict__ y,
+                                                   T* __restrict__ xo, uint64_t n, uint64_t nrows, T eps,
+                                                   uint32_t blk, const uint32_t* __restrict__ lbase,
+                                                   const uint16_t* __restrict__ tslot,
+                                                   const float* __restrict__ twt, uint32_t nblkp) {
+    constexpr uint32_t TL = TileCfg<T>::TILE;
+    constexpr int QU = TL / 256;   // entries of a tile per thread
+    const int tid = threadIdx.x;
     const uint64_t i = (uint64_t)blk * 256 + tid;
-    const bool valid = i < n;
-    uint32_t b = 0, d = 0;
+    uint32_t d = 0;
     T xi = (T)0;
-    if (valid) {
+    if (i < n) {
         xi = __builtin_nontemporal_load(x + i);
-        if (i < nrows) {
-            b = __builtin_nontemporal_load(off + i);
-            d = __builtin_nontemporal_load(off + i + 1) - b;
-        }
+        if (i < nrows) d = __builtin_nontemporal_load(off + i + 1) - __builtin_nontemporal_load(off + i);
     }
     const uint32_t lb = blk < nblkp ? lbase[blk] : 0u;
     const uint32_t le = blk < nblkp ? lbase[blk + 1] : 0u;
-    constexpr int QU = TL / 256;   // slots of a tile per thread
-    // list entries e = tid + 256 u of the next tile, fetched one tile ahead
-    // (the first tile's during the block scan); 0xFFFFFFFF past the tile
-    uint32_t nsl[QU], njj[QU];
+    // entries e = tid + 256 u of the next tile: slot (0xFFFFFFFF past the
+    // tile), weight and neighbour state, fetched one tile ahead (the first
+    // tile's during the block scan)
+    uint32_t nsl[QU];
+    float nw[QU];
+    T nv[QU];
     {
         const uint32_t c0 = (le - lb) < TL ? (le - lb) : TL;
 #pragma unroll
         for (int u = 0; u < QU; u++) {
             const uint32_t q = (uint32_t)(tid + 256 * u);
             nsl[u] = 0xFFFFFFFFu;
-            njj[u] = 0u;
+            nw[u] = 0.0f;
+            nv[u] = (T)0;
             if (q < c0) {
                 nsl[u] = __builtin_nontemporal_load(tslot + lb + q);
-                if constexpr (!PACKED) njj[u] = __builtin_nontemporal_load(ty + lb + q);
+                nw[u] = __builtin_nontemporal_load(twt + lb + q);
+                nv[u] = __builtin_nontemporal_load(y + lb + q);
             }
         }
     }
-    uint32_t nx_run = 0, nx_first = 0;   // run table entry `tid` of the next tile
-    if (PACKED && tid < OPP_RUNS && blk < nblkp) {
-        nx_run = prun[(uint64_t)blk * NT * OPP_RUNS + tid];
-        nx_first = pfirst[(uint64_t)blk * NT * OPP_RUNS + tid];
-    }
-    const bool light = valid && d <= 64u;
+    const bool light = i < n && d <= 64u;
     const uint32_t ld = light ? d : 0u;
-    uint32_t incl = ld;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y1 = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y1;
-    }
-    if (lane == 63) s_wsum[wv] = incl;
-    __syncthreads();
-    uint32_t pre = 0, total = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        uint32_t v = s_wsum[q];
-        pre += q < wv ? v : 0u;
-        total += v;
-    }
-    const uint32_t P = pre + incl - ld;
-    s_b[tid] = b;
-    s_p[tid] = P;
-    s_xi[tid] = xi;
+    uint32_t total;
+    const uint32_t P = op_light_scan(ld, sm.wsum, total);
     T num = (T)0, den = (T)0;
     for (uint32_t t0 = 0; t0 < total; t0 += TL) {
-        const uint32_t lo = P > t0 ? P : t0;
-        const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
-        const uint32_t cnt = (total - t0) < TL ? (total - t0) : TL;
+        uint32_t sl[QU];
         float wq[QU];
-        uint32_t jj[QU], sl[QU], wi[QU];
         T v[QU];
 #pragma unroll
         for (int u = 0; u < QU; u++) {
             sl[u] = nsl[u];
-            jj[u] = njj[u];
-        }
-        if constexpr (PACKED) {
-            // the weights of this tile's entries (list order) and the next
-            // tile's entries, in flight before the tile's LDS set-up
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                const uint32_t q = (uint32_t)(tid + 256 * u);
-                wq[u] = q < cnt ? __builtin_nontemporal_load(twt + lb + t0 + q) : 0.0f;
-            }
+            wq[u] = nw[u];
+            v[u] = nv[u];
         }
         {
             const uint32_t t1 = t0 + TL;
@@ -876,124 +810,54 @@ __device__ __forceinline__ void opinion_block_tiles(TileLDS<T>& sm, const uint32
             for (int u = 0; u < QU; u++) {
                 const uint32_t q = (uint32_t)(tid + 256 * u);
                 nsl[u] = 0xFFFFFFFFu;
-                njj[u] = 0u;
                 if (q < c1) {
                     nsl[u] = __builtin_nontemporal_load(tslot + lb + t1 + q);
-                    if constexpr (!PACKED) njj[u] = __builtin_nontemporal_load(ty + lb + t1 + q);
+                    nw[u] = __builtin_nontemporal_load(twt + lb + t1 + q);
+                    nv[u] = __builtin_nontemporal_load(y + lb + t1 + q);
                 }
             }
         }
-        for (uint32_t q = lo; q < hi; q++) s_own[q - t0] = (uint8_t)tid;
-        if (PACKED && tid < OPP_RUNS) {
-            sm.prun[tid] = nx_run;
-            sm.pfirst[tid] = (uint16_t)nx_first;
-            if (t0 + TL < total) {
-                const uint64_t g = (uint64_t)blk * NT + t0 / TL + 1;
-                nx_run = prun[g * OPP_RUNS + tid];
-                nx_first = pfirst[g * OPP_RUNS + tid];
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            if (sl[u] != 0xFFFFFFFFu) {
+                sm.x[sl[u]] = v[u];
+                sm.w[sl[u]] = wq[u];
             }
         }
         __syncthreads();
-        if constexpr (!PACKED) {
-            // weights of this thread's slots q = tid + 256 u (slot order):
-            // the weight index of every slot first (LDS only), then the loads
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                const uint32_t q = (uint32_t)(tid + 256 * u);
-                wi[u] = 0xFFFFFFFFu;
-                if (q < cnt) {
-                    const uint32_t a = s_own[q];
-                    wi[u] = s_b[a] + (t0 + q - s_p[a]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                const uint32_t q = (uint32_t)(tid + 256 * u);
-                wq[u] = q < cnt ? __builtin_nontemporal_load(w + wi[u]) : 0.0f;
-            }
-        }
-        if constexpr (PACKED) {
-            // entry -> slot and plan position
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                const uint32_t q = (uint32_t)(tid + 256 * u);
-                if (q < cnt) {
-                    const uint32_t pk = sl[u], pp = pk >> 11;
-                    sl[u] = pk & 2047u;
-                    jj[u] = sm.prun[pp] + (q - sm.pfirst[pp]);
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < QU; u++) v[u] = sl[u] < cnt ? y[jj[u]] : (T)0;
-        if constexpr (PACKED) {
-            // entry e's term lands at its slot
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                if (sl[u] < cnt) {
-                    const T xj = v[u];
-                    const bool acc = fabs(xj - s_xi[s_own[sl[u]]]) <= eps;
-                    s_val[sl[u]] = acc ? (T)wq[u] * xj : (T)0;
-                    s_den[sl[u]] = acc ? wq[u] : 0.0f;
-                }
-            }
-            __syncthreads();
-        } else {
-#pragma unroll
-            for (int u = 0; u < QU; u++)
-                if (sl[u] < cnt) s_val[sl[u]] = v[u];
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                const uint32_t q = (uint32_t)(tid + 256 * u);
-                if (q < cnt) {
-                    const T xj = s_val[q];
-                    const bool acc = fabs(xj - s_xi[s_own[q]]) <= eps;
-                    s_val[q] = acc ? (T)wq[u] * xj : (T)0;
-                    s_den[q] = acc ? wq[u] : 0.0f;
-                }
-            }
-            __syncthreads();
-        }
-        // in slot order; four slots' LDS reads issued together
+        // this thread's slots in order, four LDS reads issued together
+        const uint32_t lo = P > t0 ? P : t0;
+        const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
         uint32_t q = lo;
         for (; q + 4 <= hi; q += 4) {
-            const T a0 = s_val[q - t0], a1 = s_val[q + 1 - t0], a2 = s_val[q + 2 - t0], a3 = s_val[q + 3 - t0];
-            const float d0 = s_den[q - t0], d1 = s_den[q + 1 - t0], d2 = s_den[q + 2 - t0],
-                        d3 = s_den[q + 3 - t0];
-            num = num + a0;
-            num = num + a1;
-            num = num + a2;
-            num = num + a3;
-            den = den + (T)d0;
-            den = den + (T)d1;
-            den = den + (T)d2;
-            den = den + (T)d3;
+            const T a0 = sm.x[q - t0], a1 = sm.x[q + 1 - t0], a2 = sm.x[q + 2 - t0], a3 = sm.x[q + 3 - t0];
+            const float w0 = sm.w[q - t0], w1 = sm.w[q + 1 - t0], w2 = sm.w[q + 2 - t0], w3 = sm.w[q + 3 - t0];
+            op_accum(a0, w0, xi, eps, num, den);
+            op_accum(a1, w1, xi, eps, num, den);
+            op_accum(a2, w2, xi, eps, num, den);
+            op_accum(a3, w3, xi, eps, num, den);
         }
-        for (; q < hi; q++) {
-            num = num + s_val[q - t0];
-            den = den + (T)s_den[q - t0];
-        }
+        for (; q < hi; q++) op_accum(sm.x[q - t0], sm.w[q - t0], xi, eps, num, den);
         __syncthreads();
     }
     if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
 }
 
-// A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
-// h are numbered from hbase[h] in CSR order, in chunks of CHUNK slots,
-// each chunk listed in plan order (hty: plan position, hslot: slot in the
-// chunk). The wave copies a chunk's runs of y into its LDS quarter `buf`,
-// then lane l adds slots l, l + 64, ... in CSR order: the canonical
-// lane-strided partials of opinion_heavy, then the same xor butterfly.
+// A heavy agent (one wave): its in-edges are numbered from hi.w in CSR order,
+// in chunks of CHUNK slots, each chunk listed by (source partition, slot)
+// (hslot: slot in the chunk); its values sit at y[hy + list position]. The
+// wave copies a chunk's values into its LDS quarter `buf` by slot, then lane l
+// adds slots l, l + 64, ... in order: the canonical lane-strided partials,
+// then the xor butterfly.
 template <class T>
-__device__ __forceinline__ void opinion_heavy_tiles(const T* __restrict__ x, const T* __restrict__ y,
-                                                    const float* __restrict__ w, T* __restrict__ xo,
-                                                    const uint4 hi, T eps, const uint32_t* __restrict__ hty,
-                                                    const uint16_t* __restrict__ hslot, T* buf) {
+__device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, const T* __restrict__ y,
+                                                   const float* __restrict__ w, T* __restrict__ xo,
+                                                   const uint4 hi, T eps, const uint16_t* __restrict__ hslot,
+                                                   uint64_t hy, T* buf) {
     constexpr uint32_t CH = TileCfg<T>::CHUNK;
     constexpr int HU = CH / 64;
     const uint32_t lane = threadIdx.x & 63;
-    // hi: row, CSR start, degree, first chunk-list entry (built with the lists)
+    // hi: row, CSR start, degree, first chunk-list entry
     const uint64_t r = hi.x;
     const uint32_t b = hi.y, d = hi.z, hb = hi.w;
     const T xi = x[r];
@@ -1001,37 +865,30 @@ __device__ __forceinline__ void opinion_heavy_tiles(const T* __restrict__ x, con
     for (uint32_t c0 = 0; c0 < d; c0 += CH) {
         const uint32_t cnt = d - c0 < CH ? d - c0 : CH;
         float wq[HU];
-        uint32_t jj[HU], sl[HU];
+        uint32_t sl[HU];
         T v[HU];
 #pragma unroll
         for (int u = 0; u < HU; u++) {
             const uint32_t q = lane + 64u * (uint32_t)u;
             wq[u] = 0.0f;
-            jj[u] = 0u;
             sl[u] = 0xFFFFFFFFu;
+            v[u] = (T)0;
             if (q < cnt) {
                 wq[u] = __builtin_nontemporal_load(w + b + c0 + q);
-                jj[u] = __builtin_nontemporal_load(hty + hb + c0 + q);
                 sl[u] = __builtin_nontemporal_load(hslot + hb + c0 + q);
+                v[u] = __builtin_nontemporal_load(y + hy + hb + c0 + q);
             }
         }
 #pragma unroll
-        for (int u = 0; u < HU; u++) v[u] = sl[u] < cnt ? y[jj[u]] : (T)0;
-#pragma unroll
         for (int u = 0; u < HU; u++)
-            if (sl[u] < cnt) buf[sl[u]] = v[u];
+            if (sl[u] != 0xFFFFFFFFu) buf[sl[u]] = v[u];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int u = 0; u < HU; u++) {
             const uint32_t q = lane + 64u * (uint32_t)u;
-            if (q < cnt) {
-                const T xj = buf[q];
-                const bool acc = fabs(xj - xi) <= eps;
-                num = num + (acc ? (T)wq[u] * xj : (T)0);
-                den = den + (acc ? (T)wq[u] : (T)0);
-            }
+            if (q < cnt) op_accum(buf[q], wq[u], xi, eps, num, den);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1045,72 +902,73 @@ __device__ __forceinline__ void opinion_heavy_tiles(const T* __restrict__ x, con
     if (lane == 0) xo[r] = den > (T)0 ? num / den : xi;
 }
 
-// One launch per step: blocks [0, nh) run one wave per heavy agent (hub
-// waves first: they are the longest, so they start before the light blocks
-// instead of forming the tail of the launch), the rest thread-per-agent
-// blocks. With `remap` (nh a multiple of 8, 8 * ceil(nlight / 8) light
-// blocks) the light blocks of one XCD take one contiguous range of agents:
-// blocks are dispatched round-robin over the 8 XCDs, and contiguous agents
-// read the plan vector as a few sequential streams that one XCD's L2 keeps.
-// MODE 0: one-pass gather from x; 1: y[pinv[k]]; 2: tile lists for the light
-// blocks (chunk lists or y[pinv[k]] for the heavy agents); 3: as 2 with packed
-// tile lists.
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ off,
-                                                 const uint32_t* __restrict__ src,
-                                                 const T* __restrict__ x, const T* __restrict__ y,
-                                                 const float* __restrict__ w, T* __restrict__ xo, uint64_t n,
-                                                 uint64_t nrows, uint64_t rowbase, T eps,
-                                                 const uint32_t* __restrict__ heavy, uint64_t nheavy,
-                                                 uint32_t nh, uint32_t nlight, uint32_t hot, uint32_t remap,
-                                                 const uint32_t* __restrict__ lbase,
-                                                 const uint32_t* __restrict__ ty,
-                                                 const uint16_t* __restrict__ tslot, uint32_t nblkp,
-                                                 const uint4* __restrict__ hinfo,
-                                                 const uint32_t* __restrict__ hty,
-                                                 const uint16_t* __restrict__ hslot,
-                                                 const uint32_t* __restrict__ prun,
-                                                 const uint16_t* __restrict__ pfirst,
-                                                 const float* __restrict__ twt) {
-    constexpr bool PLAN = MODE != 0;
-    constexpr bool TILES = MODE >= 2;
-    // tile modes: one LDS area for the light blocks and the heavy waves
-    __shared__ __align__(16) unsigned char smraw[TILES ? sizeof(TileLDS<T>) : 16];
-    TileLDS<T>& sm = *reinterpret_cast<TileLDS<T>*>(smraw);
+// Pass 2: blocks [0, nh) run one wave per heavy agent, the others are light
+// blocks. Light blocks of one XCD take one contiguous range of agents (blocks
+// are dispatched round-robin over the 8 XCDs; the grid holds
+// 8 * ceil(nlight / 8) light blocks).
+template <class T>
+__global__ __launch_bounds__(256) void k_opinion_plan(const uint32_t* __restrict__ off,
+                                                      const T* __restrict__ x, const T* __restrict__ y,
+                                                      const float* __restrict__ w, T* __restrict__ xo,
+                                                      uint64_t n, uint64_t nrows, T eps,
+                                                      const uint4* __restrict__ hinfo, uint64_t nheavy,
+                                                      uint32_t nh, uint32_t nlight,
+                                                      const uint32_t* __restrict__ lbase,
+                                                      const uint16_t* __restrict__ tslot,
+                                                      const float* __restrict__ twt, uint32_t nblkp,
+                                                      const uint16_t* __restrict__ hslot, uint64_t hy) {
+    __shared__ __align__(16) unsigned char smraw[sizeof(PlanLDS<T>)];
+    PlanLDS<T>& sm = *reinterpret_cast<PlanLDS<T>*>(smraw);
     if (blockIdx.x >= nh) {
         uint32_t l = blockIdx.x - nh;
-        if (remap) {
-            const uint32_t per = (nlight + 7) / 8;
-            l = (l % 8) * per + l / 8;
-            if (l >= nlight) return;   // whole block
-        }
-        if constexpr (TILES) {
-            opinion_block_tiles<T, MODE == 3>(sm, off, x, y, w, xo, n, nrows, eps, l, lbase, ty, tslot, nblkp,
-                                              prun, pfirst, twt);
-        } else {
-            opinion_block<T, PLAN>(off, src, x, y, w, xo, n, nrows, rowbase, eps, l, hot);
-        }
+        const uint32_t per = (nlight + 7) / 8;
+        l = (l % 8) * per + l / 8;
+        if (l >= nlight) return;   // whole block
+        opinion_block_plan<T>(sm, off, x, y, xo, n, nrows, eps, l, lbase, tslot, twt, nblkp);
     } else {
-        uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        if constexpr (TILES) {
-            if (hinfo) {   // one quarter of the block's LDS per wave
-                if (wv < nheavy)
-                    opinion_heavy_tiles<T>(x, y, w, xo, hinfo[wv], eps, hty, hslot,
-                                           sm.val + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
-                return;
-            }
-        }
-        if (wv < nheavy) opinion_heavy<T, PLAN>(off, src, x, y, w, xo, heavy[wv], rowbase, eps, hot);
+        const uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        if (wv < nheavy)
+            opinion_heavy_plan<T>(x, y, w, xo, hinfo[wv], eps, hslot, hy,
+                                  sm.x + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
     }
 }
 
+// Pass 1: grid 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 GP_U));
+// logical block l takes plan entries [l * 1024, (l + 1) * 1024), thread t the
+// entries l * 1024 + t + 256 u, u < GP_U: four source and destination loads,
+// four gathers, four stores.
+template <class T>
+__global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc,
+                                                     const uint32_t* __restrict__ dst, const T* __restrict__ x,
+                                                     T* __restrict__ y, uint64_t ne) {
+    const uint32_t per = gridDim.x / 8;
+    const uint32_t l = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const uint64_t j0 = (uint64_t)l * (256 * GP_U) + threadIdx.x;
+    uint32_t sj[GP_U], dj[GP_U];
+    T v[GP_U];
+#pragma unroll
+    for (int u = 0; u < GP_U; u++) {
+        const uint64_t j = j0 + (uint64_t)u * 256;
+        sj[u] = 0u;
+        dj[u] = 0u;
+        if (j < ne) {
+            sj[u] = __builtin_nontemporal_load(psrc + j);
+            dj[u] = __builtin_nontemporal_load(dst + j);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
+#pragma unroll
+    for (int u = 0; u < GP_U; u++)
+        if (sj[u] && dj[u] < ne) __builtin_nontemporal_store(v[u], y + dj[u]);
+}
 
 // plan sort key of each CSR slot: source partition, then 1 for the slots of
 // a heavy row (degree > 64), so that within a partition the light slots come
 // first, in CSR order
 __global__ void k_plan_keys(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t nrows,
                             uint32_t psize, uint32_t* __restrict__ keys) {
-    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     const uint32_t b = off[r], e = off[r + 1], hv = e - b > 64u ? 1u : 0u;
     for (uint32_t k = b; k < e; k++) keys[k] = ((src[k] - 1u) / psize) << 1 | hv;
@@ -1118,7 +976,7 @@ __global__ void k_plan_keys(const uint32_t* __restrict__ off, const uint32_t* __
 
 __global__ void k_plan_fill(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint64_t ne,
                             uint32_t* __restrict__ psrc, uint32_t* __restrict__ pinv) {
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ne) return;
     const uint32_t k = perm[j];
     psrc[j] = src[k];
@@ -1130,7 +988,7 @@ __global__ __launch_bounds__(256) void k_light_totals(const uint32_t* __restrict
                                                       uint32_t* __restrict__ tot) {
     __shared__ uint32_t s_w[4];
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    uint32_t d = i < nrows ? off[i + 1] - off[i] : 0u;
+    const uint32_t d = i < nrows ? off[i + 1] - off[i] : 0u;
     uint32_t v = d <= 64u ? d : 0u;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1147,26 +1005,16 @@ __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__
                                                     uint32_t pbits, uint32_t tile, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
     __shared__ uint32_t s_w[4];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + tid;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t b = 0, d = 0;
     if (i < nrows) {
         b = off[i];
         d = off[i + 1] - b;
     }
     const uint32_t ld = d <= 64u ? d : 0u;
-    uint32_t incl = ld;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y1 = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y1;
-    }
-    if (lane == 63) s_w[wv] = incl;
-    __syncthreads();
-    uint32_t pre = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) pre += q < wv ? s_w[q] : 0u;
-    const uint32_t P = pre + incl - ld, lb = lbase[blockIdx.x];
+    uint32_t total;
+    const uint32_t P = op_light_scan(ld, s_w, total);
+    const uint32_t lb = lbase[blockIdx.x];
     const uint32_t hi = blockIdx.x << (OP_TILE_BITS + pbits);
     for (uint32_t e = 0; e < ld; e++) {
         const uint32_t q = P + e, idx = lb + q;
@@ -1176,49 +1024,25 @@ __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__
     }
 }
 
-__global__ void k_light_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ kidx,
-                             const uint16_t* __restrict__ tsl, const uint32_t* __restrict__ pinv, uint64_t nl,
-                             uint32_t* __restrict__ ty, uint16_t* __restrict__ tslot) {
-    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// list entry e (sorted light or heavy lists, positions from `base` in y):
+// its slot, its weight (light lists; 0 without 4-byte edge states) and, for
+// its plan entry, the destination base + e
+__global__ void k_list_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ kidx,
+                            const uint16_t* __restrict__ tsl, const uint32_t* __restrict__ pinv, uint64_t nl,
+                            uint64_t base, uint16_t* __restrict__ slot, const float* __restrict__ w,
+                            float* __restrict__ lw, uint32_t* __restrict__ dst) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nl) return;
-    const uint32_t idx = order[e];
-    ty[e] = pinv[kidx[idx]];
-    tslot[e] = tsl[idx];
+    const uint32_t idx = order[e], k = kidx[idx];
+    slot[e] = tsl[idx];
+    if (lw) lw[e] = w ? w[k] : 0.0f;
+    dst[pinv[k]] = (uint32_t)(base + e);
 }
-
-// packed tile lists (at most OPP_RUNS partitions): per entry of the sorted
-// light lists, slot | partition << 11 and the weight of its CSR slot (0
-// without 4-byte edge states); for tile t of block b and each partition p
-// present, the plan position of its run's first entry
-// (prun[(b * ntiles + t) * OPP_RUNS + p]) and that entry's index in the
-// tile (pfirst[...])
-__global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
-                            const uint32_t* __restrict__ kidx, const uint16_t* __restrict__ tsl,
-                            const uint32_t* __restrict__ pinv, const uint32_t* __restrict__ lbase,
-                            uint32_t pbits, uint32_t tile, uint32_t ntiles, uint64_t nl,
-                            uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
-                            uint16_t* __restrict__ pfirst, const float* __restrict__ w,
-                            float* __restrict__ twt) {
-    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nl) return;
-    const uint32_t key = keys[e], p = key & ((1u << pbits) - 1u);
-    const uint32_t blk = key >> (OP_TILE_BITS + pbits), t = (key >> pbits) & ((1u << OP_TILE_BITS) - 1u);
-    const uint32_t t0 = lbase[blk] + t * tile;
-    const uint32_t idx = order[e];
-    tpk[e] = (uint16_t)(tsl[idx] | (p << 11));
-    twt[e] = w ? w[kidx[idx]] : 0.0f;
-    if (e == t0 || keys[e - 1] != key) {
-        const uint64_t g = (uint64_t)blk * ntiles + t;
-        prun[g * OPP_RUNS + p] = pinv[kidx[idx]];
-        pfirst[g * OPP_RUNS + p] = (uint16_t)(e - t0);
-    }
-}
-
 
 // in-edge count and chunk count of each heavy row
 __global__ void k_heavy_sizes(const uint32_t* __restrict__ off, const uint32_t* __restrict__ heavy, uint64_t nheavy,
                               uint32_t chunk, uint32_t* __restrict__ deg, uint32_t* __restrict__ chunks) {
-    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nheavy) return;
     const uint32_t r = heavy[h], d = off[r + 1] - off[r];
     deg[h] = d;
@@ -1248,65 +1072,17 @@ __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__
 // row, CSR start, degree and first chunk-list entry of each heavy row
 __global__ void k_heavy_info(const uint32_t* __restrict__ off, const uint32_t* __restrict__ heavy, uint64_t nheavy,
                              const uint32_t* __restrict__ hbase, uint4* __restrict__ hinfo) {
-    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nheavy) return;
     const uint32_t r = heavy[h], b = off[r];
     hinfo[h] = make_uint4(r, b, off[r + 1] - b, hbase[h]);
 }
 
-// grid: 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 * GP_U)); logical
-// block l takes plan entries [l * 1024, (l + 1) * 1024). Float32: thread t
-// takes the 4 consecutive entries from l * 1024 + 4t (one 16-byte load of
-// their sources, four gathers, one 16-byte store; 0.731 against 0.739 ms per
-// step with the Float64 layout below). The buffers are 256-byte aligned; a
-// partial group at the end goes entry by entry.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t pu32x4 __attribute__((ext_vector_type(4)));
-static_assert(GP_U == 4, "k_plan_gather takes four plan entries per thread");
-template <class T>
-__global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc, const T* __restrict__ x,
-                                                     T* __restrict__ y, uint64_t ne) {
-    const uint32_t per = gridDim.x / 8;
-    const uint32_t l = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    const uint64_t base = (uint64_t)l * (256 * GP_U);
-    if constexpr (sizeof(T) == 4) {
-        const uint64_t j0 = base + (uint64_t)threadIdx.x * 4;
-        if (j0 >= ne) return;
-        if (j0 + 4 <= ne) {
-            const pu32x4 sv = __builtin_nontemporal_load((const pu32x4*)(psrc + j0));
-            const f32x4 v = {(float)x[sv.x - 1], (float)x[sv.y - 1], (float)x[sv.z - 1], (float)x[sv.w - 1]};
-            __builtin_nontemporal_store(v, (f32x4*)(y + j0));
-        } else {
-            for (uint64_t j = j0; j < ne; j++) y[j] = x[psrc[j] - 1];
-        }
-    } else {
-        // Float64: entries base + t + 256u, u < 4 (four 4-byte source loads,
-        // four gathers, four 8-byte stores; measured faster for Float64 than
-        // the contiguous 2 x 2 layout: 470 against 536 us at 10^8 entries)
-        const uint64_t j0 = base + threadIdx.x;
-        uint32_t sj[GP_U];
-        T v[GP_U];
-#pragma unroll
-        for (int u = 0; u < GP_U; u++) {
-            const uint64_t j = j0 + (uint64_t)u * 256;
-            sj[u] = j < ne ? __builtin_nontemporal_load(psrc + j) : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
-#pragma unroll
-        for (int u = 0; u < GP_U; u++) {
-            const uint64_t j = j0 + (uint64_t)u * 256;
-            if (j < ne) __builtin_nontemporal_store(v[u], y + j);
-        }
-    }
-}
-
-
 __global__ void k_heavy_rows(const uint32_t* __restrict__ off, uint64_t rows, uint64_t rowbase,
                              uint64_t nrows, uint32_t* __restrict__ list, uint32_t* __restrict__ cnt) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows) return;
-    uint64_t r = rowbase + i;
+    const uint64_t r = rowbase + i;
     if (off[r + 1] - off[r] > 64u) list[atomicAdd(cnt, 1u)] = (uint32_t)r;
 }
 
@@ -1671,7 +1447,6 @@ static int check_model_layout(Sim& s, int t, int et, uint32_t abytes, uint32_t e
     return VH_OK;
 }
 
-static long long env_ll(const char* name, long long dflt);
 
 // Out-edges of E's rows (one agent type, rows from 0, 1-based sources up to
 // nsrc): the transpose, by a stable radix sort of the in-edges by source;
@@ -1711,15 +1486,14 @@ int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step) {
     if (!A.writeable) return set_error(VH_ERR_ASSERT, "SIR: the agent type must be in `write`");
     if (!A.nslots) return VH_OK;
     // the exposure pass needs the in-edges of `t` only (rows from 0, sources
-    // of `t`); it runs for at least VH_SIR_EXPOSE_MIN agents (default 65536:
+    // of `t`); it runs for at least sir_expose_min agents (default 65536:
     // below, its three extra launches per step cost more than the draws it
-    // saves); VH_SIR_EXPOSE=0 turns it off
+    // saves) unless the options turn it off
     uint32_t* expo = nullptr;
     uint32_t* cnt = nullptr;
     const uint64_t nrows = E.rows ? E.nrows[t] : 0ull;
     if (nrows && E.nedges && !E.src64 && E.src_type == (uint32_t)t && E.rowbase[t] == 0 && nrows == E.rows &&
-        A.nslots < 0xFFFFFFFFull && env_ll("VH_SIR_EXPOSE", 1) != 0 &&
-        (long long)A.nslots >= env_ll("VH_SIR_EXPOSE_MIN", 65536)) {
+        A.nslots < 0xFFFFFFFFull && g_opts.sir_expose != 0 && (int64_t)A.nslots >= g_opts.sir_expose_min) {
         RC(build_out_csr(s, E, A.nslots));
         expo = E.expo.as<uint32_t>();
         cnt = E.expo_cnt.as<uint32_t>();
@@ -1737,9 +1511,10 @@ int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step) {
     return VH_OK;
 }
 
+// Rows of degree > 64 (heavy), ascending; built once per CSR version
 int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
     if (E.bins_version == E.version) return VH_OK;
-    uint64_t nrows = E.nrows[t];
+    const uint64_t nrows = E.nrows[t];
     RC(E.heavy.ensure(nrows * 4 + 8, s.stream));
     uint32_t* cnt = (uint32_t*)s.d_stats.p;
     fill_u32(cnt, 0, 1, s.stream);
@@ -1749,39 +1524,72 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
     uint32_t h = 0;
     HIPCHECK(hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, s.stream));
     HIPCHECK(hipStreamSynchronize(s.stream));
+    if (h > 1) {   // the atomic compaction's order varies between runs: sort it
+        std::vector<uint32_t> rows(h);
+        HIPCHECK(hipMemcpy(rows.data(), E.heavy.p, (size_t)h * 4, hipMemcpyDeviceToHost));
+        std::sort(rows.begin(), rows.end());
+        HIPCHECK(hipMemcpy(E.heavy.p, rows.data(), (size_t)h * 4, hipMemcpyHostToDevice));
+    }
     E.nheavy = h;
     E.bins_version = E.version;
     return VH_OK;
 }
 
-// Tile lists of the light blocks (opinion_block_tiles) from a built plan:
-// light edges numbered per block from lbase, sorted by (block, tile,
-// partition) with a stable radix sort, so each tile lists its entries in
-// plan order. Left out (E.plan_tiles false) when the key needs more than 32
-// bits.
-int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts, uint32_t tile) {
+// The gather plan of the in-edges of E (one target type at row 0, sources
+// 1-based numbers <= nsrc < 2^32) and its lists: psrc and dst in plan
+// order, the light lists (lbase, tslot, twt) and the heavy rows' chunk lists
+// (hbase, hslot, hinfo). E.plan_ok stays false when a sort key would need
+// more than 32 bits (the one-pass kernel runs instead).
+int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint32_t tile) {
+    if (E.plan_version == E.version && E.plan_psize == psize && E.plan_tile == tile) return VH_OK;
     hipStream_t st = s.stream;
-    E.plan_tiles = false;
-    E.plan_packed = false;
-    const uint64_t nrows = E.rows;
+    E.plan_ok = false;
+    E.plan_version = E.version;
+    E.plan_psize = psize;
+    E.plan_tile = tile;
+    const uint64_t ne = E.nedges, nrows = E.rows;
+    const uint64_t nparts = (nsrc + psize - 1) / psize;
     const uint32_t nblk = (uint32_t)((nrows + 255) / 256);
-    uint32_t bbits = 1, pbits = 1;
-    while ((1ull << bbits) < nblk) bbits++;
+    const uint32_t chunk = tile / 4;
+    uint32_t pbits = 1, bbits = 1;
     while ((1ull << pbits) < nparts) pbits++;
-    const uint32_t ntiles = (64 * 256 + tile - 1) / tile, chunk = tile / 4;
-    if (ntiles > (1u << OP_TILE_BITS)) return VH_OK;
-    if (bbits + OP_TILE_BITS + pbits > 32) return VH_OK;
+    while ((1ull << bbits) < nblk) bbits++;
+    if (!ne || ne >= 0xFFFFFFFFull || bbits + OP_TILE_BITS + pbits > 32 ||
+        (uint32_t)((64 * 256 + tile - 1) / tile) > (1u << OP_TILE_BITS))
+        return VH_OK;
+    const uint32_t* off = E.off.as<uint32_t>();
+    const uint32_t* src = E.src.as<uint32_t>();
+    // plan order: stable sort of the CSR slots by (partition, heavy row)
+    RC(E.psrc.ensure(ne * 4 + 8, st));
+    RC(E.dst.ensure(ne * 4 + 8, st));
+    uint32_t* pinv = (uint32_t*)s.scratch.take(ne * 4 + 4);
+    {
+        uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4);
+        uint32_t* vals = (uint32_t*)s.scratch.take(ne * 4);
+        uint32_t* ka = (uint32_t*)s.scratch.take(ne * 4);
+        uint32_t* va = (uint32_t*)s.scratch.take(ne * 4);
+        if (!pinv || !keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(nrows, 256)), dim3(256), 0, st, off, src, nrows, psize, keys);
+        if (radix_sort_pairs(&keys, &vals, ka, va, ne, pbits + 1, true, s.scratch, st))
+            return set_error(VH_ERR_DEVICE, "plan sort");
+        hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, src, vals, ne,
+                           E.psrc.as<uint32_t>(), pinv);
+        HIPCHECK(hipGetLastError());
+    }
+    // light lists: the light edges of each 256-row block numbered from
+    // lbase[block], sorted by (block, tile, partition) (stable: slot order
+    // within a partition)
     RC(E.lbase.ensure(((uint64_t)nblk + 1) * 4 + 8, st));
     uint32_t* tot = (uint32_t*)s.scratch.take((uint64_t)nblk * 4 + 4);
     if (!tot) return set_error(VH_ERR_OOM, "scratch");
-    const uint32_t* off = E.off.as<uint32_t>();
     hipLaunchKernelGGL(k_light_totals, dim3(nblk), dim3(256), 0, st, off, nrows, tot);
     if (scan_excl_u32(tot, E.lbase.as<uint32_t>(), nblk, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
     uint32_t nl = 0;
     HIPCHECK(hipMemcpyAsync(&nl, E.lbase.as<uint32_t>() + nblk, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    RC(E.ty.ensure((uint64_t)nl * 4 + 8, st));
     RC(E.tslot.ensure((uint64_t)nl * 2 + 8, st));
+    RC(E.twt.ensure((uint64_t)nl * 4 + 8, st));
+    const float* w4 = E.bytes == 4 ? E.st.as<float>() : nullptr;
     if (nl) {
         uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
         uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
@@ -1790,110 +1598,62 @@ int build_tile_lists(Sim& s, EdgeStore& E, uint32_t psize, uint32_t nparts, uint
         uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
         uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2);
         if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
-        hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, E.src.as<uint32_t>(), nrows,
-                           E.lbase.as<uint32_t>(), psize, pbits, tile, keys, kidx, tsl);
+        hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, src, nrows, E.lbase.as<uint32_t>(),
+                           psize, pbits, tile, keys, kidx, tsl);
         if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + OP_TILE_BITS + pbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "tile sort");
-        hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, vals, kidx, tsl,
-                           E.pinv.as<uint32_t>(), (uint64_t)nl, E.ty.as<uint32_t>(), E.tslot.as<uint16_t>());
+        hipLaunchKernelGGL(k_list_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, vals, kidx, tsl, pinv,
+                           (uint64_t)nl, (uint64_t)0, E.tslot.as<uint16_t>(), w4, E.twt.as<float>(),
+                           E.dst.as<uint32_t>());
         HIPCHECK(hipGetLastError());
-        if (nparts <= (uint32_t)OPP_RUNS) {
-            // packed lists: 16-bit entries, a run table per tile (a fixed
-            // ntiles x OPP_RUNS slot per block)
-            const uint64_t nt = (uint64_t)nblk * ntiles * OPP_RUNS;
-            RC(E.prun.ensure(nt * 4 + 8, st));
-            RC(E.pfirst.ensure(nt * 2 + 8, st));
-            RC(E.tpk.ensure((uint64_t)nl * 2 + 8, st));
-            RC(E.twt.ensure((uint64_t)nl * 4 + 8, st));
-            hipLaunchKernelGGL(k_pack_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, tsl,
-                               E.pinv.as<uint32_t>(), E.lbase.as<uint32_t>(), pbits, tile, ntiles, (uint64_t)nl,
-                               E.tpk.as<uint16_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
-                               E.bytes == 4 ? E.st.as<float>() : nullptr, E.twt.as<float>());
-            HIPCHECK(hipGetLastError());
-            E.plan_packed = true;
-        }
     }
-    E.plan_tiles = true;
-    // heavy rows (E.heavy, in its order): chunk lists
-    E.plan_htiles = false;
+    E.plan_hy = nl;
+    // heavy rows (E.heavy, ascending): chunk lists from nl on
     const uint64_t nheavy = E.nheavy;
-    if (!nheavy) return VH_OK;
-    uint32_t* hd = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
-    uint32_t* hc = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
-    uint32_t* cbase = (uint32_t*)s.scratch.take(nheavy * 4 + 8);
-    if (!hd || !hc || !cbase) return set_error(VH_ERR_OOM, "scratch");
-    RC(E.hbase.ensure(nheavy * 4 + 8, st));
-    hipLaunchKernelGGL(k_heavy_sizes, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off, E.heavy.as<uint32_t>(),
-                       nheavy, chunk, hd, hc);
-    if (scan_excl_u32(hd, E.hbase.as<uint32_t>(), nheavy, s.scratch, st) ||
-        scan_excl_u32(hc, cbase, nheavy, s.scratch, st))
-        return set_error(VH_ERR_DEVICE, "scan");
-    uint32_t htot[2] = {0, 0};
-    HIPCHECK(hipMemcpyAsync(&htot[0], E.hbase.as<uint32_t>() + nheavy, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(&htot[1], cbase + nheavy, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    const uint32_t nhe = htot[0], nch = htot[1];
-    uint32_t cbits = 1;
-    while ((1ull << cbits) < nch) cbits++;
-    if (cbits + pbits > 32 || !nhe) return VH_OK;
-    RC(E.hty.ensure((uint64_t)nhe * 4 + 8, st));
-    RC(E.hslot.ensure((uint64_t)nhe * 2 + 8, st));
-    uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
-    uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
-    uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
-    uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
-    uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
-    uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nhe * 2);
-    if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
-    hipLaunchKernelGGL(k_heavy_keys, dim3(grid_for(nheavy * 64, 256)), dim3(256), 0, st, off, E.src.as<uint32_t>(),
-                       E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, psize, pbits, chunk, keys, kidx, tsl);
-    if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
-        return set_error(VH_ERR_DEVICE, "chunk sort");
-    hipLaunchKernelGGL(k_light_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl,
-                       E.pinv.as<uint32_t>(), (uint64_t)nhe, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
-    RC(E.hinfo.ensure(nheavy * 16 + 16, st));
-    hipLaunchKernelGGL(k_heavy_info, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off, E.heavy.as<uint32_t>(),
-                       nheavy, E.hbase.as<uint32_t>(), E.hinfo.as<uint4>());
-    HIPCHECK(hipGetLastError());
-    E.plan_htiles = true;
-    return VH_OK;
-}
-
-// The plan of the in-edges of E (one target type at row 0; sources 1-based
-// numbers <= nsrc < 2^32).
-int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint32_t tile) {
-    if (E.plan_version == E.version && E.plan_psize == psize && E.plan_tile == tile) return VH_OK;
-    hipStream_t st = s.stream;
-    const uint64_t ne = E.nedges;
-    RC(E.psrc.ensure(ne * 4 + 8, st));
-    RC(E.pinv.ensure(ne * 4 + 8, st));
-    if (ne) {
-        const uint64_t nparts = (nsrc + psize - 1) / psize;
-        uint32_t bits = 1;
-        while (bits < 32 && (1ull << bits) < nparts) bits++;
-        uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4);
-        uint32_t* vals = (uint32_t*)s.scratch.take(ne * 4);
-        uint32_t* ka = (uint32_t*)s.scratch.take(ne * 4);
-        uint32_t* va = (uint32_t*)s.scratch.take(ne * 4);
-        if (!keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
-        hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(E.rows, 256)), dim3(256), 0, st, E.off.as<uint32_t>(),
-                           E.src.as<uint32_t>(), E.rows, psize, keys);
-        if (radix_sort_pairs(&keys, &vals, ka, va, ne, bits + 1, true, s.scratch, st))
-            return set_error(VH_ERR_DEVICE, "plan sort");
-        hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, E.src.as<uint32_t>(), vals, ne,
-                           E.psrc.as<uint32_t>(), E.pinv.as<uint32_t>());
+    if (nheavy) {
+        uint32_t* hd = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
+        uint32_t* hc = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
+        uint32_t* cbase = (uint32_t*)s.scratch.take(nheavy * 4 + 8);
+        if (!hd || !hc || !cbase) return set_error(VH_ERR_OOM, "scratch");
+        RC(E.hbase.ensure(nheavy * 4 + 8, st));
+        hipLaunchKernelGGL(k_heavy_sizes, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off,
+                           E.heavy.as<uint32_t>(), nheavy, chunk, hd, hc);
+        if (scan_excl_u32(hd, E.hbase.as<uint32_t>(), nheavy, s.scratch, st) ||
+            scan_excl_u32(hc, cbase, nheavy, s.scratch, st))
+            return set_error(VH_ERR_DEVICE, "scan");
+        uint32_t htot[2] = {0, 0};
+        HIPCHECK(hipMemcpyAsync(&htot[0], E.hbase.as<uint32_t>() + nheavy, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&htot[1], cbase + nheavy, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        const uint32_t nhe = htot[0], nch = htot[1];
+        uint32_t cbits = 1;
+        while ((1ull << cbits) < nch) cbits++;
+        if (cbits + pbits > 32 || (uint64_t)nl + nhe != ne) return VH_OK;
+        RC(E.hslot.ensure((uint64_t)nhe * 2 + 8, st));
+        uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+        uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+        uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+        uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+        uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
+        uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nhe * 2);
+        if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_heavy_keys, dim3(grid_for(nheavy * 64, 256)), dim3(256), 0, st, off, src,
+                           E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, psize, pbits, chunk,
+                           keys, kidx, tsl);
+        if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
+            return set_error(VH_ERR_DEVICE, "chunk sort");
+        hipLaunchKernelGGL(k_list_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl, pinv,
+                           (uint64_t)nhe, (uint64_t)nl, E.hslot.as<uint16_t>(), nullptr, nullptr,
+                           E.dst.as<uint32_t>());
+        RC(E.hinfo.ensure(nheavy * 16 + 16, st));
+        hipLaunchKernelGGL(k_heavy_info, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off,
+                           E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), E.hinfo.as<uint4>());
         HIPCHECK(hipGetLastError());
-        RC(build_tile_lists(s, E, psize, (uint32_t)nparts, tile));
+    } else if (nl != ne) {
+        return VH_OK;
     }
-    E.plan_version = E.version;
-    E.plan_psize = psize;
-    E.plan_tile = tile;
+    E.plan_ok = true;
     return VH_OK;
-}
-
-long long env_ll(const char* name, long long dflt) {
-    const char* v = getenv(name);
-    return v ? atoll(v) : dflt;
 }
 
 int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step) {
@@ -1904,79 +1664,66 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     if (!A.writeable) return set_error(VH_ERR_ASSERT, "opinion: the agent type must be in `write`");
     if (!A.nslots) return VH_OK;
     const uint32_t tb = f32 ? 4 : 8;
-    uint64_t nrows = E.rows ? E.nrows[t] : 0;
+    const uint64_t nrows = E.rows ? E.nrows[t] : 0;
     if (E.rows) RC(build_heavy_bins(s, E, t));
-    uint64_t nheavy = E.rows ? E.nheavy : 0;
-    uint32_t nlight = grid_for(A.nslots, 256);
-    // hot prefix of the state kept in L2 (VH_OPINION_HOT_BYTES; default: off)
-    const long long hot_bytes = env_ll("VH_OPINION_HOT_BYTES", -1);
-    // two-pass gather from a state vector of at least VH_OPINION_PLAN_BYTES
-    // (< 0: never) with source partitions of VH_OPINION_PART_BYTES (default
-    // 2 MiB, widened up to 1.5x to make at most 32 partitions, which the
-    // packed tile lists need)
-    const long long plan_min = env_ll("VH_OPINION_PLAN_BYTES", 16ll << 20);
-    const bool part_set = getenv("VH_OPINION_PART_BYTES") != nullptr;
-    const long long part_bytes = env_ll("VH_OPINION_PART_BYTES", 2ll << 20);
-    uint32_t hot = hot_bytes < 0 ? 0xFFFFFFFFu : (uint32_t)(hot_bytes / tb);
-    // the plan needs one target type at row 0 and 32-bit sources, all of `t`
-    const bool plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
-                      E.rowbase[t] == 0 && nrows == E.rows && (uint64_t)A.nslots * tb >= (uint64_t)plan_min;
-    uint32_t nh = (uint32_t)((nheavy + 3) / 4);
-    uint32_t nl = nlight;
+    const uint64_t nheavy = E.rows ? E.nheavy : 0;
+    const uint32_t nlight = grid_for(A.nslots, 256);
+    // two-pass gather through a plan from a state vector of at least
+    // opinion_plan_bytes (< 0: never), with source partitions of
+    // opinion_part_bytes (0: 2 MiB, widened up to 1.5x to make at most 32
+    // partitions: fewer partitions give longer runs per tile)
+    const int64_t plan_min = g_opts.opinion_plan_bytes;
+    bool plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
+                E.rowbase[t] == 0 && nrows == E.rows && A.nslots < 0xFFFFFFFFull &&
+                (uint64_t)A.nslots * tb >= (uint64_t)plan_min;
+    const uint32_t tile = f32 ? (uint32_t)TileCfg<float>::TILE : (uint32_t)TileCfg<double>::TILE;
     if (plan) {
-        uint64_t psize = part_bytes / tb;
+        const bool auto_part = g_opts.opinion_part_bytes == 0;
+        uint64_t psize = (auto_part ? (2ull << 20) : (uint64_t)g_opts.opinion_part_bytes) / tb;
         if (psize < 1) psize = 1;
         if (psize > 0x7FFFFFFFull) psize = 0x7FFFFFFFull;
         const uint64_t np = (A.nslots + psize - 1) / psize;
-        if (!part_set && np > (uint64_t)OPP_RUNS && np * 2 <= (uint64_t)OPP_RUNS * 3)
-            psize = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;
-        RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots,
-                             f32 ? (uint32_t)TileCfg<float>::TILE : (uint32_t)TileCfg<double>::TILE));
-        RC(E.pval.ensure(E.nedges * tb + 8, s.stream));
-        uint64_t nb = (E.nedges + 256 * GP_U - 1) / (256 * GP_U);
-        dim3 gg((uint32_t)((nb + 7) / 8 * 8)), bb(256);
-        if (f32)
-            hipLaunchKernelGGL(k_plan_gather<float>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
-                               (const float*)A.rstate(), E.pval.as<float>(), E.nedges);
-        else
-            hipLaunchKernelGGL(k_plan_gather<double>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
-                               (const double*)A.rstate(), E.pval.as<double>(), E.nedges);
-        nh = (nh + 7) / 8 * 8;
-        nl = (nlight + 7) / 8 * 8;
+        if (auto_part && np > 32 && np * 2 <= 32 * 3) psize = (A.nslots + 31) / 32;
+        RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots, tile));
+        plan = E.plan_ok;
     }
-    dim3 g(nl + nh), b(256);
-    const uint32_t* src = plan ? E.pinv.as<uint32_t>() : E.src.as<uint32_t>();
-    // pass 2 with tile lists unless VH_OPINION_TILES=0
-    int mode = !plan ? 0 : (E.plan_tiles && env_ll("VH_OPINION_TILES", 1) != 0) ? 2 : 1;
-    // packed tile lists unless VH_OPINION_PACKED=0
-    if (mode == 2 && E.plan_packed && env_ll("VH_OPINION_PACKED", 1) != 0) mode = 3;
-    const uint16_t* tsl_arg = mode == 3 ? E.tpk.as<uint16_t>() : E.tslot.as<uint16_t>();
+    const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+    if (!plan) {
+        dim3 g(nlight + nh), b(256);
+        if (f32)
+            hipLaunchKernelGGL(k_opinion<float>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
+                               (const float*)A.rstate(), E.st.as<float>(), (float*)A.wstate(), A.nslots, nrows,
+                               E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh);
+        else
+            hipLaunchKernelGGL(k_opinion<double>, g, b, 0, s.stream, E.off.as<uint32_t>(), E.src.as<uint32_t>(),
+                               (const double*)A.rstate(), E.st.as<float>(), (double*)A.wstate(), A.nslots, nrows,
+                               E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh);
+        HIPCHECK(hipGetLastError());
+        return VH_OK;
+    }
+    RC(E.pval.ensure(E.nedges * tb + 8, s.stream));
+    const uint64_t nb = (E.nedges + 256 * GP_U - 1) / (256 * GP_U);
+    const dim3 gg((uint32_t)((nb + 7) / 8 * 8)), bb(256);
+    // light blocks: 8 * ceil(nlight / 8) (XCD-aware order), heavy blocks a multiple of 8
+    const uint32_t nh8 = (nh + 7) / 8 * 8, nl8 = (nlight + 7) / 8 * 8;
     const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
-    // heavy rows from chunk lists unless VH_OPINION_HEAVY_TILES=0
-    const uint4* hb = mode >= 2 && E.plan_htiles && env_ll("VH_OPINION_HEAVY_TILES", 1) != 0
-                          ? E.hinfo.as<uint4>() : nullptr;
+    const dim3 g2(nh8 + nl8);
     if (f32) {
-        auto k = mode == 3   ? k_opinion<float, 3>
-                 : mode == 2 ? k_opinion<float, 2>
-                 : mode == 1 ? k_opinion<float, 1>
-                             : k_opinion<float, 0>;
-        hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const float*)A.rstate(),
-                           E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(), A.nslots, nrows,
-                           E.rowbase[t], (float)p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot,
-                           (uint32_t)plan, E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), tsl_arg, nblkp, hb,
-                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
-                           E.pfirst.as<uint16_t>(), E.twt.as<float>());
+        hipLaunchKernelGGL(k_plan_gather<float>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(), E.dst.as<uint32_t>(),
+                           (const float*)A.rstate(), E.pval.as<float>(), E.nedges);
+        hipLaunchKernelGGL(k_opinion_plan<float>, g2, bb, 0, s.stream, E.off.as<uint32_t>(),
+                           (const float*)A.rstate(), E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(),
+                           A.nslots, nrows, (float)p.eps, E.hinfo.as<uint4>(), nheavy, nh8, nlight,
+                           E.lbase.as<uint32_t>(), E.tslot.as<uint16_t>(), E.twt.as<float>(), nblkp,
+                           E.hslot.as<uint16_t>(), E.plan_hy);
     } else {
-        auto k = mode == 3   ? k_opinion<double, 3>
-                 : mode == 2 ? k_opinion<double, 2>
-                 : mode == 1 ? k_opinion<double, 1>
-                             : k_opinion<double, 0>;
-        hipLaunchKernelGGL(k, g, b, 0, s.stream, E.off.as<uint32_t>(), src, (const double*)A.rstate(),
-                           E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(), A.nslots, nrows,
-                           E.rowbase[t], p.eps, E.heavy.as<uint32_t>(), nheavy, nh, nlight, hot, (uint32_t)plan,
-                           E.lbase.as<uint32_t>(), E.ty.as<uint32_t>(), tsl_arg, nblkp, hb,
-                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(), E.prun.as<uint32_t>(),
-                           E.pfirst.as<uint16_t>(), E.twt.as<float>());
+        hipLaunchKernelGGL(k_plan_gather<double>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(), E.dst.as<uint32_t>(),
+                           (const double*)A.rstate(), E.pval.as<double>(), E.nedges);
+        hipLaunchKernelGGL(k_opinion_plan<double>, g2, bb, 0, s.stream, E.off.as<uint32_t>(),
+                           (const double*)A.rstate(), E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(),
+                           A.nslots, nrows, p.eps, E.hinfo.as<uint4>(), nheavy, nh8, nlight,
+                           E.lbase.as<uint32_t>(), E.tslot.as<uint16_t>(), E.twt.as<float>(), nblkp,
+                           E.hslot.as<uint16_t>(), E.plan_hy);
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

@@ -3,6 +3,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "vh_common.h"
@@ -59,6 +60,9 @@ struct EdgeStore {
     int32_t target = 0;
     std::vector<uint64_t> h_from, h_to;   // init-phase edges in add_edge! order
     std::vector<uint8_t> h_st;
+    // :SingleEdge init phase: index in h_to of the first edge to each target
+    // (kept across add_edge! calls, cleared at finish_init!)
+    std::unordered_map<uint64_t, size_t> h_first;
     uint64_t rowbase[VH_MAXA + 1] = {0}, nrows[VH_MAXA + 1] = {0};
     uint64_t rows = 0;             // total rows of the CSR (0: never built)
     uint64_t nedges = 0;
@@ -70,17 +74,17 @@ struct EdgeStore {
     DevBuf heavy;                  // uint32 rows with degree > 64
     uint64_t nheavy = 0, bins_version = ~0ull;
     // gather plan of the opinion kernels (build_gather_plan), valid for
-    // `plan_version` with partitions of 2^plan_shift sources
-    DevBuf psrc, pinv, pval;       // uint32 sources in plan order, plan position per CSR slot, states
-    DevBuf lbase, ty, tslot;       // light-edge base per block, tile lists (plan position, slot)
-    DevBuf hbase, hty, hslot;      // the same for the heavy rows (base per heavy row, chunk lists)
+    // `plan_version` with partitions of plan_psize sources and tiles of
+    // plan_tile light slots
+    DevBuf psrc, dst, pval;        // per plan entry: source, position in pval; pval: gathered states
+    DevBuf lbase, tslot, twt;      // light lists: first entry per block, slot and weight per entry
+    DevBuf hbase, hslot;           // heavy rows' chunk lists: first entry per heavy row, slot per entry
     DevBuf hinfo;                  // per heavy row: row, CSR start, degree, hbase (one 16-byte load)
-    DevBuf prun, pfirst, tpk;      // packed tile lists: run tables per block and tile, entries
-    DevBuf twt;                    // edge weight of each packed tile-list entry, in list order
-    bool plan_tiles = false, plan_htiles = false, plan_packed = false;
+    bool plan_ok = false;          // the lists fit their 32-bit sort keys (else the one-pass kernel)
     uint64_t plan_version = ~0ull;
     uint32_t plan_psize = 0;       // sources per partition
-    uint32_t plan_tile = 0;        // light slots per tile of the tile lists
+    uint32_t plan_tile = 0;        // light slots per tile of the lists
+    uint64_t plan_hy = 0;          // position of the heavy rows' first entry in pval
     // out-edges of the SIR exposure pass (build_out_csr), valid for
     // `out_version`; `expo`: one bit per agent, set when it has an infected
     // in-neighbour in this step; `expo_cnt`: infected counts by step parity
@@ -89,8 +93,8 @@ struct EdgeStore {
     // every device buffer of the store (release, accounting)
     template <class F>
     void each_buf(F f) {
-        DevBuf* all[] = {&off, &src, &st, &heavy, &psrc, &pinv, &pval, &lbase, &ty, &tslot, &hbase, &hty,
-                         &hslot, &hinfo, &prun, &pfirst, &tpk, &twt, &out_off, &out_dst, &expo, &expo_cnt};
+        DevBuf* all[] = {&off, &src, &st, &heavy, &psrc, &dst, &pval, &lbase, &tslot, &twt, &hbase, &hslot,
+                         &hinfo, &out_off, &out_dst, &expo, &expo_cnt};
         for (DevBuf* b : all) f(*b);
     }
     bool writeable = false;
@@ -158,6 +162,9 @@ int readback(Sim& s, int n);                           // d_stats -> h_stats, sy
 int ensure_slots(Sim& s, int t, uint64_t slots);
 // degree bins for the opinion kernels
 int build_heavy_bins(Sim& s, EdgeStore& E, int t);
+
+// process-wide kernel options (vh_set_options)
+extern vh_options g_opts;
 
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* where);

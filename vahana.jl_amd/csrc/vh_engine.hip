@@ -16,6 +16,7 @@ namespace vh {
 
 static thread_local char g_err[1024];
 static int g_asserts = 1;
+vh_options g_opts = {16ll << 20, 0, 1, 0, 65536};
 
 int set_error(int code, const char* fmt, ...) {
     va_list ap;
@@ -119,6 +120,20 @@ extern "C" int vh_edgetype_hints(vh_model_t mh, uint8_t et, uint32_t* hints) {
     return VH_OK;
 }
 extern "C" const char* vh_last_error(void) { return g_err; }
+extern "C" int vh_get_options(vh_options* out) {
+    if (!out) return set_error(VH_ERR_ARG, "null argument");
+    *out = g_opts;
+    return VH_OK;
+}
+extern "C" int vh_set_options(const vh_options* in) {
+    if (!in) return set_error(VH_ERR_ARG, "null argument");
+    if (in->opinion_part_bytes < 0) return set_error(VH_ERR_ARG, "opinion_part_bytes must be >= 0");
+    if (in->sir_expose != 0 && in->sir_expose != 1) return set_error(VH_ERR_ARG, "sir_expose must be 0 or 1");
+    if (in->sir_expose_min < 0) return set_error(VH_ERR_ARG, "sir_expose_min must be >= 0");
+    g_opts = *in;
+    g_opts.pad = 0;
+    return VH_OK;
+}
 extern "C" int vh_enable_asserts(int enable) {
     g_asserts = enable ? 1 : 0;
     return VH_OK;
@@ -342,30 +357,38 @@ extern "C" int vh_add_edges(vh_sim_t h, uint8_t et, const uint64_t* from, const 
                 return set_error(VH_ERR_ASSERT, "invalid source id %llu", (unsigned long long)from[i]);
         }
     }
+    if (E.bytes && !states && n)
+        return set_error(VH_ERR_ARG, "add_edge!: the edge type has a state; `states` must not be NULL");
+    size_t old = E.h_to.size();
     // _can_add src/EdgeMethods.jl:257-294: a second, different edge to the
-    // same target of a :SingleEdge type (not :SingleType, not S+I) asserts
+    // same target of a :SingleEdge type (not :SingleType, not S+I) asserts.
+    // E.h_first maps a target to the index of its first edge in add_edge!
+    // order; an index >= old is an entry of this call (from[k - old]).
     if (has(E.hints, VH_HINT_SINGLEEDGE) && !singletype && !(stateless && ignorefrom)) {
-        std::unordered_map<uint64_t, size_t> first;
-        for (size_t k = 0; k < E.h_to.size(); k++) first.emplace(E.h_to[k], k);
+        std::vector<uint64_t> added;   // targets first seen in this call (undone on error)
         for (uint64_t i = 0; i < n; i++) {
-            auto it = first.find(to[i]);
-            if (it != first.end()) {
-                size_t k = it->second;
-                bool same = ignorefrom || E.h_from[k] == from[i];
-                if (!stateless && memcmp(E.h_st.data() + k * E.bytes,
-                                         (const uint8_t*)states + i * E.bytes, E.bytes) != 0)
-                    same = false;
-                if (!same)
+            auto it = E.h_first.find(to[i]);
+            if (it != E.h_first.end()) {
+                const size_t k = it->second;
+                const bool batch = k >= old;
+                const uint64_t kf = batch ? from[k - old] : E.h_from[k];
+                const uint8_t* ks = batch ? (const uint8_t*)states + (k - old) * E.bytes
+                                          : E.h_st.data() + k * E.bytes;
+                bool same = ignorefrom || kf == from[i];
+                if (E.bytes && memcmp(ks, (const uint8_t*)states + i * E.bytes, E.bytes) != 0) same = false;
+                if (!same) {
+                    for (uint64_t tgt : added) E.h_first.erase(tgt);
                     return set_error(VH_ERR_ASSERT,
                                      "An edge has already been added to the agent with the id %llu "
                                      "(and the edgetype hints containing the :SingleEdge hint).",
                                      (unsigned long long)to[i]);
+                }
             } else {
-                first.emplace(to[i], E.h_to.size() + i);
+                E.h_first.emplace(to[i], old + i);
+                added.push_back(to[i]);
             }
         }
     }
-    size_t old = E.h_to.size();
     E.h_to.insert(E.h_to.end(), to, to + n);
     E.h_from.resize(old + n);
     if (!ignorefrom) memcpy(E.h_from.data() + old, from, n * 8);
@@ -1053,6 +1076,7 @@ extern "C" int vh_finish_init(vh_sim_t h) {
         std::vector<uint64_t>().swap(E.h_to);
         std::vector<uint64_t>().swap(E.h_from);
         std::vector<uint8_t>().swap(E.h_st);
+        std::unordered_map<uint64_t, size_t>().swap(E.h_first);
     }
     // degree bins of the neighbour-gathering bodies, so that their first
     // apply! needs no host round trip (rebuilt when the edges change)

@@ -605,15 +605,13 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
     }
     if (!e->table_valid || e->table_block != e->block) RC(build_table(e));
     size_t nm = e->members.size();
-    // exposure pass unless VH_SIR_EXPOSE=0, for ensembles of at least
-    // VH_SIR_EXPOSE_MIN agents (default 65536: below, its two extra launches
-    // per step cost more than the draws it saves); the members' out-edges
-    // are built on the first such step, on the step stream
-    const char* ev = getenv("VH_SIR_EXPOSE");
-    const char* evm = getenv("VH_SIR_EXPOSE_MIN");
+    // exposure pass unless the options turn it off, for ensembles of at
+    // least sir_expose_min agents (default 65536: below, its two extra
+    // launches per step cost more than the draws it saves); the members'
+    // out-edges are built on the first such step, on the step stream
     uint64_t total = 0;
     for (auto& kv : e->members) total += kv.second->n;
-    const bool expose = !(ev && atoi(ev) == 0) && (long long)total >= (evm ? atoll(evm) : 65536ll);
+    const bool expose = g_opts.sir_expose != 0 && (int64_t)total >= g_opts.sir_expose_min;
     if (expose) {
         for (auto& kv : e->members) {
             Member* m = kv.second;

@@ -492,6 +492,17 @@ extern "C" int vh_sim_copy(vh_sim_t h, vh_sim_t* out) {
         F.h_from = E.h_from;
         F.h_to = E.h_to;
         F.h_st = E.h_st;
+        F.h_first = E.h_first;
+        // device-generated in-edges requested before finish_init!, and the
+        // CSR version the stencil fast path keys on
+        F.gen = E.gen;
+        F.gen_mean_deg = E.gen_mean_deg;
+        F.gen_seed = E.gen_seed;
+        F.gen_wseed = E.gen_wseed;
+        F.stencil = E.stencil;
+        F.sspec = E.sspec;
+        F.stencil_version = E.stencil_version;
+        F.version = E.version;
         memcpy(F.rowbase, E.rowbase, sizeof F.rowbase);
         memcpy(F.nrows, E.nrows, sizeof F.nrows);
         F.rows = E.rows;

@@ -43,7 +43,7 @@ def make_requests(count, rate, seed=1, n_min=10_000, n_max=10_000_000, h_min=50,
 
 
 # state vectors from this size on are gathered in two passes by the opinion
-# kernels (the library's default of VH_OPINION_PLAN_BYTES)
+# kernels (the default of the option opinion_plan_bytes)
 OPINION_PLAN_MIN_BYTES = 16 << 20
 
 

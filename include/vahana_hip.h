@@ -154,6 +154,20 @@ typedef struct vh_contact_params {   /* state {UInt8 health, Float32 susceptibil
  * src/Simulation.jl:720-821.  Type lists are bit masks over registered type
  * ids (agent type ids are 1-based like src/ModelTypes.jl:86; bit i <=> id i),
  * except `call`, whose order fixes the side-effect order. */
+/* A field of the body's parameter blob taken from a simulation value (see
+ * vh_sim_set_value) when apply! is enqueued: param(sim, name) and
+ * get_global(sim, name) read inside a transition (src/Simulation.jl:587,
+ * src/Global.jl:10), both constant while a transition runs (set_param! only
+ * before finish_init!, set_global! never inside a transition). The value is
+ * converted to the field's type as Julia's convert does (an integer field
+ * takes only integral values in its range). */
+typedef struct vh_value_bind {
+    uint32_t offset;                 /* byte offset of the field in the blob   */
+    uint8_t  kind;                   /* VH_VALUE_PARAM or VH_VALUE_GLOBAL      */
+    uint8_t  dtype;                  /* enum vh_dtype of the field             */
+    uint16_t slot;
+} vh_value_bind;
+
 typedef struct vh_apply_desc {
     uint32_t transition;             /* enum vh_transition                    */
     uint32_t n_call;
@@ -166,7 +180,8 @@ typedef struct vh_apply_desc {
     int8_t   role_edge[4];           /* body's edge-type parameters  (-1 = unused) */
     const void* params;              /* body parameter blob (copied)           */
     uint32_t params_bytes;
-    uint32_t pad;
+    uint32_t n_bind;                 /* blob fields taken from simulation values */
+    const vh_value_bind* bind;
 } vh_apply_desc;
 
 /* ------------------------------------------------------------- mapreduce
@@ -273,6 +288,20 @@ int vh_stream_destroy(void* hip_stream);
  * on one shared compute stream reads finished ones while the others run;
  * NULL: reads use the compute stream */
 int vh_set_io_stream(vh_sim_t s, void* hip_stream);
+/* param / set_param! (src/Simulation.jl:587-630, register_param!
+ * src/ModelTypes.jl:244-266) and get_global / set_global! / modify_global!
+ * (src/Global.jl:10-59, register_global! src/ModelTypes.jl:268-288) as
+ * numbered scalar slots per simulation; the host layer keeps the names and
+ * non-scalar globals (the time series of push_global!). `value` points to one
+ * value of `dtype` (enum vh_dtype). A parameter can be set only before
+ * finish_init!, as set_param! asserts. */
+#define VH_MAX_VALUES   32
+#define VH_VALUE_PARAM  0
+#define VH_VALUE_GLOBAL 1
+int vh_sim_set_value(vh_sim_t s, int kind, uint32_t slot, int dtype, const void* value);
+/* *dtype receives the stored type, `value` (8 bytes) the value; VH_ERR_ARG
+ * for a slot never set */
+int vh_sim_get_value(vh_sim_t s, int kind, uint32_t slot, int* dtype, void* value);
 /* sim.num_transitions (src/Simulation.jl:473,816) */
 int vh_num_transitions(vh_sim_t s, uint64_t* out);
 /* read_snapshot! src/HDF5.jl:1594-1646 (_read_agents_restore! :801-850), after

@@ -163,6 +163,10 @@ struct or_sim {
     uint64_t num_transitions;
     const vh_apply_desc* d;            /* desc of the running apply! */
     uint8_t params[256];
+    /* param / get_global values: raw 8 bytes, vh_dtype, set mask per kind */
+    uint64_t vals[2][VH_MAX_VALUES];
+    uint8_t vdt[2][VH_MAX_VALUES];
+    uint32_t vset[2];
 };
 
 #define ETH(s, et) ((s)->m->e[et].hints)
@@ -1099,8 +1103,105 @@ static int write_result(or_sim* s, int t, uint64_t idx, int keep, const uint8_t*
 
 static int in_mask(uint32_t mask, int i) { return (mask >> i) & 1u; }
 
-/* apply! src/Simulation.jl:720-821 */
+/* param(sim, name) src/Simulation.jl:587, set_param! :602-625 (asserts
+ * after finish_init!), get_global / set_global! src/Global.jl:10-59, as
+ * numbered slots holding one scalar each */
+static uint32_t dt_bytes(int dt) {
+    switch (dt) {
+    case VH_DT_I64: case VH_DT_F64: case VH_DT_U64: return 8;
+    case VH_DT_F32: case VH_DT_U32: case VH_DT_I32: return 4;
+    case VH_DT_U16: return 2;
+    case VH_DT_BOOL: case VH_DT_U8: return 1;
+    default: return 0;
+    }
+}
+int or_sim_set_value(or_sim* s, int kind, uint32_t slot, int dtype, const void* value) {
+    if (kind != VH_VALUE_PARAM && kind != VH_VALUE_GLOBAL) return fail(VH_ERR_ARG, "kind");
+    if (slot >= VH_MAX_VALUES || !dt_bytes(dtype) || !value) return fail(VH_ERR_ARG, "slot/dtype");
+    if (kind == VH_VALUE_PARAM && s->initialized)
+        return fail(VH_ERR_ASSERT, "You can call `set_param!` only before `finish_init!`.");
+    uint64_t raw = 0;
+    memcpy(&raw, value, dt_bytes(dtype));
+    s->vals[kind][slot] = raw;
+    s->vdt[kind][slot] = (uint8_t)dtype;
+    s->vset[kind] |= 1u << slot;
+    return VH_OK;
+}
+int or_sim_get_value(or_sim* s, int kind, uint32_t slot, int* dtype, void* value) {
+    if ((kind != VH_VALUE_PARAM && kind != VH_VALUE_GLOBAL) || slot >= VH_MAX_VALUES ||
+        !((s->vset[kind] >> slot) & 1u))
+        return fail(VH_ERR_ARG, "value slot not set");
+    if (dtype) *dtype = s->vdt[kind][slot];
+    memcpy(value, &s->vals[kind][slot], 8);
+    return VH_OK;
+}
+/* Julia's convert of a stored scalar to a field type: any number to a float
+ * type; to an integer type only an integral value in its range (else an
+ * InexactError); Bool takes 0 and 1. The value is read as long double, which
+ * holds every Int64/UInt64 and Float64 exactly (x86 80-bit extended). */
+static int convert_to(int from, uint64_t raw, int to, uint8_t* out) {
+    long double v;
+    switch (from) {
+    case VH_DT_F64: { double x; memcpy(&x, &raw, 8); v = x; break; }
+    case VH_DT_F32: { float x; memcpy(&x, &raw, 4); v = x; break; }
+    case VH_DT_I64: { int64_t x; memcpy(&x, &raw, 8); v = (long double)x; break; }
+    case VH_DT_I32: { int32_t x; memcpy(&x, &raw, 4); v = x; break; }
+    case VH_DT_U64: { uint64_t x; memcpy(&x, &raw, 8); v = (long double)x; break; }
+    case VH_DT_U32: { uint32_t x; memcpy(&x, &raw, 4); v = x; break; }
+    case VH_DT_U16: { uint16_t x; memcpy(&x, &raw, 2); v = x; break; }
+    case VH_DT_U8: case VH_DT_BOOL: { uint8_t x; memcpy(&x, &raw, 1); v = x; break; }
+    default: return 0;
+    }
+    if (to == VH_DT_F64) { double x = (double)v; memcpy(out, &x, 8); return 1; }
+    if (to == VH_DT_F32) { float x = (float)v; memcpy(out, &x, 4); return 1; }
+    if (v != floorl(v)) return 0;
+    switch (to) {
+    case VH_DT_I64: if (v < -9223372036854775808.0L || v > 9223372036854775807.0L) return 0;
+                    { int64_t x = (int64_t)v; memcpy(out, &x, 8); } return 1;
+    case VH_DT_U64: if (v < 0 || v > 18446744073709551615.0L) return 0;
+                    { uint64_t x = (uint64_t)v; memcpy(out, &x, 8); } return 1;
+    case VH_DT_I32: if (v < -2147483648.0L || v > 2147483647.0L) return 0;
+                    { int32_t x = (int32_t)v; memcpy(out, &x, 4); } return 1;
+    case VH_DT_U32: if (v < 0 || v > 4294967295.0L) return 0;
+                    { uint32_t x = (uint32_t)v; memcpy(out, &x, 4); } return 1;
+    case VH_DT_U16: if (v < 0 || v > 65535.0L) return 0;
+                    { uint16_t x = (uint16_t)v; memcpy(out, &x, 2); } return 1;
+    case VH_DT_U8: if (v < 0 || v > 255.0L) return 0;
+                   { uint8_t x = (uint8_t)v; memcpy(out, &x, 1); } return 1;
+    case VH_DT_BOOL: if (v < 0 || v > 1.0L) return 0;
+                     { uint8_t x = (uint8_t)v; memcpy(out, &x, 1); } return 1;
+    default: return 0;
+    }
+}
+
+static int or_apply_resolved(or_sim* s, const vh_apply_desc* d);
+
+/* apply! src/Simulation.jl:720-821; blob fields bound to param/global
+ * values are filled in first (the values are constant during a transition) */
 int or_apply(or_sim* s, const vh_apply_desc* d) {
+    if (!d->n_bind) return or_apply_resolved(s, d);
+    uint8_t blob[256];
+    vh_apply_desc r = *d;
+    if (d->params_bytes > sizeof blob || !d->bind) return fail(VH_ERR_ARG, "params/bind");
+    memset(blob, 0, sizeof blob);
+    if (d->params_bytes) memcpy(blob, d->params, d->params_bytes);
+    for (uint32_t k = 0; k < d->n_bind; k++) {
+        const vh_value_bind* b = &d->bind[k];
+        uint32_t sz = dt_bytes(b->dtype);
+        if ((b->kind != VH_VALUE_PARAM && b->kind != VH_VALUE_GLOBAL) || b->slot >= VH_MAX_VALUES || !sz ||
+            (uint64_t)b->offset + sz > d->params_bytes)
+            return fail(VH_ERR_ARG, "invalid value binding");
+        if (!((s->vset[b->kind] >> b->slot) & 1u)) return fail(VH_ERR_ASSERT, "value not set");
+        if (!convert_to(s->vdt[b->kind][b->slot], s->vals[b->kind][b->slot], b->dtype, blob + b->offset))
+            return fail(VH_ERR_ASSERT, "InexactError");
+    }
+    r.params = blob;
+    r.n_bind = 0;
+    r.bind = NULL;
+    return or_apply_resolved(s, &r);
+}
+
+static int or_apply_resolved(or_sim* s, const vh_apply_desc* d) {
     or_model* m = s->m;
     if (!s->initialized) return fail(VH_ERR_ASSERT, "You must call finish_init! before apply!");
     if (d->with_edge >= 0 && HAS(m->e[d->with_edge].hints, VH_HINT_SINGLETYPE))

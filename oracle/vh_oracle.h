@@ -32,6 +32,8 @@ int or_add_edges(or_sim* s, uint8_t et, const uint64_t* from, const uint64_t* to
                  const void* states, uint64_t n);
 int or_finish_init(or_sim* s);
 int or_apply(or_sim* s, const vh_apply_desc* d);
+int or_sim_set_value(or_sim* s, int kind, uint32_t slot, int dtype, const void* value);
+int or_sim_get_value(or_sim* s, int kind, uint32_t slot, int* dtype, void* value);
 int or_num_transitions(or_sim* s, uint64_t* out);
 int or_restore_died(or_sim* s, uint8_t t, const uint8_t* died, uint64_t n);
 int or_set_num_transitions(or_sim* s, uint64_t n);

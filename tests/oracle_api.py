@@ -66,7 +66,7 @@ def create_model(name):
 
 
 def create_simulation(model, params=None, globals_=None, name=None):
-    return OracleSim(model, name)
+    return OracleSim(model, name, params=params, globals_=globals_)
 
 
 # the product's API functions dispatch on the simulation object
@@ -83,5 +83,7 @@ snapshot = vh.snapshot
 for _name in ["add_agent_", "add_agents_", "add_edge_", "add_edges_", "finish_init_", "apply_",
               "apply", "mapreduce", "num_agents", "num_edges", "all_agents", "all_agentids",
               "agentstate", "edges", "neighborids", "edgestates", "has_edge", "copy_simulation",
-              "finish_simulation_", "agent_id", "agent_nr", "type_nr", "Field"]:
+              "finish_simulation_", "agent_id", "agent_nr", "type_nr", "Field", "register_param_",
+              "register_global_", "param", "set_param_", "get_global", "set_global_", "modify_global_",
+              "push_global_", "Record", "param_ref", "global_ref"]:
     globals()[_name] = getattr(vh, _name)

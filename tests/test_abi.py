@@ -29,7 +29,9 @@ def test_library_exports_every_declared_symbol(vh):
 
 def test_struct_layouts_match_header(vh):
     A = vh._abi
-    assert C.sizeof(A.ApplyDesc) == 72
+    assert C.sizeof(A.ApplyDesc) == 80
+    assert C.sizeof(A.ValueBind) == 8
+    assert C.sizeof(A.Options) == 32
     assert C.sizeof(A.MapDesc) == 24
     assert C.sizeof(A.TestParams) == 32
     assert C.sizeof(A.SirParams) == 24

@@ -72,3 +72,14 @@ def test_stateful_edges_need_states(vh, gpu_ready):
     sim2._add_edges_array(E2, [ids2[0], ids2[0]], [ids2[1], ids2[1]], None)
     vh.finish_init_(sim2)
     assert vh.neighborids(sim2, ids2[1], E2) == ids2[0]
+
+
+def test_bound_values_on_device(vh, ora, gpu_ready):
+    """SIR reading p_inf and seed from parameters and the recovery days from
+    a global changed mid-run (vh_value_bind): equal to the literal run on the
+    device, step by step, and to the checker's bound run."""
+    import test_globals as TG
+    print("[gpu] bound params/globals, SIR", flush=True)
+    got = TG.bound_sir_matches_literal(vh)
+    want = TG.bound_sir_matches_literal(ora)
+    assert np.array_equal(got, want)

@@ -27,7 +27,12 @@ __all__ = [
     "num_edges", "all_agents", "all_agentids", "agentstate", "edges", "neighborids",
     "edgestates", "has_edge", "copy_simulation", "finish_simulation_", "agent_id",
     "agent_nr", "type_nr", "register_transition", "enable_asserts", "VahanaError",
+    "register_param_", "register_global_", "param", "set_param_", "get_global", "set_global_",
+    "modify_global_", "push_global_", "param_ref", "global_ref", "get_options", "set_options",
+    "options",
 ]
+param_ref = transitions.param_ref
+global_ref = transitions.global_ref
 
 
 class VahanaError(RuntimeError):
@@ -170,6 +175,8 @@ class ModelTypes:
     def __init__(self):
         self.agents = []       # (StructType, hints)
         self.edges = []        # (StructType, hints, target, size)
+        self.params = []       # (name, default) of register_param!
+        self.globals_ = []     # (name, initial value) of register_global!
 
     def __or__(self, step):
         return step(self)
@@ -203,12 +210,30 @@ def register_edgetype_(T, *hints, target=None, size=0):
     return step
 
 
+def register_param_(name, default):
+    """register_param! src/ModelTypes.jl:260-266 (pipeable)."""
+    def step(types):
+        types.params.append((str(name).lstrip(":"), default))
+        return types
+    return step
+
+
+def register_global_(name, init):
+    """register_global! src/ModelTypes.jl:282-288 (pipeable)."""
+    def step(types):
+        types.globals_.append((str(name).lstrip(":"), init))
+        return types
+    return step
+
+
 class Model:
     """create_model src/Simulation.jl:115-207: registered type ids and hints."""
 
     def __init__(self, types, name, lib):
         self.name = name
         self.lib = lib
+        self.params = list(getattr(types, "params", ()))
+        self.globals_ = list(getattr(types, "globals_", ()))
         self.handle = C.c_void_p()
         _check(lib.model_create(C.byref(self.handle)), lib)
         self.agent_types = []
@@ -284,10 +309,62 @@ def _as_list(x):
 
 
 # ------------------------------------------------------------------ simulation
+class Record:
+    """A mutable struct of named fields: the Params_/Globals_ structs that
+    create_model generates from register_param!/register_global!
+    (src/Simulation.jl:180-200)."""
+
+    def __init__(self, fields):
+        self.__dict__.update(fields)
+
+    def __repr__(self):
+        return "Record(" + ", ".join(f"{k}={v!r}" for k, v in vars(self).items()) + ")"
+
+
+def _fields_of(obj):
+    if obj is None:
+        return {}
+    if isinstance(obj, dict):
+        return dict(obj)
+    if hasattr(obj, "_asdict"):
+        return dict(obj._asdict())
+    return dict(vars(obj))
+
+
+def _getf(obj, name):
+    return obj[name] if isinstance(obj, dict) else getattr(obj, name)
+
+
+def _hasf(obj, name):
+    return name in obj if isinstance(obj, dict) else hasattr(obj, name)
+
+
+def _setf(obj, name, value):
+    if isinstance(obj, dict):
+        obj[name] = value
+    else:
+        setattr(obj, name, value)
+
+
+def _scalar_dtype(v):
+    """(vh_dtype, numpy dtype) of a scalar value, None otherwise."""
+    if isinstance(v, (bool, np.bool_)):
+        return _abi.DT["b1"], np.uint8
+    if isinstance(v, np.unsignedinteger):
+        return _abi.DT["u8"], np.uint64
+    if isinstance(v, (int, np.integer)):
+        return _abi.DT["i8"], np.int64
+    if isinstance(v, np.float32):
+        return _abi.DT["f4"], np.float32
+    if isinstance(v, (float, np.floating)):
+        return _abi.DT["f8"], np.float64
+    return None
+
+
 class Simulation:
     """create_simulation src/Simulation.jl:261-339 on the device backend."""
 
-    def __init__(self, model, name=None, handle=None):
+    def __init__(self, model, name=None, handle=None, params=None, globals_=None):
         self.model = model
         self.lib = model.lib
         self.name = name or model.name
@@ -297,6 +374,53 @@ class Simulation:
         else:
             _check(self.lib.sim_create(model.handle, C.byref(self.handle)), self.lib)
         self.initialized = False
+        self._init_values(params, globals_)
+
+    # -- params / globals (src/Simulation.jl:270-292, src/Global.jl)
+    def _init_values(self, params, globals_):
+        import copy
+        m = self.model
+        if getattr(m, "params", None):
+            assert params is None, ("You can not use `register_param!` and the `param` argument of "
+                                    "`create_simulation` at the same time.")
+            params = Record(copy.deepcopy(dict(m.params)))
+        if getattr(m, "globals_", None):
+            assert globals_ is None, ("You can not use `register_globals!` and the `global` argument "
+                                      "of `create_simulation` at the same time.")
+            globals_ = Record(copy.deepcopy(dict(m.globals_)))
+        self.params, self.globals = params, globals_
+        self.globals_last_change = 0
+        # scalar fields get numbered value slots in the library (the values a
+        # transition body can read); others stay on the host
+        self._slots = ({}, {})
+        for kind, obj in ((_abi.VALUE_PARAM, params), (_abi.VALUE_GLOBAL, globals_)):
+            for k, v in _fields_of(obj).items():
+                if _scalar_dtype(v) is not None and len(self._slots[kind]) < _abi.MAX_VALUES:
+                    self._slots[kind][k] = len(self._slots[kind])
+                    self._push_value(kind, k, v)
+
+    def _push_value(self, kind, name, v):
+        slot = self._slots[kind].get(name)
+        if slot is None:
+            return
+        dt = _scalar_dtype(v)
+        if dt is None:
+            raise TypeError(f"{name} holds a scalar; {type(v).__name__} can not be stored in it")
+        buf = np.array([v], dtype=dt[1])
+        _check(self.lib.sim_set_value(self.handle, kind, slot, dt[0], buf.ctypes.data_as(C.c_void_p)),
+               self.lib)
+
+    def _value(self, kind, name):
+        """The library's copy of a scalar param/global (what bodies read)."""
+        slot = self._slots[kind][name]
+        dt = C.c_int()
+        raw = (C.c_uint8 * 8)()
+        _check(self.lib.sim_get_value(self.handle, kind, slot, C.byref(dt), raw), self.lib)
+        code = {v: k for k, v in _abi.DT.items()}[dt.value]
+        npt = {"b1": np.uint8, "u8": np.uint64, "i8": np.int64, "f4": np.float32, "f8": np.float64,
+               "u4": np.uint32, "i4": np.int32, "u2": np.uint16, "u1": np.uint8}[code]
+        v = np.frombuffer(bytes(raw), dtype=npt, count=1)[0]
+        return bool(v) if code == "b1" else v.item()
 
     # -- type ids
     def _aid(self, T):
@@ -424,7 +548,20 @@ class Simulation:
         keep = C.create_string_buffer(pbuf, max(len(pbuf), 1))
         d.params = C.cast(keep, C.c_void_p)
         d.params_bytes = len(pbuf)
-        return d, keep
+        binds = getattr(tr, "binds", ())
+        barr = None
+        if binds:
+            barr = (_abi.ValueBind * len(binds))()
+            for i, (off, dt, ref) in enumerate(binds):
+                slot = self._slots[ref.kind].get(ref.name)
+                if slot is None:
+                    raise AssertionError(f"{ref!r}: the simulation has no scalar "
+                                         f"{'parameter' if ref.kind == _abi.VALUE_PARAM else 'global'} "
+                                         f"named {ref.name}")
+                barr[i].offset, barr[i].kind, barr[i].dtype, barr[i].slot = off, ref.kind, dt, slot
+            d.n_bind = len(binds)
+            d.bind = C.cast(barr, C.POINTER(_abi.ValueBind))
+        return d, (keep, barr)
 
     def _call_apply(self, d):
         return self.lib.apply(self.handle, C.byref(d))
@@ -505,6 +642,10 @@ class Simulation:
         c = type(self).__new__(type(self))
         c.model, c.lib, c.name, c.handle = self.model, self.lib, self.name, h
         c.initialized = self.initialized
+        import copy
+        c.params, c.globals = copy.deepcopy(self.params), copy.deepcopy(self.globals)
+        c.globals_last_change = self.globals_last_change
+        c._slots = (dict(self._slots[0]), dict(self._slots[1]))
         return c
 
     def _destroy(self):
@@ -520,9 +661,63 @@ class Simulation:
 
 
 def create_simulation(model, params=None, globals_=None, name=None):
-    """create_simulation src/Simulation.jl:261 (params/globals are carried by the
-    transition objects here)."""
-    return Simulation(model, name)
+    """create_simulation src/Simulation.jl:261-339. `params` / `globals_`: an
+    object with fields (or a dict), or None when the model registered them
+    (register_param_ / register_global_); scalar fields are what transition
+    bodies can read (transitions.param_ref / global_ref)."""
+    return Simulation(model, name, params=params, globals_=globals_)
+
+
+def param(sim, name):
+    """param(sim, name) src/Simulation.jl:587."""
+    return _getf(sim.params, str(name).lstrip(":"))
+
+
+def set_param_(*args):
+    """set_param!(sim, name, value) src/Simulation.jl:602-625, or its pipeable
+    form set_param_(name, value)(sim). Only before finish_init!; an unknown
+    name is a warning."""
+    if len(args) == 2:
+        name, value = args
+        return lambda sim: (set_param_(sim, name, value), sim)[1]
+    sim, name, value = args
+    name = str(name).lstrip(":")
+    assert not sim.initialized, ("You can call `set_param!` only before `finish_init!`. For values "
+                                 "that change while the simulation is running, register them as "
+                                 "globals and use `set_global!` instead.")
+    if sim.params is None or not _hasf(sim.params, name):
+        import warnings
+        warnings.warn(f"Parameter {name} is unknown")
+        return
+    _setf(sim.params, name, value)
+    sim._push_value(_abi.VALUE_PARAM, name, value)
+
+
+def get_global(sim, name):
+    """get_global(sim, name) src/Global.jl:10."""
+    return _getf(sim.globals, str(name).lstrip(":"))
+
+
+def set_global_(sim, name, value):
+    """set_global!(sim, name, value) src/Global.jl:26-29 (not inside a
+    transition: here globals change only between apply! calls)."""
+    name = str(name).lstrip(":")
+    sim.globals_last_change = sim._num_transitions() - 1
+    _setf(sim.globals, name, value)
+    sim._push_value(_abi.VALUE_GLOBAL, name, value)
+
+
+def modify_global_(sim, name, f):
+    """modify_global!(sim, name, f) src/Global.jl:53-55."""
+    set_global_(sim, name, f(get_global(sim, name)))
+
+
+def push_global_(sim, name, value):
+    """push_global!(sim, name, value) src/Global.jl:86-89: appends to a
+    vector-valued global (a time series)."""
+    name = str(name).lstrip(":")
+    sim.globals_last_change = sim._num_transitions() - 1
+    _getf(sim.globals, name).append(value)
 
 
 def add_agent_(sim, agent):

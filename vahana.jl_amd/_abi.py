@@ -55,6 +55,15 @@ MAP_FIELD, MAP_FIELD_EQ, MAP_ONE = 0, 1, 2
 S, I, R = 0, 1, 2
 
 
+class ValueBind(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("kind", C.c_uint8), ("dtype", C.c_uint8),
+                ("slot", C.c_uint16)]
+
+
+VALUE_PARAM, VALUE_GLOBAL = 0, 1
+MAX_VALUES = 32
+
+
 class ApplyDesc(C.Structure):
     _fields_ = [("transition", C.c_uint32), ("n_call", C.c_uint32),
                 ("call", C.c_uint8 * MAX_CALL),
@@ -63,7 +72,8 @@ class ApplyDesc(C.Structure):
                 ("add_existing_agents", C.c_uint32), ("add_existing_edges", C.c_uint32),
                 ("with_edge", C.c_int32),
                 ("role_agent", C.c_int8 * 4), ("role_edge", C.c_int8 * 4),
-                ("params", C.c_void_p), ("params_bytes", C.c_uint32), ("pad", C.c_uint32)]
+                ("params", C.c_void_p), ("params_bytes", C.c_uint32), ("n_bind", C.c_uint32),
+                ("bind", C.POINTER(ValueBind))]
 
 
 class Options(C.Structure):
@@ -129,6 +139,8 @@ SIGNATURES = [
     ("finish_init", C.c_int, [_vp]),
     ("apply", C.c_int, [_vp, C.POINTER(ApplyDesc)]),
     ("num_transitions", C.c_int, [_vp, _u64p]),
+    ("sim_set_value", C.c_int, [_vp, C.c_int, C.c_uint32, C.c_int, _vp]),
+    ("sim_get_value", C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_int), _vp]),
     ("num_agents", C.c_int, [_vp, C.c_uint8, _u64p]),
     ("num_edges", C.c_int, [_vp, C.c_uint8, _u64p]),
     ("read_agents", C.c_int, [_vp, C.c_uint8, _vp, _u8p, C.c_uint64, _u64p]),

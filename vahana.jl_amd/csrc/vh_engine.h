@@ -125,6 +125,10 @@ struct Sim {
     DevBuf d_stats;                // small readback area
     uint64_t* h_stats = nullptr;   // pinned mirror of d_stats
     Scratch scratch;
+    // param / get_global values (vh_sim_set_value): raw 8 bytes and vh_dtype
+    uint64_t vals[2][VH_MAX_VALUES] = {};
+    uint8_t vdt[2][VH_MAX_VALUES] = {};
+    uint32_t vset[2] = {0, 0};     // slots set, per kind
 };
 
 // model-body launchers (vh_bodies.hip)

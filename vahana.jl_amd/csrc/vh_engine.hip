@@ -1,6 +1,7 @@
 // vh_engine.hip — the C ABI: model/simulation registry, finish_init!, the
 // phases of apply! (src/Simulation.jl:720-821) as device passes, CSR builds,
 // death/birth bookkeeping and queries.
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -1525,15 +1526,138 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
     return VH_OK;
 }
 
+// ============================================================ params / globals
+static uint32_t dt_size(int dt) {
+    switch (dt) {
+    case VH_DT_I64: case VH_DT_F64: case VH_DT_U64: return 8;
+    case VH_DT_F32: case VH_DT_U32: case VH_DT_I32: return 4;
+    case VH_DT_U16: return 2;
+    case VH_DT_BOOL: case VH_DT_U8: return 1;
+    default: return 0;
+    }
+}
+static bool dt_float(int dt) { return dt == VH_DT_F64 || dt == VH_DT_F32; }
+
+// the stored value (raw 8 bytes of type `from`) converted to `to` and
+// written to `out`; false when Julia's convert would throw (a non-integral
+// or out-of-range value for an integer field)
+static bool convert_value(int from, uint64_t raw, int to, void* out) {
+    double f = 0.0;
+    int64_t i = 0;
+    uint64_t u = 0;
+    bool isf = dt_float(from), neg = false;
+    switch (from) {
+    case VH_DT_F64: memcpy(&f, &raw, 8); break;
+    case VH_DT_F32: { float g; memcpy(&g, &raw, 4); f = g; break; }
+    case VH_DT_I64: memcpy(&i, &raw, 8); neg = i < 0; u = (uint64_t)i; break;
+    case VH_DT_I32: { int32_t v; memcpy(&v, &raw, 4); i = v; neg = v < 0; u = (uint64_t)i; break; }
+    case VH_DT_U64: memcpy(&u, &raw, 8); i = (int64_t)u; break;
+    case VH_DT_U32: { uint32_t v; memcpy(&v, &raw, 4); u = v; i = v; break; }
+    case VH_DT_U16: { uint16_t v; memcpy(&v, &raw, 2); u = v; i = v; break; }
+    case VH_DT_U8: case VH_DT_BOOL: { uint8_t v; memcpy(&v, &raw, 1); u = v; i = v; break; }
+    default: return false;
+    }
+    if (dt_float(to)) {
+        const double v = isf ? f : (neg ? (double)i : (double)u);
+        if (to == VH_DT_F64) memcpy(out, &v, 8);
+        else { const float g = (float)v; memcpy(out, &g, 4); }
+        return true;
+    }
+    // integer target: the value as a sign and magnitude
+    if (isf) {
+        if (!(f == floor(f)) || f < -9.2233720368547758e18 || f >= 1.8446744073709552e19) return false;
+        neg = f < 0;
+        if (neg) i = (int64_t)f; else u = (uint64_t)f;
+        if (!neg) i = (int64_t)u;
+    }
+    auto fits = [&](int64_t lo, uint64_t hi) { return neg ? (lo < 0 && i >= lo) : u <= hi; };
+    switch (to) {
+    case VH_DT_I64: if (!fits(INT64_MIN, INT64_MAX)) return false; { int64_t v = neg ? i : (int64_t)u; memcpy(out, &v, 8); } return true;
+    case VH_DT_U64: if (!fits(0, UINT64_MAX)) return false; memcpy(out, &u, 8); return true;
+    case VH_DT_I32: if (!fits(INT32_MIN, INT32_MAX)) return false; { int32_t v = (int32_t)(neg ? i : (int64_t)u); memcpy(out, &v, 4); } return true;
+    case VH_DT_U32: if (!fits(0, UINT32_MAX)) return false; { uint32_t v = (uint32_t)u; memcpy(out, &v, 4); } return true;
+    case VH_DT_U16: if (!fits(0, UINT16_MAX)) return false; { uint16_t v = (uint16_t)u; memcpy(out, &v, 2); } return true;
+    case VH_DT_U8: if (!fits(0, UINT8_MAX)) return false; { uint8_t v = (uint8_t)u; memcpy(out, &v, 1); } return true;
+    case VH_DT_BOOL: if (!fits(0, 1)) return false; { uint8_t v = (uint8_t)u; memcpy(out, &v, 1); } return true;
+    default: return false;
+    }
+}
+
+extern "C" int vh_sim_set_value(vh_sim_t h, int kind, uint32_t slot, int dtype, const void* value) {
+    if (!h || !value) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (kind != VH_VALUE_PARAM && kind != VH_VALUE_GLOBAL) return set_error(VH_ERR_ARG, "kind must be 0 or 1");
+    if (slot >= VH_MAX_VALUES) return set_error(VH_ERR_ARG, "slot %u >= %d", slot, VH_MAX_VALUES);
+    const uint32_t sz = dt_size(dtype);
+    if (!sz) return set_error(VH_ERR_ARG, "unknown dtype %d", dtype);
+    if (kind == VH_VALUE_PARAM && s.initialized)
+        return set_error(VH_ERR_ASSERT,
+                         "You can call `set_param!` only before `finish_init!`. For values that change "
+                         "while the simulation is running, register them as globals and use "
+                         "`set_global!` instead.");
+    uint64_t raw = 0;
+    memcpy(&raw, value, sz);
+    s.vals[kind][slot] = raw;
+    s.vdt[kind][slot] = (uint8_t)dtype;
+    s.vset[kind] |= 1u << slot;
+    return VH_OK;
+}
+
+extern "C" int vh_sim_get_value(vh_sim_t h, int kind, uint32_t slot, int* dtype, void* value) {
+    if (!h || !value) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if ((kind != VH_VALUE_PARAM && kind != VH_VALUE_GLOBAL) || slot >= VH_MAX_VALUES ||
+        !((s.vset[kind] >> slot) & 1u))
+        return set_error(VH_ERR_ARG, "value slot %u of kind %d is not set", slot, kind);
+    if (dtype) *dtype = s.vdt[kind][slot];
+    memcpy(value, &s.vals[kind][slot], 8);
+    return VH_OK;
+}
+
+// the desc with its bound blob fields filled in from the simulation's values
+// (`blob` holds the copy the returned desc points to)
+static int resolve_binds(const Sim& s, const vh_apply_desc* d, vh_apply_desc& out, uint8_t* blob) {
+    out = *d;
+    if (!d->n_bind) return VH_OK;
+    if (!d->bind) return set_error(VH_ERR_ARG, "n_bind > 0 without bind");
+    if (d->params_bytes > VH_PARAMS_BYTES || (d->params_bytes && !d->params))
+        return set_error(VH_ERR_ARG, "params too large");
+    memset(blob, 0, VH_PARAMS_BYTES);
+    if (d->params_bytes) memcpy(blob, d->params, d->params_bytes);
+    for (uint32_t k = 0; k < d->n_bind; k++) {
+        const vh_value_bind& b = d->bind[k];
+        const uint32_t sz = dt_size(b.dtype);
+        if ((b.kind != VH_VALUE_PARAM && b.kind != VH_VALUE_GLOBAL) || b.slot >= VH_MAX_VALUES || !sz ||
+            (uint64_t)b.offset + sz > d->params_bytes)
+            return set_error(VH_ERR_ARG, "invalid value binding %u", k);
+        if (!((s.vset[b.kind] >> b.slot) & 1u))
+            return set_error(VH_ERR_ASSERT, "the %s of slot %u is not set", b.kind ? "global" : "parameter",
+                             (unsigned)b.slot);
+        if (!convert_value(s.vdt[b.kind][b.slot], s.vals[b.kind][b.slot], b.dtype, blob + b.offset))
+            return set_error(VH_ERR_ASSERT, "InexactError: the %s of slot %u does not convert to the field's type",
+                             b.kind ? "global" : "parameter", (unsigned)b.slot);
+    }
+    out.params = blob;
+    out.n_bind = 0;
+    out.bind = nullptr;
+    return VH_OK;
+}
+
 extern "C" int vh_apply(vh_sim_t h, const vh_apply_desc* d) {
     if (!h || !d) return set_error(VH_ERR_ARG, "null argument");
-    RC(apply_one(h->s, d));
+    vh_apply_desc r;
+    uint8_t blob[VH_PARAMS_BYTES];
+    RC(resolve_binds(h->s, d, r, blob));
+    RC(apply_one(h->s, &r));
     HIPCHECK(hipEventRecord(h->s.last, h->s.stream));
     return VH_OK;
 }
 extern "C" int vh_apply_n(vh_sim_t h, const vh_apply_desc* d, uint32_t steps) {
     if (!h || !d) return set_error(VH_ERR_ARG, "null argument");
-    for (uint32_t i = 0; i < steps; i++) RC(apply_one(h->s, d));
+    vh_apply_desc r;
+    uint8_t blob[VH_PARAMS_BYTES];
+    RC(resolve_binds(h->s, d, r, blob));
+    for (uint32_t i = 0; i < steps; i++) RC(apply_one(h->s, &r));
     HIPCHECK(hipEventRecord(h->s.last, h->s.stream));
     return VH_OK;
 }

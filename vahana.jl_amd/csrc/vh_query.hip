@@ -486,6 +486,10 @@ extern "C" int vh_sim_copy(vh_sim_t h, vh_sim_t* out) {
         RC(copy_buf(B.died_now, A.died_now, A.cap, d.stream));
         RC(copy_buf(B.reuse, A.reuse, A.nreuse * 4, d.stream));
     }
+    memcpy(d.vals, s.vals, sizeof d.vals);
+    memcpy(d.vdt, s.vdt, sizeof d.vdt);
+    d.vset[0] = s.vset[0];
+    d.vset[1] = s.vset[1];
     for (int et = 0; et < s.m->ne; et++) {
         EdgeStore& E = s.e[et];
         EdgeStore& F = d.e[et];

@@ -9,13 +9,55 @@ import ctypes as C
 from . import _abi
 
 
+class ValueRef:
+    """param(sim, name) (a parameter) or get_global(sim, name) (a global)
+    read by a transition body: the parameter-blob field that receives it is
+    filled from the simulation's value when apply! is enqueued
+    (vh_value_bind). Both are constant while a transition runs."""
+
+    def __init__(self, kind, name):
+        self.kind = kind
+        self.name = str(name).lstrip(":")
+
+    def __repr__(self):
+        return f"{'param' if self.kind == _abi.VALUE_PARAM else 'get_global'}(sim, :{self.name})"
+
+
+def param_ref(name):
+    return ValueRef(_abi.VALUE_PARAM, name)
+
+
+def global_ref(name):
+    return ValueRef(_abi.VALUE_GLOBAL, name)
+
+
+# vh_dtype of the ctypes field types of the parameter blobs
+_CT_DT = {C.c_double: _abi.DT["f8"], C.c_float: _abi.DT["f4"], C.c_int64: _abi.DT["i8"],
+          C.c_uint64: _abi.DT["u8"], C.c_uint32: _abi.DT["u4"], C.c_int32: _abi.DT["i4"],
+          C.c_uint16: _abi.DT["u2"], C.c_uint8: _abi.DT["u1"]}
+
+
+def _set_fields(p, **fields):
+    """Set the blob fields; a ValueRef leaves the field 0 and becomes a
+    binding (byte offset, field dtype, reference)."""
+    binds = []
+    ftypes = dict(p._fields_)
+    for k, v in fields.items():
+        if isinstance(v, ValueRef):
+            binds.append((getattr(type(p), k).offset, _CT_DT[ftypes[k]], v))
+        else:
+            setattr(p, k, float(v) if ftypes[k] in (C.c_double, C.c_float) else int(v))
+    return binds
+
+
 class Transition:
-    def __init__(self, id_, name, role_agents=(), role_edges=(), params=None):
+    def __init__(self, id_, name, role_agents=(), role_edges=(), params=None, binds=()):
         self.id = id_
         self.name = name
         self.role_agents = list(role_agents)
         self.role_edges = list(role_edges)
         self.params = params
+        self.binds = list(binds)   # (offset, dtype, ValueRef)
 
     def params_bytes(self):
         if self.params is None:
@@ -47,9 +89,11 @@ def sum_neighbors(edgetype):
 
 def die_if(cmp, value=0, edgetype=None):
     """`nothing` (death) when the predicate on the Int64 field `foo` holds
-    (cmp: always, never, lt, ge, eq, odd, even, num_edges_eq)."""
-    return Transition(_abi.TR_DIE_IF, f"die_if_{cmp}", role_edges=[edgetype],
-                      params=_tp(value, cmp))
+    (cmp: always, never, lt, ge, eq, odd, even, num_edges_eq); `value` may be
+    a param_ref / global_ref."""
+    p = _tp(0, cmp)
+    binds = _set_fields(p, value=value)
+    return Transition(_abi.TR_DIE_IF, f"die_if_{cmp}", role_edges=[edgetype], params=p, binds=binds)
 
 
 def add_self_edge(edgetype, state=None):
@@ -98,23 +142,30 @@ def count_edges(edgetype):
 
 
 def sir(edgetype, p_inf, seed, recovery=7):
-    """SIR contagion over Stateless in-edges."""
-    p = _abi.SirParams(p_inf=float(p_inf), seed=int(seed), recovery=int(recovery))
-    return Transition(_abi.TR_SIR, "sir", role_edges=[edgetype], params=p)
+    """SIR contagion over Stateless in-edges (any argument may be a
+    param_ref / global_ref)."""
+    p = _abi.SirParams()
+    binds = _set_fields(p, p_inf=p_inf, seed=seed, recovery=recovery)
+    return Transition(_abi.TR_SIR, "sir", role_edges=[edgetype], params=p, binds=binds)
 
 
 def opinion(edgetype, eps, f32=False):
-    """Weighted bounded confidence (Float64 opinions, or Float32 with f32=True)."""
-    p = _abi.OpinionParams(eps=float(eps))
+    """Weighted bounded confidence (Float64 opinions, or Float32 with f32=True);
+    `eps` may be a param_ref / global_ref."""
+    p = _abi.OpinionParams()
+    binds = _set_fields(p, eps=eps)
     return Transition(_abi.TR_OPINION_F32 if f32 else _abi.TR_OPINION_F64,
-                      "opinion_f32" if f32 else "opinion_f64", role_edges=[edgetype], params=p)
+                      "opinion_f32" if f32 else "opinion_f64", role_edges=[edgetype], params=p,
+                      binds=binds)
 
 
 def contact(edgetype, beta, gamma, p_death, seed, n_contacts=20, max_redraw=16):
-    """Dynamic contact network with deaths, 1:1 births and rebuilt contacts."""
-    p = _abi.ContactParams(beta=float(beta), gamma=float(gamma), p_death=float(p_death),
-                           seed=int(seed), n_contacts=int(n_contacts), max_redraw=int(max_redraw))
-    return Transition(_abi.TR_CONTACT, "contact", role_edges=[edgetype], params=p)
+    """Dynamic contact network with deaths, 1:1 births and rebuilt contacts
+    (any argument may be a param_ref / global_ref)."""
+    p = _abi.ContactParams()
+    binds = _set_fields(p, beta=beta, gamma=gamma, p_death=p_death, seed=seed, n_contacts=n_contacts,
+                        max_redraw=max_redraw)
+    return Transition(_abi.TR_CONTACT, "contact", role_edges=[edgetype], params=p, binds=binds)
 
 
 def diffuse(edgetype, T, field="active"):

@@ -39,24 +39,28 @@ def sir_bytes(n, e):
     return n * (4 + 4) + (n + 1) * 4 + e * 4
 
 
-def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0, relabel=True):
+def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0, relabel=True, n=None, edges=None, steps=5,
+                warmup=1):
     """The checker's apply! (a restatement of the reference's data structures)
-    over a bounded sample of the workload: same model, 10^6 agents (2*10^5
-    for the contact model); median of up to 5 steps after a warm-up."""
+    over the workload: `n` agents (default: 10^6; 2*10^5 for the contact
+    model), median of up to `steps` steps after `warmup` warm-up steps, at
+    most `seconds_cap` seconds of timed steps. `edges`: the opinion graph to
+    use (otherwise generated from `seed`)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as ora
     from __graft_entry__ import load_package
     vh = load_package()
     M = vh.models
-    n = 1_000_000
+    n = n or 1_000_000
     if workload.startswith("opinion"):
-        a, b = vh.inputs.graph_ba(n, 5, seed)
-        edges = vh.inputs.symmetric_weighted(a, b, seed + 1)
+        if edges is None:
+            a, b = vh.inputs.graph_ba(n, 5, seed)
+            edges = vh.inputs.symmetric_weighted(a, b, seed + 1)
         m = M.build_opinion(ora, n, seed=seed, f32=f32, edges=edges, relabel=relabel)
         step = lambda: M.step_opinion(ora, m, eps=0.2)  # noqa: E731
         ne = len(edges[0])
     elif workload == "contact":
-        n = 200_000
+        n = min(n, 200_000)
         m = M.build_contact(ora, n, seed=seed)
         step = lambda: M.step_contact(ora, m, seed=seed)  # noqa: E731
         ne = 20 * n
@@ -70,10 +74,11 @@ def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0, relabel=True):
         m = M.build_sir(ora, n, seed=seed, edges=edges)
         step = lambda: M.step_sir(ora, m)  # noqa: E731
         ne = len(edges[0])
-    step()   # warm-up
+    for _ in range(warmup):
+        step()
     times = []
     t_all = time.perf_counter()
-    while len(times) < 5 and time.perf_counter() - t_all < seconds_cap:
+    while len(times) < steps and time.perf_counter() - t_all < seconds_cap:
         t0 = time.perf_counter()
         step()
         times.append(time.perf_counter() - t0)
@@ -81,23 +86,29 @@ def _cpu_sample(workload, f32, seed=1, seconds_cap=25.0, relabel=True):
     return n, ne, times
 
 
-def cpu_baseline(workload, f32, relabel=True):
-    """Setting (i): one host core (the same agent numbering as the GPU run)."""
-    n, ne, times = _cpu_sample(workload, f32, relabel=relabel)
+def cpu_baseline(workload, f32, relabel=True, n=None, edges=None):
+    """Setting (i): one host core (the same agent numbering as the GPU run).
+    At 10^7 opinion agents (the headline workload itself, same graph): the
+    median of 2 steps, no warm-up (a step takes seconds; the checker has no
+    JIT)."""
+    big = n is not None and n >= 5_000_000
+    n_, ne, times = _cpu_sample(workload, f32, relabel=relabel, n=n, edges=edges,
+                                steps=2 if big else 5, warmup=0 if big else 1,
+                                seconds_cap=40.0 if big else 25.0)
     med = times[len(times) // 2]
-    return {"value": n / med, "unit": "agent-updates/s", "cores": 1, "kind": "port",
+    return {"value": n_ / med, "unit": "agent-updates/s", "cores": 1, "kind": "port",
             "edges_per_s": ne / med,
-            "sample": f"{workload} N={n} E={ne}, median of {len(times)} apply! steps after 1 warm-up, "
-                      "1 host core"}
+            "sample": f"{workload} N={n_} E={ne}, median of {len(times)} apply! steps"
+                      f"{'' if big else ' after 1 warm-up'}, 1 host core"}
 
 
-def cpu_baseline_all_cores(workload, f32, relabel=True):
+def cpu_baseline_all_cores(workload, f32, relabel=True, n=None):
     """Setting (ii): the host cores as P independent processes, each owning a
-    contiguous range of agents (its own sample of the workload, rank-seeded;
-    the exchange between ranks is not modelled), run at the same time. The
-    aggregate rate is all ranks' agents over the slowest rank's median step,
-    as ranks advance in lockstep. P = the CPUs this process may use, at most
-    15. The workers make no GPU calls."""
+    contiguous share of the agents (N / P of them: its own rank-seeded graph
+    of that size; the exchange between ranks is not modelled), run at the
+    same time. The aggregate rate is all ranks' agents over the slowest
+    rank's median step, as ranks advance in lockstep. P = the CPUs this
+    process may use, at most 15. The workers make no GPU calls."""
     import subprocess
     try:
         ncpu = len(os.sched_getaffinity(0))
@@ -106,6 +117,8 @@ def cpu_baseline_all_cores(workload, f32, relabel=True):
     P = max(1, min(ncpu, 15))
     env = dict(os.environ, ROCR_VISIBLE_DEVICES="", HIP_VISIBLE_DEVICES="", OMP_NUM_THREADS="1")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--workload", workload]
+    if n:
+        cmd += ["--n", str(max(1000, n // P))]
     if not relabel:
         cmd.append("--no-relabel")
     procs = [subprocess.Popen(cmd + ["--rank", str(r)], stdout=subprocess.PIPE, env=env)
@@ -117,17 +130,18 @@ def cpu_baseline_all_cores(workload, f32, relabel=True):
             raise RuntimeError(f"cpu worker failed ({pr.returncode})")
         res.append(json.loads(out.decode().strip().splitlines()[-1]))
     slow = max(r["median_s"] for r in res)
-    n = sum(r["n"] for r in res)
+    n_all = sum(r["n"] for r in res)
     ne = sum(r["edges"] for r in res)
-    return {"value": n / slow, "unit": "agent-updates/s", "cores": P, "kind": "port",
+    return {"value": n_all / slow, "unit": "agent-updates/s", "cores": P, "kind": "port",
             "edges_per_s": ne / slow,
-            "sample": f"{workload}: {P} processes x N={res[0]['n']} (rank-seeded samples, no halo "
-                      f"exchange), slowest rank's median apply! step {slow:.3f} s"}
+            "sample": f"{workload}: {P} processes x N={res[0]['n']} (N={n_all} in all; rank-seeded "
+                      f"graphs, no halo exchange), slowest rank's median apply! step {slow:.3f} s"}
 
 
 def cpu_worker(args):
     f32 = args.workload == "opinion32"
-    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank, relabel=not args.no_relabel)
+    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank, relabel=not args.no_relabel,
+                               n=args.n if args.n != 10_000_000 else None)
     print(json.dumps({"n": n, "edges": ne, "median_s": times[len(times) // 2]}), flush=True)
 
 
@@ -148,6 +162,8 @@ def main():
     ap.add_argument("--no-relabel", action="store_true",
                     help="opinion: keep the generation order of the agents (default: renumbered "
                          "breadth-first at finish_init!)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="opinion: skip the second timing in generation order")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--rank", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calib", action="store_true",
@@ -221,13 +237,12 @@ def main():
         nbytes = sir_bytes(n, m.nedges)
         dtype = "u8"
         workload = f"SIR contagion, directed G(n,p), N={n}, E={m.nedges}, p_inf=0.05"
-    del edges
+    if not args.workload.startswith("opinion"):
+        del edges
     log(f"init {time.perf_counter() - t0:.1f}s: {workload}")
 
     stream = torch.cuda.Stream(device=dev)
     L = vh._abi.lib()
-    L.set_stream(m.sim.handle, stream.cuda_stream)
-    vh.enable_asserts(False)       # production mode (enable_asserts(false) in the reference)
 
     if args.calib:
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
@@ -235,30 +250,50 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        starts[k].record(stream)
-        step()
-        ends[k].record(stream)
-    L.sync(m.sim.handle)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
-    ev_mean = sum(ev) / len(ev)
-    ms = wall * 1e3 / args.steps
-    value = n / (wall / args.steps)
-    edges_s = m.nedges / (wall / args.steps)
+
+    def timed(sim, step):
+        """W warm-up steps, then K steps between two device synchronisations,
+        with HIP events on the simulation's stream around every step: (wall
+        ms per step, mean event ms, sorted event ms)."""
+        L.set_stream(sim.handle, stream.cuda_stream)
+        vh.enable_asserts(False)   # production mode (enable_asserts(false) in the reference)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            starts[k].record(stream)
+            step()
+            ends[k].record(stream)
+        L.sync(sim.handle)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ev = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
+        vh.enable_asserts(True)
+        assert sim._num_transitions() == 1 + args.warmup + args.steps
+        return wall * 1e3 / args.steps, sum(ev) / len(ev), ev
+
+    ms, ev_mean, ev = timed(m.sim, step)
+    value = n / (ms * 1e-3)
+    edges_s = m.nedges / (ms * 1e-3)
     achieved = nbytes / (ev_mean * 1e-3) / 1e9
     log(f"step wall {ms:.3f} ms, kernel events mean {ev_mean:.3f} ms (min {ev[0]:.3f} max {ev[-1]:.3f})")
-    vh.enable_asserts(True)
-    num_trans = m.sim._num_transitions()
-    assert num_trans == 1 + args.warmup + args.steps
+
+    gen_order = None
+    if args.workload.startswith("opinion") and not args.no_relabel and not args.no_compare:
+        # the same graph and states in generation order (no renumbering at
+        # finish_init!), timed the same way
+        x_relabel = m
+        m2 = M.build_opinion(vh, n, seed=1, f32=f32, edges=edges, relabel=False)
+        ms2, ev2, _ = timed(m2.sim, lambda: M.step_opinion(vh, m2, eps=0.2))
+        gen_order = {"ms_per_step": ms2, "kernel_ms": ev2, "value": n / (ms2 * 1e-3),
+                     "roofline_frac": nbytes / (ev2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"generation order: step wall {ms2:.3f} ms, kernel events mean {ev2:.3f} ms")
+        m2.sim._destroy()
+        del m2, x_relabel
 
     traffic = None
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
@@ -277,11 +312,7 @@ def main():
     drift = None
     if f32:
         # the Float32 variant against the Float64 run of the same inputs
-        a, b = vh.inputs.graph_ba(n, 5, 1)
-        edges = vh.inputs.symmetric_weighted(a, b, 2)
-        del a, b
         m64 = M.build_opinion(vh, n, seed=1, f32=False, edges=edges, relabel=not args.no_relabel)
-        del edges
         for _ in range(1 + args.warmup + args.steps - 1):
             M.step_opinion(vh, m64, eps=0.2)
         x64 = np.asarray(m64.sim._read_agents(m64.T)[0]["opinion"])
@@ -294,11 +325,19 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        # opinion: the checker on the headline workload itself (same N, same
+        # graph); other workloads: their bounded samples
+        opin = args.workload.startswith("opinion")
+        m.sim._destroy()
         t0 = time.perf_counter()
-        cpu = cpu_baseline(args.workload, f32, relabel=not args.no_relabel)
+        cpu = cpu_baseline(args.workload, f32, relabel=not args.no_relabel, n=n if opin else None,
+                           edges=edges if opin else None)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {cpu}")
+        if opin:
+            del edges
         t0 = time.perf_counter()
-        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32, relabel=not args.no_relabel)
+        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32, relabel=not args.no_relabel,
+                                                  n=n if opin else None)
         log(f"cpu baseline, all cores {time.perf_counter() - t0:.1f}s: {cpu['all_cores']}")
 
     out = {
@@ -321,6 +360,8 @@ def main():
                      "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},
         "cpu_baseline": cpu,
     }
+    if gen_order is not None:
+        out["generation_order"] = gen_order
     if drift is not None:
         out["f32_drift_vs_f64"] = drift
     print(json.dumps(out), flush=True)

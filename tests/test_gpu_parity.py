@@ -65,8 +65,8 @@ def test_sir_parity(vh, ora, gpu_ready, expose, opts):
 # neighbour-state gather (options opinion_plan_bytes, opinion_part_bytes):
 # one pass; two passes through a plan with 2 KiB source partitions (16 for
 # Float64, 8 for Float32 states), with 4 KiB partitions, with the default
-# 2 MiB (one partition at this size), and with one source per partition
-# (4000 partitions: a two-digit plan sort, one-entry runs)
+# 2 MiB (one partition at this size), and with 1-byte partitions requested
+# (widened to the most the lists allow: 32 partitions of 125 agents)
 GATHER_MODES = {"direct": (-1, 0), "plan": (0, 2048), "plan_4k": (0, 4096), "plan_auto": (0, 0),
                 "plan_1src": (0, 1)}
 

@@ -716,20 +716,16 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
 // ----------------------------------------------------------------- gather plan
 // For a large state vector the neighbour states of a static edge store are
 // gathered in two passes. The plan lists the CSR slots sorted stably by
-// source partition (`psize` consecutive sources, sized so that one XCD's L2
-// holds a partition's states), light rows' slots before heavy rows' within a
-// partition. Pass 1 (k_plan_gather) walks the plan, each XCD one contiguous
-// eighth of it (so its gathers hit the partitions its L2 holds), and stores
-// y[dst[j]] = x[psrc[j] - 1]. dst puts the values in list order: the light
-// in-edges of a 256-agent block are numbered from lbase[block] in agent order,
-// in tiles of TILE slots, and the entries of a tile are listed by (source
-// partition, slot), so pass 1 writes a tile's values as one run per
-// partition; the heavy rows' chunk lists follow from position `hy` on. Pass 2
-// (k_opinion_plan) reads y, each entry's slot in its tile (tslot) and weight
-// (twt) as three sequential streams, one tile ahead, stages value and weight
-// by slot in LDS, and every thread adds its own agent's slots. The values are
-// exactly x[src[k]] and the sums keep their canonical order, so the results
-// are the one-pass kernel's.
+// source partition (`psize` consecutive sources, at most 32 partitions, sized
+// so that one XCD's L2 holds a partition's states), light rows' slots before
+// heavy rows' within a partition. Pass 1 (k_plan_gather) walks the plan, each
+// XCD one contiguous eighth of it (its gathers hit the partitions its L2
+// holds), and writes y[j] = x[psrc[j] - 1] sequentially. Within a partition
+// the plan keeps CSR order, so the light in-edges of a tile of consecutive
+// agents are one run of y per partition. Pass 2 (k_opinion_plan) reads those
+// runs through per-tile run tables and adds each agent's terms in canonical
+// order; the values are exactly x[src[k]], so the results are the one-pass
+// kernel's.
 constexpr int GP_U = 4;   // plan entries per thread in pass 1 (measured: 4 beats 1, 2, 8, 16)
 
 template <class T>
@@ -739,13 +735,24 @@ struct TileCfg {
     static constexpr int TILES = (64 * 256 + TILE - 1) / TILE;   // tiles of a light block at most
 };
 constexpr int OP_TILE_BITS = 4;   // bits of a tile number within a light block (list sort keys)
+constexpr int OPP_RUNS = 32;      // partitions at most (5 bits of a list entry beside the 11-bit slot)
 
-// LDS of a pass-2 block; a heavy block gives each of its 4 waves a quarter of x
+// Light lists: the light in-edges of a 256-agent block are numbered from
+// lbase[block] in agent order, in tiles of TILE slots; every tile lists its
+// entries by (partition, slot). Entry e: tpk[e] = slot | partition << 11 and
+// twt[e] its weight. Its value is y[prun[t][p] + (e - pfirst[t][p])] for its
+// tile t and partition p (the tile's run table). Per tile: the owners mark
+// their slots, the entries' terms are computed in parallel and staged in LDS
+// by slot, then each thread adds its own agent's slots in order.
 template <class T>
 struct PlanLDS {
-    T x[TileCfg<T>::TILE];       // neighbour state by slot
-    float w[TileCfg<T>::TILE];   // edge weight by slot
+    T num[TileCfg<T>::TILE];     // numerator term by slot (heavy waves: a quarter each for x_j)
+    float den[TileCfg<T>::TILE]; // weight term by slot
+    uint8_t own[TileCfg<T>::TILE];
+    T xi[256];                   // agent states
     uint32_t wsum[4];
+    uint32_t prun[OPP_RUNS];     // run table of the tile
+    uint16_t pfirst[OPP_RUNS];
 };
 
 template <class T>
@@ -753,9 +760,11 @@ __device__ __forceinline__ void opinion_block_plan(PlanLDS<T>& sm, const uint32_
                                                    const T* __restrict__ x, const T* __restrict__ y,
                                                    T* __restrict__ xo, uint64_t n, uint64_t nrows, T eps,
                                                    uint32_t blk, const uint32_t* __restrict__ lbase,
-                                                   const uint16_t* __restrict__ tslot,
-                                                   const float* __restrict__ twt, uint32_t nblkp) {
-    constexpr uint32_t TL = TileCfg<T>::TILE;
+                                                   const uint16_t* __restrict__ tpk,
+                                                   const float* __restrict__ twt, uint32_t nblkp,
+                                                   const uint32_t* __restrict__ prun,
+                                                   const uint16_t* __restrict__ pfirst) {
+    constexpr uint32_t TL = TileCfg<T>::TILE, NT = TileCfg<T>::TILES;
     constexpr int QU = TL / 256;   // entries of a tile per thread
     const int tid = threadIdx.x;
     const uint64_t i = (uint64_t)blk * 256 + tid;
@@ -765,43 +774,48 @@ __device__ __forceinline__ void opinion_block_plan(PlanLDS<T>& sm, const uint32_
         xi = __builtin_nontemporal_load(x + i);
         if (i < nrows) d = __builtin_nontemporal_load(off + i + 1) - __builtin_nontemporal_load(off + i);
     }
-    const uint32_t lb = blk < nblkp ? lbase[blk] : 0u;
-    const uint32_t le = blk < nblkp ? lbase[blk + 1] : 0u;
-    // entries e = tid + 256 u of the next tile: slot (0xFFFFFFFF past the
-    // tile), weight and neighbour state, fetched one tile ahead (the first
-    // tile's during the block scan)
-    uint32_t nsl[QU];
+    const bool inblk = blk < nblkp;
+    const uint32_t lb = inblk ? lbase[blk] : 0u;
+    const uint32_t le = inblk ? lbase[blk + 1] : 0u;
+    // list entries e = tid + 256 u of the next tile (packed entry, weight),
+    // fetched one tile ahead (the first during the block scan); 0xFFFF past
+    // the tile
+    uint32_t npk[QU];
     float nw[QU];
-    T nv[QU];
     {
         const uint32_t c0 = (le - lb) < TL ? (le - lb) : TL;
 #pragma unroll
         for (int u = 0; u < QU; u++) {
             const uint32_t q = (uint32_t)(tid + 256 * u);
-            nsl[u] = 0xFFFFFFFFu;
+            npk[u] = 0xFFFFu;
             nw[u] = 0.0f;
-            nv[u] = (T)0;
             if (q < c0) {
-                nsl[u] = __builtin_nontemporal_load(tslot + lb + q);
+                npk[u] = __builtin_nontemporal_load(tpk + lb + q);
                 nw[u] = __builtin_nontemporal_load(twt + lb + q);
-                nv[u] = __builtin_nontemporal_load(y + lb + q);
             }
         }
+    }
+    uint32_t nx_run = 0, nx_first = 0;   // run table entry `tid` of the next tile
+    if (tid < OPP_RUNS && inblk) {
+        nx_run = prun[(uint64_t)blk * NT * OPP_RUNS + tid];
+        nx_first = pfirst[(uint64_t)blk * NT * OPP_RUNS + tid];
     }
     const bool light = i < n && d <= 64u;
     const uint32_t ld = light ? d : 0u;
     uint32_t total;
     const uint32_t P = op_light_scan(ld, sm.wsum, total);
+    sm.xi[tid] = xi;
     T num = (T)0, den = (T)0;
     for (uint32_t t0 = 0; t0 < total; t0 += TL) {
-        uint32_t sl[QU];
+        const uint32_t lo = P > t0 ? P : t0;
+        const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
+        const uint32_t cnt = (total - t0) < TL ? (total - t0) : TL;
+        uint32_t pk[QU];
         float wq[QU];
-        T v[QU];
 #pragma unroll
         for (int u = 0; u < QU; u++) {
-            sl[u] = nsl[u];
+            pk[u] = npk[u];
             wq[u] = nw[u];
-            v[u] = nv[u];
         }
         {
             const uint32_t t1 = t0 + TL;
@@ -809,51 +823,82 @@ __device__ __forceinline__ void opinion_block_plan(PlanLDS<T>& sm, const uint32_
 #pragma unroll
             for (int u = 0; u < QU; u++) {
                 const uint32_t q = (uint32_t)(tid + 256 * u);
-                nsl[u] = 0xFFFFFFFFu;
+                npk[u] = 0xFFFFu;
                 if (q < c1) {
-                    nsl[u] = __builtin_nontemporal_load(tslot + lb + t1 + q);
+                    npk[u] = __builtin_nontemporal_load(tpk + lb + t1 + q);
                     nw[u] = __builtin_nontemporal_load(twt + lb + t1 + q);
-                    nv[u] = __builtin_nontemporal_load(y + lb + t1 + q);
                 }
             }
         }
-#pragma unroll
-        for (int u = 0; u < QU; u++) {
-            if (sl[u] != 0xFFFFFFFFu) {
-                sm.x[sl[u]] = v[u];
-                sm.w[sl[u]] = wq[u];
+        for (uint32_t q = lo; q < hi; q++) sm.own[q - t0] = (uint8_t)tid;
+        if (tid < OPP_RUNS) {
+            sm.prun[tid] = nx_run;
+            sm.pfirst[tid] = (uint16_t)nx_first;
+            if (t0 + TL < total) {
+                const uint64_t g = (uint64_t)blk * NT + t0 / TL + 1;
+                nx_run = prun[g * OPP_RUNS + tid];
+                nx_first = pfirst[g * OPP_RUNS + tid];
             }
         }
         __syncthreads();
-        // this thread's slots in order, four LDS reads issued together
-        const uint32_t lo = P > t0 ? P : t0;
-        const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
+        // every entry's plan position first (LDS only), then the loads, then
+        // each entry's terms at its slot
+        uint32_t jj[QU];
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            const uint32_t q = (uint32_t)(tid + 256 * u);
+            const uint32_t pp = (pk[u] >> 11) & 31u;
+            jj[u] = q < cnt ? sm.prun[pp] + (q - sm.pfirst[pp]) : 0u;
+        }
+        T v[QU];
+#pragma unroll
+        for (int u = 0; u < QU; u++) v[u] = pk[u] != 0xFFFFu ? y[jj[u]] : (T)0;
+#pragma unroll
+        for (int u = 0; u < QU; u++) {
+            if (pk[u] != 0xFFFFu) {
+                const uint32_t sl = pk[u] & 2047u;
+                const bool acc = fabs(v[u] - sm.xi[sm.own[sl]]) <= eps;
+                sm.num[sl] = acc ? (T)wq[u] * v[u] : (T)0;
+                sm.den[sl] = acc ? wq[u] : 0.0f;
+            }
+        }
+        __syncthreads();
+        // this thread's slots in order; four LDS reads issued together
         uint32_t q = lo;
         for (; q + 4 <= hi; q += 4) {
-            const T a0 = sm.x[q - t0], a1 = sm.x[q + 1 - t0], a2 = sm.x[q + 2 - t0], a3 = sm.x[q + 3 - t0];
-            const float w0 = sm.w[q - t0], w1 = sm.w[q + 1 - t0], w2 = sm.w[q + 2 - t0], w3 = sm.w[q + 3 - t0];
-            op_accum(a0, w0, xi, eps, num, den);
-            op_accum(a1, w1, xi, eps, num, den);
-            op_accum(a2, w2, xi, eps, num, den);
-            op_accum(a3, w3, xi, eps, num, den);
+            const T a0 = sm.num[q - t0], a1 = sm.num[q + 1 - t0], a2 = sm.num[q + 2 - t0],
+                    a3 = sm.num[q + 3 - t0];
+            const float d0 = sm.den[q - t0], d1 = sm.den[q + 1 - t0], d2 = sm.den[q + 2 - t0],
+                        d3 = sm.den[q + 3 - t0];
+            num = num + a0;
+            num = num + a1;
+            num = num + a2;
+            num = num + a3;
+            den = den + (T)d0;
+            den = den + (T)d1;
+            den = den + (T)d2;
+            den = den + (T)d3;
         }
-        for (; q < hi; q++) op_accum(sm.x[q - t0], sm.w[q - t0], xi, eps, num, den);
+        for (; q < hi; q++) {
+            num = num + sm.num[q - t0];
+            den = den + (T)sm.den[q - t0];
+        }
         __syncthreads();
     }
     if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
 }
 
-// A heavy agent (one wave): its in-edges are numbered from hi.w in CSR order,
-// in chunks of CHUNK slots, each chunk listed by (source partition, slot)
-// (hslot: slot in the chunk); its values sit at y[hy + list position]. The
-// wave copies a chunk's values into its LDS quarter `buf` by slot, then lane l
-// adds slots l, l + 64, ... in order: the canonical lane-strided partials,
-// then the xor butterfly.
+// A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
+// h are numbered from hbase[h] in CSR order, in chunks of CHUNK slots, each
+// chunk listed in plan order (hty: plan position, hslot: slot in the chunk).
+// The wave copies a chunk's values into its LDS quarter `buf` by slot, then
+// lane l adds slots l, l + 64, ... in order: the canonical lane-strided
+// partials, then the xor butterfly.
 template <class T>
 __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, const T* __restrict__ y,
                                                    const float* __restrict__ w, T* __restrict__ xo,
-                                                   const uint4 hi, T eps, const uint16_t* __restrict__ hslot,
-                                                   uint64_t hy, T* buf) {
+                                                   const uint4 hi, T eps, const uint32_t* __restrict__ hty,
+                                                   const uint16_t* __restrict__ hslot, T* buf) {
     constexpr uint32_t CH = TileCfg<T>::CHUNK;
     constexpr int HU = CH / 64;
     const uint32_t lane = threadIdx.x & 63;
@@ -865,20 +910,22 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
     for (uint32_t c0 = 0; c0 < d; c0 += CH) {
         const uint32_t cnt = d - c0 < CH ? d - c0 : CH;
         float wq[HU];
-        uint32_t sl[HU];
+        uint32_t jj[HU], sl[HU];
         T v[HU];
 #pragma unroll
         for (int u = 0; u < HU; u++) {
             const uint32_t q = lane + 64u * (uint32_t)u;
             wq[u] = 0.0f;
+            jj[u] = 0u;
             sl[u] = 0xFFFFFFFFu;
-            v[u] = (T)0;
             if (q < cnt) {
                 wq[u] = __builtin_nontemporal_load(w + b + c0 + q);
+                jj[u] = __builtin_nontemporal_load(hty + hb + c0 + q);
                 sl[u] = __builtin_nontemporal_load(hslot + hb + c0 + q);
-                v[u] = __builtin_nontemporal_load(y + hy + hb + c0 + q);
             }
         }
+#pragma unroll
+        for (int u = 0; u < HU; u++) v[u] = sl[u] != 0xFFFFFFFFu ? y[jj[u]] : (T)0;
 #pragma unroll
         for (int u = 0; u < HU; u++)
             if (sl[u] != 0xFFFFFFFFu) buf[sl[u]] = v[u];
@@ -902,10 +949,11 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
     if (lane == 0) xo[r] = den > (T)0 ? num / den : xi;
 }
 
-// Pass 2: blocks [0, nh) run one wave per heavy agent, the others are light
-// blocks. Light blocks of one XCD take one contiguous range of agents (blocks
-// are dispatched round-robin over the 8 XCDs; the grid holds
-// 8 * ceil(nlight / 8) light blocks).
+// Pass 2: blocks [0, nh) run one wave per heavy agent (the longest work
+// starts first), the others are light blocks. Light blocks of one XCD take
+// one contiguous range of agents (blocks are dispatched round-robin over the
+// 8 XCDs; the grid holds 8 * ceil(nlight / 8) light blocks), so that they
+// read y as a few sequential streams.
 template <class T>
 __global__ __launch_bounds__(256) void k_opinion_plan(const uint32_t* __restrict__ off,
                                                       const T* __restrict__ x, const T* __restrict__ y,
@@ -914,9 +962,12 @@ __global__ __launch_bounds__(256) void k_opinion_plan(const uint32_t* __restrict
                                                       const uint4* __restrict__ hinfo, uint64_t nheavy,
                                                       uint32_t nh, uint32_t nlight,
                                                       const uint32_t* __restrict__ lbase,
-                                                      const uint16_t* __restrict__ tslot,
+                                                      const uint16_t* __restrict__ tpk,
                                                       const float* __restrict__ twt, uint32_t nblkp,
-                                                      const uint16_t* __restrict__ hslot, uint64_t hy) {
+                                                      const uint32_t* __restrict__ prun,
+                                                      const uint16_t* __restrict__ pfirst,
+                                                      const uint32_t* __restrict__ hty,
+                                                      const uint16_t* __restrict__ hslot) {
     __shared__ __align__(16) unsigned char smraw[sizeof(PlanLDS<T>)];
     PlanLDS<T>& sm = *reinterpret_cast<PlanLDS<T>*>(smraw);
     if (blockIdx.x >= nh) {
@@ -924,43 +975,60 @@ __global__ __launch_bounds__(256) void k_opinion_plan(const uint32_t* __restrict
         const uint32_t per = (nlight + 7) / 8;
         l = (l % 8) * per + l / 8;
         if (l >= nlight) return;   // whole block
-        opinion_block_plan<T>(sm, off, x, y, xo, n, nrows, eps, l, lbase, tslot, twt, nblkp);
+        opinion_block_plan<T>(sm, off, x, y, xo, n, nrows, eps, l, lbase, tpk, twt, nblkp, prun, pfirst);
     } else {
         const uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
         if (wv < nheavy)
-            opinion_heavy_plan<T>(x, y, w, xo, hinfo[wv], eps, hslot, hy,
-                                  sm.x + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
+            opinion_heavy_plan<T>(x, y, w, xo, hinfo[wv], eps, hty, hslot,
+                                  sm.num + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
     }
 }
 
 // Pass 1: grid 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 GP_U));
-// logical block l takes plan entries [l * 1024, (l + 1) * 1024), thread t the
-// entries l * 1024 + t + 256 u, u < GP_U: four source and destination loads,
-// four gathers, four stores.
+// logical block l takes plan entries [l * 1024, (l + 1) * 1024). Float64:
+// thread t the entries l * 1024 + t + 256 u, u < 4 (four 4-byte source
+// loads, four gathers, four 8-byte stores; measured faster than a contiguous
+// 2 x 2 layout, 470 against 536 us at 10^8 entries). Float32: the 4
+// consecutive entries from l * 1024 + 4 t (one 16-byte load of their
+// sources, four gathers, one 16-byte store; 0.731 against 0.739 ms per step
+// with the Float64 layout). The buffers are 256-byte aligned; a partial group
+// at the end goes entry by entry.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t pu32x4 __attribute__((ext_vector_type(4)));
+static_assert(GP_U == 4, "k_plan_gather takes four plan entries per thread");
 template <class T>
-__global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc,
-                                                     const uint32_t* __restrict__ dst, const T* __restrict__ x,
+__global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc, const T* __restrict__ x,
                                                      T* __restrict__ y, uint64_t ne) {
     const uint32_t per = gridDim.x / 8;
     const uint32_t l = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    const uint64_t j0 = (uint64_t)l * (256 * GP_U) + threadIdx.x;
-    uint32_t sj[GP_U], dj[GP_U];
-    T v[GP_U];
+    const uint64_t base = (uint64_t)l * (256 * GP_U);
+    if constexpr (sizeof(T) == 4) {
+        const uint64_t j0 = base + (uint64_t)threadIdx.x * 4;
+        if (j0 >= ne) return;
+        if (j0 + 4 <= ne) {
+            const pu32x4 sv = __builtin_nontemporal_load((const pu32x4*)(psrc + j0));
+            const f32x4 v = {(float)x[sv.x - 1], (float)x[sv.y - 1], (float)x[sv.z - 1], (float)x[sv.w - 1]};
+            __builtin_nontemporal_store(v, (f32x4*)(y + j0));
+        } else {
+            for (uint64_t j = j0; j < ne; j++) y[j] = x[psrc[j] - 1];
+        }
+    } else {
+        const uint64_t j0 = base + threadIdx.x;
+        uint32_t sj[GP_U];
+        T v[GP_U];
 #pragma unroll
-    for (int u = 0; u < GP_U; u++) {
-        const uint64_t j = j0 + (uint64_t)u * 256;
-        sj[u] = 0u;
-        dj[u] = 0u;
-        if (j < ne) {
-            sj[u] = __builtin_nontemporal_load(psrc + j);
-            dj[u] = __builtin_nontemporal_load(dst + j);
+        for (int u = 0; u < GP_U; u++) {
+            const uint64_t j = j0 + (uint64_t)u * 256;
+            sj[u] = j < ne ? __builtin_nontemporal_load(psrc + j) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
+#pragma unroll
+        for (int u = 0; u < GP_U; u++) {
+            const uint64_t j = j0 + (uint64_t)u * 256;
+            if (j < ne) __builtin_nontemporal_store(v[u], y + j);
         }
     }
-#pragma unroll
-    for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
-#pragma unroll
-    for (int u = 0; u < GP_U; u++)
-        if (sj[u] && dj[u] < ne) __builtin_nontemporal_store(v[u], y + dj[u]);
 }
 
 // plan sort key of each CSR slot: source partition, then 1 for the slots of
@@ -1024,19 +1092,41 @@ __global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__
     }
 }
 
-// list entry e (sorted light or heavy lists, positions from `base` in y):
-// its slot, its weight (light lists; 0 without 4-byte edge states) and, for
-// its plan entry, the destination base + e
-__global__ void k_list_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ kidx,
-                            const uint16_t* __restrict__ tsl, const uint32_t* __restrict__ pinv, uint64_t nl,
-                            uint64_t base, uint16_t* __restrict__ slot, const float* __restrict__ w,
-                            float* __restrict__ lw, uint32_t* __restrict__ dst) {
+// packed light lists: per entry of the sorted lists, slot | partition << 11
+// and its weight (0 without 4-byte edge states); for tile t of block b
+// and each partition p present, the plan position of its run's first entry
+// (prun[(b * ntiles + t) * OPP_RUNS + p]) and that entry's index in the tile
+// (pfirst[...])
+__global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+                            const uint32_t* __restrict__ kidx, const uint16_t* __restrict__ tsl,
+                            const uint32_t* __restrict__ pinv,
+                            const uint32_t* __restrict__ lbase, uint32_t pbits, uint32_t tile, uint32_t ntiles,
+                            uint64_t nl, uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
+                            uint16_t* __restrict__ pfirst, const float* __restrict__ w, float* __restrict__ twt) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nl) return;
-    const uint32_t idx = order[e], k = kidx[idx];
-    slot[e] = tsl[idx];
-    if (lw) lw[e] = w ? w[k] : 0.0f;
-    dst[pinv[k]] = (uint32_t)(base + e);
+    const uint32_t key = keys[e], p = key & ((1u << pbits) - 1u);
+    const uint32_t blk = key >> (OP_TILE_BITS + pbits), t = (key >> pbits) & ((1u << OP_TILE_BITS) - 1u);
+    const uint32_t t0 = lbase[blk] + t * tile;
+    const uint32_t idx = order[e];
+    tpk[e] = (uint16_t)(tsl[idx] | (p << 11));
+    twt[e] = w ? w[kidx[idx]] : 0.0f;
+    if (e == t0 || keys[e - 1] != key) {
+        const uint64_t g = (uint64_t)blk * ntiles + t;
+        prun[g * OPP_RUNS + p] = pinv[kidx[idx]];
+        pfirst[g * OPP_RUNS + p] = (uint16_t)(e - t0);
+    }
+}
+
+// heavy chunk lists: plan position and chunk slot per entry
+__global__ void k_heavy_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ kidx,
+                             const uint16_t* __restrict__ tsl, const uint32_t* __restrict__ pinv, uint64_t nl,
+                             uint32_t* __restrict__ hty, uint16_t* __restrict__ hslot) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nl) return;
+    const uint32_t idx = order[e];
+    hty[e] = pinv[kidx[idx]];
+    hslot[e] = tsl[idx];
 }
 
 // in-edge count and chunk count of each heavy row
@@ -1536,10 +1626,10 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
 }
 
 // The gather plan of the in-edges of E (one target type at row 0, sources
-// 1-based numbers <= nsrc < 2^32) and its lists: psrc and dst in plan
-// order, the light lists (lbase, tslot, twt) and the heavy rows' chunk lists
-// (hbase, hslot, hinfo). E.plan_ok stays false when a sort key would need
-// more than 32 bits (the one-pass kernel runs instead).
+// 1-based numbers <= nsrc < 2^32, at most OPP_RUNS partitions) and its lists:
+// psrc in plan order; the light lists (lbase, tpk, twt) and their run tables
+// (prun, pfirst); the heavy rows' chunk lists (hbase, hty, hslot, hinfo). E.plan_ok stays false
+// when a sort key would need more than 32 bits (the one-pass kernel runs).
 int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint32_t tile) {
     if (E.plan_version == E.version && E.plan_psize == psize && E.plan_tile == tile) return VH_OK;
     hipStream_t st = s.stream;
@@ -1550,18 +1640,17 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
     const uint64_t ne = E.nedges, nrows = E.rows;
     const uint64_t nparts = (nsrc + psize - 1) / psize;
     const uint32_t nblk = (uint32_t)((nrows + 255) / 256);
-    const uint32_t chunk = tile / 4;
+    const uint32_t chunk = tile / 4, ntiles = (64 * 256 + tile - 1) / tile;
     uint32_t pbits = 1, bbits = 1;
     while ((1ull << pbits) < nparts) pbits++;
     while ((1ull << bbits) < nblk) bbits++;
-    if (!ne || ne >= 0xFFFFFFFFull || bbits + OP_TILE_BITS + pbits > 32 ||
-        (uint32_t)((64 * 256 + tile - 1) / tile) > (1u << OP_TILE_BITS))
+    if (!ne || ne >= 0xFFFFFFFFull || nparts > (uint64_t)OPP_RUNS || bbits + OP_TILE_BITS + pbits > 32 ||
+        ntiles > (1u << OP_TILE_BITS))
         return VH_OK;
     const uint32_t* off = E.off.as<uint32_t>();
     const uint32_t* src = E.src.as<uint32_t>();
     // plan order: stable sort of the CSR slots by (partition, heavy row)
     RC(E.psrc.ensure(ne * 4 + 8, st));
-    RC(E.dst.ensure(ne * 4 + 8, st));
     uint32_t* pinv = (uint32_t*)s.scratch.take(ne * 4 + 4);
     {
         uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4);
@@ -1578,7 +1667,7 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
     }
     // light lists: the light edges of each 256-row block numbered from
     // lbase[block], sorted by (block, tile, partition) (stable: slot order
-    // within a partition)
+    // within a partition); the blocks' degree orders
     RC(E.lbase.ensure(((uint64_t)nblk + 1) * 4 + 8, st));
     uint32_t* tot = (uint32_t*)s.scratch.take((uint64_t)nblk * 4 + 4);
     if (!tot) return set_error(VH_ERR_OOM, "scratch");
@@ -1587,28 +1676,30 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
     uint32_t nl = 0;
     HIPCHECK(hipMemcpyAsync(&nl, E.lbase.as<uint32_t>() + nblk, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    RC(E.tslot.ensure((uint64_t)nl * 2 + 8, st));
+    const uint64_t nt = (uint64_t)nblk * ntiles * OPP_RUNS;
+    RC(E.tpk.ensure((uint64_t)nl * 2 + 8, st));
     RC(E.twt.ensure((uint64_t)nl * 4 + 8, st));
+    RC(E.prun.ensure(nt * 4 + 8, st));
+    RC(E.pfirst.ensure(nt * 2 + 8, st));
     const float* w4 = E.bytes == 4 ? E.st.as<float>() : nullptr;
     if (nl) {
-        uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
-        uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
-        uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
-        uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
-        uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nl * 4);
-        uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2);
+        uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
+        uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
+        uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
+        uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
+        uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
+        uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2 + 4);
         if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
         hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, src, nrows, E.lbase.as<uint32_t>(),
                            psize, pbits, tile, keys, kidx, tsl);
         if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + OP_TILE_BITS + pbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "tile sort");
-        hipLaunchKernelGGL(k_list_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, vals, kidx, tsl, pinv,
-                           (uint64_t)nl, (uint64_t)0, E.tslot.as<uint16_t>(), w4, E.twt.as<float>(),
-                           E.dst.as<uint32_t>());
+        hipLaunchKernelGGL(k_pack_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, tsl, pinv,
+                           E.lbase.as<uint32_t>(), pbits, tile, ntiles, (uint64_t)nl, E.tpk.as<uint16_t>(),
+                           E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(), w4, E.twt.as<float>());
         HIPCHECK(hipGetLastError());
     }
-    E.plan_hy = nl;
-    // heavy rows (E.heavy, ascending): chunk lists from nl on
+    // heavy rows (E.heavy, ascending): chunk lists
     const uint64_t nheavy = E.nheavy;
     if (nheavy) {
         uint32_t* hd = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
@@ -1629,6 +1720,7 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
         uint32_t cbits = 1;
         while ((1ull << cbits) < nch) cbits++;
         if (cbits + pbits > 32 || (uint64_t)nl + nhe != ne) return VH_OK;
+        RC(E.hty.ensure((uint64_t)nhe * 4 + 8, st));
         RC(E.hslot.ensure((uint64_t)nhe * 2 + 8, st));
         uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
         uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
@@ -1642,9 +1734,8 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
                            keys, kidx, tsl);
         if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "chunk sort");
-        hipLaunchKernelGGL(k_list_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl, pinv,
-                           (uint64_t)nhe, (uint64_t)nl, E.hslot.as<uint16_t>(), nullptr, nullptr,
-                           E.dst.as<uint32_t>());
+        hipLaunchKernelGGL(k_heavy_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl, pinv,
+                           (uint64_t)nhe, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
         RC(E.hinfo.ensure(nheavy * 16 + 16, st));
         hipLaunchKernelGGL(k_heavy_info, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off,
                            E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), E.hinfo.as<uint4>());
@@ -1671,7 +1762,7 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     // two-pass gather through a plan from a state vector of at least
     // opinion_plan_bytes (< 0: never), with source partitions of
     // opinion_part_bytes (0: 2 MiB, widened up to 1.5x to make at most 32
-    // partitions: fewer partitions give longer runs per tile)
+    // partitions), never more than 32 partitions
     const int64_t plan_min = g_opts.opinion_plan_bytes;
     bool plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
                 E.rowbase[t] == 0 && nrows == E.rows && A.nslots < 0xFFFFFFFFull &&
@@ -1681,9 +1772,11 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
         const bool auto_part = g_opts.opinion_part_bytes == 0;
         uint64_t psize = (auto_part ? (2ull << 20) : (uint64_t)g_opts.opinion_part_bytes) / tb;
         if (psize < 1) psize = 1;
-        if (psize > 0x7FFFFFFFull) psize = 0x7FFFFFFFull;
         const uint64_t np = (A.nslots + psize - 1) / psize;
-        if (auto_part && np > 32 && np * 2 <= 32 * 3) psize = (A.nslots + 31) / 32;
+        if (auto_part && np > OPP_RUNS && np * 2 <= OPP_RUNS * 3) psize = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;
+        const uint64_t pmin = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;   // at most OPP_RUNS partitions
+        if (psize < pmin) psize = pmin;
+        if (psize > 0x7FFFFFFFull) psize = 0x7FFFFFFFull;
         RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots, tile));
         plan = E.plan_ok;
     }
@@ -1709,21 +1802,23 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
     const dim3 g2(nh8 + nl8);
     if (f32) {
-        hipLaunchKernelGGL(k_plan_gather<float>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(), E.dst.as<uint32_t>(),
+        hipLaunchKernelGGL(k_plan_gather<float>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
                            (const float*)A.rstate(), E.pval.as<float>(), E.nedges);
         hipLaunchKernelGGL(k_opinion_plan<float>, g2, bb, 0, s.stream, E.off.as<uint32_t>(),
                            (const float*)A.rstate(), E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(),
                            A.nslots, nrows, (float)p.eps, E.hinfo.as<uint4>(), nheavy, nh8, nlight,
-                           E.lbase.as<uint32_t>(), E.tslot.as<uint16_t>(), E.twt.as<float>(), nblkp,
-                           E.hslot.as<uint16_t>(), E.plan_hy);
+                           E.lbase.as<uint32_t>(), E.tpk.as<uint16_t>(), E.twt.as<float>(), nblkp,
+                           E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
+                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
     } else {
-        hipLaunchKernelGGL(k_plan_gather<double>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(), E.dst.as<uint32_t>(),
+        hipLaunchKernelGGL(k_plan_gather<double>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
                            (const double*)A.rstate(), E.pval.as<double>(), E.nedges);
         hipLaunchKernelGGL(k_opinion_plan<double>, g2, bb, 0, s.stream, E.off.as<uint32_t>(),
                            (const double*)A.rstate(), E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(),
                            A.nslots, nrows, p.eps, E.hinfo.as<uint4>(), nheavy, nh8, nlight,
-                           E.lbase.as<uint32_t>(), E.tslot.as<uint16_t>(), E.twt.as<float>(), nblkp,
-                           E.hslot.as<uint16_t>(), E.plan_hy);
+                           E.lbase.as<uint32_t>(), E.tpk.as<uint16_t>(), E.twt.as<float>(), nblkp,
+                           E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
+                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

@@ -76,15 +76,15 @@ struct EdgeStore {
     // gather plan of the opinion kernels (build_gather_plan), valid for
     // `plan_version` with partitions of plan_psize sources and tiles of
     // plan_tile light slots
-    DevBuf psrc, dst, pval;        // per plan entry: source, position in pval; pval: gathered states
-    DevBuf lbase, tslot, twt;      // light lists: first entry per block, slot and weight per entry
-    DevBuf hbase, hslot;           // heavy rows' chunk lists: first entry per heavy row, slot per entry
+    DevBuf psrc, pval;             // per plan entry: source; gathered states
+    DevBuf lbase, tpk, twt;        // light lists: first entry per block; packed entry, weight
+    DevBuf prun, pfirst;           // run table per block and tile: plan position, list index
+    DevBuf hbase, hty, hslot;      // heavy rows' chunk lists: first entry per row; plan position, slot
     DevBuf hinfo;                  // per heavy row: row, CSR start, degree, hbase (one 16-byte load)
-    bool plan_ok = false;          // the lists fit their 32-bit sort keys (else the one-pass kernel)
+    bool plan_ok = false;          // the lists fit their sort keys (else the one-pass kernel)
     uint64_t plan_version = ~0ull;
     uint32_t plan_psize = 0;       // sources per partition
     uint32_t plan_tile = 0;        // light slots per tile of the lists
-    uint64_t plan_hy = 0;          // position of the heavy rows' first entry in pval
     // out-edges of the SIR exposure pass (build_out_csr), valid for
     // `out_version`; `expo`: one bit per agent, set when it has an infected
     // in-neighbour in this step; `expo_cnt`: infected counts by step parity
@@ -93,8 +93,8 @@ struct EdgeStore {
     // every device buffer of the store (release, accounting)
     template <class F>
     void each_buf(F f) {
-        DevBuf* all[] = {&off, &src, &st, &heavy, &psrc, &dst, &pval, &lbase, &tslot, &twt, &hbase, &hslot,
-                         &hinfo, &out_off, &out_dst, &expo, &expo_cnt};
+        DevBuf* all[] = {&off,   &src, &st,  &heavy,  &psrc,    &pval,    &lbase, &tpk,  &twt,     &prun,
+                         &pfirst, &hbase, &hty, &hslot, &hinfo, &out_off, &out_dst, &expo, &expo_cnt};
         for (DevBuf* b : all) f(*b);
     }
     bool writeable = false;

@@ -314,6 +314,15 @@ int vh_set_num_transitions(vh_sim_t sim, uint64_t n);
 /* mapreduce(sim, f, op, T; init, datatype) — kind 0: agents, 1: edges */
 int vh_mapreduce(vh_sim_t s, int kind, uint8_t type_id, const vh_map_desc* f,
                  int op, int dtype, const void* init, void* out);
+/* vh_mapreduce's reduction without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (tight
+ * serving loops, code under a HIP graph capture, harnesses that forbid
+ * hipMalloc). Same arguments and results as vh_mapreduce; the block partials
+ * live in a fixed library array of (2^17 + 256) 8-byte words per GPU, which
+ * limits a call to 2^28 records (VH_ERR_CAPACITY beyond). Calls that overlap
+ * on one GPU share that array and race. */
+int vh_mapreduce_noalloc(vh_sim_t s, int kind, uint8_t type_id, const vh_map_desc* f,
+                         int op, int dtype, const void* init, void* out);
 /* num_agents src/Agent.jl:324-343 */
 int vh_num_agents(vh_sim_t s, uint8_t type_id, uint64_t* out);
 /* num_edges(sim, T) src/Edge.jl:373-389 */

@@ -110,6 +110,7 @@ struct EdgeStore {
 
 struct Sim {
     Model* m = nullptr;
+    int device = 0;                // the device current at vh_sim_create
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipStream_t io = nullptr;      // reads, when set (vh_set_io_stream)

@@ -160,7 +160,12 @@ extern "C" int vh_sim_create(vh_model_t mh, vh_sim_t* out) {
     vh_sim* h = new vh_sim();
     Sim& s = h->s;
     s.m = &mh->m;
-    hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&s.device);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipGetDevice");
+    }
+    e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete h;
         return hip_fail(e, "hipStreamCreate");

@@ -2,6 +2,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "vh_engine.h"
@@ -282,6 +283,113 @@ extern "C" int vh_mapreduce(vh_sim_t h, int kind, uint8_t t, const vh_map_desc* 
         RC(reduce_levels(s, o, nb, op, dtype, &tot));
     }
     uint64_t r = h_op(op, dtype, tot, acc);
+    memcpy(out, &r, 8);
+    return VH_OK;
+}
+
+// ========================================== mapreduce without allocation
+// vh_mapreduce_noalloc: vh_mapreduce's kernels and reduction tree with the
+// block partials in `scratch`, a file-scope device array, instead of the
+// simulation's scratch arena (which grows with hipMalloc). Level 1 holds at
+// most MR_L1 block results (MR_L1 * MTILE = 2^28 records), levels 2 and 3 at
+// most 64 and 1. Each GPU holds its own copy of the array until the process
+// ends; its address comes from hipGetSymbolAddress once per device (the
+// array's name is not a device address in host code). Every partial is
+// written before it is read, so nothing needs clearing; calls that overlap on
+// one GPU share the array and race.
+constexpr uint64_t MR_L1 = 131072;
+constexpr uint64_t MR_SCRATCH_WORDS = MR_L1 + 256;
+constexpr int MR_MAX_DEVICES = 64;
+static __device__ __align__(256) uint64_t scratch[MR_SCRATCH_WORDS];
+
+static std::mutex g_scratch_mu;
+static uint64_t* g_scratch_ptr[MR_MAX_DEVICES] = {};
+
+// the device address of `scratch` on `dev` (looked up once per device, with
+// that device current)
+static int scratch_on(int dev, uint64_t** out) {
+    if (dev < 0 || dev >= MR_MAX_DEVICES) return set_error(VH_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPCHECK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHECK(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIPCHECK(hipSetDevice(cur));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = (uint64_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return VH_OK;
+}
+
+extern "C" int vh_mapreduce_noalloc(vh_sim_t h, int kind, uint8_t t, const vh_map_desc* f, int op, int dtype,
+                                    const void* init, void* out) {
+    if (!h || !f || !out) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (!s.initialized) return set_error(VH_ERR_ASSERT, "mapreduce needs an initialized simulation");
+    if (op < VH_OP_ADD || op > VH_OP_OR) return set_error(VH_ERR_ASSERT, "unknown operator");
+    uint64_t acc;
+    if (init) memcpy(&acc, init, 8);
+    else RC(val4empty(op, dtype, &acc));
+    MapSpec ms;
+    memset(&ms, 0, sizeof ms);
+    ms.kind = f->kind;
+    ms.offset = f->offset;
+    ms.fdt = f->field_dtype;
+    ms.value = f->value;
+    ms.op = op;
+    ms.dtype = dtype;
+    const uint8_t* recs;
+    const uint8_t* died = nullptr;
+    uint64_t n;
+    uint32_t bytes;
+    if (kind == 0) {
+        if (t < 1 || t > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+        AgentStore& A = s.a[t];
+        recs = A.rstate();
+        bytes = A.bytes;
+        n = A.nextid - 1;
+        if (A.mortal) died = A.died.as<uint8_t>();
+    } else {
+        if (t >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+        EdgeStore& E = s.e[t];
+        if (has(E.hints, VH_HINT_STATELESS))
+            return set_error(VH_ERR_ASSERT, "mapreduce is not defined for the hint combination");
+        recs = E.st.as<uint8_t>();
+        bytes = E.bytes;
+        n = E.nedges;
+    }
+    if (n > MR_L1 * MTILE)
+        return set_error(VH_ERR_CAPACITY, "mapreduce_noalloc: %llu records exceed its maximum of %llu",
+                         (unsigned long long)n, (unsigned long long)(MR_L1 * MTILE));
+    uint64_t* area;
+    RC(scratch_on(s.device, &area));
+    hipStream_t st = s.stream;
+    uint64_t* buf = area;
+    uint64_t m;
+    if (n == 0) {
+        hipLaunchKernelGGL(k_identity, dim3(1), dim3(1), 0, st, buf, op, dtype);
+        m = 1;
+    } else {
+        m = (n + MTILE - 1) / MTILE;   // <= MR_L1
+        hipLaunchKernelGGL(k_mr_records, dim3((unsigned)m), dim3(MB), 0, st, recs, bytes, died, n, ms, buf);
+    }
+    // levels 2 (<= 64 values, at MR_L1) and 3 (1 value, at MR_L1 + 128)
+    uint64_t* next = area + MR_L1;
+    while (m > 1) {
+        const uint64_t nb = (m + MTILE - 1) / MTILE;
+        hipLaunchKernelGGL(k_mr_values, dim3((unsigned)nb), dim3(MB), 0, st, buf, m, op, dtype, next);
+        buf = next;
+        next += 128;
+        m = nb;
+    }
+    HIPCHECK(hipGetLastError());
+    uint64_t tot = 0;
+    HIPCHECK(hipMemcpyAsync(&tot, buf, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const uint64_t r = h_op(op, dtype, tot, acc);
     memcpy(out, &r, 8);
     return VH_OK;
 }

@@ -382,7 +382,8 @@ def bench_ensemble(vh, args):
     info = E.info()
     log(f"ensemble built in {time.perf_counter() - t0:.1f}s: {info}")
     block, remap, tms = E.autotune(2)
-    log(f"autotune: block {block}, xcd_remap {remap}, {tms:.3f} ms/step")
+    log(f"autotune: block {block}, xcd_remap {remap}, agents per thread by size class {E.class_apt}, "
+        f"{tms:.3f} ms/step")
     steps = args.steps if args.steps != 50 else 200
     E.step(1, use_graph=False)
     E.sync()
@@ -399,6 +400,7 @@ def bench_ensemble(vh, args):
     sample = list(range(0, R, max(1, R // 64)))
     solo_time, solo_agents = 0.0, 0
     solo_by_mode = {"1": 0.0, "0": 0.0}
+    service = []   # each sampled replica's run time alone (its faster mode)
     saved_opts = vh.get_options()
     for i in sample:
         n, md, p, seed = specs[i]
@@ -421,14 +423,29 @@ def bench_ensemble(vh, args):
             best = mode_best if best is None else min(best, mode_best)
         solo_time += best
         solo_agents += n
+        service.append(best)
     vh.set_options(**saved_opts)
     solo_rate = solo_agents * steps / solo_time
     solo_total = len(specs) * (solo_time / len(sample))
+    # latency of every replica when all R arrive at t = 0: in the ensemble all
+    # finish with the last launch; one at a time, replica k finishes after the
+    # run times of those before it (each sampled replica stands for R / sample
+    # replicas in spec order)
+    rep = max(1, R // len(sample))
+    seq_done = np.cumsum(np.repeat(np.asarray(service), rep))
+    pct = lambda a, q: float(np.percentile(a, q))  # noqa: E731
     out = {"metric": "ensemble agent-updates/s (1024 SIR replicas, one launch per step)",
            "value": agent_steps / wall, "unit": "agent-updates/s",
            "edges_per_s": info["edges"] * steps / wall, "steps": steps,
            "ms_per_step": wall * 1e3 / steps, "replicas": R, "agents": info["agents"],
-           "edges": info["edges"], "autotune": {"block": block, "xcd_remap": remap},
+           "edges": info["edges"],
+           "autotune": {"block": block, "xcd_remap": remap,
+                        "agents_per_thread_by_size_class": {"<1e4": E.class_apt[0], "1e4-1e5": E.class_apt[1],
+                                                            ">=1e5": E.class_apt[2]}},
+           "latency_s": {"ensemble": {"p50": wall, "p99": wall},
+                         "one_at_a_time": {"p50": pct(seq_done, 50), "p99": pct(seq_done, 99)},
+                         "one_at_a_time_alone": {"p50": pct(service, 50), "p99": pct(service, 99)},
+                         "note": "all replicas submitted at t=0; 'alone' is a replica's own run time"},
            "one_at_a_time": {"agent_updates_per_s": solo_rate, "sample_replicas": len(sample),
                              "est_total_s": solo_total,
                              "est_total_s_exposure_on": len(specs) * solo_by_mode["1"] / len(sample),

@@ -420,9 +420,15 @@ int vh_ensemble_read(vh_ensemble_t e, uint32_t member_id, void* states, uint64_t
                      uint64_t* step);
 int vh_ensemble_info(vh_ensemble_t e, uint64_t* members, uint64_t* agents, uint64_t* edges,
                      uint64_t* bytes);
-/* times every (block, order) candidate on a snapshot, keeps the fastest */
+/* agents per thread (1, 2 or 4) of the members of each size class
+ * ([0, 10^4), [10^4, 10^5), [10^5, ...) agents): a member's logical blocks
+ * cover block * apt agents */
+int vh_ensemble_set_tiles(vh_ensemble_t e, const uint32_t* apt3);
+/* times, on a snapshot restored afterwards, every (block, order) candidate
+ * with one agent per thread, then 2 and 4 agents per thread for each size
+ * class present; keeps the fastest (class_apt: 3 entries, may be NULL) */
 int vh_ensemble_autotune(vh_ensemble_t e, uint32_t trial_steps, uint32_t* block, uint32_t* xcd_remap,
-                         float* ms_per_step);
+                         uint32_t* class_apt, float* ms_per_step);
 
 /* ------------------------------------------------------ graph generators
  * Host-side sources for add_graph! (src/GraphsSupport.jl:34-57) with a

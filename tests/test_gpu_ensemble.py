@@ -107,3 +107,56 @@ def test_ensemble_join_and_leave(vh, gpu_ready, expose_min, opts):
         assert nxt == st + 1
         want = _solo(vh, n, md, p, seed, gseed, st, n // 100)
         assert np.array_equal(np.ascontiguousarray(got).view(np.uint8), want)
+
+
+@pytest.mark.parametrize("apt", [(4, 2, 1), (1, 4, 2), (2, 1, 4)])
+def test_ensemble_tiles_per_size_class(vh, gpu_ready, apt, opts):
+    """Logical blocks of 1, 2 or 4 agents per thread per size class (<1e4,
+    <1e5, >=1e5 agents), with the exposure pass on: every member equals its
+    solo run."""
+    opts(sir_expose=1, sir_expose_min=0)
+    print(f"[gpu] ensemble tiles per size class {apt}", flush=True)
+    specs = [(700, 9.0, 0.08, 41, 401), (9999, 6.0, 0.05, 42, 402), (10000, 12.0, 0.03, 43, 403),
+             (54321, 7.0, 0.06, 44, 404), (100000, 5.0, 0.04, 45, 405), (180000, 10.0, 0.02, 46, 406)]
+    E = vh.ensemble.Ensemble()
+    ids = [E.add_sir(n, md, p, seed, gseed, recovery=7, n_init=max(1, n // 200))
+           for n, md, p, seed, gseed in specs]
+    E.set_tiles(apt)
+    E.step(9, use_graph=True)
+    E.step(3, use_graph=False)
+    for mid, (n, md, p, seed, gseed) in zip(ids, specs):
+        got, nxt = E.read(mid)
+        assert nxt == 13
+        assert np.array_equal(np.ascontiguousarray(got).view(np.uint8),
+                              _solo(vh, n, md, p, seed, gseed, 12, max(1, n // 200))), f"member n={n}"
+
+
+def test_ensemble_1024_replicas_config3(vh, gpu_ready):
+    """Config c3 at its stated size: 1024 SIR replicas, N ~ logU[1e3, 1e6]
+    (1.5e8 agents, 1.9e9 edges), mean in-degree U[4, 20], p U[0.01, 0.1],
+    distinct seeds; autotuned (block, order, tiles per size class); 6 steps
+    in one graph launch; 16 members spread over the sizes (the smallest and
+    largest included) equal their solo runs."""
+    rng = np.random.default_rng(7)
+    R = 1024
+    specs = [(int(10 ** rng.uniform(3, 6)), float(rng.uniform(4, 20)), float(rng.uniform(0.01, 0.1)), 1 + i)
+             for i in range(R)]
+    print("[gpu] ensemble 1024 replicas: building", flush=True)
+    E = vh.ensemble.Ensemble()
+    ids = [E.add_sir(n, md, p, seed, 10_000 + seed, 7, max(1, n // 1000)) for n, md, p, seed in specs]
+    info = E.info()
+    assert info["members"] == R and info["agents"] == sum(s[0] for s in specs)
+    block, remap, ms = E.autotune(1)
+    print(f"[gpu] ensemble 1024: {info}, autotune block {block} remap {remap} tiles {E.class_apt} "
+          f"{ms:.3f} ms/step", flush=True)
+    E.step(6, use_graph=True)
+    E.sync()
+    by_size = np.argsort([s[0] for s in specs])
+    pick = sorted(set(by_size[np.linspace(0, R - 1, 16).astype(int)].tolist()))
+    for k in pick:
+        n, md, p, seed = specs[k]
+        got, nxt = E.read(ids[k])
+        assert nxt == 7
+        assert np.array_equal(np.ascontiguousarray(got).view(np.uint8),
+                              _solo(vh, n, md, p, seed, 10_000 + seed, 6, max(1, n // 1000))), f"member {k} n={n}"
+    E.close()

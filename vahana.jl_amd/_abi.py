@@ -186,7 +186,8 @@ DEVICE_ONLY = [
     ("ensemble_set_streams", C.c_int, [_vp, _vp, _vp]),
     ("ensemble_read", C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64, _u64p, _u64p]),
     ("ensemble_info", C.c_int, [_vp, _u64p, _u64p, _u64p, _u64p]),
-    ("ensemble_autotune", C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_float)]),
+    ("ensemble_autotune", C.c_int, [_vp, C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(C.c_float)]),
+    ("ensemble_set_tiles", C.c_int, [_vp, _u32p]),
     ("mapreduce_noalloc", C.c_int, [_vp, C.c_int, C.c_uint8, C.POINTER(MapDesc), C.c_int, C.c_int, _vp,
                                     _vp]),
 ]

@@ -106,6 +106,8 @@ __global__ void k_member_offsets(const uint32_t* __restrict__ keys, uint64_t ne,
     out_off[v] = (uint32_t)lo;
 }
 
+constexpr int ENS_CLASSES = 3;   // member size classes of the tile choice
+
 struct MemberDesc {
     const uint32_t* off;
     const uint32_t* src;
@@ -120,6 +122,8 @@ struct MemberDesc {
     const uint32_t* out_dst;
     uint32_t* bits;            // two exposure bitmaps of nw words (launch parity)
     uint64_t nw;
+    uint32_t tile;             // agents per logical block (the launch block size times 1, 2 or 4)
+    uint32_t pad;
 };
 
 // Exposure (as in the solo SIR step): per launch, k_ens_decide turns the
@@ -185,15 +189,15 @@ __device__ __forceinline__ uint32_t ens_block(uint32_t nblocks, uint32_t remap) 
     return b;   // >= nblocks: no block
 }
 
-// grid-stride over the logical blocks (of `bsize` agents): members whose
-// pass is off, and blocks without an agent to mark (by the block's counts of
-// the previous launch: infected, and infected in it), cost a read or two
-// per logical block, not a block launch
+// grid-stride over the logical blocks (of their member's `tile` agents):
+// members whose pass is off, and blocks without an agent to mark (by the
+// block's counts of the previous launch: infected, and infected in it), cost
+// a read or two per logical block, not a block launch
 __global__ __launch_bounds__(256) void k_ens_expose(const MemberDesc* __restrict__ md,
                                                     const uint32_t* __restrict__ bmember,
                                                     const uint32_t* __restrict__ bstart, uint32_t nblocks,
-                                                    uint32_t bsize, const uint32_t* __restrict__ dec,
-                                                    uint32_t parity, const uint32_t* __restrict__ blkcnt) {
+                                                    const uint32_t* __restrict__ dec, uint32_t parity,
+                                                    const uint32_t* __restrict__ blkcnt) {
     for (uint32_t lb = blockIdx.x; lb < nblocks; lb += gridDim.x) {
         const uint32_t mi = bmember[lb];
         const uint32_t dm = dec[mi];
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(256) void k_ens_expose(const MemberDesc* __restrict
         const MemberDesc& m = md[mi];
         uint32_t* bits = m.bits + parity * m.nw;
         const uint64_t a0 = bstart[lb];
-        for (uint32_t o = threadIdx.x; o < bsize; o += 256) {
+        for (uint32_t o = threadIdx.x; o < m.tile; o += 256) {
             const uint64_t i = a0 + o;
             if (i >= m.n) continue;
             const uint32_t st = m.rs[i];
@@ -220,9 +224,10 @@ __global__ __launch_bounds__(256) void k_ens_expose(const MemberDesc* __restrict
 }
 
 // one launch for every member: logical block b belongs to member
-// bmember[b] and covers agents [bstart[b], bstart[b] + block) of it
-// (dec null: no exposure; else per-block infected counts of the written
-// state go to blkcnt[(parity ^ 1) * nblocks + b])
+// bmember[b] and covers agents [bstart[b], bstart[b] + tile) of it, thread t
+// the agents bstart[b] + t + k * blockDim, k < tile / blockDim (dec null: no
+// exposure; else per-block infected counts of the written state go to
+// blkcnt[(parity ^ 1) * nblocks + b])
 __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t* __restrict__ bmember,
                                const uint32_t* __restrict__ bstart, uint32_t nblocks, uint32_t remap,
                                const uint32_t* __restrict__ dec, uint32_t parity, uint64_t lidx,
@@ -234,41 +239,48 @@ __global__ void k_sir_ensemble(const MemberDesc* __restrict__ md, const uint32_t
     const MemberDesc& m = md[mi];
     const uint64_t a0 = bstart[b];
     if (a0 >= m.n) return;      // whole block (a member past its horizon has n = 0)
-    const uint64_t i = a0 + threadIdx.x;
-    const bool in = i < m.n;
     const uint32_t dm = dec ? dec[mi] : 0u;
     const bool ex = (dm & 1u) != 0u;
-    uint32_t s = in ? m.rs[i] : 0u;
-    uint32_t h = s & 0xFFu;
-    const bool skip = in && h == VH_S && ex && !((m.bits[parity * m.nw + (i >> 5)] >> (i & 31u)) & 1u);
-    if (in && h == VH_S && !skip) {
-        const uint64_t pre = vh_hash_pre(m.seed, vh_make_id(m.type, i + 1), m.step);
-        const uint64_t pt = vh_u01_below(m.p);
-        uint32_t bb = m.off[i], e = m.off[i + 1];
-        // the draw first, the source's state only below p (as in k_sir)
-        if (vh_any_infecting(pre, pt, bb, e, [&](uint32_t k) { return (m.rs[m.src[k] - 1] & 0xFFu) == VH_I; }))
-            s = (s & 0xFF00u) | VH_I;
-    } else if (in && h == VH_I) {
-        uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
-        s = (s & 0xFF00u) | (days << 16) | (days >= m.rec ? VH_R : VH_I);
+    const uint32_t apt = m.tile / blockDim.x;
+    uint32_t c_inf = 0, c_new = 0;   // this wave's infected (and newly infected) agents
+    for (uint32_t k = 0; k < apt; k++) {
+        const uint64_t i = a0 + threadIdx.x + (uint64_t)k * blockDim.x;
+        const bool in = i < m.n;
+        uint32_t s = in ? m.rs[i] : 0u;
+        const uint32_t h = s & 0xFFu;
+        const bool skip = in && h == VH_S && ex && !((m.bits[parity * m.nw + (i >> 5)] >> (i & 31u)) & 1u);
+        if (in && h == VH_S && !skip) {
+            const uint64_t pre = vh_hash_pre(m.seed, vh_make_id(m.type, i + 1), m.step);
+            const uint64_t pt = vh_u01_below(m.p);
+            const uint32_t bb = m.off[i], e = m.off[i + 1];
+            // the draw first, the source's state only below p (as in k_sir)
+            if (vh_any_infecting(pre, pt, bb, e,
+                                 [&](uint32_t q) { return (m.rs[m.src[q] - 1] & 0xFFu) == VH_I; }))
+                s = (s & 0xFF00u) | VH_I;
+        } else if (in && h == VH_I) {
+            const uint32_t days = ((s >> 16) + 1u) & 0xFFFFu;
+            s = (s & 0xFF00u) | (days << 16) | (days >= m.rec ? VH_R : VH_I);
+        }
+        if (in) m.ws[i] = s;
+        if (dec) {
+            c_inf += (uint32_t)__popcll(__ballot(in && (s & 0xFFu) == VH_I));
+            c_new += (uint32_t)__popcll(__ballot(in && (s & 0xFFu) == VH_I && (s >> 16) == 0u));
+        }
     }
-    if (in) m.ws[i] = s;
     if (dec) {
         // this block's words of the other bitmap, for the next launch:
         // cleared before a full launch (this one off, or the next launch
         // number a multiple of EXPO_FULL), else this launch's words
         const uint64_t w0 = a0 >> 5;
         const bool next_full = !ex || (lidx + 1) % EXPO_FULL == 0;
-        if (threadIdx.x < (blockDim.x >> 5) && w0 + threadIdx.x < m.nw)
-            m.bits[(parity ^ 1u) * m.nw + w0 + threadIdx.x] =
-                next_full ? 0u : m.bits[parity * m.nw + w0 + threadIdx.x];
+        for (uint32_t q = threadIdx.x; q < (m.tile >> 5); q += blockDim.x)
+            if (w0 + q < m.nw)
+                m.bits[(parity ^ 1u) * m.nw + w0 + q] = next_full ? 0u : m.bits[parity * m.nw + w0 + q];
         // infected agents of the written state, and those infected in this
         // launch (days 0): count | new count << 16
-        const uint64_t msk = __ballot(in && (s & 0xFFu) == VH_I);
-        const uint64_t nmsk = __ballot(in && (s & 0xFFu) == VH_I && (s >> 16) == 0u);
         if ((threadIdx.x & 63u) == 0) {
-            s_inf[threadIdx.x >> 6] = (uint32_t)__popcll(msk);
-            s_new[threadIdx.x >> 6] = (uint32_t)__popcll(nmsk);
+            s_inf[threadIdx.x >> 6] = c_inf;
+            s_new[threadIdx.x >> 6] = c_new;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -369,6 +381,7 @@ struct vh_ensemble {
     void* h_descs = nullptr;
     size_t h_descs_cap = 0;
     uint32_t block = 256, remap = 1;
+    uint32_t apt[ENS_CLASSES] = {1, 1, 1};   // agents per thread of each size class
     DevBuf d_desc, d_bmember, d_bstart;
     DevBuf d_mblk, d_blkcnt, d_dec;   // exposure: member block ranges, per-block counts, decisions
     Scratch oscr;                     // scratch of the members' out-edge builds (step stream)
@@ -379,6 +392,7 @@ struct vh_ensemble {
     uint32_t nblocks = 0;
     bool table_valid = false;
     uint32_t table_block = 0;
+    uint32_t table_apt[ENS_CLASSES] = {0, 0, 0};
     Scratch scratch;
     DevBuf d_cdf;
 };
@@ -394,6 +408,16 @@ static int pinned(void** p, size_t* cap, size_t bytes) {
     return VH_OK;
 }
 
+// size class of a member: [0, 10^4), [10^4, 10^5), [10^5, ...) agents
+static int size_class(uint64_t n) { return n < 10000 ? 0 : n < 100000 ? 1 : 2; }
+static uint32_t member_tile(const vh_ensemble* e, uint64_t n) { return e->block * e->apt[size_class(n)]; }
+static bool table_current(const vh_ensemble* e) {
+    if (!e->table_valid || e->table_block != e->block) return false;
+    for (int c = 0; c < ENS_CLASSES; c++)
+        if (e->table_apt[c] != e->apt[c]) return false;
+    return true;
+}
+
 static int build_table(vh_ensemble* e) {
     e->h_bmember.clear();
     e->h_bstart.clear();
@@ -403,7 +427,8 @@ static int build_table(vh_ensemble* e) {
     for (auto& kv : e->members) {
         Member* m = kv.second;
         mblk.push_back((uint32_t)e->h_bmember.size());
-        for (uint64_t a = 0; a < m->n; a += e->block) {
+        const uint32_t tile = member_tile(e, m->n);
+        for (uint64_t a = 0; a < m->n; a += tile) {
             e->h_bmember.push_back(mi);
             e->h_bstart.push_back((uint32_t)a);
         }
@@ -434,6 +459,7 @@ static int build_table(vh_ensemble* e) {
     }
     e->table_valid = true;
     e->table_block = e->block;
+    for (int c = 0; c < ENS_CLASSES; c++) e->table_apt[c] = e->apt[c];
     return VH_OK;
 }
 
@@ -457,6 +483,8 @@ static int upload_desc(vh_ensemble* e) {
         d.out_dst = m->out_dst.as<uint32_t>();
         d.bits = m->bits.as<uint32_t>();
         d.nw = m->nw;
+        d.tile = member_tile(e, m->n);
+        d.pad = 0;
         e->h_desc.push_back(d);
     }
     return VH_OK;
@@ -582,6 +610,15 @@ extern "C" int vh_ensemble_set_horizon(vh_ensemble_t e, uint32_t id, uint64_t la
     return VH_OK;
 }
 
+extern "C" int vh_ensemble_set_tiles(vh_ensemble_t e, const uint32_t* apt) {
+    if (!e || !apt) return set_error(VH_ERR_ARG, "null argument");
+    for (int c = 0; c < ENS_CLASSES; c++)
+        if (apt[c] != 1 && apt[c] != 2 && apt[c] != 4)
+            return set_error(VH_ERR_ARG, "agents per thread must be 1, 2 or 4");
+    for (int c = 0; c < ENS_CLASSES; c++) e->apt[c] = apt[c];
+    return VH_OK;
+}
+
 extern "C" int vh_ensemble_set_config(vh_ensemble_t e, uint32_t block, uint32_t remap) {
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return set_error(VH_ERR_ARG, "block must be 64..1024 (power of two)");
@@ -603,7 +640,7 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
         (void)hipGraphExecDestroy(e->exec);
         e->exec = nullptr;
     }
-    if (!e->table_valid || e->table_block != e->block) RC(build_table(e));
+    if (!table_current(e)) RC(build_table(e));
     size_t nm = e->members.size();
     // exposure pass unless the options turn it off, for ensembles of at
     // least sir_expose_min agents (default 65536: below, its two extra
@@ -664,7 +701,7 @@ extern "C" int vh_ensemble_step(vh_ensemble_t e, uint32_t steps, int use_graph) 
                                (uint64_t)(launch0 + s), e->d_dec.as<uint32_t>());
             hipLaunchKernelGGL(k_ens_expose, dim3(std::min<uint32_t>(e->nblocks, 8192u)), dim3(256), 0,
                                e->stream, md, e->d_bmember.as<uint32_t>(), e->d_bstart.as<uint32_t>(),
-                               e->nblocks, e->block, (const uint32_t*)e->d_dec.as<uint32_t>(), parity,
+                               e->nblocks, (const uint32_t*)e->d_dec.as<uint32_t>(), parity,
                                (const uint32_t*)e->d_blkcnt.as<uint32_t>());
         }
         hipLaunchKernelGGL(k_sir_ensemble, g, b, 0, e->stream, md, e->d_bmember.as<uint32_t>(),
@@ -762,44 +799,73 @@ extern "C" int vh_ensemble_info(vh_ensemble_t e, uint64_t* members, uint64_t* ag
 // order): times `trial_steps` steps per candidate on the members' current
 // state and keeps the fastest.  Trial steps advance a snapshot, not the
 // members: their buffers are restored afterwards.
+// The autotuner: every trial runs `trial_steps` launches (after one launch
+// that builds the block table) from a snapshot of the members' states and
+// step numbers, restored afterwards. First the launch block size (128..1024)
+// and block order (XCD-aware or not) with one agent per thread, then, per
+// size class present, 2 or 4 agents per thread (larger logical blocks) where
+// that is faster. Restoring invalidates the block table, so the first launch
+// after it runs without the exposure pass and the next one rebuilds its
+// bitmaps (no stale exposure bits).
 extern "C" int vh_ensemble_autotune(vh_ensemble_t e, uint32_t trial_steps, uint32_t* best_block,
-                                    uint32_t* best_remap, float* best_ms) {
+                                    uint32_t* best_remap, uint32_t* class_apt, float* best_ms) {
     if (e->members.empty()) return set_error(VH_ERR_ARG, "empty ensemble");
     if (trial_steps == 0) trial_steps = 3;
     // snapshot of the current state of every member
     std::vector<DevBuf> snap(e->members.size());
     size_t k = 0;
     std::vector<std::pair<int, uint64_t>> saved;
+    bool present[ENS_CLASSES] = {false, false, false};
     for (auto& kv : e->members) {
         Member* m = kv.second;
-        RC(snap[k].ensure(m->n * 4, e->stream));
+        RC(snap[k].ensure(m->n * 4 + 4, e->stream));
         HIPCHECK(hipMemcpyAsync(snap[k].p, m->st[m->cur].p, m->n * 4, hipMemcpyDeviceToDevice, e->stream));
         saved.push_back({m->cur, m->step});
+        present[size_class(m->n)] = true;
         k++;
     }
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
+    auto trial = [&](uint32_t blk, uint32_t rm, const uint32_t* apt, float* ms) -> int {
+        RC(vh_ensemble_set_config(e, blk, rm));
+        RC(vh_ensemble_set_tiles(e, apt));
+        RC(vh_ensemble_step(e, 1, 0));   // warm-up, table build
+        HIPCHECK(hipEventRecord(a, e->stream));
+        RC(vh_ensemble_step(e, trial_steps, 0));
+        HIPCHECK(hipEventRecord(b, e->stream));
+        HIPCHECK(hipEventSynchronize(b));
+        HIPCHECK(hipEventElapsedTime(ms, a, b));
+        *ms /= trial_steps;
+        return VH_OK;
+    };
     float best = 1e30f;
     uint32_t bb = e->block, br = e->remap;
+    uint32_t bapt[ENS_CLASSES] = {1, 1, 1};
     const uint32_t blocks[] = {128, 256, 512, 1024};
     for (uint32_t blk : blocks)
         for (uint32_t rm = 0; rm < 2; rm++) {
-            RC(vh_ensemble_set_config(e, blk, rm));
-            RC(vh_ensemble_step(e, 1, 0));   // warm-up, table build
-            HIPCHECK(hipEventRecord(a, e->stream));
-            RC(vh_ensemble_step(e, trial_steps, 0));
-            HIPCHECK(hipEventRecord(b, e->stream));
-            HIPCHECK(hipEventSynchronize(b));
             float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, a, b));
-            ms /= trial_steps;
+            RC(trial(blk, rm, bapt, &ms));
             if (ms < best) {
                 best = ms;
                 bb = blk;
                 br = rm;
             }
         }
+    for (int c = 0; c < ENS_CLASSES; c++) {
+        if (!present[c]) continue;
+        for (uint32_t v : {2u, 4u}) {
+            uint32_t apt[ENS_CLASSES] = {bapt[0], bapt[1], bapt[2]};
+            apt[c] = v;
+            float ms = 0;
+            RC(trial(bb, br, apt, &ms));
+            if (ms < best) {
+                best = ms;
+                bapt[c] = v;
+            }
+        }
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     // restore
@@ -812,10 +878,13 @@ extern "C" int vh_ensemble_autotune(vh_ensemble_t e, uint32_t trial_steps, uint3
         k++;
     }
     HIPCHECK(hipStreamSynchronize(e->stream));
-    for (auto& s : snap) s.release();
-    RC(vh_ensemble_set_config(e, bb, br));
+    for (auto& sb : snap) sb.release();
+    RC(vh_ensemble_set_config(e, bb, br));   // (invalidates the block table)
+    RC(vh_ensemble_set_tiles(e, bapt));
     if (best_block) *best_block = bb;
     if (best_remap) *best_remap = br;
+    if (class_apt)
+        for (int c = 0; c < ENS_CLASSES; c++) class_apt[c] = bapt[c];
     if (best_ms) *best_ms = best;
     return VH_OK;
 }

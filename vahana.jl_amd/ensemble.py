@@ -54,10 +54,19 @@ class Ensemble:
         self._check(self.lib.ensemble_set_streams(self.handle, step_stream, aux_stream))
 
     def autotune(self, trial_steps=3):
+        """(block, xcd_remap, ms per step); the chosen agents per thread of
+        the size classes <1e4, <1e5, >=1e5 agents are in self.class_apt."""
         b, r, ms = C.c_uint32(), C.c_uint32(), C.c_float()
-        self._check(self.lib.ensemble_autotune(self.handle, trial_steps, C.byref(b), C.byref(r),
+        apt = (C.c_uint32 * 3)()
+        self._check(self.lib.ensemble_autotune(self.handle, trial_steps, C.byref(b), C.byref(r), apt,
                                                C.byref(ms)))
+        self.class_apt = [apt[i] for i in range(3)]
         return b.value, bool(r.value), ms.value
+
+    def set_tiles(self, apt):
+        """Agents per thread (1, 2, 4) for the size classes <1e4, <1e5, >=1e5."""
+        arr = (C.c_uint32 * 3)(*[int(v) for v in apt])
+        self._check(self.lib.ensemble_set_tiles(self.handle, arr))
 
     def read(self, mid):
         n = self.members[mid].n

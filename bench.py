@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrivals/s (0: 0.8 x the measured one-at-a-time capacity)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-classes", action="store_true",
+                    help="serve: also measure capacity per size class (30 requests per decade)")
     ap.add_argument("--no-relabel", action="store_true",
                     help="opinion: keep the generation order of the agents (default: renumbered "
                          "breadth-first at finish_init!)")
@@ -484,6 +486,18 @@ def bench_serve(vh, args):
     s_cap, wall = S.run_sequential(vh, burst, stats=ph)
     log(f"one-at-a-time phases (host s): {({k: round(v, 3) for k, v in ph.items()})}")
     cap_seq = S.summarize(s_cap, wall)
+    # GPU-busy accounting of the one-at-a-time run: the time a request's steps
+    # hold the GPU (enqueue to completion, nothing else running), per size
+    # class; no schedule serves the mix faster than the sum of the classes'
+    # step time that batching cannot remove (the large requests fill the GPU
+    # alone)
+    by_class = {}
+    for lo, hi, name in ((1e4, 1e5, "1e4-1e5"), (1e5, 1e6, "1e5-1e6"), (1e6, 1e7 + 1, "1e6-1e7")):
+        rs = [r for r in s_cap if lo <= r.n < hi]
+        by_class[name] = {"requests": len(rs), "agent_steps": int(sum(r.n * r.horizon for r in rs)),
+                          "run_s": sum(r.run_s for r in rs), "build_s": sum(r.build_s for r in rs),
+                          "read_s": sum(r.read_s for r in rs)}
+    run_sum = sum(r.run_s for r in s_cap)
     log(f"capacity, one at a time: {cap_seq}")
     same = {r.id: r.digest for r in d_cap} == {r.id: r.digest for r in s_cap}
     rate = args.rate if args.rate > 0 else 0.8 * cap_seq["throughput_rps"]
@@ -494,6 +508,20 @@ def bench_serve(vh, args):
     lat_seq = S.summarize(s_lat, wall)
     log(f"Poisson {rate:.2f}/s, one at a time: {lat_seq}")
     same = same and {r.id: r.digest for r in d_lat} == {r.id: r.digest for r in s_lat}
+    classes = None
+    if args.serve_classes:
+        # capacity by size class: 30 requests of one decade each, all at t = 0
+        classes = {}
+        for lo, hi, name in ((1e4, 1e5, "1e4-1e5"), (1e5, 1e6, "1e5-1e6"), (1e6, 1e7, "1e6-1e7")):
+            reqs_c = S.make_requests(30, rate=0, seed=21, n_min=lo, n_max=hi)
+            dc, srv_c = served(reqs_c)
+            sc, wall_c = S.run_sequential(vh, reqs_c)
+            seq_c = S.summarize(sc, wall_c)
+            classes[name] = {"served_rps": srv_c["throughput_rps"], "one_at_a_time_rps": seq_c["throughput_rps"],
+                             "ratio": srv_c["throughput_rps"] / seq_c["throughput_rps"],
+                             "served_p99_s": srv_c["p99_s"], "one_at_a_time_p99_s": seq_c["p99_s"],
+                             "identical": {r.id: r.digest for r in dc} == {r.id: r.digest for r in sc}}
+            log(f"class {name}: {classes[name]}")
     out = {"metric": "served requests/s with p50/p99 latency (SIR + opinion mix)",
            "value": lat_srv["throughput_rps"], "unit": "requests/s",
            "capacity_served_rps": cap_srv["throughput_rps"],
@@ -503,9 +531,18 @@ def bench_serve(vh, args):
            "p50_ratio": lat_srv["p50_s"] / lat_seq["p50_s"],
            "p99_ratio": lat_srv["p99_s"] / lat_seq["p99_s"],
            "results_identical": same, "requests": args.requests, "hbm_budget_bytes": budget,
+           "gpu_busy": {"one_at_a_time_wall_s": cap_seq["wall_s"], "one_at_a_time_step_s": run_sum,
+                        "served_wall_s": cap_srv["wall_s"], "by_size_class": by_class,
+                        "capacity_ratio_bound_large_classes": cap_seq["wall_s"] / max(
+                            1e-9, by_class["1e6-1e7"]["run_s"] + by_class["1e5-1e6"]["run_s"]),
+                        "note": "step_s: enqueue-to-completion time of each request's steps when it runs "
+                                "alone; the 1e5-1e7 classes' step time is GPU-bound work that batching "
+                                "cannot remove"},
            "n_gpus": 1, "higher_is_better": True,
            "config": {"workload": "serving: Poisson SIR/opinion requests, N ~ logU[1e4, 1e7], "
                                   "horizon ~ U[50, 500], mean degree ~ U[4, 20], device-built G(n, p)"}}
+    if classes is not None:
+        out["capacity_by_size_class"] = classes
     print(json.dumps(out), flush=True)
 
 

@@ -275,6 +275,19 @@ int vh_finish_init_relabel(vh_sim_t sim, uint8_t t, uint64_t* new_nr);
 int vh_apply(vh_sim_t s, const vh_apply_desc* desc);
 /* apply! repeated `steps` times with the same desc (one host call) */
 int vh_apply_n(vh_sim_t s, const vh_apply_desc* desc, uint32_t steps);
+/* apply! of one transition to many simulations (no reference counterpart:
+ * the reference runs one simulation per process): simulation i takes part in
+ * the first steps[i] steps, and the one-pass opinion bodies of all
+ * simulations of a step run as one launch per state type (simulations whose
+ * state needs the two-pass gather, and other bodies, run their own kernels).
+ * All simulations must run on one stream (vh_set_stream) and have the type
+ * ids the desc names. Asynchronous like vh_apply; a call first waits for the
+ * batch's previous call. */
+typedef struct vh_batch* vh_batch_t;
+int vh_batch_create(vh_batch_t* out);
+int vh_batch_destroy(vh_batch_t b);
+int vh_batch_apply(vh_batch_t b, const vh_sim_t* sims, uint32_t n, const vh_apply_desc* desc,
+                   const uint32_t* steps);
 /* wait for the simulation's stream */
 int vh_sync(vh_sim_t s);
 /* attach an external hipStream_t (NULL: the library's own stream) */

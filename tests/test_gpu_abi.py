@@ -1,5 +1,7 @@
 """Init-phase edge checks of the C ABI on a simulation (GPU: a simulation
 owns a HIP stream from vh_sim_create on). The checker runs the same calls."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -83,3 +85,45 @@ def test_bound_values_on_device(vh, ora, gpu_ready):
     got = TG.bound_sir_matches_literal(vh)
     want = TG.bound_sir_matches_literal(ora)
     assert np.array_equal(got, want)
+
+
+def test_batch_apply_opinion_matches_solo(vh, ora, gpu_ready, opts):
+    """vh_batch_apply: opinion simulations of several sizes and step counts
+    advanced together (one fused launch per step for the one-pass ones; one
+    of them large enough for the two-pass gather, which runs its own
+    kernels): every state equals the same simulation stepped alone, and one
+    the checker's."""
+    print("[gpu] batched opinion apply vs solo", flush=True)
+    opts(opinion_plan_bytes=200_000 * 8)   # the 300k-agent simulation uses the plan
+    M = vh.models
+    specs = [(3000, 8.0, 5), (20000, 12.0, 9), (777, 5.0, 7), (300_000, 10.0, 6), (64, 4.0, 9)]
+    lib = vh._abi.lib()
+    st = C.c_void_p()
+    assert lib.stream_create(C.byref(st)) == 0
+    sims = []
+    for k, (n, md, steps) in enumerate(specs):
+        m = M.build_opinion_gnp(vh, n, mean_deg=md, seed=50 + k, graph_seed=60 + k, weight_seed=70 + k)
+        m.sim._set_streams(st, st)
+        sims.append(m)
+    B = vh.Batch()
+    B.apply([m.sim for m in sims], vh.transitions.opinion(sims[0].E, 0.2), sims[0].T, [sims[0].T, sims[0].E],
+            sims[0].T, steps=[s[2] for s in specs])
+    lib.sync(sims[0].sim.handle)
+    for k, ((n, md, steps), m) in enumerate(zip(specs, sims)):
+        solo = M.build_opinion_gnp(vh, n, mean_deg=md, seed=50 + k, graph_seed=60 + k, weight_seed=70 + k)
+        for _ in range(steps):
+            M.step_opinion(vh, solo)
+        a = np.asarray(m.sim._read_agents(m.T)[0]).view(np.uint8)
+        b = np.asarray(solo.sim._read_agents(solo.T)[0]).view(np.uint8)
+        assert np.array_equal(a, b), f"member {k} n={n}"
+        assert m.sim._num_transitions() == steps + 1
+    n, md, steps = specs[0]
+    c = M.build_opinion_gnp(ora, n, mean_deg=md, seed=50, graph_seed=60, weight_seed=70, device_graph=False)
+    for _ in range(steps):
+        M.step_opinion(ora, c)
+    assert np.array_equal(np.asarray(sims[0].sim._read_agents(sims[0].T)[0]).view(np.uint8),
+                          np.asarray(c.sim._read_agents(c.T)[0]).view(np.uint8))
+    B.close()
+    for m in sims:
+        m.sim._destroy()
+    lib.stream_destroy(st)

@@ -29,7 +29,7 @@ __all__ = [
     "agent_nr", "type_nr", "register_transition", "enable_asserts", "VahanaError",
     "register_param_", "register_global_", "param", "set_param_", "get_global", "set_global_",
     "modify_global_", "push_global_", "param_ref", "global_ref", "get_options", "set_options",
-    "options",
+    "options", "Batch",
 ]
 param_ref = transitions.param_ref
 global_ref = transitions.global_ref
@@ -956,6 +956,44 @@ def has_edge(sim, id_, T):
           "has_edge")
     fr, _ = sim._edges_of(id_, T)
     return len(fr) > 0
+
+
+class Batch:
+    """apply! of one transition to many simulations at once (vh_batch_apply):
+    the one-pass opinion bodies of all of them run as one launch per step.
+    The simulations must run on one stream and come from models with the
+    same type ids (e.g. built by the same builder)."""
+
+    def __init__(self):
+        self.lib = _abi.lib()
+        self.handle = C.c_void_p()
+        _check(self.lib.batch_create(C.byref(self.handle)), self.lib)
+
+    def apply(self, sims, func, call, read, write, steps=1, add_existing=(), with_edge=None):
+        """`steps`: one count for all, or one per simulation (simulation i
+        takes part in the first steps[i] steps)."""
+        n = len(sims)
+        if not n:
+            return
+        steps = [int(steps)] * n if np.isscalar(steps) else [int(k) for k in steps]
+        d, keep = sims[0]._desc(func, call, read, write, add_existing, with_edge)
+        hs = (C.c_void_p * n)(*[s.handle.value for s in sims])
+        st = (C.c_uint32 * n)(*steps)
+        _check(self.lib.batch_apply(self.handle, hs, n, C.byref(d), st), self.lib)
+        del keep
+        for s in sims:
+            s.initialized = True
+
+    def close(self):
+        if self.handle:
+            self.lib.batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def get_options():

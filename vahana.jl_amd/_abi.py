@@ -188,6 +188,9 @@ DEVICE_ONLY = [
     ("ensemble_info", C.c_int, [_vp, _u64p, _u64p, _u64p, _u64p]),
     ("ensemble_autotune", C.c_int, [_vp, C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(C.c_float)]),
     ("ensemble_set_tiles", C.c_int, [_vp, _u32p]),
+    ("batch_create", C.c_int, [C.POINTER(_vp)]),
+    ("batch_destroy", C.c_int, [_vp]),
+    ("batch_apply", C.c_int, [_vp, C.POINTER(_vp), C.c_uint32, C.POINTER(ApplyDesc), _u32p]),
     ("mapreduce_noalloc", C.c_int, [_vp, C.c_int, C.c_uint8, C.POINTER(MapDesc), C.c_int, C.c_int, _vp,
                                     _vp]),
 ]

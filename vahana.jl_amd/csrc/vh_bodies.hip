@@ -713,6 +713,29 @@ __global__ __launch_bounds__(256) void k_opinion(const uint32_t* __restrict__ of
     }
 }
 
+// Many simulations' one-pass opinion steps in one launch: block b belongs to
+// the member with the largest blk0 <= b (its heavy-wave blocks first, then
+// its light blocks), every member with its own CSR, states and eps.
+template <class T>
+__global__ __launch_bounds__(256) void k_opinion_multi(const OpMember* __restrict__ mem, uint32_t nm) {
+    uint32_t lo = 0, hi = nm;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (mem[mid].blk0 <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const OpMember& m = mem[lo];
+    const uint32_t local = blockIdx.x - m.blk0;
+    if (local >= m.nh) {
+        opinion_block<T>(m.off, m.src, (const T*)m.x, m.w, (T*)m.xo, m.n, m.nrows, m.rowbase, (T)m.eps,
+                         local - m.nh);
+    } else {
+        const uint64_t wv = ((uint64_t)local * blockDim.x + threadIdx.x) >> 6;
+        if (wv < m.nheavy) opinion_heavy<T>(m.off, m.src, (const T*)m.x, m.w, (T*)m.xo, m.heavy[wv], m.rowbase,
+                                           (T)m.eps);
+    }
+}
+
 // ----------------------------------------------------------------- gather plan
 // For a large state vector the neighbour states of a static edge store are
 // gathered in two passes. The plan lists the CSR slots sorted stably by
@@ -1781,6 +1804,24 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
         plan = E.plan_ok;
     }
     const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
+    if (!plan && g_opinion_defer && g_opinion_defer->stream == s.stream) {
+        OpMember om;
+        om.off = E.off.as<uint32_t>();
+        om.src = E.src.as<uint32_t>();
+        om.x = A.rstate();
+        om.w = E.st.as<float>();
+        om.xo = A.wstate();
+        om.heavy = E.heavy.as<uint32_t>();
+        om.n = A.nslots;
+        om.nrows = nrows;
+        om.rowbase = E.rowbase[t];
+        om.nheavy = nheavy;
+        om.eps = f32 ? (double)(float)p.eps : p.eps;
+        om.nh = nh;
+        om.blk0 = nlight + nh;   // (the block count until the launch assigns offsets)
+        g_opinion_defer->m[f32 ? 1 : 0].push_back(om);
+        return VH_OK;
+    }
     if (!plan) {
         dim3 g(nlight + nh), b(256);
         if (f32)
@@ -1821,6 +1862,39 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
                            E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
     }
     HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+thread_local OpinionDefer* g_opinion_defer = nullptr;
+
+int opinion_defer_launch(OpinionDefer& c, hipStream_t st, DevBuf& dev, void** host, size_t* host_cap,
+                         size_t* host_off) {
+    for (int k = 0; k < 2; k++) {
+        std::vector<OpMember>& v = c.m[k];
+        if (v.empty()) continue;
+        uint64_t blocks = 0;
+        for (OpMember& om : v) {
+            const uint32_t nb = om.blk0;   // the member's block count
+            om.blk0 = (uint32_t)blocks;
+            blocks += nb;
+        }
+        if (blocks > 0x7FFFFFFFull) return set_error(VH_ERR_CAPACITY, "batched opinion launch too large");
+        const size_t bytes = v.size() * sizeof(OpMember);
+        if (*host_off + bytes > *host_cap) return set_error(VH_ERR_CAPACITY, "batch staging");
+        memcpy((uint8_t*)*host + *host_off, v.data(), bytes);
+        HIPCHECK(hipMemcpyAsync((uint8_t*)dev.p + *host_off, (uint8_t*)*host + *host_off, bytes,
+                                hipMemcpyHostToDevice, st));
+        const OpMember* d = (const OpMember*)((uint8_t*)dev.p + *host_off);
+        *host_off += (bytes + 255) & ~(size_t)255;
+        if (k == 0)
+            hipLaunchKernelGGL(k_opinion_multi<double>, dim3((uint32_t)blocks), dim3(256), 0, st, d,
+                               (uint32_t)v.size());
+        else
+            hipLaunchKernelGGL(k_opinion_multi<float>, dim3((uint32_t)blocks), dim3(256), 0, st, d,
+                               (uint32_t)v.size());
+        HIPCHECK(hipGetLastError());
+        v.clear();
+    }
     return VH_OK;
 }
 

@@ -137,6 +137,30 @@ int launch_generic(int transition, const DCtx* d_ctx, int ci, int call_t, uint64
                    int with_edge, hipStream_t st);
 int launch_sir(Sim& s, int t, int et, const vh_sir_params& p, uint64_t step);
 int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step);
+// one-pass opinion launches of many simulations collected into one fused
+// launch per state type (vh_batch_apply): while `g_opinion_defer` is set,
+// launch_opinion records a simulation's launch instead of enqueuing it when
+// the simulation runs on the collector's stream and needs no plan
+struct OpMember {
+    const uint32_t* off;
+    const uint32_t* src;
+    const void* x;
+    const float* w;
+    void* xo;
+    const uint32_t* heavy;
+    uint64_t n, nrows, rowbase, nheavy;
+    double eps;
+    uint32_t blk0, nh;   // first block of the member in the launch; its heavy-wave blocks
+};
+struct OpinionDefer {
+    hipStream_t stream = nullptr;
+    std::vector<OpMember> m[2];   // [0] Float64, [1] Float32 states
+};
+extern thread_local OpinionDefer* g_opinion_defer;
+// enqueue the fused launches of the collected members on `st`, member tables
+// staged through `host` (pinned) into `dev` (both grown as needed)
+int opinion_defer_launch(OpinionDefer& c, hipStream_t st, DevBuf& dev, void** host, size_t* host_cap,
+                         size_t* host_off);
 // diffuse over an unchanged device stencil: neighbours from the stencil
 // arithmetic instead of the CSR (same neighbour set; the body is an OR)
 int launch_diffuse_stencil(Sim& s, int t, int et, uint32_t field);

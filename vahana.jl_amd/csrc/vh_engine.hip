@@ -1657,6 +1657,86 @@ extern "C" int vh_apply(vh_sim_t h, const vh_apply_desc* d) {
     HIPCHECK(hipEventRecord(h->s.last, h->s.stream));
     return VH_OK;
 }
+// ============================================================ batched apply!
+struct vh_batch {
+    hipStream_t stream = nullptr;     // the stream of the last call's simulations
+    hipEvent_t done = nullptr;        // end of the last call's launches (staging reuse)
+    DevBuf dev;                       // member tables of one call
+    void* host = nullptr;             // their pinned staging
+    size_t host_cap = 0;
+};
+
+extern "C" int vh_batch_create(vh_batch_t* out) {
+    if (!out) return set_error(VH_ERR_ARG, "null argument");
+    vh_batch* b = new vh_batch();
+    hipError_t e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipEventCreate");
+    }
+    *out = b;
+    return VH_OK;
+}
+
+extern "C" int vh_batch_destroy(vh_batch_t b) {
+    if (!b) return VH_OK;
+    (void)hipEventSynchronize(b->done);
+    b->dev.release();
+    if (b->host) (void)hipHostFree(b->host);
+    (void)hipEventDestroy(b->done);
+    delete b;
+    return VH_OK;
+}
+
+// apply! of `desc` to every simulation, step by step: simulation i takes
+// part in the first steps[i] steps. The one-pass opinion bodies of all
+// simulations of a step run as one launch per state type; everything else of
+// an apply! (validation, bookkeeping, other bodies) is each simulation's own.
+extern "C" int vh_batch_apply(vh_batch_t b, const vh_sim_t* sims, uint32_t n, const vh_apply_desc* desc,
+                              const uint32_t* steps) {
+    if (!b || !desc || (n && (!sims || !steps))) return set_error(VH_ERR_ARG, "null argument");
+    if (!n) return VH_OK;
+    hipStream_t st = sims[0]->s.stream;
+    uint32_t smax = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!sims[i]) return set_error(VH_ERR_ARG, "null simulation");
+        if (sims[i]->s.stream != st) return set_error(VH_ERR_ARG, "batched simulations must share one stream");
+        smax = std::max(smax, steps[i]);
+    }
+    // the previous call's staging is free once its copies ran
+    HIPCHECK(hipEventSynchronize(b->done));
+    const size_t need = (size_t)smax * 2 * (((size_t)n * sizeof(OpMember) + 255) & ~(size_t)255);
+    if (need > b->host_cap) {
+        if (b->host) (void)hipHostFree(b->host);
+        b->host = nullptr;
+        b->host_cap = 0;
+        HIPCHECK(hipHostMalloc(&b->host, need, hipHostMallocDefault));
+        b->host_cap = need;
+    }
+    RC(b->dev.ensure(need + 256, st));
+    size_t off = 0;
+    OpinionDefer col;
+    col.stream = st;
+    int rc = VH_OK;
+    for (uint32_t k = 0; k < smax && rc == VH_OK; k++) {
+        g_opinion_defer = &col;
+        for (uint32_t i = 0; i < n && rc == VH_OK; i++) {
+            if (steps[i] <= k) continue;
+            vh_apply_desc r;
+            uint8_t blob[VH_PARAMS_BYTES];
+            rc = resolve_binds(sims[i]->s, desc, r, blob);
+            if (rc == VH_OK) rc = apply_one(sims[i]->s, &r);
+        }
+        g_opinion_defer = nullptr;
+        if (rc == VH_OK) rc = opinion_defer_launch(col, st, b->dev, &b->host, &b->host_cap, &off);
+    }
+    g_opinion_defer = nullptr;
+    RC(rc);
+    HIPCHECK(hipEventRecord(b->done, st));
+    for (uint32_t i = 0; i < n; i++) HIPCHECK(hipEventRecord(sims[i]->s.last, st));
+    return VH_OK;
+}
+
 extern "C" int vh_apply_n(vh_sim_t h, const vh_apply_desc* d, uint32_t steps) {
     if (!h || !d) return set_error(VH_ERR_ARG, "null argument");
     vh_apply_desc r;

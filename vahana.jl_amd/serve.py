@@ -7,10 +7,11 @@ Both models build their graphs on the device (directed G(n, p), mean degree
 from the request): SIR requests as ensemble members, opinion requests as
 simulations whose edges come from vh_add_edges_gnp.
 Continuous batching: every scheduler iteration advances all SIR requests by
-one segmented, graph-captured ensemble launch per step and every opinion
-request by one vh_apply_n call on its own stream, so their work overlaps on
-the GPU; requests that arrive meanwhile join at the next iteration as budget
-frees up.
+one segmented, graph-captured ensemble launch per step, all small opinion
+requests (state below the two-pass threshold) by one fused launch per step
+(vh_batch_apply) and every large opinion request by one vh_apply_n call, all
+on one compute stream; requests that arrive meanwhile join at the next
+iteration as budget frees up.
 
 `run_sequential` is the one-at-a-time baseline: the same requests, each built,
 run to its horizon and read back before the next one starts.
@@ -104,6 +105,10 @@ class Server:
         self.io = _stream(lib)
         self.ens = Ensemble()
         self.ens.set_streams(self.compute, self.io)
+        # small opinion requests (one-pass kernel) advance together, one
+        # fused launch per step
+        from . import Batch
+        self.batch = Batch()
         # host seconds per scheduler phase of the last run (observability)
         self.stats = {}
         lib.enable_asserts(0)
@@ -150,8 +155,13 @@ class Server:
                 if r.kind == "sir" and r.ens is not self.ens:
                     r.ens.step(min(q, r.left), use_graph=True)
             ta2 = time.perf_counter()
+            small = [r for r in stepping if r.kind == "opinion" and r.n * 8 < OPINION_PLAN_MIN_BYTES]
+            if small:
+                m0 = small[0].model
+                self.batch.apply([r.model.sim for r in small], vh.transitions.opinion(m0.E, 0.2), m0.T,
+                                 [m0.T, m0.E], m0.T, steps=[min(q, r.left) for r in small])
             for r in stepping:
-                if r.kind == "opinion":
+                if r.kind == "opinion" and r.n * 8 >= OPINION_PLAN_MIN_BYTES:
                     models.step_opinion_n(vh, r.model, min(q, r.left), eps=0.2)
             for r in stepping:
                 r.left -= min(q, r.left)
@@ -279,6 +289,7 @@ class Server:
         done.append(r)
 
     def close(self):
+        self.batch.close()
         self.ens.close()
         lib = _abi.lib()
         lib.stream_destroy(self.io)
@@ -325,6 +336,7 @@ def run_sequential(vh, requests, time_scale=1.0, keep_results=False, stats=None)
         ph["run"] += tc - tb
         ph["read"] += td - tc
         r2 = types.SimpleNamespace(**{k: v for k, v in vars(r).items()})
+        r2.build_s, r2.run_s, r2.read_s = tb - ta, tc - tb, td - tc
         r2.status = "done"
         r2.digest = _digest(st)
         if keep_results:

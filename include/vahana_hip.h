@@ -324,6 +324,13 @@ int vh_num_transitions(vh_sim_t s, uint64_t* out);
 int vh_restore_died(vh_sim_t sim, uint8_t t, const uint8_t* died, uint64_t n);
 /* read_snapshot! sets sim.num_transitions to the snapshot's transition */
 int vh_set_num_transitions(vh_sim_t sim, uint64_t n);
+/* last_change of an agent (kind 0) or edge (kind 1) type: the transition that
+ * last wrote it (finish_write! src/AgentMethods.jl:433, src/EdgeMethods.jl:682)
+ * or whose deaths removed some of its edges (src/AgentMethods.jl:347-353); 0
+ * after finish_init!. write_snapshot skips types unchanged since their last
+ * write (src/HDF5.jl:497-503,592-598); read_snapshot! restores the value. */
+int vh_last_change(vh_sim_t sim, int kind, uint8_t type_id, uint64_t* out);
+int vh_set_last_change(vh_sim_t sim, int kind, uint8_t type_id, uint64_t value);
 /* mapreduce(sim, f, op, T; init, datatype) — kind 0: agents, 1: edges */
 int vh_mapreduce(vh_sim_t s, int kind, uint8_t type_id, const vh_map_desc* f,
                  int op, int dtype, const void* init, void* out);

@@ -141,7 +141,7 @@ typedef struct {
     u64vec reuse;                      /* reuseable slot numbers            */
     uint64_t cap;                      /* allocated slots (host amortisation) */
 } rw_t;
-typedef struct { rw_t read, write; uint64_t nextid; int prepared, writeable; } agents_t;
+typedef struct { rw_t read, write; uint64_t nextid; int prepared, writeable; uint64_t last_change; } agents_t;
 
 /* One per-target container (the element of the Vector/Dict field).  For
  * Stateless+IgnoreFrom layouts `n` is the Int64 count (or the Bool). */
@@ -153,6 +153,7 @@ typedef struct {
     int readable, writeable, add_existing;
     /* agentsontarget: source id -> targets (src/EdgeMethods.jl:375-384) */
     u64vec* aot[MAXA + 1]; uint8_t* aot_has[MAXA + 1]; uint64_t aot_n[MAXA + 1];
+    uint64_t last_change;
 } edges_t;
 
 struct or_sim {
@@ -511,6 +512,15 @@ static void remove_agent_source(or_sim* s, int et, const u64vec* died) {
 }
 
 /* ------------------------------------------------------------ finish_write */
+/* edges (or counts) held by the containers of a field */
+static uint64_t field_edges(const field_t* f) {
+    uint64_t c = 0;
+    if (!f) return 0;
+    for (int t = 0; t <= MAXA; t++)
+        for (uint64_t r = 0; f->rows[t] && r < f->nrows[t]; r++) c += f->rows[t][r].n;
+    return c;
+}
+
 /* agent finish_write! src/AgentMethods.jl:300-439 */
 static void finish_write_agents(or_sim* s, int t) {
     agents_t* A = &s->a[t];
@@ -521,10 +531,19 @@ static void finish_write_agents(or_sim* s, int t) {
         u64vec aids = {0, 0, 0};
         for (uint64_t i = 0; i < A->write.reuse.n; i++)
             u64_push(&aids, agent_id((uint64_t)t, A->write.reuse.v[i]));
-        for (uint64_t i = 0; i < aids.n; i++)
-            for (int et = 0; et < s->m->ne; et++) remove_agent_target(s, et, aids.v[i]);
-        if (aids.n)
+        if (aids.n) {
+            /* network_changed: an edge type that lost edges to the deaths
+             * gets last_change = num_transitions (src/AgentMethods.jl:347-353;
+             * there the flag is set when the dead agent had a container or
+             * a source entry, here when edges were removed) */
+            uint64_t before[MAXE];
+            for (int et = 0; et < s->m->ne; et++) before[et] = field_edges(s->e[et].write);
+            for (uint64_t i = 0; i < aids.n; i++)
+                for (int et = 0; et < s->m->ne; et++) remove_agent_target(s, et, aids.v[i]);
             for (int et = 0; et < s->m->ne; et++) remove_agent_source(s, et, &aids);
+            for (int et = 0; et < s->m->ne; et++)
+                if (field_edges(s->e[et].write) != before[et]) s->e[et].last_change = s->num_transitions;
+        }
         free(aids.v);
     }
     if (bytes && must_copy) {
@@ -545,6 +564,7 @@ static void finish_write_agents(or_sim* s, int t) {
     }
     for (uint64_t i = 0; i < A->write.reuse.n; i++) u64_push(&A->read.reuse, A->write.reuse.v[i]);
     A->write.reuse.n = 0;
+    A->last_change = s->num_transitions;
     A->writeable = 0;
 }
 
@@ -552,7 +572,29 @@ static void finish_write_agents(or_sim* s, int t) {
 static void finish_write_edges(or_sim* s, int et) {
     edges_t* X = &s->e[et];
     if (X->write != X->read) { field_free(X->read); X->read = X->write; }
+    X->last_change = s->num_transitions;
     X->writeable = 0;
+}
+
+int or_last_change(or_sim* s, int kind, uint8_t t, uint64_t* out) {
+    if (kind == 0) {
+        if (t < 1 || t > s->m->na) return fail(VH_ERR_UNREGISTERED, "unregistered agent type");
+        *out = s->a[t].last_change;
+    } else {
+        if (t >= s->m->ne) return fail(VH_ERR_UNREGISTERED, "unregistered edge type");
+        *out = s->e[t].last_change;
+    }
+    return VH_OK;
+}
+int or_set_last_change(or_sim* s, int kind, uint8_t t, uint64_t v) {
+    if (kind == 0) {
+        if (t < 1 || t > s->m->na) return fail(VH_ERR_UNREGISTERED, "unregistered agent type");
+        s->a[t].last_change = v;
+    } else {
+        if (t >= s->m->ne) return fail(VH_ERR_UNREGISTERED, "unregistered edge type");
+        s->e[t].last_change = v;
+    }
+    return VH_OK;
 }
 
 /* finish_init! src/Simulation.jl:403-476 (one process: no distribute!) */

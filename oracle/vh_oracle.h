@@ -35,6 +35,8 @@ int or_apply(or_sim* s, const vh_apply_desc* d);
 int or_sim_set_value(or_sim* s, int kind, uint32_t slot, int dtype, const void* value);
 int or_sim_get_value(or_sim* s, int kind, uint32_t slot, int* dtype, void* value);
 int or_num_transitions(or_sim* s, uint64_t* out);
+int or_last_change(or_sim* s, int kind, uint8_t t, uint64_t* out);
+int or_set_last_change(or_sim* s, int kind, uint8_t t, uint64_t v);
 int or_restore_died(or_sim* s, uint8_t t, const uint8_t* died, uint64_t n);
 int or_set_num_transitions(or_sim* s, uint64_t n);
 /* order: 0 = the reference's sequential left fold, 1 = the device's fixed tree */

@@ -843,3 +843,120 @@ def remove_edges(V):
             V.finish_simulation_(x)
         out["remove_from/" + name] = r
     return out
+
+
+# ------------------------------------------- snapshots per edge hint (HDF5)
+def snapshot_hint_model(V, variant):
+    """The three models of test/hdf5_snapshot.jl:79-112 (types of
+    test/hdf5_common.jl:19-60): default hints; Agent :Immortal, EdgeState
+    :IgnoreFrom, StatelessEdge :Stateless; Agent :Immortal, EdgeState
+    :NumEdgesOnly, StatelessEdge :HasEdgeOnly + :SingleType (target Agent).
+    Agent keeps InnerStruct's fields flat (foo = inner.i); RasterAgent holds
+    its cell number, which the die_if body reads."""
+    Agent = V.struct("Agent", [("foo", "<i8"), ("inner_f", "<f8"), ("f", "<f8")])
+    RasterAgent = V.struct("RasterAgent", [("foo", "<i8")])
+    EmptyAgentVector = V.struct("EmptyAgentVector")
+    EdgeState = V.struct("EdgeState", [("f", "<u8"), ("t", "<u8")])
+    StatelessEdge = V.struct("StatelessEdge")
+    RasterEdge = V.struct("RasterEdge", [("a", "<i8")])
+    EmptyEdgeVector = V.struct("EmptyEdgeVector")
+    ah = {"default": (), "immortal": ("Immortal",), "neighbors": ("Immortal",)}[variant]
+    eh = {"default": (), "immortal": ("IgnoreFrom",), "neighbors": ("NumEdgesOnly",)}[variant]
+    sh = {"default": (), "immortal": ("Stateless",), "neighbors": ("HasEdgeOnly", "SingleType")}[variant]
+    model = (V.ModelTypes() | V.register_agenttype_(Agent, *ah) | V.register_agenttype_(RasterAgent)
+             | V.register_edgetype_(EdgeState, *eh) | V.register_edgetype_(RasterEdge)
+             | V.register_edgetype_(StatelessEdge, *sh,
+                                    target=Agent if variant == "neighbors" else None)
+             | V.register_agenttype_(EmptyAgentVector) | V.register_edgetype_(EmptyEdgeVector)
+             | V.create_model(f"hdf5_{variant}"))
+    return model, dict(Agent=Agent, RasterAgent=RasterAgent, EmptyAgentVector=EmptyAgentVector,
+                       EdgeState=EdgeState, StatelessEdge=StatelessEdge, RasterEdge=RasterEdge,
+                       EmptyEdgeVector=EmptyEdgeVector)
+
+
+def _snap_params():
+    return ({"pb": 1, "pa": [2.0, 2.1], "pos": {"x": 3, "y": 4, "z": 5}},
+            {"pos": {"x": 1, "y": 2}, "ga": 3, "gb": [4.0, 4.1], "gc": [5, 6, 7], "empty": []})
+
+
+def snapshot_hint_createsim(V, model, t):
+    """createsim test/hdf5_common.jl:66-99: 16 agents, each with an
+    EdgeState edge to itself and to agent 2, StatelessEdges to agents 2 and
+    5, and a RasterEdge to cell mod1(id, 8) of a (2,2,2,1) raster."""
+    p, g = _snap_params()
+    sim = V.create_simulation(model, params=p, globals_=g)
+    A, ES, SE, RE = t["Agent"], t["EdgeState"], t["StatelessEdge"], t["RasterEdge"]
+    ids = [int(i) for i in V.add_agents_(sim, [A(i + 1, float(i), float(i + 2)) for i in range(1, 17)])]
+    for i in ids:
+        V.add_edge_(sim, i, i, ES(i, i))
+        V.add_edge_(sim, i, ids[1], SE())
+        V.add_edge_(sim, i, ids[4], SE())
+        V.add_edge_(sim, i, ids[1], ES(i, ids[0]))
+    cells = iter(range(1, 9))
+    rids = V.raster.add_raster_(sim, "fourdspace", (2, 2, 2, 1), lambda _p: t["RasterAgent"](next(cells)))
+    for i in ids:
+        nr = V.agent_nr(i)
+        V.add_edge_(sim, i, int(rids[(nr - 1) % 8]), RE(nr % 8 + 1))
+    V.finish_init_(sim)
+    return sim
+
+
+def snapshot_hint_raster_step(V, sim, t):
+    """In place of remove_some_rasternodes (test/hdf5_common.jl:129-134):
+    the raster agents re-add their RasterEdges (RasterEdge written), then
+    the odd cells die (RasterAgent written; their RasterEdges go with them,
+    which changes RasterEdge once more)."""
+    RA, RE = t["RasterAgent"], t["RasterEdge"]
+    V.apply_(sim, V.transitions.readd_edges(RE), [RA], [RA, RE], [RE])
+    V.apply_(sim, V.transitions.die_if("odd"), [RA], [RA], [RA])
+
+
+def snapshot_hint_runsim(V, model, t, path):
+    """runsim test/hdf5_common.jl:101-127: snapshots "Initial state", after
+    the Agent transition and push_global!, and "Final state"."""
+    sim = snapshot_hint_createsim(V, model, t)
+    V.snapshot.write_snapshot(sim, path, "Initial state")
+    A = t["Agent"]
+    V.apply_(sim, V.transitions.count_edges(t["StatelessEdge"]), [A], [A, t["StatelessEdge"]], [A])
+    V.push_global_(sim, "gb", 2.1)
+    V.push_global_(sim, "gc", 4)
+    V.snapshot.write_snapshot(sim, path)
+    snapshot_hint_raster_step(V, sim, t)
+    V.snapshot.write_snapshot(sim, path, "Final state")
+    return sim
+
+
+def snapshot_hint_restore(V, model, path, **kw):
+    """restore test/hdf5_common.jl:136-142: a simulation created with other
+    params/globals, then read_snapshot!."""
+    sim = V.create_simulation(model, params={"pb": 0, "pa": [0.0, 0.0], "pos": {"x": 0, "y": 0, "z": 0}},
+                              globals_={"pos": {"x": 0, "y": 0}, "ga": 0, "gb": [0.0], "gc": [0],
+                                        "empty": []})
+    V.snapshot.read_snapshot(sim, path, **kw)
+    return sim
+
+
+def snapshot_hint_state(V, sim, t):
+    """What test(sim, restored) of test/hdf5_snapshot.jl:29-75 compares: per
+    agent type died flags, states (live slots), slot count and last_change;
+    per edge type the edges and last_change; params and
+    globals_last_change."""
+    out = {}
+    for name in ("Agent", "RasterAgent", "EmptyAgentVector"):
+        T = t[name]
+        st, died = sim._read_agents(T)
+        raw = (np.ascontiguousarray(st).view(np.uint8).reshape(len(died), -1)[~died].tobytes()
+               if T.fields else b"")
+        out[name] = (died.tobytes(), raw, len(died), V.snapshot.last_change(sim, T))
+    for name in ("EdgeState", "StatelessEdge", "RasterEdge", "EmptyEdgeVector"):
+        E = t[name]
+        to, fr, st = sim._read_edges(E)
+        ignorefrom = sim.model.has_hint(E, "IgnoreFrom")
+        raw = np.ascontiguousarray(st).view(np.uint8).tobytes() if st is not None else b""
+        out[name] = (to.tobytes(), b"" if ignorefrom else fr.tobytes(), raw,
+                     V.snapshot.last_change(sim, E))
+    out["params"] = repr(sorted(vars(sim.params).items()) if not isinstance(sim.params, dict)
+                         else sorted(sim.params.items()))
+    out["globals_last_change"] = sim.globals_last_change
+    out["num_transitions"] = sim._num_transitions()
+    return out

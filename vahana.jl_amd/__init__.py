@@ -245,7 +245,9 @@ class Model:
             _check(lib.register_agenttype(self.handle, hints, T.nbytes, C.byref(tid)), lib)
             self.aid[T] = tid.value
             self.agent_types.append((T, hints))
+        self.etarget = {}      # edge type -> target agent type (:SingleType), or None
         for T, hints, target, size in types.edges:
+            self.etarget[T] = target
             eid = C.c_uint8()
             tgt = self.aid[target] if target is not None else -1
             _check(lib.register_edgetype(self.handle, hints, T.nbytes, tgt, int(size),

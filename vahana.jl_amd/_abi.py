@@ -134,6 +134,8 @@ SIGNATURES = [
     ("sim_copy", C.c_int, [_vp, C.POINTER(_vp)]),
     ("restore_died", C.c_int, [_vp, C.c_uint8, _u8p, C.c_uint64]),
     ("set_num_transitions", C.c_int, [_vp, C.c_uint64]),
+    ("last_change", C.c_int, [_vp, C.c_int, C.c_uint8, _u64p]),
+    ("set_last_change", C.c_int, [_vp, C.c_int, C.c_uint8, C.c_uint64]),
     ("add_agents", C.c_int, [_vp, C.c_uint8, _vp, C.c_uint64, _u64p]),
     ("add_edges", C.c_int, [_vp, C.c_uint8, _u64p, _u64p, _vp, C.c_uint64]),
     ("finish_init", C.c_int, [_vp]),

@@ -40,6 +40,7 @@ struct AgentStore {
     DevBuf reuse;                   // LIFO stack of free slots (uint32, bottom first)
     uint64_t nreuse = 0;
     bool writeable = false;
+    uint64_t last_change = 0;       // transition of the last write (src/AgentMethods.jl:433)
     const uint8_t* rstate() const { return state[rd].as<uint8_t>(); }
     uint8_t* wstate() const { return indep ? state[rd].as<uint8_t>() : state[rd ^ 1].as<uint8_t>(); }
 };
@@ -98,6 +99,9 @@ struct EdgeStore {
         for (DevBuf* b : all) f(*b);
     }
     bool writeable = false;
+    // transition of the last write or death removal (src/EdgeMethods.jl:682,
+    // src/AgentMethods.jl:347-353)
+    uint64_t last_change = 0;
     // in-edges requested as a raster stencil during init (vh_add_edges_stencil)
     bool stencil = false;
     StencilSpec sspec;

@@ -309,6 +309,36 @@ extern "C" int vh_set_num_transitions(vh_sim_t h, uint64_t n) {
     return VH_OK;
 }
 
+// last_change of an agent (kind 0) or edge (kind 1) type: the transition
+// that last wrote it (src/AgentMethods.jl:433, src/EdgeMethods.jl:682), or
+// whose deaths removed edges of it (src/AgentMethods.jl:347-353); 0 after
+// initialisation
+extern "C" int vh_last_change(vh_sim_t h, int kind, uint8_t type_id, uint64_t* out) {
+    if (!h || !out) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (kind == 0) {
+        if (type_id < 1 || type_id > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+        *out = s.a[type_id].last_change;
+    } else {
+        if (type_id >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+        *out = s.e[type_id].last_change;
+    }
+    return VH_OK;
+}
+// read_snapshot! restores last_change from the snapshot (src/HDF5.jl:1267,1434-1436)
+extern "C" int vh_set_last_change(vh_sim_t h, int kind, uint8_t type_id, uint64_t value) {
+    if (!h) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (kind == 0) {
+        if (type_id < 1 || type_id > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+        s.a[type_id].last_change = value;
+    } else {
+        if (type_id >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+        s.e[type_id].last_change = value;
+    }
+    return VH_OK;
+}
+
 extern "C" int vh_num_transitions(vh_sim_t h, uint64_t* out) {
     *out = h->s.num_transitions;
     return VH_OK;
@@ -1516,16 +1546,24 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
     // deaths remove edges of every edge type (src/AgentMethods.jl:314-345)
     if (any_death)
         for (int et = 0; et < m.ne; et++)
-            if (!(kind == K_CONTACT && et == d->role_edge[0])) RC(filter_deaths(s, s.e[et], dn));
+            if (!(kind == K_CONTACT && et == d->role_edge[0])) {
+                const uint64_t before = s.e[et].nedges;
+                RC(filter_deaths(s, s.e[et], dn));
+                if (s.e[et].nedges != before) s.e[et].last_change = step;   // network_changed
+            }
 
     // ---- publish (double-buffer swap instead of the reference's memcpy)
     for (int t = 1; t <= m.na; t++) {
         AgentStore& A = s.a[t];
         if (A.writeable) A.nslots = std::max(A.nslots, A.nextid - 1);
         if (A.writeable && !A.indep) A.rd ^= 1;
+        if (A.writeable) A.last_change = step;
         A.writeable = false;
     }
-    for (int et = 0; et < m.ne; et++) s.e[et].writeable = false;
+    for (int et = 0; et < m.ne; et++) {
+        if (s.e[et].writeable) s.e[et].last_change = step;
+        s.e[et].writeable = false;
+    }
     s.num_transitions++;
     if (g_asserts) RC(check_errors(s));
     return VH_OK;

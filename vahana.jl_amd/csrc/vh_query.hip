@@ -587,6 +587,7 @@ extern "C" int vh_sim_copy(vh_sim_t h, vh_sim_t* out) {
         B.cap = A.cap;
         B.rd = A.rd;
         B.nreuse = A.nreuse;
+        B.last_change = A.last_change;
         size_t sb = A.cap * A.bytes;
         RC(copy_buf(B.state[0], A.state[0], sb, d.stream));
         RC(copy_buf(B.state[1], A.state[1], sb, d.stream));
@@ -615,6 +616,7 @@ extern "C" int vh_sim_copy(vh_sim_t h, vh_sim_t* out) {
         F.sspec = E.sspec;
         F.stencil_version = E.stencil_version;
         F.version = E.version;
+        F.last_change = E.last_change;
         memcpy(F.rowbase, E.rowbase, sizeof F.rowbase);
         memcpy(F.nrows, E.nrows, sizeof F.nrows);
         F.rows = E.rows;

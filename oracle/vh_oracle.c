@@ -157,7 +157,8 @@ typedef struct {
 } edges_t;
 
 struct or_sim {
-    or_model* m;
+    or_model* m;                       /* -> mown */
+    or_model mown;                     /* own copy: the model may be destroyed first */
     agents_t a[MAXA + 1];
     edges_t e[MAXE];
     int initialized, intransition;
@@ -243,7 +244,8 @@ static void cont_push(cont_t* c, uint64_t from, const uint8_t* st, uint32_t byte
 int or_sim_create(or_model* m, or_sim** out) {
     or_sim* s = (or_sim*)calloc(1, sizeof(or_sim));
     if (!s) return fail(VH_ERR_OOM, "oom");
-    s->m = m;
+    s->mown = *m;
+    s->m = &s->mown;
     for (int t = 1; t <= m->na; t++) s->a[t].nextid = 1;
     for (int e = 0; e < m->ne; e++) {
         s->e[e].read = field_new();
@@ -289,6 +291,7 @@ static void rw_copy(rw_t* dst, const rw_t* src, uint32_t bytes, int mortal) {
 int or_sim_copy(or_sim* s, or_sim** out) {
     or_sim* c = (or_sim*)calloc(1, sizeof(or_sim));
     *c = *s;
+    c->m = &c->mown;
     for (int t = 1; t <= s->m->na; t++) {
         rw_copy(&c->a[t].read, &s->a[t].read, s->m->a[t].bytes, mortal_t(s, t));
         rw_copy(&c->a[t].write, &s->a[t].write, s->m->a[t].bytes, mortal_t(s, t));

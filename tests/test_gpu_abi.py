@@ -127,3 +127,18 @@ def test_batch_apply_opinion_matches_solo(vh, ora, gpu_ready, opts):
     for m in sims:
         m.sim._destroy()
     lib.stream_destroy(st)
+
+
+def test_model_destroyed_before_simulation_device(vh, gpu_ready):
+    """The device simulation keeps its own copy of the model's types: the
+    model handle destroyed first leaves it usable and destroyable."""
+    print("[gpu] model destroyed before its simulation", flush=True)
+    sim, A, E, ids = _single_edge_sim(vh)
+    sim.model.lib.model_destroy(sim.model.handle)
+    sim.model.handle = type(sim.model.handle)()
+    sim._add_edges_array(E, [ids[0]], [ids[1]], _st(E, [3]))
+    vh.finish_init_(sim)
+    c = vh.copy_simulation(sim)
+    assert vh.num_edges(sim, E) == 1 and vh.num_edges(c, E) == 1
+    c._destroy()
+    sim._destroy()

@@ -261,3 +261,18 @@ def test_snapshot_raster_ids(ora, tmp_path):
     assert ora.raster.cellid(sim2, "grid", (2, 2)) == ora.raster.cellid(sim, "grid", (2, 2))
     ora.apply_(sim2, ora.transitions.diffuse(GridE, GridA), [GridA], [GridA, GridE], [GridA])
     assert ora.mapreduce(sim2, "active", "+", GridA) == 9
+
+
+def test_model_destroyed_before_simulation(ora):
+    """A simulation keeps its own copy of the model's types: destroying the
+    model handle first (as a garbage collector may, for objects in a
+    reference cycle) leaves the simulation usable and destroyable."""
+    model, SIR, Contact = SC.sir_model(ora)
+    sim = ora.create_simulation(model)
+    ids = sim._add_agents_array(SIR, np.zeros(3, dtype=SIR.dtype))
+    model.lib.model_destroy(model.handle)
+    model.handle = type(model.handle)()      # Model.__del__ then skips it
+    sim._add_edges_array(Contact, [ids[0]], [ids[1]])
+    ora.finish_init_(sim)
+    assert ora.num_edges(sim, Contact) == 1
+    sim._destroy()

@@ -113,6 +113,9 @@ struct EdgeStore {
 };
 
 struct Sim {
+    // the simulation's own copy of its model's types: a model handle may be
+    // destroyed while simulations of it live
+    Model model{};
     Model* m = nullptr;
     int device = 0;                // the device current at vh_sim_create
     hipStream_t stream = nullptr;

@@ -159,7 +159,8 @@ extern "C" int vh_sim_create(vh_model_t mh, vh_sim_t* out) {
     if (!mh || !out) return set_error(VH_ERR_ARG, "null argument");
     vh_sim* h = new vh_sim();
     Sim& s = h->s;
-    s.m = &mh->m;
+    s.model = mh->m;
+    s.m = &s.model;
     hipError_t e = hipGetDevice(&s.device);
     if (e != hipSuccess) {
         delete h;

@@ -574,7 +574,9 @@ extern "C" int vh_sim_copy(vh_sim_t h, vh_sim_t* out) {
     Sim& s = h->s;
     HIPCHECK(hipStreamSynchronize(s.stream));
     vh_sim* c = nullptr;
-    RC(vh_sim_create((vh_model_t)((char*)s.m - offsetof(vh_model, m)), &c));
+    vh_model types;
+    types.m = *s.m;
+    RC(vh_sim_create(&types, &c));
     Sim& d = c->s;
     d.initialized = s.initialized;
     d.num_transitions = s.num_transitions;

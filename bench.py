@@ -194,11 +194,12 @@ def main():
     f32 = args.workload == "opinion32"
     t0 = time.perf_counter()
     if args.workload.startswith("opinion"):
-        a, b = vh.inputs.graph_ba(n, 5, 1)
-        edges = vh.inputs.symmetric_weighted(a, b, 2)
-        del a, b
+        # the graph and the breadth-first numbering built on the device at
+        # finish_init! (the same graph the host generator gives; the CPU
+        # baseline generates its copy on the host)
+        edges = None
         relabel = not args.no_relabel
-        m = M.build_opinion(vh, n, seed=1, f32=f32, edges=edges, relabel=relabel)
+        m = M.build_opinion(vh, n, seed=1, f32=f32, relabel=relabel, device_graph=True)
         step = lambda: M.step_opinion(vh, m, eps=0.2)  # noqa: E731
         nbytes = opinion_bytes(n, m.nedges, f32)
         dtype = "f32" if f32 else "f64"
@@ -241,7 +242,8 @@ def main():
         workload = f"SIR contagion, directed G(n,p), N={n}, E={m.nedges}, p_inf=0.05"
     if not args.workload.startswith("opinion"):
         del edges
-    log(f"init {time.perf_counter() - t0:.1f}s: {workload}")
+    init_s = time.perf_counter() - t0
+    log(f"init {init_s:.2f}s: {workload}")
 
     stream = torch.cuda.Stream(device=dev)
     L = vh._abi.lib()
@@ -289,7 +291,7 @@ def main():
         # the same graph and states in generation order (no renumbering at
         # finish_init!), timed the same way
         x_relabel = m
-        m2 = M.build_opinion(vh, n, seed=1, f32=f32, edges=edges, relabel=False)
+        m2 = M.build_opinion(vh, n, seed=1, f32=f32, relabel=False, device_graph=True)
         ms2, ev2, _ = timed(m2.sim, lambda: M.step_opinion(vh, m2, eps=0.2))
         gen_order = {"ms_per_step": ms2, "kernel_ms": ev2, "value": n / (ms2 * 1e-3),
                      "roofline_frac": nbytes / (ev2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -314,7 +316,7 @@ def main():
     drift = None
     if f32:
         # the Float32 variant against the Float64 run of the same inputs
-        m64 = M.build_opinion(vh, n, seed=1, f32=False, edges=edges, relabel=not args.no_relabel)
+        m64 = M.build_opinion(vh, n, seed=1, f32=False, relabel=not args.no_relabel, device_graph=True)
         for _ in range(1 + args.warmup + args.steps - 1):
             M.step_opinion(vh, m64, eps=0.2)
         x64 = np.asarray(m64.sim._read_agents(m64.T)[0]["opinion"])
@@ -335,8 +337,6 @@ def main():
         cpu = cpu_baseline(args.workload, f32, relabel=not args.no_relabel, n=n if opin else None,
                            edges=edges if opin else None)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {cpu}")
-        if opin:
-            del edges
         t0 = time.perf_counter()
         cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32, relabel=not args.no_relabel,
                                                   n=n if opin else None)
@@ -356,7 +356,8 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic (seeded counter-hash graph and states)",
-        "config": {"workload": workload, "agents": n, "edges": m.nedges, "parallelism": "single GPU"},
+        "config": {"workload": workload, "agents": n, "edges": m.nedges, "parallelism": "single GPU",
+                   "init_s": round(init_s, 3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},

@@ -248,6 +248,16 @@ int vh_add_edges(vh_sim_t s, uint8_t edge_type_id, const uint64_t* from,
  * give the same edges on the host. No other edges may be added to `et`. */
 int vh_add_edges_gnp(vh_sim_t sim, uint8_t et, double mean_deg, uint64_t graph_seed,
                      uint64_t weight_seed);
+/* Bulk init-phase edges for add_graph! of a generated Barabasi-Albert graph
+ * (src/GraphsSupport.jl:34-57; the edges add_edge!, src/EdgeMethods.jl:
+ * 388-523, would add): the undirected pairs vh_graph_ba(n, m, graph_seed)
+ * gives over the n agents of the edge type's target type, each added in
+ * both directions (pair k: edge 2k a_k -> b_k, edge 2k + 1 b_k -> a_k); a
+ * Float32 edge state is the weight of vh_fill_open01f(weight_seed, 0,
+ * 6 << 40) at that edge position. Generated on the device at finish_init!
+ * (and numbered with the other init edges by vh_finish_init_relabel). No
+ * other edges may be added to `et`; 1 <= m <= 255, 2 m n < 2^32. */
+int vh_add_edges_ba(vh_sim_t sim, uint8_t et, uint32_t m, uint64_t graph_seed, uint64_t weight_seed);
 
 /* connect_raster_neighbors! src/Raster.jl:134-164 for a raster whose
  * prod(dims) cells are the agents first_nr, first_nr + 1, ... of cell_type

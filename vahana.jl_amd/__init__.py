@@ -515,6 +515,14 @@ class Simulation:
         _check(self.lib.add_edges_gnp(self.handle, self._eid(T), float(mean_deg), int(graph_seed),
                                       int(weight_seed)), self.lib)
 
+    def _add_edges_ba(self, T, m, graph_seed, weight_seed=0):
+        """The symmetrised Barabasi-Albert graph of edge type T generated on
+        the device at finish_init! (vh_add_edges_ba); the host twin is
+        inputs.symmetric_weighted(*inputs.graph_ba(n, m, graph_seed),
+        weight_seed)."""
+        _check(self.lib.add_edges_ba(self.handle, self._eid(T), int(m), int(graph_seed), int(weight_seed)),
+               self.lib)
+
     def _desc(self, func, call, read, write, add_existing=(), with_edge=None):
         tr = _resolve(func)
         call, read, write = _as_list(call), _as_list(read), _as_list(write)

@@ -153,6 +153,7 @@ SIGNATURES = [
 DEVICE_ONLY = [
     ("apply_n", C.c_int, [_vp, C.POINTER(ApplyDesc), C.c_uint32]),
     ("add_edges_gnp", C.c_int, [_vp, C.c_uint8, C.c_double, C.c_uint64, C.c_uint64]),
+    ("add_edges_ba", C.c_int, [_vp, C.c_uint8, C.c_uint32, C.c_uint64, C.c_uint64]),
     ("finish_init_relabel", C.c_int, [_vp, C.c_uint8, _u64p]),
     ("add_edges_stencil", C.c_int, [_vp, C.c_uint8, C.c_uint32, C.c_uint64, C.c_uint32,
                                     C.POINTER(C.c_int64), C.c_double, C.c_int, C.c_int,

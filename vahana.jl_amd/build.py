@@ -14,7 +14,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("VH_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["vh_prims.hip", "vh_engine.hip", "vh_query.hip", "vh_bodies.hip", "vh_ensemble.hip",
-           "vh_graph.cpp"]
+           "vh_graphgen.hip", "vh_graph.cpp"]
 
 # -ffp-contract=off: no fused multiply-add, the model bodies round every
 # product and sum like the reference's IEEE arithmetic; packed FP32 ops are

@@ -110,6 +110,9 @@ struct EdgeStore {
     bool gen = false;
     double gen_mean_deg = 0.0;
     uint64_t gen_seed = 0, gen_wseed = 0;
+    // in-edges requested as a device Barabasi-Albert graph, both directions
+    // of every pair (vh_add_edges_ba; seeds in gen_seed / gen_wseed)
+    uint32_t ba_m = 0;             // 0: none
 };
 
 struct Sim {
@@ -137,6 +140,10 @@ struct Sim {
     uint64_t vals[2][VH_MAX_VALUES] = {};
     uint8_t vdt[2][VH_MAX_VALUES] = {};
     uint32_t vset[2] = {0, 0};     // slots set, per kind
+    // vh_finish_init_relabel: the new numbers of the agents of relabel_type
+    // (uint32, 1-based, by old number - 1), applied to device-generated edges
+    DevBuf relabel_map;
+    int relabel_type = 0;
 };
 
 // model-body launchers (vh_bodies.hip)
@@ -188,6 +195,16 @@ int gnp_device(uint64_t n, double mean_deg, uint64_t seed, DevBuf& off, DevBuf& 
 // CSR of a raster stencil (vh_bodies.hip): off[rows + 1] with the cell rows
 // at rowbase, sources as AgentIDs (src64) or numbers, uniform states
 int stencil_csr(Sim& s, EdgeStore& E, uint64_t rowbase, uint64_t rows);
+// Barabasi-Albert pairs on the device (vh_graphgen.hip), equal to
+// vh_graph_ba: a[k], b[k] for k < ba_pairs(n, m)
+uint64_t ba_pairs(uint64_t n, uint32_t m);
+int ba_device(uint64_t n, uint32_t m, uint64_t seed, uint32_t* a, uint32_t* b, Scratch& sc, hipStream_t st);
+// breadth-first numbering on the device, equal to vh_bfs_order: newnr[v - 1]
+// for the undirected edges (ea[j], eb[j]) (1-based)
+int bfs_device(uint64_t n, uint64_t M, const uint32_t* ea, const uint32_t* eb, uint32_t* newnr, Scratch& sc,
+               hipStream_t st);
+// scratch bytes bfs_device takes (reserve them beforehand: it takes many pieces)
+size_t bfs_scratch_bytes(uint64_t n, uint64_t M);
 // Float32 weights in (0, 1) by CSR position (vh_ensemble.hip)
 void fill_weights(float* w, uint64_t n, uint64_t wseed, hipStream_t st);
 // host graph helper (vh_graph.cpp)

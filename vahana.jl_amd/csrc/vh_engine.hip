@@ -224,6 +224,7 @@ static void sim_release(Sim& s) {
     }
     s.d_ctx.release(); s.d_err.release(); s.d_stats.release();
     s.scratch.release();
+    s.relabel_map.release();
     if (s.h_stats) (void)hipHostFree(s.h_stats);
     s.h_stats = nullptr;
     if (s.own_stream && s.stream) (void)hipStreamDestroy(s.stream);
@@ -872,6 +873,32 @@ int vh::ensure_slots(Sim& s, int t, uint64_t slots) {
     return VH_OK;
 }
 
+// Bulk init-phase edges: the symmetrised Barabasi-Albert graph that
+// vh_graph_ba + add_edge! of both directions of every pair (a -> b, then
+// b -> a, pair by pair) would give, Float32 weights by that position,
+// generated on the device at finish_init! (add_graph! of a generated graph,
+// src/GraphsSupport.jl:34-57).
+extern "C" int vh_add_edges_ba(vh_sim_t h, uint8_t et, uint32_t m, uint64_t graph_seed, uint64_t weight_seed) {
+    if (!h) return set_error(VH_ERR_ARG, "null simulation");
+    Sim& s = h->s;
+    if (et >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+    if (s.initialized)
+        return set_error(VH_ERR_ASSERT, "You can call add_edge! only in the initialization phase");
+    EdgeStore& E = s.e[et];
+    if (E.target < 1) return set_error(VH_ERR_ARG, "device Barabasi-Albert: the edge type needs a target type");
+    if (has(E.hints, VH_HINT_IGNOREFROM) || has(E.hints, VH_HINT_SINGLEEDGE))
+        return set_error(VH_ERR_ARG, "device Barabasi-Albert: the edge type must keep sources and allow many edges");
+    if (E.bytes != 0 && E.bytes != 4)
+        return set_error(VH_ERR_ARG, "device Barabasi-Albert: edge state must be empty or one Float32 weight");
+    if (!E.h_to.empty() || E.gen || E.ba_m || E.stencil)
+        return set_error(VH_ERR_ARG, "device Barabasi-Albert: edges already added");
+    if (m < 1 || m > 255) return set_error(VH_ERR_ARG, "device Barabasi-Albert: 1 <= m <= 255");
+    E.ba_m = m;
+    E.gen_seed = graph_seed;
+    E.gen_wseed = weight_seed;
+    return VH_OK;
+}
+
 // Bulk init-phase edges: the G(n, p) graph that vh_graph_gnp + add_edge!
 // in target order would give, generated on the device at finish_init!.
 extern "C" int vh_add_edges_gnp(vh_sim_t h, uint8_t et, double mean_deg, uint64_t graph_seed,
@@ -1017,6 +1044,71 @@ static int build_gnp_csr(Sim& s, EdgeStore& E) {
     return VH_OK;
 }
 
+// edges 2k: a_k -> b_k and 2k+1: b_k -> a_k of the Barabasi-Albert pairs, as
+// AgentIDs of type t (numbers through `map` when given: map[nr - 1])
+__global__ void k_ba_log(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t K, uint32_t t,
+                         const uint32_t* __restrict__ map, uint64_t* __restrict__ to, uint64_t* __restrict__ from) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 2 * K) return;
+    uint32_t x = a[j >> 1], y = b[j >> 1];
+    if (map) {
+        x = map[x - 1];
+        y = map[y - 1];
+    }
+    const uint32_t f = (j & 1) ? y : x, g = (j & 1) ? x : y;
+    to[j] = vh_make_id(t, g);
+    from[j] = vh_make_id(t, f);
+}
+// the same edges as 1-based number pairs (ea = from, eb = to) at `ea`, `eb`
+__global__ void k_ba_ends(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t K,
+                          uint32_t* __restrict__ ea, uint32_t* __restrict__ eb) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 2 * K) return;
+    const uint32_t x = a[j >> 1], y = b[j >> 1];
+    ea[j] = (j & 1) ? y : x;
+    eb[j] = (j & 1) ? x : y;
+}
+
+static int ba_agents(Sim& s, EdgeStore& E, uint64_t* n, uint64_t* K) {
+    *n = s.a[E.target].nextid - 1;
+    if (*n < E.ba_m) return set_error(VH_ERR_ARG, "device Barabasi-Albert: fewer agents than m");
+    *K = ba_pairs(*n, E.ba_m);
+    return VH_OK;
+}
+
+// right after a scratch reset: room for the pairs, the log, the weights, the
+// rounds' state and build_csr's sort of the 2K edges
+static int ba_reserve(Sim& s, EdgeStore& E) {
+    uint64_t n, K;
+    RC(ba_agents(s, E, &n, &K));
+    if (s.scratch.reserve(200 * K + 8 * n + ((size_t)64 << 20), s.stream))
+        return set_error(VH_ERR_OOM, "scratch for the Barabasi-Albert graph");
+    return VH_OK;
+}
+
+static int build_ba_csr(Sim& s, EdgeStore& E) {
+    hipStream_t st = s.stream;
+    Scratch& sc = s.scratch;
+    uint64_t n, K;
+    RC(ba_agents(s, E, &n, &K));
+    uint32_t* a = (uint32_t*)sc.take(K * 4 + 4);
+    uint32_t* b = (uint32_t*)sc.take(K * 4 + 4);
+    uint64_t* to = (uint64_t*)sc.take(2 * K * 8 + 8);
+    uint64_t* fr = (uint64_t*)sc.take(2 * K * 8 + 8);
+    float* w = E.bytes ? (float*)sc.take(2 * K * 4 + 8) : nullptr;
+    if (!a || !b || !to || !fr || (E.bytes && !w)) return set_error(VH_ERR_OOM, "scratch");
+    RC(ba_device(n, E.ba_m, E.gen_seed, a, b, sc, st));
+    const uint32_t* map =
+        (s.relabel_type == E.target && s.relabel_map.p) ? s.relabel_map.as<uint32_t>() : nullptr;
+    if (K)
+        hipLaunchKernelGGL(k_ba_log, dim3(grid_for(2 * K, 256)), dim3(256), 0, st, a, b, K, (uint32_t)E.target,
+                           map, to, fr);
+    if (w) fill_weights(w, 2 * K, E.gen_wseed, st);
+    RC(build_csr(s, E, false, to, fr, (const uint8_t*)w, 2 * K));
+    E.ba_m = 0;
+    return VH_OK;
+}
+
 // finish_init! with the agents of type t renumbered for locality, as the
 // reference renumbers them when it distributes them (src/Simulation.jl:414-453,
 // `return_idmapping`): vh_bfs_order over the init edges among agents of
@@ -1034,21 +1126,69 @@ extern "C" int vh_finish_init_relabel(vh_sim_t h, uint8_t t, uint64_t* new_nr) {
     const uint64_t n = A.nextid - 1;
     if (n >= 0xFFFFFFFFull) return set_error(VH_ERR_CAPACITY, "too many agents");
     // the endpoints of every init edge among agents of type t, in edge-type
-    // and insertion order
-    std::vector<uint32_t> ea, eb;
+    // and insertion order (device-generated Barabasi-Albert edges in their
+    // edge type's place), numbered breadth-first on the device (bfs_device,
+    // the order of vh_bfs_order)
+    hipStream_t st = s.stream;
+    std::vector<std::vector<uint32_t>> ha(s.m->ne), hb(s.m->ne);
+    std::vector<uint64_t> cnt(s.m->ne, 0), bak(s.m->ne, 0);
+    uint64_t M = 0;
     for (int et = 0; et < s.m->ne; et++) {
         EdgeStore& E = s.e[et];
         if (has(E.hints, VH_HINT_IGNOREFROM)) continue;
-        for (size_t k = 0; k < E.h_to.size(); k++) {
-            uint64_t to = E.h_to[k], fr = E.h_from[k];
-            if (vh_type_of(to) != t || vh_type_of(fr) != t) continue;
-            ea.push_back((uint32_t)vh_nr_of(fr));
-            eb.push_back((uint32_t)vh_nr_of(to));
+        if (E.ba_m && E.target == (int)t) {
+            uint64_t nn, K;
+            RC(ba_agents(s, E, &nn, &K));
+            bak[et] = K;
+            cnt[et] = 2 * K;
+        } else {
+            for (size_t k = 0; k < E.h_to.size(); k++) {
+                uint64_t to = E.h_to[k], fr = E.h_from[k];
+                if (vh_type_of(to) != t || vh_type_of(fr) != t) continue;
+                ha[et].push_back((uint32_t)vh_nr_of(fr));
+                hb[et].push_back((uint32_t)vh_nr_of(to));
+            }
+            cnt[et] = ha[et].size();
         }
+        M += cnt[et];
     }
-    RC(vh_bfs_order(n, ea.size(), ea.data(), eb.data(), new_nr));
-    std::vector<uint32_t>().swap(ea);
-    std::vector<uint32_t>().swap(eb);
+    Scratch& sc = s.scratch;
+    sc.reset(st);
+    uint64_t bak_max = 0;
+    for (uint64_t k : bak) bak_max = k > bak_max ? k : bak_max;
+    if (sc.reserve(8 * M + 8 * bak_max + 5 * n + bfs_scratch_bytes(n, M) + ((size_t)1 << 20), st))
+        return set_error(VH_ERR_OOM, "scratch for the breadth-first numbering");
+    uint32_t* ea = (uint32_t*)sc.take(M * 4 + 4);
+    uint32_t* eb = (uint32_t*)sc.take(M * 4 + 4);
+    if (!ea || !eb) return set_error(VH_ERR_OOM, "scratch");
+    uint64_t o = 0;
+    for (int et = 0; et < s.m->ne; et++) {
+        if (!cnt[et]) continue;
+        if (bak[et]) {
+            const uint64_t K = bak[et];
+            uint32_t* pa = (uint32_t*)sc.take(K * 4 + 4);
+            uint32_t* pb = (uint32_t*)sc.take(K * 4 + 4);
+            if (!pa || !pb) return set_error(VH_ERR_OOM, "scratch");
+            RC(ba_device(n, s.e[et].ba_m, s.e[et].gen_seed, pa, pb, sc, st));
+            hipLaunchKernelGGL(k_ba_ends, dim3(grid_for(2 * K, 256)), dim3(256), 0, st, pa, pb, K, ea + o, eb + o);
+        } else {
+            HIPCHECK(hipMemcpyAsync(ea + o, ha[et].data(), cnt[et] * 4, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(eb + o, hb[et].data(), cnt[et] * 4, hipMemcpyHostToDevice, st));
+        }
+        o += cnt[et];
+    }
+    RC(s.relabel_map.ensure(n * 4 + 4, st));
+    RC(bfs_device(n, M, ea, eb, s.relabel_map.as<uint32_t>(), sc, st));
+    s.relabel_type = t;
+    {
+        std::vector<uint32_t> nr32(n);
+        if (n) HIPCHECK(hipMemcpyAsync(nr32.data(), s.relabel_map.p, n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        for (uint64_t k = 0; k < n; k++) new_nr[k] = nr32[k];
+    }
+    sc.reset(st);
+    ha.clear();
+    hb.clear();
     // states to their new slots
     if (A.bytes && n) {
         std::vector<uint8_t> ns(n * A.bytes);
@@ -1063,7 +1203,10 @@ extern "C" int vh_finish_init_relabel(vh_sim_t h, uint8_t t, uint64_t* new_nr) {
             for (auto& id : *v)
                 if (id && vh_type_of(id) == t) id = vh_make_id(t, new_nr[vh_nr_of(id) - 1]);
     }
-    return vh_finish_init(h);
+    const int rc = vh_finish_init(h);
+    s.relabel_map.release();
+    s.relabel_type = 0;
+    return rc;
 }
 
 // finish_init! src/Simulation.jl:403-476 for one process: publish the
@@ -1092,6 +1235,7 @@ extern "C" int vh_finish_init(vh_sim_t h) {
         EdgeStore& E = s.e[et];
         uint64_t n = E.h_to.size();
         s.scratch.reset(st);
+        if (E.ba_m) RC(ba_reserve(s, E));
         uint64_t* dto = (uint64_t*)s.scratch.take(n * 8 + 8);
         uint64_t* dfrom = has(E.hints, VH_HINT_IGNOREFROM) ? nullptr : (uint64_t*)s.scratch.take(n * 8 + 8);
         uint8_t* dst = E.bytes ? (uint8_t*)s.scratch.take(n * E.bytes + 8) : nullptr;
@@ -1105,6 +1249,8 @@ extern "C" int vh_finish_init(vh_sim_t h) {
         if (s.m->na != 1) E.src64 = true;
         if (E.gen)
             RC(build_gnp_csr(s, E));
+        else if (E.ba_m)
+            RC(build_ba_csr(s, E));
         else if (E.stencil)
             RC(build_stencil(s, E));
         else

@@ -26,6 +26,9 @@ struct Scratch {
     int nextra = 0;
     void* take(size_t bytes);            // nullptr on allocation failure
     void reset(hipStream_t st);
+    // after reset: grow the main block to at least `bytes` (work that takes
+    // many pieces before its next reset); 0 or -3
+    int reserve(size_t bytes, hipStream_t st);
     void release();
     size_t bytes_held() const { return cap + spilled; }
 };

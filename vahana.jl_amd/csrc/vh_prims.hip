@@ -50,6 +50,16 @@ void Scratch::reset(hipStream_t st) {
     }
     used = 0;
 }
+int Scratch::reserve(size_t bytes, hipStream_t st) {
+    if (used || nextra || cap >= bytes) return 0;
+    hipStreamSynchronize(st);
+    if (main) hipFree(main);
+    main = nullptr;
+    cap = 0;
+    if (device_alloc(&main, bytes) != hipSuccess) return -3;
+    cap = bytes;
+    return 0;
+}
 void Scratch::release() {
     for (int i = 0; i < nextra; i++) hipFree(extra[i]);
     nextra = 0;

@@ -611,6 +611,7 @@ extern "C" int vh_sim_copy(vh_sim_t h, vh_sim_t* out) {
         // device-generated in-edges requested before finish_init!, and the
         // CSR version the stencil fast path keys on
         F.gen = E.gen;
+        F.ba_m = E.ba_m;
         F.gen_mean_deg = E.gen_mean_deg;
         F.gen_seed = E.gen_seed;
         F.gen_wseed = E.gen_wseed;

@@ -58,11 +58,13 @@ def step_sir(V, m, p_inf=0.05, seed=1, recovery=7):
     V.apply_(m.sim, V.transitions.sir(m.E, p_inf, seed, recovery), m.T, [m.T, m.E], m.T)
 
 
-def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None, relabel=False):
+def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None, relabel=False, device_graph=False):
     """`relabel`: agents renumbered breadth-first at finish_init! (m.idmap =
     new number of each agent number); a simulation without
     vh_finish_init_relabel (the checker) gets its inputs renumbered the same
-    way (inputs.bfs_order)."""
+    way (inputs.bfs_order). `device_graph`: the BA graph (m_ba, seed, weights
+    seed + 1) is generated on the device at finish_init! (vh_add_edges_ba)
+    instead of being added from `edges` / the host generator; the same graph."""
     _, _, HK, HK32, Knows, *_ = _types(V)
     A = HK32 if f32 else HK
     model = (V.ModelTypes()
@@ -73,6 +75,12 @@ def build_opinion(V, n, m_ba=5, seed=1, f32=False, edges=None, relabel=False):
     x = inputs.opinion_init(n, seed, f32)
     st = np.zeros(n, dtype=A.dtype)
     st["opinion"] = x
+    if device_graph and hasattr(sim.lib, "add_edges_ba"):
+        ids = sim._add_agents_array(A, st)
+        sim._add_edges_ba(Knows, m_ba, seed, seed + 1)
+        idmap = V.finish_init_(sim, relabel=A) if relabel else V.finish_init_(sim)
+        nedges = V.num_edges(sim, Knows)
+        return types.SimpleNamespace(sim=sim, T=A, E=Knows, ids=ids, nedges=nedges, f32=f32, idmap=idmap)
     if edges is None:
         a, b = inputs.graph_ba(n, m_ba, seed)
         edges = inputs.symmetric_weighted(a, b, seed + 1)

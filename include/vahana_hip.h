@@ -195,6 +195,33 @@ enum vh_dtype { VH_DT_I64 = 0, VH_DT_F64 = 1, VH_DT_BOOL = 2, VH_DT_F32 = 3,
 enum vh_map_kind { VH_MAP_FIELD = 0,      /* f(x) = x.field                     */
                    VH_MAP_FIELD_EQ = 1,   /* f(x) = x.field == value (Bool/Int) */
                    VH_MAP_ONE = 2 };      /* f(x) = 1                           */
+/* A computed f for mapreduce (the reference takes any function of the
+ * state): a postfix program over one record. Values are Int64, Float64 or
+ * Bool; FIELD pushes the field at byte offset `arg` of type `dtype` (floats
+ * as Float64, Bool as Bool, other integers as Int64); CONST_I / CONST_F /
+ * CONST_B push `arg` (Int64 bits, Float64 bits, 0 or 1). ADD SUB MUL MIN
+ * MAX: Float64 if an operand is a float, else Int64 (wrapping); DIV: Float64
+ * (Julia's /); MIN / MAX on floats as Julia's min / max (NaN wins, -0.0 <
+ * +0.0); NEG / ABS keep the type (Bool as Int64); comparisons compare as
+ * Float64 if an operand is a float, else as Int64, and give Bool; AND / OR /
+ * NOT are logical (nonzero is true) and give Bool; SELECT pops c, a, b
+ * (pushed in that order) and gives c ? a : b (Float64 if a or b is a
+ * float). The program's value is converted to the reduction's dtype: a
+ * float to an integer dtype truncates (NaN gives 0, values out of range
+ * saturate), any value to Bool is != 0. */
+enum vh_expr_code { VH_EX_FIELD = 0, VH_EX_CONST_I = 1, VH_EX_CONST_F = 2, VH_EX_CONST_B = 3,
+                    VH_EX_ADD = 10, VH_EX_SUB = 11, VH_EX_MUL = 12, VH_EX_DIV = 13, VH_EX_MIN = 14,
+                    VH_EX_MAX = 15, VH_EX_NEG = 20, VH_EX_ABS = 21, VH_EX_NOT = 22, VH_EX_EQ = 30,
+                    VH_EX_NE = 31, VH_EX_LT = 32, VH_EX_LE = 33, VH_EX_GT = 34, VH_EX_GE = 35,
+                    VH_EX_AND = 40, VH_EX_OR = 41, VH_EX_SELECT = 50 };
+#define VH_EXPR_MAX_OPS   64
+#define VH_EXPR_MAX_DEPTH 16
+typedef struct vh_expr_op {
+    uint32_t code;                   /* enum vh_expr_code                     */
+    uint32_t dtype;                  /* FIELD: enum vh_dtype of the field     */
+    uint64_t arg;                    /* FIELD: byte offset; CONST_*: value bits */
+} vh_expr_op;
+
 typedef struct vh_map_desc {
     uint32_t kind;                   /* enum vh_map_kind                      */
     uint32_t offset;                 /* byte offset of the field in the state */
@@ -344,6 +371,12 @@ int vh_set_last_change(vh_sim_t sim, int kind, uint8_t type_id, uint64_t value);
 /* mapreduce(sim, f, op, T; init, datatype) — kind 0: agents, 1: edges */
 int vh_mapreduce(vh_sim_t s, int kind, uint8_t type_id, const vh_map_desc* f,
                  int op, int dtype, const void* init, void* out);
+/* mapreduce(sim, f, op, T; init, datatype) with f a vh_expr_op program of
+ * nops operations (at most VH_EXPR_MAX_OPS, stack depth at most
+ * VH_EXPR_MAX_DEPTH, fields inside the record); same reduction and results
+ * as vh_mapreduce otherwise. VH_ERR_ARG for an invalid program. */
+int vh_mapreduce_expr(vh_sim_t s, int kind, uint8_t type_id, const vh_expr_op* prog, uint32_t nops,
+                      int op, int dtype, const void* init, void* out);
 /* vh_mapreduce's reduction without any device allocation: no counterpart in
  * the reference; for callers that cannot allocate device memory (tight
  * serving loops, code under a HIP graph capture, harnesses that forbid

@@ -1613,6 +1613,196 @@ static acc_t tree_reduce(acc_t* v, uint64_t n, int op, int dtype) {
     }
 }
 
+/* ---- computed maps (vh_mapreduce_expr): the postfix program of
+ * include/vahana_hip.h evaluated per record with a tagged stack */
+enum { EX_I = 0, EX_F = 1, EX_B = 2 };
+typedef struct { int tag; int64_t i; double f; } exv_t;
+
+static double exv_f(exv_t v) { return v.tag == EX_F ? v.f : (double)v.i; }
+static int exv_true(exv_t v) { return v.tag == EX_F ? v.f != 0.0 : v.i != 0; }
+static uint32_t ex_field_bytes(uint32_t dt) {
+    switch (dt) {
+    case VH_DT_I64: case VH_DT_F64: case VH_DT_U64: return 8;
+    case VH_DT_F32: case VH_DT_U32: case VH_DT_I32: return 4;
+    case VH_DT_U16: return 2;
+    case VH_DT_BOOL: case VH_DT_U8: return 1;
+    }
+    return 0;
+}
+static int ex_check(const vh_expr_op* p, uint32_t n, uint32_t bytes) {
+    int d = 0;
+    if (!p || n == 0 || n > VH_EXPR_MAX_OPS) return fail(VH_ERR_ARG, "mapreduce program: 1 to 64 operations");
+    for (uint32_t k = 0; k < n; k++) {
+        int pop;
+        switch (p[k].code) {
+        case VH_EX_FIELD:
+            if (!ex_field_bytes(p[k].dtype) || p[k].arg + ex_field_bytes(p[k].dtype) > bytes)
+                return fail(VH_ERR_ARG, "mapreduce program: field outside the state");
+            pop = 0; break;
+        case VH_EX_CONST_I: case VH_EX_CONST_F: case VH_EX_CONST_B: pop = 0; break;
+        case VH_EX_NEG: case VH_EX_ABS: case VH_EX_NOT: pop = 1; break;
+        case VH_EX_SELECT: pop = 3; break;
+        case VH_EX_ADD: case VH_EX_SUB: case VH_EX_MUL: case VH_EX_DIV: case VH_EX_MIN: case VH_EX_MAX:
+        case VH_EX_EQ: case VH_EX_NE: case VH_EX_LT: case VH_EX_LE: case VH_EX_GT: case VH_EX_GE:
+        case VH_EX_AND: case VH_EX_OR: pop = 2; break;
+        default: return fail(VH_ERR_ARG, "mapreduce program: unknown operation");
+        }
+        if (d < pop) return fail(VH_ERR_ARG, "mapreduce program: missing operands");
+        d += 1 - pop;
+        if (d > VH_EXPR_MAX_DEPTH) return fail(VH_ERR_ARG, "mapreduce program: stack too deep");
+    }
+    if (d != 1) return fail(VH_ERR_ARG, "mapreduce program: not one value");
+    return VH_OK;
+}
+static exv_t ex_eval(const vh_expr_op* p, uint32_t n, const uint8_t* st) {
+    exv_t stk[VH_EXPR_MAX_DEPTH];
+    int sp = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t c = p[k].code;
+        exv_t r = {EX_I, 0, 0.0};
+        if (c == VH_EX_FIELD) {
+            const uint8_t* q = st + p[k].arg;
+            if (is_fdt((int)p[k].dtype)) { r.tag = EX_F; r.f = field_f(q, p[k].dtype); }
+            else if (p[k].dtype == VH_DT_BOOL) { r.tag = EX_B; r.i = q[0] != 0; }
+            else { r.tag = EX_I; r.i = field_i(q, p[k].dtype); }
+            stk[sp++] = r;
+            continue;
+        }
+        if (c == VH_EX_CONST_I) { r.tag = EX_I; memcpy(&r.i, &p[k].arg, 8); stk[sp++] = r; continue; }
+        if (c == VH_EX_CONST_F) { r.tag = EX_F; memcpy(&r.f, &p[k].arg, 8); stk[sp++] = r; continue; }
+        if (c == VH_EX_CONST_B) { r.tag = EX_B; r.i = p[k].arg != 0; stk[sp++] = r; continue; }
+        if (c == VH_EX_NEG || c == VH_EX_ABS || c == VH_EX_NOT) {
+            exv_t a = stk[sp - 1];
+            if (c == VH_EX_NOT) { r.tag = EX_B; r.i = !exv_true(a); }
+            else if (a.tag == EX_F) { r.tag = EX_F; r.f = c == VH_EX_NEG ? -a.f : fabs(a.f); }
+            else {
+                uint64_t u = (uint64_t)a.i, neg = 0ull - u;
+                r.tag = EX_I;
+                r.i = (int64_t)(c == VH_EX_NEG ? neg : (a.i < 0 ? neg : u));
+            }
+            stk[sp - 1] = r;
+            continue;
+        }
+        if (c == VH_EX_SELECT) {
+            exv_t b = stk[sp - 1], a = stk[sp - 2], cc = stk[sp - 3];
+            sp -= 2;
+            exv_t x = exv_true(cc) ? a : b;
+            if (a.tag == EX_F || b.tag == EX_F) { r.tag = EX_F; r.f = exv_f(x); }
+            else { r.tag = (a.tag == EX_B && b.tag == EX_B) ? EX_B : EX_I; r.i = x.i; }
+            stk[sp - 1] = r;
+            continue;
+        }
+        {
+            exv_t b = stk[sp - 1], a = stk[sp - 2];
+            int fl = a.tag == EX_F || b.tag == EX_F;
+            sp--;
+            if (c >= VH_EX_EQ && c <= VH_EX_GE) {
+                int t;
+                if (fl) {
+                    double x = exv_f(a), y = exv_f(b);
+                    t = c == VH_EX_EQ ? x == y : c == VH_EX_NE ? x != y : c == VH_EX_LT ? x < y
+                      : c == VH_EX_LE ? x <= y : c == VH_EX_GT ? x > y : x >= y;
+                } else {
+                    t = c == VH_EX_EQ ? a.i == b.i : c == VH_EX_NE ? a.i != b.i : c == VH_EX_LT ? a.i < b.i
+                      : c == VH_EX_LE ? a.i <= b.i : c == VH_EX_GT ? a.i > b.i : a.i >= b.i;
+                }
+                r.tag = EX_B;
+                r.i = t;
+            } else if (c == VH_EX_AND || c == VH_EX_OR) {
+                r.tag = EX_B;
+                r.i = c == VH_EX_AND ? (exv_true(a) && exv_true(b)) : (exv_true(a) || exv_true(b));
+            } else if (fl || c == VH_EX_DIV) {
+                double x = exv_f(a), y = exv_f(b);
+                r.tag = EX_F;
+                switch (c) {
+                case VH_EX_ADD: r.f = x + y; break;
+                case VH_EX_SUB: r.f = x - y; break;
+                case VH_EX_MUL: r.f = x * y; break;
+                case VH_EX_DIV: r.f = x / y; break;
+                case VH_EX_MIN: r.f = jl_fmin(x, y); break;
+                case VH_EX_MAX: r.f = jl_fmax(x, y); break;
+                }
+            } else {
+                uint64_t x = (uint64_t)a.i, y = (uint64_t)b.i;
+                r.tag = EX_I;
+                switch (c) {
+                case VH_EX_ADD: r.i = (int64_t)(x + y); break;
+                case VH_EX_SUB: r.i = (int64_t)(x - y); break;
+                case VH_EX_MUL: r.i = (int64_t)(x * y); break;
+                case VH_EX_MIN: r.i = a.i < b.i ? a.i : b.i; break;
+                case VH_EX_MAX: r.i = a.i > b.i ? a.i : b.i; break;
+                }
+            }
+            stk[sp - 1] = r;
+        }
+    }
+    return stk[0];
+}
+/* the program's value in the reduction's type */
+static acc_t ex_to_acc(exv_t v, int dtype) {
+    acc_t a;
+    if (is_fdt(dtype)) a.f = exv_f(v);
+    else if (dtype == VH_DT_BOOL) a.i = exv_true(v);
+    else if (v.tag != EX_F) a.i = v.i;
+    else if (v.f != v.f) a.i = 0;                       /* NaN */
+    else if (v.f >= 9223372036854775808.0) a.i = INT64_MAX;
+    else if (v.f < -9223372036854775808.0) a.i = INT64_MIN;
+    else a.i = (int64_t)v.f;
+    return a;
+}
+
+/* mapreduce with a computed f: or_mapreduce's folds over the program's
+ * values (order 0: the reference's sequential fold, 1: the device tree) */
+int or_mapreduce_expr(or_sim* s, int kind, uint8_t t, const vh_expr_op* prog, uint32_t nops, int op, int dtype,
+                      const void* init, void* out, int order) {
+    if (s->intransition) return fail(VH_ERR_ASSERT, "You can not call mapreduce inside of a transition function.");
+    acc_t acc = {0};
+    if (init) { if (is_fdt(dtype)) memcpy(&acc.f, init, 8); else memcpy(&acc.i, init, 8); }
+    else { int r = val4empty(op, dtype, &acc); if (r) return r; }
+    acc_t* vals = NULL;
+    uint64_t n = 0, cap = 0;
+#define PUSHX(x) do { if (n == cap) { cap = cap ? cap * 2 : 1024; vals = (acc_t*)realloc(vals, cap * sizeof(acc_t)); } vals[n++] = (x); } while (0)
+    if (kind == 0) {
+        if (t < 1 || t > s->m->na) return fail(VH_ERR_UNREGISTERED, "unregistered agent type");
+        agents_t* A = &s->a[t];
+        uint32_t bytes = s->m->a[t].bytes;
+        int r = ex_check(prog, nops, bytes);
+        if (r) return r;
+        int mortal = mortal_t(s, t);
+        acc_t id = op_identity(op, dtype);
+        for (uint64_t i = 1; i <= A->nextid - 1; i++) {
+            if (mortal && A->read.died[i - 1]) { if (order) PUSHX(id); continue; }
+            PUSHX(ex_to_acc(ex_eval(prog, nops, A->read.state + (i - 1) * bytes), dtype));
+        }
+    } else {
+        if (t >= s->m->ne) return fail(VH_ERR_UNREGISTERED, "unregistered edge type");
+        if (HAS(ETH(s, t), VH_HINT_STATELESS))
+            return fail(VH_ERR_ASSERT, "mapreduce is not defined for the hint combination");
+        uint32_t bytes = s->m->e[t].bytes;
+        int r = ex_check(prog, nops, bytes);
+        if (r) return r;
+        field_t* fl = s->e[t].read;
+        for (int at = 0; at <= MAXA; at++)
+            for (uint64_t i = 0; i < fl->nrows[at]; i++) {
+                cont_t* c = &fl->rows[at][i];
+                if (!c->present) continue;
+                for (uint32_t j = 0; j < c->n; j++)
+                    PUSHX(ex_to_acc(ex_eval(prog, nops, c->st + (size_t)j * bytes), dtype));
+            }
+    }
+    if (order == 0) {
+        for (uint64_t i = 0; i < n; i++) acc = op_apply(op, dtype, vals[i], acc);
+    } else {
+        acc_t tot = tree_reduce(vals, n, op, dtype);
+        acc = op_apply(op, dtype, tot, acc);
+    }
+    free(vals);
+    if (is_fdt(dtype)) memcpy(out, &acc.f, 8);
+    else memcpy(out, &acc.i, 8);
+    return VH_OK;
+#undef PUSHX
+}
+
 int or_mapreduce(or_sim* s, int kind, uint8_t t, const vh_map_desc* f, int op, int dtype,
                  const void* init, void* out, int order) {
     if (s->intransition) return fail(VH_ERR_ASSERT, "You can not call mapreduce inside of a transition function.");

@@ -40,6 +40,8 @@ int or_set_last_change(or_sim* s, int kind, uint8_t t, uint64_t v);
 int or_restore_died(or_sim* s, uint8_t t, const uint8_t* died, uint64_t n);
 int or_set_num_transitions(or_sim* s, uint64_t n);
 /* order: 0 = the reference's sequential left fold, 1 = the device's fixed tree */
+int or_mapreduce_expr(or_sim* s, int kind, uint8_t t, const vh_expr_op* prog, uint32_t nops, int op,
+                      int dtype, const void* init, void* out, int order);
 int or_mapreduce(or_sim* s, int kind, uint8_t t, const vh_map_desc* f, int op,
                  int dtype, const void* init, void* out, int order);
 int or_num_agents(or_sim* s, uint8_t t, uint64_t* out);

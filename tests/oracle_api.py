@@ -36,6 +36,7 @@ def lib():
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         ])
         L.mapreduce.argtypes = list(L.mapreduce.argtypes) + [C.c_int]
+        L.mapreduce_expr.argtypes = list(L.mapreduce_expr.argtypes) + [C.c_int]
         _LIB = L
     return _LIB
 
@@ -49,6 +50,9 @@ class OracleSim(vh.Simulation):
     def _mapreduce(self, kind, tid, md, op, dt, initp, outp):
         return self.lib.mapreduce(self.handle, kind, tid, C.byref(md), op, dt, initp, outp,
                                   self.order)
+
+    def _mapreduce_expr(self, kind, tid, prog, nops, op, dt, initp, outp):
+        return self.lib.mapreduce_expr(self.handle, kind, tid, prog, nops, op, dt, initp, outp, self.order)
 
     def reuse_stack(self, T):
         n = C.c_uint64()
@@ -80,6 +84,7 @@ raster = vh.raster
 models = vh.models
 inputs = vh.inputs
 snapshot = vh.snapshot
+expr = vh.expr
 for _name in ["add_agent_", "add_agents_", "add_edge_", "add_edges_", "finish_init_", "apply_",
               "apply", "mapreduce", "num_agents", "num_edges", "all_agents", "all_agentids",
               "agentstate", "edges", "neighborids", "edgestates", "has_edge", "copy_simulation",

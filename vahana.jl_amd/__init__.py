@@ -641,6 +641,9 @@ class Simulation:
     def _mapreduce(self, kind, tid, md, op, dt, initp, outp):
         return self.lib.mapreduce(self.handle, kind, tid, C.byref(md), op, dt, initp, outp)
 
+    def _mapreduce_expr(self, kind, tid, prog, nops, op, dt, initp, outp):
+        return self.lib.mapreduce_expr(self.handle, kind, tid, prog, nops, op, dt, initp, outp)
+
     def _num_transitions(self):
         n = C.c_uint64()
         _check(self.lib.num_transitions(self.handle, C.byref(n)), self.lib)
@@ -859,6 +862,8 @@ def mapreduce(sim, f, op, T, init=None, datatype=None):
     op: '+', '*', 'max', 'min', '&', '|'; datatype: 'i8' (Int), 'f8', 'b1' (Bool)."""
     kind = 0 if T in sim.model.aid else 1
     tid = sim._aid(T) if kind == 0 else sim._eid(T)
+    if isinstance(f, expr.Expr):
+        return _mapreduce_expr(sim, f, op, T, kind, tid, init, datatype)
     md, fcode = _map_desc(T, f)
     if datatype is None:
         if init is not None:
@@ -886,6 +891,41 @@ def mapreduce(sim, f, op, T, init=None, datatype=None):
     if datatype == "b1":
         return bool(raw)
     return int(np.array([raw], dtype=np.uint64).view(np.int64)[0])
+
+
+def _init_arg(init, datatype):
+    if init is None:
+        return None, None
+    iv = C.c_double(float(init)) if datatype == "f8" else C.c_int64(int(init))
+    return iv, C.cast(C.pointer(iv), C.c_void_p)
+
+
+def _result(raw, datatype):
+    if datatype == "f8":
+        return float(np.array([raw], dtype=np.uint64).view(np.float64)[0])
+    if datatype == "b1":
+        return bool(raw)
+    return int(np.array([raw], dtype=np.uint64).view(np.int64)[0])
+
+
+def _mapreduce_expr(sim, f, op, T, kind, tid, init, datatype):
+    """mapreduce with a computed f (vahana_amd.expr): vh_mapreduce_expr."""
+    prog, nops, vkind = expr.compile_expr(f, T, sim)
+    if datatype is None:
+        if init is not None:
+            datatype = "f8" if isinstance(init, float) else ("b1" if isinstance(init, bool) else "i8")
+        elif vkind == "f":
+            datatype = "f8"
+        elif vkind == "b":
+            datatype = "i8" if op in ("+", "*") else "b1"   # Bool + Bool is an Int
+        else:
+            datatype = "i8"
+    out = C.c_uint64()
+    _iv, initp = _init_arg(init, datatype)
+    rc = sim._mapreduce_expr(kind, tid, prog, nops, _abi.OP[op], _abi.DT[datatype], initp,
+                             C.cast(C.pointer(out), C.c_void_p))
+    _check(rc, sim.lib)
+    return _result(out.value, datatype)
 
 
 # ------------------------------------------------------------------ queries
@@ -1052,4 +1092,4 @@ def enable_asserts(enable=True):
     _abi.lib().enable_asserts(1 if enable else 0)
 
 
-from . import inputs, models, ensemble, serve, raster, snapshot  # noqa: E402,F401
+from . import inputs, models, ensemble, serve, raster, snapshot, expr  # noqa: E402,F401

@@ -86,6 +86,17 @@ class MapDesc(C.Structure):
                 ("pad", C.c_uint32), ("value", C.c_int64)]
 
 
+class ExprOp(C.Structure):
+    _fields_ = [("code", C.c_uint32), ("dtype", C.c_uint32), ("arg", C.c_uint64)]
+
+
+EX = {"field": 0, "const_i": 1, "const_f": 2, "const_b": 3, "+": 10, "-": 11, "*": 12, "/": 13,
+      "min": 14, "max": 15, "neg": 20, "abs": 21, "not": 22, "==": 30, "!=": 31, "<": 32, "<=": 33,
+      ">": 34, ">=": 35, "and": 40, "or": 41, "select": 50}
+EXPR_MAX_OPS = 64
+EXPR_MAX_DEPTH = 16
+
+
 class TestParams(C.Structure):
     _fields_ = [("value", C.c_int64), ("cmp", C.c_int32), ("pad", C.c_uint32),
                 ("edge_state", C.c_uint8 * 16)]
@@ -199,6 +210,8 @@ DEVICE_ONLY = [
 ]
 MAPREDUCE = ("mapreduce", C.c_int, [_vp, C.c_int, C.c_uint8, C.POINTER(MapDesc), C.c_int,
                                      C.c_int, _vp, _vp])
+MAPREDUCE_EXPR = ("mapreduce_expr", C.c_int, [_vp, C.c_int, C.c_uint8, C.POINTER(ExprOp), C.c_uint32,
+                                              C.c_int, C.c_int, _vp, _vp])
 
 
 class Lib:
@@ -212,7 +225,7 @@ class Lib:
         self.path = path
         self.dll = C.CDLL(path)
         self.prefix = prefix
-        for name, res, args in list(SIGNATURES) + [MAPREDUCE] + list(extra):
+        for name, res, args in list(SIGNATURES) + [MAPREDUCE, MAPREDUCE_EXPR] + list(extra):
             fn = getattr(self.dll, prefix + name)
             fn.restype = res
             fn.argtypes = args

@@ -145,6 +145,152 @@ __global__ __launch_bounds__(MB) void k_mr_values(const uint64_t* __restrict__ i
 }
 __global__ void k_identity(uint64_t* out, int op, int dt) { *out = identity(op, dt); }
 
+// ---- computed maps (vh_mapreduce_expr): a postfix program per record;
+// values are 8 bits + a tag (0 Int64, 1 Float64, 2 Bool)
+struct ExprProg {
+    vh_expr_op op[VH_EXPR_MAX_OPS];
+    uint32_t n;
+};
+constexpr uint8_t TI = 0, TF = 1, TB = 2;
+
+// a float to Int64: truncated; NaN 0, out of range saturated
+__device__ __forceinline__ int64_t f2i(double x) {
+    if (x != x) return 0;
+    if (x >= 9223372036854775808.0) return INT64_MAX;
+    if (x < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+__device__ __forceinline__ double ev_f(uint8_t tag, uint64_t v) {
+    return tag == TF ? from_bits(v) : (double)(int64_t)v;
+}
+__device__ uint64_t expr_val(const ExprProg& p, const uint8_t* st, int dt, uint8_t& tag_out) {
+    uint64_t sv[VH_EXPR_MAX_DEPTH];
+    uint8_t sg[VH_EXPR_MAX_DEPTH];
+    int sp = 0;
+    for (uint32_t k = 0; k < p.n; k++) {
+        const uint32_t c = p.op[k].code;
+        const uint64_t arg = p.op[k].arg;
+        if (c == VH_EX_FIELD) {
+            const uint32_t fdt = p.op[k].dtype;
+            const uint8_t* q = st + arg;
+            if (is_f(fdt)) {
+                sv[sp] = to_bits(field_f(q, fdt));
+                sg[sp] = TF;
+            } else {
+                sv[sp] = (uint64_t)field_i(q, fdt);
+                sg[sp] = fdt == VH_DT_BOOL ? TB : TI;
+                if (fdt == VH_DT_BOOL) sv[sp] = sv[sp] != 0;
+            }
+            sp++;
+        } else if (c == VH_EX_CONST_I || c == VH_EX_CONST_F || c == VH_EX_CONST_B) {
+            sv[sp] = c == VH_EX_CONST_B ? (arg != 0) : arg;
+            sg[sp] = c == VH_EX_CONST_I ? TI : (c == VH_EX_CONST_F ? TF : TB);
+            sp++;
+        } else if (c == VH_EX_NEG || c == VH_EX_ABS || c == VH_EX_NOT) {
+            const uint64_t a = sv[sp - 1];
+            const uint8_t g = sg[sp - 1];
+            if (c == VH_EX_NOT) {
+                sv[sp - 1] = g == TF ? (uint64_t)(from_bits(a) == 0.0) : (uint64_t)(a == 0);
+                sg[sp - 1] = TB;
+            } else if (g == TF) {
+                const double x = from_bits(a);
+                sv[sp - 1] = to_bits(c == VH_EX_NEG ? -x : fabs(x));
+            } else {
+                const int64_t x = (int64_t)a;
+                sv[sp - 1] = (uint64_t)(c == VH_EX_NEG ? (int64_t)(0ull - (uint64_t)x) : (x < 0 ? (int64_t)(0ull - (uint64_t)x) : x));
+                sg[sp - 1] = TI;
+            }
+        } else if (c == VH_EX_SELECT) {
+            const uint64_t b = sv[sp - 1], a = sv[sp - 2], cc = sv[sp - 3];
+            const uint8_t gb = sg[sp - 1], ga = sg[sp - 2], gc = sg[sp - 3];
+            const bool t = gc == TF ? from_bits(cc) != 0.0 : cc != 0;
+            sp -= 2;
+            if (ga == TF || gb == TF) {
+                sv[sp - 1] = to_bits(t ? ev_f(ga, a) : ev_f(gb, b));
+                sg[sp - 1] = TF;
+            } else {
+                sv[sp - 1] = t ? a : b;
+                sg[sp - 1] = (ga == TB && gb == TB) ? TB : TI;
+            }
+        } else {   // binary
+            const uint64_t b = sv[sp - 1], a = sv[sp - 2];
+            const uint8_t gb = sg[sp - 1], ga = sg[sp - 2];
+            sp--;
+            const bool fl = ga == TF || gb == TF;
+            uint64_t r = 0;
+            uint8_t g = TI;
+            if (c >= VH_EX_EQ && c <= VH_EX_GE) {
+                bool t = false;
+                if (fl) {
+                    const double x = ev_f(ga, a), y = ev_f(gb, b);
+                    t = c == VH_EX_EQ ? x == y : c == VH_EX_NE ? x != y : c == VH_EX_LT ? x < y
+                      : c == VH_EX_LE ? x <= y : c == VH_EX_GT ? x > y : x >= y;
+                } else {
+                    const int64_t x = (int64_t)a, y = (int64_t)b;
+                    t = c == VH_EX_EQ ? x == y : c == VH_EX_NE ? x != y : c == VH_EX_LT ? x < y
+                      : c == VH_EX_LE ? x <= y : c == VH_EX_GT ? x > y : x >= y;
+                }
+                r = t;
+                g = TB;
+            } else if (c == VH_EX_AND || c == VH_EX_OR) {
+                const bool x = ga == TF ? from_bits(a) != 0.0 : a != 0;
+                const bool y = gb == TF ? from_bits(b) != 0.0 : b != 0;
+                r = c == VH_EX_AND ? (x && y) : (x || y);
+                g = TB;
+            } else if (fl || c == VH_EX_DIV) {
+                const double x = ev_f(ga, a), y = ev_f(gb, b);
+                double z = 0.0;
+                switch (c) {
+                case VH_EX_ADD: z = x + y; break;
+                case VH_EX_SUB: z = x - y; break;
+                case VH_EX_MUL: z = x * y; break;
+                case VH_EX_DIV: z = x / y; break;
+                case VH_EX_MIN: z = jl_fmin(x, y); break;
+                case VH_EX_MAX: z = jl_fmax(x, y); break;
+                }
+                r = to_bits(z);
+                g = TF;
+            } else {
+                const int64_t x = (int64_t)a, y = (int64_t)b;
+                switch (c) {
+                case VH_EX_ADD: r = a + b; break;
+                case VH_EX_SUB: r = a - b; break;
+                case VH_EX_MUL: r = a * b; break;
+                case VH_EX_MIN: r = (uint64_t)(x < y ? x : y); break;
+                case VH_EX_MAX: r = (uint64_t)(x > y ? x : y); break;
+                }
+                g = TI;
+            }
+            sv[sp - 1] = r;
+            sg[sp - 1] = g;
+        }
+    }
+    tag_out = sg[0];
+    const uint64_t v = sv[0];
+    const uint8_t g = sg[0];
+    if (is_f(dt)) return to_bits(ev_f(g, v));
+    if (dt == VH_DT_BOOL) return g == TF ? (uint64_t)(from_bits(v) != 0.0) : (uint64_t)(v != 0);
+    return g == TF ? (uint64_t)f2i(from_bits(v)) : v;
+}
+
+__global__ __launch_bounds__(MB) void k_mr_expr(const uint8_t* __restrict__ st, uint32_t bytes,
+                                                const uint8_t* __restrict__ died, uint64_t n, ExprProg prog,
+                                                int op, int dt, uint64_t* __restrict__ out) {
+    __shared__ uint64_t lds[MB];
+    const uint64_t id = identity(op, dt);
+    uint64_t acc = id;
+    for (int j = 0; j < MI; j++) {
+        const uint64_t e = (uint64_t)blockIdx.x * MTILE + threadIdx.x + (uint64_t)MB * j;
+        if (e < n) {
+            uint8_t tg;
+            const uint64_t v = (died && died[e]) ? id : expr_val(prog, st + e * bytes, dt, tg);
+            acc = op_apply(op, dt, acc, v);
+        }
+    }
+    const uint64_t r = tree_block(acc, lds, op, dt);
+    if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
 }  // namespace
 
 // host restatement of op_apply for the final combination with init
@@ -283,6 +429,105 @@ extern "C" int vh_mapreduce(vh_sim_t h, int kind, uint8_t t, const vh_map_desc* 
         RC(reduce_levels(s, o, nb, op, dtype, &tot));
     }
     uint64_t r = h_op(op, dtype, tot, acc);
+    memcpy(out, &r, 8);
+    return VH_OK;
+}
+
+static uint32_t field_bytes(uint32_t dt) {
+    switch (dt) {
+    case VH_DT_I64: case VH_DT_F64: case VH_DT_U64: return 8;
+    case VH_DT_F32: case VH_DT_U32: case VH_DT_I32: return 4;
+    case VH_DT_U16: return 2;
+    case VH_DT_BOOL: case VH_DT_U8: return 1;
+    }
+    return 0;
+}
+
+// an expression program: known codes, fields inside `bytes`, stack depth
+// within limits and one value at the end
+static int check_expr(const vh_expr_op* prog, uint32_t nops, uint32_t bytes) {
+    if (!prog || nops == 0 || nops > VH_EXPR_MAX_OPS)
+        return set_error(VH_ERR_ARG, "mapreduce program: 1 to %d operations", VH_EXPR_MAX_OPS);
+    int d = 0;
+    for (uint32_t k = 0; k < nops; k++) {
+        const uint32_t c = prog[k].code;
+        int pop = 0, push = 1;
+        switch (c) {
+        case VH_EX_FIELD: {
+            const uint32_t fb = field_bytes(prog[k].dtype);
+            if (!fb || prog[k].arg + fb > bytes)
+                return set_error(VH_ERR_ARG, "mapreduce program: field %u outside the state", k);
+            break;
+        }
+        case VH_EX_CONST_I: case VH_EX_CONST_F: case VH_EX_CONST_B: break;
+        case VH_EX_NEG: case VH_EX_ABS: case VH_EX_NOT: pop = 1; break;
+        case VH_EX_ADD: case VH_EX_SUB: case VH_EX_MUL: case VH_EX_DIV: case VH_EX_MIN: case VH_EX_MAX:
+        case VH_EX_EQ: case VH_EX_NE: case VH_EX_LT: case VH_EX_LE: case VH_EX_GT: case VH_EX_GE:
+        case VH_EX_AND: case VH_EX_OR: pop = 2; break;
+        case VH_EX_SELECT: pop = 3; break;
+        default: return set_error(VH_ERR_ARG, "mapreduce program: unknown operation %u", c);
+        }
+        if (d < pop) return set_error(VH_ERR_ARG, "mapreduce program: operation %u lacks operands", k);
+        d += push - pop;
+        if (d > VH_EXPR_MAX_DEPTH) return set_error(VH_ERR_ARG, "mapreduce program: stack deeper than %d",
+                                                    VH_EXPR_MAX_DEPTH);
+    }
+    if (d != 1) return set_error(VH_ERR_ARG, "mapreduce program: leaves %d values", d);
+    return VH_OK;
+}
+
+// mapreduce with a computed f (vh_mapreduce_expr): vh_mapreduce's reduction
+// tree with the program evaluated per record
+extern "C" int vh_mapreduce_expr(vh_sim_t h, int kind, uint8_t t, const vh_expr_op* prog, uint32_t nops, int op,
+                                 int dtype, const void* init, void* out) {
+    if (!h || !out) return set_error(VH_ERR_ARG, "null argument");
+    Sim& s = h->s;
+    if (!s.initialized) return set_error(VH_ERR_ASSERT, "mapreduce needs an initialized simulation");
+    if (op < VH_OP_ADD || op > VH_OP_OR) return set_error(VH_ERR_ASSERT, "unknown operator");
+    uint64_t acc;
+    if (init) memcpy(&acc, init, 8);
+    else RC(val4empty(op, dtype, &acc));
+    const uint8_t* recs;
+    const uint8_t* died = nullptr;
+    uint64_t n;
+    uint32_t bytes;
+    if (kind == 0) {
+        if (t < 1 || t > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+        AgentStore& A = s.a[t];
+        recs = A.rstate();
+        bytes = A.bytes;
+        n = A.nextid - 1;
+        if (A.mortal) died = A.died.as<uint8_t>();
+    } else {
+        if (t >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+        EdgeStore& E = s.e[t];
+        if (has(E.hints, VH_HINT_STATELESS))
+            return set_error(VH_ERR_ASSERT, "mapreduce is not defined for the hint combination");
+        recs = E.st.as<uint8_t>();
+        bytes = E.bytes;
+        n = E.nedges;
+    }
+    RC(check_expr(prog, nops, bytes));
+    ExprProg P;
+    memset(&P, 0, sizeof P);
+    memcpy(P.op, prog, nops * sizeof(vh_expr_op));
+    P.n = nops;
+    s.scratch.reset(s.stream);
+    uint64_t tot;
+    if (n == 0) {
+        uint64_t* o = (uint64_t*)s.scratch.take(8);
+        if (!o) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_identity, dim3(1), dim3(1), 0, s.stream, o, op, dtype);
+        RC(reduce_levels(s, o, 1, op, dtype, &tot));
+    } else {
+        const uint64_t nb = (n + MTILE - 1) / MTILE;
+        uint64_t* o = (uint64_t*)s.scratch.take(nb * 8 + 8);
+        if (!o) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_mr_expr, dim3((unsigned)nb), dim3(MB), 0, s.stream, recs, bytes, died, n, P, op,
+                           dtype, o);
+        RC(reduce_levels(s, o, nb, op, dtype, &tot));
+    }
+    const uint64_t r = h_op(op, dtype, tot, acc);
     memcpy(out, &r, 8);
     return VH_OK;
 }

@@ -79,7 +79,8 @@ enum vh_transition {
     VH_TR_OPINION_F64 = 17,  /* weighted bounded confidence, Float64 opinions      */
     VH_TR_OPINION_F32 = 18,  /* same, Float32 opinions                              */
     VH_TR_CONTACT = 19,      /* dynamic contact network with deaths/births         */
-    VH_TR_COUNT_ = 20
+    VH_TR_COUNT_ = 20,
+    VH_TR_USER = 64          /* first id of the bodies from vh_register_body       */
 };
 
 /* predicates of VH_TR_DIE_IF (applied to the Int64 field at offset 0) */
@@ -298,6 +299,19 @@ int vh_add_edges_ba(vh_sim_t sim, uint8_t et, uint32_t m, uint64_t graph_seed, u
 int vh_add_edges_stencil(vh_sim_t sim, uint8_t et, uint32_t cell_type, uint64_t first_nr, uint32_t ndims,
                          const int64_t* dims, double distance, int metric, int periodic,
                          const uint8_t* edge_state);
+/* A transition body loaded at run time (the reference's apply! takes any
+ * function, src/Simulation.jl:720-730): kernel `kernel` of the code object
+ * file `code_object`, defined with VH_BODY_KERNEL of include/vahana_device.h
+ * and compiled for gfx950 (hipcc --genco). `flags`: VH_BODY_ADDS_EDGES when
+ * the body calls add_edge! / remove_edges! (entries of edge role 0),
+ * VH_BODY_ADDS_AGENTS when it calls add_agent! (agent role 0); such bodies
+ * run twice per apply! (count, then write). *transition_id receives the id
+ * to put in vh_apply_desc.transition (VH_TR_USER + k); registering the same
+ * kernel again gives the same id. At most VH_MAX_USER_BODIES bodies. */
+#define VH_BODY_ADDS_EDGES  1u
+#define VH_BODY_ADDS_AGENTS 2u
+#define VH_MAX_USER_BODIES  64
+int vh_register_body(const char* code_object, const char* kernel, uint32_t flags, uint32_t* transition_id);
 /* finish_init! src/Simulation.jl:403-476: upload and build the CSR */
 int vh_finish_init(vh_sim_t s);
 /* finish_init! with the agents of type t renumbered in breadth-first order

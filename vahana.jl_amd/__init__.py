@@ -13,6 +13,7 @@ src/Agent.jl / src/AgentMethods.jl (agents, mapreduce), src/Edge.jl /
 src/EdgeMethods.jl (edges and accessors).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -29,7 +30,7 @@ __all__ = [
     "agent_nr", "type_nr", "register_transition", "enable_asserts", "VahanaError",
     "register_param_", "register_global_", "param", "set_param_", "get_global", "set_global_",
     "modify_global_", "push_global_", "param_ref", "global_ref", "get_options", "set_options",
-    "options", "Batch",
+    "options", "Batch", "register_body",
 ]
 param_ref = transitions.param_ref
 global_ref = transitions.global_ref
@@ -926,6 +927,26 @@ def _mapreduce_expr(sim, f, op, T, kind, tid, init, datatype):
                              C.cast(C.pointer(out), C.c_void_p))
     _check(rc, sim.lib)
     return _result(out.value, datatype)
+
+
+def register_body(code_object, kernel, adds_edges=False, adds_agents=False):
+    """A transition body compiled against include/vahana_device.h into a code
+    object (hipcc --genco --offload-arch=gfx950), loaded at run time
+    (vh_register_body): apply! then takes any such body, as the reference's
+    takes any function (src/Simulation.jl:720-730). `adds_edges`: the body
+    calls add_edge! / remove_edges! (edge role 0); `adds_agents`: add_agent!
+    (agent role 0). Returns a transitions.UserBody."""
+    lib = _abi.lib()
+    flags = (_abi.BODY_ADDS_EDGES if adds_edges else 0) | (_abi.BODY_ADDS_AGENTS if adds_agents else 0)
+    tid = C.c_uint32()
+    _check(lib.register_body(os.fsencode(os.path.abspath(code_object)), kernel.encode(), flags,
+                             C.byref(tid)), lib)
+    return transitions.UserBody(tid.value, kernel, flags)
+
+
+def example_bodies():
+    """Path of the code object built from examples/bodies.hip."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "examples", "bodies.co")
 
 
 # ------------------------------------------------------------------ queries

@@ -94,6 +94,9 @@ EX = {"field": 0, "const_i": 1, "const_f": 2, "const_b": 3, "+": 10, "-": 11, "*
       "min": 14, "max": 15, "neg": 20, "abs": 21, "not": 22, "==": 30, "!=": 31, "<": 32, "<=": 33,
       ">": 34, ">=": 35, "and": 40, "or": 41, "select": 50}
 EXPR_MAX_OPS = 64
+TR_USER = 64
+BODY_ADDS_EDGES = 1
+BODY_ADDS_AGENTS = 2
 EXPR_MAX_DEPTH = 16
 
 
@@ -165,6 +168,7 @@ DEVICE_ONLY = [
     ("apply_n", C.c_int, [_vp, C.POINTER(ApplyDesc), C.c_uint32]),
     ("add_edges_gnp", C.c_int, [_vp, C.c_uint8, C.c_double, C.c_uint64, C.c_uint64]),
     ("add_edges_ba", C.c_int, [_vp, C.c_uint8, C.c_uint32, C.c_uint64, C.c_uint64]),
+    ("register_body", C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _u32p]),
     ("finish_init_relabel", C.c_int, [_vp, C.c_uint8, _u64p]),
     ("add_edges_stencil", C.c_int, [_vp, C.c_uint8, C.c_uint32, C.c_uint64, C.c_uint32,
                                     C.POINTER(C.c_int64), C.c_double, C.c_int, C.c_int,

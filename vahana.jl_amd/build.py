@@ -14,7 +14,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("VH_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["vh_prims.hip", "vh_engine.hip", "vh_query.hip", "vh_bodies.hip", "vh_ensemble.hip",
-           "vh_graphgen.hip", "vh_graph.cpp"]
+           "vh_graphgen.hip", "vh_userbody.hip", "vh_graph.cpp"]
 
 # -ffp-contract=off: no fused multiply-add, the model bodies round every
 # product and sum like the reference's IEEE arithmetic; packed FP32 ops are
@@ -36,6 +36,26 @@ def _compile(src):
     return obj
 
 
+EXAMPLES = os.path.join(HERE, "examples")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_example_bodies():
+    """Code objects of the run-time bodies in examples/ (vh_register_body)."""
+    out = []
+    for f in sorted(os.listdir(EXAMPLES)):
+        if not f.endswith(".hip"):
+            continue
+        co = os.path.join(EXAMPLES, f[:-4] + ".co")
+        cmd = (["hipcc", "--genco"] + [x for x in HIPFLAGS if x not in ("-fPIC",)] +
+               ["-I", INCLUDE, os.path.join(EXAMPLES, f), "-o", co])
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile {f} failed:\n{r.stdout}\n{r.stderr}")
+        out.append(co)
+    return out
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
@@ -47,6 +67,7 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_example_bodies()
     if verbose:
         print("built", OUT)
     return OUT

@@ -15,131 +15,6 @@ using namespace vh;
 
 namespace {
 
-// ----------------------------------------------------------------- accessors
-// in-edges [b, e) of `to` for edge type et (false: none / not readable)
-__device__ __forceinline__ bool d_row(const DCtx* c, int et, uint64_t to, uint32_t& b, uint32_t& e) {
-    b = e = 0;
-    if (et < 0 || et >= VH_MAXE) {
-        vh_flag(c, VH_DERR_HINT);
-        return false;
-    }
-    const DEdge& E = c->e[et];
-    if (!E.readable) {
-        vh_flag(c, VH_DERR_READ_EDGE);
-        return false;
-    }
-    if (!E.off) return false;
-    uint64_t r = vh_row(E, to);
-    if (r == ~0ull) return false;
-    b = E.off[r];
-    e = E.off[r + 1];
-    return e > b;
-}
-// agentstate / agentstate_flexible (src/AgentMethods.jl:91-123)
-__device__ __forceinline__ const uint8_t* d_state(const DCtx* c, uint64_t id) {
-    uint32_t t = vh_type_of(id);
-    if (t == 0 || t > VH_MAXA) {
-        vh_flag(c, VH_DERR_TARGET);
-        return nullptr;
-    }
-    const DAgent& A = c->a[t];
-    if (!A.readable) {
-        vh_flag(c, VH_DERR_READ_AGENT);
-        return nullptr;
-    }
-    uint64_t nr = vh_nr_of(id);
-    if (nr == 0 || nr > A.nslots) {
-        vh_flag(c, VH_DERR_TARGET);
-        return nullptr;
-    }
-    if (A.mortal && A.died[nr - 1]) {
-        vh_flag(c, VH_DERR_DIED);
-        return nullptr;
-    }
-    return A.rstate + (nr - 1) * A.bytes;
-}
-__device__ __forceinline__ int64_t ld_i64(const uint8_t* p) {
-    int64_t v;
-    memcpy(&v, p, 8);
-    return v;
-}
-__device__ __forceinline__ void st_i64(uint8_t* p, int64_t v) { memcpy(p, &v, 8); }
-
-// add_edge! / add_agent! inside a body: pass 0 counts, pass 1 writes at the
-// position given by the scan of the counts (caller order, then call order)
-struct Emitter {
-    const DCtx* c;
-    uint64_t caller;
-    int pass;
-    uint32_t ne, nb;
-
-    __device__ void edge(uint64_t from, uint64_t to, const void* st) {
-        const DEmitEdge& E = c->ee[0];
-        if (E.et < 0 || !c->e[E.et].writeable) {
-            vh_flag(c, VH_DERR_WRITE);
-            return;
-        }
-        if (pass == 1) {
-            uint64_t p = E.offs[caller] + ne;
-            E.to[p] = to;
-            E.from[p] = from;
-            for (uint32_t k = 0; k < E.bytes; k++) E.st[p * E.bytes + k] = ((const uint8_t*)st)[k];
-            if (E.kind) E.kind[p] = VH_LOG_ADD;
-        }
-        ne++;
-    }
-    // remove_edges!(sim, to, ET) (all) or remove_edges!(sim, from, to, ET)
-    // src/EdgeMethods.jl:527-599, logged in call order with the add_edge!s
-    // (_can_remove_edges :101-121: ET in `write` and in `add_existing`)
-    __device__ void remove(uint64_t from, uint64_t to, bool all) {
-        const DEmitEdge& E = c->ee[0];
-        if (E.et < 0 || !c->e[E.et].writeable) {
-            vh_flag(c, VH_DERR_WRITE);
-            return;
-        }
-        if (!c->e[E.et].add_existing) {
-            vh_flag(c, VH_DERR_ADDEX);
-            return;
-        }
-        if (!all && VH_HAS(c->e[E.et].hints, VH_HINT_IGNOREFROM)) {
-            vh_flag(c, VH_DERR_HINT);
-            return;
-        }
-        if (pass == 1) {
-            uint64_t p = E.offs[caller] + ne;
-            E.to[p] = to;
-            E.from[p] = from;
-            for (uint32_t k = 0; k < E.bytes; k++) E.st[p * E.bytes + k] = 0;
-            E.kind[p] = all ? VH_LOG_CLEAR : VH_LOG_REMOVE_FROM;
-        }
-        ne++;
-    }
-    // _get_next_id src/AgentMethods.jl:37-63: LIFO reuse, then fresh slots
-    __device__ uint64_t agent(const uint8_t* st) {
-        const DEmitAgent& B = c->ea[0];
-        if (B.at <= 0 || !c->a[B.at].writeable) {
-            vh_flag(c, VH_DERR_WRITE);
-            return 0;
-        }
-        const DAgent& A = c->a[B.at];
-        uint64_t id = vh_make_id(B.at, 0);
-        if (pass == 1) {
-            uint64_t g = B.offs[caller] + nb;
-            uint64_t slot = g < A.nreuse ? A.reuse[A.nreuse - 1 - g] : A.nextid + (g - A.nreuse);
-            uint8_t* dst = A.wstate + (slot - 1) * A.bytes;
-            for (uint32_t k = 0; k < A.bytes; k++) dst[k] = st[k];
-            id = vh_make_id(B.at, slot);
-        }
-        nb++;
-        return id;
-    }
-    __device__ void finish() {
-        if (pass != 0) return;
-        if (c->ee[0].cnt && ne) c->ee[0].cnt[caller] = ne;
-        if (c->ea[0].cnt && nb) c->ea[0].cnt[caller] = nb;
-    }
-};
-
 // ----------------------------------------------------------------- bodies
 template <int TR>
 struct Body;
@@ -408,43 +283,7 @@ struct Body<VH_TR_REMOVE_EDGES> {
 template <int TR>
 __global__ __launch_bounds__(256) void k_generic(const DCtx* __restrict__ c, int ci, int t, int pass,
                                                  int with_edge) {
-    uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const DAgent& A = c->a[t];
-    if (idx > A.nslots) return;
-    if (A.mortal && A.died[idx - 1]) return;
-    const uint64_t id = vh_make_id(t, idx);
-    const uint32_t bytes = A.bytes;
-    const uint8_t* rs = A.rstate + (idx - 1) * bytes;
-    if (with_edge >= 0) {
-        // transition_with_read_with_edge! src/AgentMethods.jl:209-228: only
-        // targets of with_edge are called; the others keep their state
-        const DEdge& E = c->e[with_edge];
-        uint64_t r = vh_row(E, id);
-        bool hasedge = E.off && r != ~0ull && E.off[r + 1] > E.off[r];
-        if (!hasedge) {
-            if (pass == 1 && A.writeable && !A.indep) {
-                uint8_t* ws = A.wstate + (idx - 1) * bytes;
-                for (uint32_t k = 0; k < bytes; k++) ws[k] = rs[k];
-            }
-            return;
-        }
-    }
-    uint8_t st[VH_MAX_STATE], out[VH_MAX_STATE];
-    for (uint32_t k = 0; k < bytes; k++) st[k] = out[k] = rs[k];
-    Emitter em{c, c->cbase[ci] + idx - 1, pass, 0u, 0u};
-    bool keep = Body<TR>::run(c, st, id, em, out);
-    if (pass == 0) {
-        em.finish();
-        return;
-    }
-    if (!A.writeable) return;
-    if (!keep) {
-        if (!A.mortal) vh_flag(c, VH_DERR_IMMORTAL);
-        else A.died_now[idx - 1] = 1;
-        return;
-    }
-    uint8_t* ws = A.wstate + (idx - 1) * bytes;
-    for (uint32_t k = 0; k < bytes; k++) ws[k] = out[k];
+    vh_generic_call<Body<TR>>(c, ci, t, pass, with_edge);
 }
 
 // ----------------------------------------------------------------- SIR
@@ -1522,6 +1361,7 @@ namespace vh {
 int launch_generic(int tr, const DCtx* d_ctx, int ci, int call_t, uint64_t nslots, int pass,
                    int with_edge, hipStream_t st) {
     if (!nslots) return VH_OK;
+    if (tr >= (int)VH_TR_USER) return launch_user_body((uint32_t)tr, d_ctx, ci, call_t, nslots, pass, with_edge, st);
     dim3 g(grid_for(nslots, 256)), b(256);
     switch (tr) {
 #define CASE(X)                                                                            \

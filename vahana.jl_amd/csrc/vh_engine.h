@@ -195,6 +195,11 @@ int gnp_device(uint64_t n, double mean_deg, uint64_t seed, DevBuf& off, DevBuf& 
 // CSR of a raster stencil (vh_bodies.hip): off[rows + 1] with the cell rows
 // at rowbase, sources as AgentIDs (src64) or numbers, uniform states
 int stencil_csr(Sim& s, EdgeStore& E, uint64_t rowbase, uint64_t rows);
+// bodies registered at run time (vh_userbody.hip): flags (-1: not one) and
+// the launch of its agent-loop kernel
+int user_body_flags(uint32_t tr, uint32_t* flags);
+int launch_user_body(uint32_t tr, const DCtx* d_ctx, int ci, int call_t, uint64_t nslots, int pass,
+                     int with_edge, hipStream_t st);
 // Barabasi-Albert pairs on the device (vh_graphgen.hip), equal to
 // vh_graph_ba: a[k], b[k] for k < ba_pairs(n, m)
 uint64_t ba_pairs(uint64_t n, uint32_t m);

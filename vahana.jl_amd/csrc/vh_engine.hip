@@ -1323,6 +1323,8 @@ static int kind_of(uint32_t tr) {
     case VH_TR_OPINION_F64: case VH_TR_OPINION_F32: return K_OPINION;
     case VH_TR_CONTACT: return K_CONTACT;
     }
+    uint32_t f;
+    if (user_body_flags(tr, &f) == 0) return (f & (VH_BODY_ADDS_EDGES | VH_BODY_ADDS_AGENTS)) ? K_TWOPASS : K_PLAIN;
     return -1;
 }
 
@@ -1517,12 +1519,18 @@ static int apply_one(Sim& s, const vh_apply_desc* d) {
         // births to role_agent[0] (SPAWN).
         int erole = d->transition == VH_TR_NEIGHBOR_FOO_EDGES ? d->role_edge[1] : d->role_edge[0];
         // remove_edges! bodies log removals beside the add_edge!s; the
-        // SendId body of that test adds to its second edge role
-        const bool removals = d->transition == VH_TR_REMOVE_EDGES;
-        if (removals && d->params && d->params_bytes >= sizeof(vh_test_params) &&
+        // SendId body of that test adds to its second edge role; a
+        // registered body adds to edge role 0 when it says so, and may
+        // remove there
+        uint32_t uflags = 0;
+        const bool user = user_body_flags(d->transition, &uflags) == 0;
+        if (user && !(uflags & VH_BODY_ADDS_EDGES)) erole = -1;
+        const bool removals = d->transition == VH_TR_REMOVE_EDGES || (user && (uflags & VH_BODY_ADDS_EDGES));
+        if (d->transition == VH_TR_REMOVE_EDGES && d->params && d->params_bytes >= sizeof(vh_test_params) &&
             ((const vh_test_params*)d->params)->value == VH_RM_SEND_ID)
             erole = d->role_edge[1];
-        bool births = d->transition == VH_TR_SPAWN && d->role_agent[0] > 0;
+        bool births = (d->transition == VH_TR_SPAWN || (user && (uflags & VH_BODY_ADDS_AGENTS))) &&
+                      d->role_agent[0] > 0;
         uint32_t* ecnt = nullptr;
         uint64_t* eoff = nullptr;
         uint32_t* bcnt = nullptr;

@@ -62,6 +62,8 @@ class Transition:
     def params_bytes(self):
         if self.params is None:
             return b""
+        if isinstance(self.params, (bytes, bytearray)):
+            return bytes(self.params)
         return bytes(C.string_at(C.addressof(self.params), C.sizeof(self.params)))
 
     def __repr__(self):
@@ -198,3 +200,21 @@ def move_to_value(sim, raster, edgetype, from_raster=True, to_raster=True, edge_
     for i, b in enumerate(raw):
         p.edge_state[i] = b
     return Transition(_abi.TR_MOVE_TO, "move_to_value", role_edges=[edgetype], params=p)
+
+
+class UserBody:
+    """A transition body loaded at run time (register_body): calling it with
+    the edge / agent types of its roles and its parameter blob (a ctypes
+    structure or bytes) gives the Transition to apply."""
+
+    def __init__(self, id_, name, flags):
+        self.id = id_
+        self.name = name
+        self.flags = flags
+
+    def __call__(self, role_edges=(), role_agents=(), params=None):
+        return Transition(self.id, self.name, role_agents=list(role_agents), role_edges=list(role_edges),
+                          params=params)
+
+    def __repr__(self):
+        return f"UserBody({self.name}, id={self.id})"

@@ -116,3 +116,8 @@ def test_register_body_errors(vh, gpu_ready, tmp_path):
     assert a.id == b.id >= vh._abi.TR_USER
     with pytest.raises(vh.VahanaError):
         vh.register_body(co, "vh_ex_add_f64", adds_edges=True)
+    # the failed loads leave no HIP error behind for later launches to report
+    m = vh.models.build_sir(vh, 200_000, seed=3, n_init=2000)
+    for _ in range(3):
+        vh.models.step_sir(vh, m, p_inf=0.2, seed=3)
+    m.sim._destroy()

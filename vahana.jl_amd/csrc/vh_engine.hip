@@ -27,6 +27,7 @@ int set_error(int code, const char* fmt, ...) {
     return code;
 }
 int hip_fail(hipError_t e, const char* where) {
+    (void)hipGetLastError();   // reported here, not again by a later launch check
     return set_error(e == hipErrorOutOfMemory ? VH_ERR_OOM : VH_ERR_DEVICE, "%s: %s", where,
                      hipGetErrorString(e));
 }

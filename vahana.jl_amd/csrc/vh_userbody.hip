@@ -34,14 +34,20 @@ std::vector<UserBody>& bodies() {
 int function_on(UserBody& b, int dev, hipFunction_t* out) {
     if (dev < 0 || dev >= UB_MAX_DEVICES) return set_error(VH_ERR_DEVICE, "device %d out of range", dev);
     if (!b.fn[dev]) {
+        // a failed call leaves its error as the thread's last HIP error, which
+        // a later launch check would report as its own: it is cleared here
         hipModule_t m = nullptr;
         hipError_t e = hipModuleLoad(&m, b.path.c_str());
-        if (e != hipSuccess) return set_error(VH_ERR_ARG, "can not load code object %s (%s)", b.path.c_str(),
-                                              hipGetErrorString(e));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(VH_ERR_ARG, "can not load code object %s (%s)", b.path.c_str(), hipGetErrorString(e));
+        }
         hipFunction_t f = nullptr;
         e = hipModuleGetFunction(&f, m, b.kernel.c_str());
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             (void)hipModuleUnload(m);
+            (void)hipGetLastError();
             return set_error(VH_ERR_ARG, "no kernel %s in %s (%s)", b.kernel.c_str(), b.path.c_str(),
                              hipGetErrorString(e));
         }

@@ -342,6 +342,20 @@ def main():
                                                   n=n if opin else None)
         log(f"cpu baseline, all cores {time.perf_counter() - t0:.1f}s: {cpu['all_cores']}")
 
+    # the metric's second half (ensemble p50/p99 against one at a time):
+    # config c3, 1024 SIR replicas x 200 steps in one launch per step
+    ens = None
+    if args.workload == "opinion" and not args.no_compare:
+        t0 = time.perf_counter()
+        e = ensemble_run(vh)
+        ens = {k: e[k] for k in ("replicas", "steps", "ms_per_step", "replicas_per_s",
+                                 "one_at_a_time_replicas_per_s", "speedup_vs_one_at_a_time", "value",
+                                 "edges_per_s", "autotune")}
+        ens["agent_updates_per_s"] = ens.pop("value")
+        ens["latency_s"] = {k: e["latency_s"][k] for k in ("ensemble", "one_at_a_time")}
+        log(f"ensemble {time.perf_counter() - t0:.1f}s: {ens['speedup_vs_one_at_a_time']:.2f}x one at a time, "
+            f"p99 {ens['latency_s']['ensemble']['p99']:.3f} s vs {ens['latency_s']['one_at_a_time']['p99']:.3f} s")
+
     out = {
         "metric": "agent-updates/s per apply! (edges/s alongside)",
         "value": value,
@@ -365,18 +379,24 @@ def main():
     }
     if gen_order is not None:
         out["generation_order"] = gen_order
+    if ens is not None:
+        out["ensemble"] = ens
     if drift is not None:
         out["f32_drift_vs_f64"] = drift
     print(json.dumps(out), flush=True)
 
 
 def bench_ensemble(vh, args):
+    out = ensemble_run(vh, args.replicas, args.steps if args.steps != 50 else 200)
+    print(json.dumps(out), flush=True)
+
+
+def ensemble_run(vh, R=1024, steps=200):
     """1024 SIR replicas (sizes log-uniform in [1e3, 1e6], mean in-degree
     U[4, 20], p_inf U[0.01, 0.1], distinct seeds), 200 steps in one
     graph-captured launch sequence, against the same replicas one at a time."""
     import numpy as np
     rng = np.random.default_rng(7)
-    R = args.replicas
     specs = [(int(10 ** rng.uniform(3, 6)), float(rng.uniform(4, 20)), float(rng.uniform(0.01, 0.1)),
               1 + i) for i in range(R)]
     E = vh.ensemble.Ensemble()
@@ -387,7 +407,6 @@ def bench_ensemble(vh, args):
     block, remap, tms = E.autotune(2)
     log(f"autotune: block {block}, xcd_remap {remap}, agents per thread by size class {E.class_apt}, "
         f"{tms:.3f} ms/step")
-    steps = args.steps if args.steps != 50 else 200
     E.step(1, use_graph=False)
     E.sync()
     t0 = time.perf_counter()
@@ -457,8 +476,8 @@ def bench_ensemble(vh, args):
            "replicas_per_s": R / wall, "one_at_a_time_replicas_per_s": R / solo_total,
            "speedup_vs_one_at_a_time": solo_total / wall, "dtype": "u8", "n_gpus": 1,
            "higher_is_better": True}
-    print(json.dumps(out), flush=True)
     E.close()
+    return out
 
 
 def bench_serve(vh, args):

@@ -555,9 +555,11 @@ def bench_serve(vh, args):
                         "served_wall_s": cap_srv["wall_s"], "by_size_class": by_class,
                         "capacity_ratio_bound_large_classes": cap_seq["wall_s"] / max(
                             1e-9, by_class["1e6-1e7"]["run_s"] + by_class["1e5-1e6"]["run_s"]),
+                        "capacity_ratio_bound_steps_and_builds": cap_seq["wall_s"] / max(
+                            1e-9, sum(c["run_s"] + c["build_s"] for c in by_class.values())),
                         "note": "step_s: enqueue-to-completion time of each request's steps when it runs "
                                 "alone; the 1e5-1e7 classes' step time is GPU-bound work that batching "
-                                "cannot remove"},
+                                "cannot remove; graph builds are device kernels as well"},
            "n_gpus": 1, "higher_is_better": True,
            "config": {"workload": "serving: Poisson SIR/opinion requests, N ~ logU[1e4, 1e7], "
                                   "horizon ~ U[50, 500], mean degree ~ U[4, 20], device-built G(n, p)"}}

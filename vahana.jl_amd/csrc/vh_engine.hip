@@ -1082,7 +1082,7 @@ static int ba_agents(Sim& s, EdgeStore& E, uint64_t* n, uint64_t* K) {
 static int ba_reserve(Sim& s, EdgeStore& E) {
     uint64_t n, K;
     RC(ba_agents(s, E, &n, &K));
-    if (s.scratch.reserve(200 * K + 8 * n + ((size_t)64 << 20), s.stream))
+    if (s.scratch.reserve(128 * K + 8 * n + ((size_t)64 << 20), s.stream))
         return set_error(VH_ERR_OOM, "scratch for the Barabasi-Albert graph");
     return VH_OK;
 }
@@ -1271,6 +1271,9 @@ extern "C" int vh_finish_init(vh_sim_t h) {
     }
     s.scratch.reset(st);
     HIPCHECK(hipStreamSynchronize(st));
+    // the arena grown for the init-time sorts (device graphs, numbering) is
+    // not kept for the steps, which size it for themselves
+    if (s.scratch.bytes_held() > ((size_t)1 << 30)) s.scratch.release();
     uint32_t e = 0;
     HIPCHECK(hipMemcpy(&e, s.d_err.p, 4, hipMemcpyDeviceToHost));
     if (e) return set_error(VH_ERR_ASSERT, "invalid edge target during finish_init! (flags %u)", e);

@@ -94,8 +94,8 @@ struct EdgeStore {
     // every device buffer of the store (release, accounting)
     template <class F>
     void each_buf(F f) {
-        DevBuf* all[] = {&off,   &src, &st,  &heavy,  &psrc,    &pval,    &lbase, &tpk,  &twt,     &prun,
-                         &pfirst, &hbase, &hty, &hslot, &hinfo, &out_off, &out_dst, &expo, &expo_cnt};
+        DevBuf* all[] = {&off,   &src,   &st,  &heavy, &psrc,  &pval,    &lbase,   &tpk,  &twt,      &prun,
+                         &pfirst, &hbase, &hty, &hslot, &hinfo, &out_off, &out_dst, &expo, &expo_cnt, &ba_pairs};
         for (DevBuf* b : all) f(*b);
     }
     bool writeable = false;
@@ -113,6 +113,8 @@ struct EdgeStore {
     // in-edges requested as a device Barabasi-Albert graph, both directions
     // of every pair (vh_add_edges_ba; seeds in gen_seed / gen_wseed)
     uint32_t ba_m = 0;             // 0: none
+    DevBuf ba_pairs;               // its pairs when the numbering generated them (a[K], then b[K])
+    bool ba_pairs_ok = false;
 };
 
 struct Sim {

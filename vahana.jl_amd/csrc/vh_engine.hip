@@ -1092,13 +1092,14 @@ static int build_ba_csr(Sim& s, EdgeStore& E) {
     Scratch& sc = s.scratch;
     uint64_t n, K;
     RC(ba_agents(s, E, &n, &K));
-    uint32_t* a = (uint32_t*)sc.take(K * 4 + 4);
-    uint32_t* b = (uint32_t*)sc.take(K * 4 + 4);
+    const bool have = E.ba_pairs_ok;   // generated by the numbering
+    uint32_t* a = have ? E.ba_pairs.as<uint32_t>() : (uint32_t*)sc.take(K * 4 + 4);
+    uint32_t* b = have ? a + K : (uint32_t*)sc.take(K * 4 + 4);
     uint64_t* to = (uint64_t*)sc.take(2 * K * 8 + 8);
     uint64_t* fr = (uint64_t*)sc.take(2 * K * 8 + 8);
     float* w = E.bytes ? (float*)sc.take(2 * K * 4 + 8) : nullptr;
     if (!a || !b || !to || !fr || (E.bytes && !w)) return set_error(VH_ERR_OOM, "scratch");
-    RC(ba_device(n, E.ba_m, E.gen_seed, a, b, sc, st));
+    if (!have) RC(ba_device(n, E.ba_m, E.gen_seed, a, b, sc, st));
     const uint32_t* map =
         (s.relabel_type == E.target && s.relabel_map.p) ? s.relabel_map.as<uint32_t>() : nullptr;
     if (K)
@@ -1106,6 +1107,9 @@ static int build_ba_csr(Sim& s, EdgeStore& E) {
                            map, to, fr);
     if (w) fill_weights(w, 2 * K, E.gen_wseed, st);
     RC(build_csr(s, E, false, to, fr, (const uint8_t*)w, 2 * K));
+    HIPCHECK(hipStreamSynchronize(st));
+    E.ba_pairs.release();
+    E.ba_pairs_ok = false;
     E.ba_m = 0;
     return VH_OK;
 }
@@ -1166,11 +1170,14 @@ extern "C" int vh_finish_init_relabel(vh_sim_t h, uint8_t t, uint64_t* new_nr) {
     for (int et = 0; et < s.m->ne; et++) {
         if (!cnt[et]) continue;
         if (bak[et]) {
+            // the pairs are kept for finish_init!'s CSR build
             const uint64_t K = bak[et];
-            uint32_t* pa = (uint32_t*)sc.take(K * 4 + 4);
-            uint32_t* pb = (uint32_t*)sc.take(K * 4 + 4);
-            if (!pa || !pb) return set_error(VH_ERR_OOM, "scratch");
-            RC(ba_device(n, s.e[et].ba_m, s.e[et].gen_seed, pa, pb, sc, st));
+            EdgeStore& E = s.e[et];
+            RC(E.ba_pairs.ensure(2 * K * 4 + 8, st));
+            uint32_t* pa = E.ba_pairs.as<uint32_t>();
+            uint32_t* pb = pa + K;
+            RC(ba_device(n, E.ba_m, E.gen_seed, pa, pb, sc, st));
+            E.ba_pairs_ok = true;
             hipLaunchKernelGGL(k_ba_ends, dim3(grid_for(2 * K, 256)), dim3(256), 0, st, pa, pb, K, ea + o, eb + o);
         } else {
             HIPCHECK(hipMemcpyAsync(ea + o, ha[et].data(), cnt[et] * 4, hipMemcpyHostToDevice, st));

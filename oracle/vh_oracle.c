@@ -1107,7 +1107,48 @@ static int body_remove_edges(ctx_t* c, const uint8_t* st, uint64_t id, uint8_t* 
     return 1;
 }
 
+/* ------------------------------------------------ caller-supplied bodies
+ * The checker's counterpart of vh_register_body: a C restatement of a
+ * run-time body, registered as a function over the accessors below. */
+#define OR_MAX_USER 64
+static body_fn g_user[OR_MAX_USER];
+static uint32_t g_nuser;
+
+int or_register_body(or_body_fn fn, uint32_t flags, uint32_t* id) {
+    (void)flags;   /* the checker applies add_edge!/add_agent! as they come */
+    if (!fn || !id) return fail(VH_ERR_ARG, "null argument");
+    for (uint32_t k = 0; k < g_nuser; k++)
+        if (g_user[k] == (body_fn)fn) { *id = VH_TR_USER + k; return VH_OK; }
+    if (g_nuser >= OR_MAX_USER) return fail(VH_ERR_CAPACITY, "too many registered bodies");
+    g_user[g_nuser] = (body_fn)fn;
+    *id = VH_TR_USER + g_nuser++;
+    return VH_OK;
+}
+const void* or_ctx_params(or_ctx* c) { return ((ctx_t*)c)->s->params; }
+int or_ctx_role_edge(or_ctx* c, int k) { return ((ctx_t*)c)->d->role_edge[k]; }
+int or_ctx_role_agent(or_ctx* c, int k) { return ((ctx_t*)c)->d->role_agent[k]; }
+uint32_t or_ctx_in_edges(or_ctx* cc, int et, uint64_t id, const uint64_t** from, const uint8_t** states) {
+    ctx_t* c = (ctx_t*)cc;
+    cont_t* k = rd_cont(c, et, id);
+    if (!k) return 0;
+    if (from) *from = k->from;
+    if (states) *states = k->st;
+    return k->n;
+}
+const uint8_t* or_ctx_agentstate(or_ctx* c, uint64_t id) { return agentstate_of((ctx_t*)c, id); }
+void or_ctx_add_edge(or_ctx* c, int et, uint64_t from, uint64_t to, const void* st) {
+    ctx_add_edge((ctx_t*)c, et, from, to, (const uint8_t*)st);
+}
+uint64_t or_ctx_add_agent(or_ctx* c, int t, const void* st) {
+    return ctx_add_agent((ctx_t*)c, t, (const uint8_t*)st);
+}
+void or_ctx_remove_edges(or_ctx* c, int et, uint64_t from, uint64_t to, int all) {
+    ctx_remove_edges((ctx_t*)c, et, from, to, all);
+}
+int or_ctx_failed(or_ctx* c) { return ((ctx_t*)c)->err != 0; }
+
 static body_fn body_of(uint32_t tr) {
+    if (tr >= VH_TR_USER && tr - VH_TR_USER < g_nuser) return g_user[tr - VH_TR_USER];
     switch (tr) {
     case VH_TR_NOOP: return body_noop;
     case VH_TR_SUM_NEIGHBORS: return body_sum_neighbors;

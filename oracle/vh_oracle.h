@@ -40,6 +40,23 @@ int or_set_last_change(or_sim* s, int kind, uint8_t t, uint64_t v);
 int or_restore_died(or_sim* s, uint8_t t, const uint8_t* died, uint64_t n);
 int or_set_num_transitions(or_sim* s, uint64_t n);
 /* order: 0 = the reference's sequential left fold, 1 = the device's fixed tree */
+/* bodies supplied by the caller (the checker's counterpart of
+ * vh_register_body): fn returns 1 with the new state in `out` (a copy of
+ * `st` on entry) or 0 for `nothing`; it reaches the simulation through the
+ * or_ctx_* accessors (the reference's transition API) */
+typedef struct or_ctx or_ctx;
+typedef int (*or_body_fn)(or_ctx* c, const uint8_t* st, uint64_t id, uint8_t* out);
+int or_register_body(or_body_fn fn, uint32_t flags, uint32_t* id);
+const void* or_ctx_params(or_ctx* c);
+int or_ctx_role_edge(or_ctx* c, int k);
+int or_ctx_role_agent(or_ctx* c, int k);
+/* edges(sim, id, ET): count, sources and states in insertion order */
+uint32_t or_ctx_in_edges(or_ctx* c, int et, uint64_t id, const uint64_t** from, const uint8_t** states);
+const uint8_t* or_ctx_agentstate(or_ctx* c, uint64_t id);
+void or_ctx_add_edge(or_ctx* c, int et, uint64_t from, uint64_t to, const void* st);
+uint64_t or_ctx_add_agent(or_ctx* c, int t, const void* st);
+void or_ctx_remove_edges(or_ctx* c, int et, uint64_t from, uint64_t to, int all);
+int or_ctx_failed(or_ctx* c);
 int or_mapreduce_expr(or_sim* s, int kind, uint8_t t, const vh_expr_op* prog, uint32_t nops, int op,
                       int dtype, const void* init, void* out, int order);
 int or_mapreduce(or_sim* s, int kind, uint8_t t, const vh_map_desc* f, int op,

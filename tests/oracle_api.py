@@ -34,6 +34,7 @@ def lib():
             ("hash4", C.c_uint64, [C.c_uint64] * 4),
             ("canon_sum_f64", None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint32,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            ("register_body", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
         ])
         L.mapreduce.argtypes = list(L.mapreduce.argtypes) + [C.c_int]
         L.mapreduce_expr.argtypes = list(L.mapreduce_expr.argtypes) + [C.c_int]
@@ -61,6 +62,24 @@ class OracleSim(vh.Simulation):
         buf = (C.c_uint64 * max(n.value, 1))()
         self.lib.reuse_stack(self.handle, self._aid(T), buf, n.value, C.byref(n))
         return [buf[i] for i in range(n.value)]
+
+
+def register_body(code_object, kernel, adds_edges=False, adds_agents=False):
+    """The checker's counterpart of vh.register_body: the C restatement of
+    device kernel `vh_<name>` is the checker function `or_<name>`
+    (oracle/example_bodies.c), registered with or_register_body."""
+    L = lib()
+    name = "or_" + (kernel[3:] if kernel.startswith("vh_") else kernel)
+    fn = C.cast(getattr(L.dll, name), C.c_void_p)
+    flags = (vh._abi.BODY_ADDS_EDGES if adds_edges else 0) | (vh._abi.BODY_ADDS_AGENTS if adds_agents else 0)
+    tid = C.c_uint32()
+    rc = L.register_body(fn, flags, C.byref(tid))
+    if rc != 0:
+        raise vh.VahanaError(rc, L.last_error().decode())
+    return vh.transitions.UserBody(tid.value, kernel, flags)
+
+
+example_bodies = vh.example_bodies
 
 
 def create_model(name):

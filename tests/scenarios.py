@@ -960,3 +960,53 @@ def snapshot_hint_state(V, sim, t):
     out["globals_last_change"] = sim.globals_last_change
     out["num_transitions"] = sim._num_transitions()
     return out
+
+
+# ------------------- test/hdf5_snapshot.jl with its own transitions (bodies
+# loaded at run time on the device, their C restatements in the checker)
+import ctypes as _C
+
+
+class AddF64Params(_C.Structure):
+    _fields_ = [("offset", _C.c_uint32), ("pad", _C.c_uint32), ("delta", _C.c_double)]
+
+
+class RasterParams(_C.Structure):
+    _fields_ = [("f_offset", _C.c_uint32), ("pad", _C.c_uint32), ("modulus", _C.c_double),
+                ("threshold", _C.c_double)]
+
+
+def reference_bodies(V):
+    """(Agent(state.inner, state.f - 2), remove_some_rasternodes) of
+    test/hdf5_common.jl:107-109, 129-134 as registered bodies."""
+    co = V.example_bodies()
+    return (V.register_body(co, "vh_ex_add_f64"),
+            V.register_body(co, "vh_ex_remove_some_rasternodes", adds_edges=True))
+
+
+def reference_raster_step(V, sim, t, ras):
+    A, RA, RE = t["Agent"], t["RasterAgent"], t["RasterEdge"]
+    p = RasterParams(A.dtype.fields["f"][1], 0, 8.0, 4.5)
+    V.apply_(sim, ras(role_edges=[RE], params=p), [RA], [RA, RE, A], [RA, RE])
+
+
+def reference_runsim(V, model, t, path, bodies):
+    """runsim test/hdf5_common.jl:101-127 with its own transitions."""
+    sub, ras = bodies
+    A = t["Agent"]
+    sim = snapshot_hint_createsim(V, model, t)
+    V.snapshot.write_snapshot(sim, path, "Initial state")
+    V.apply_(sim, sub(params=AddF64Params(A.dtype.fields["f"][1], 0, -2.0)), [A], [A], [A])
+    V.push_global_(sim, "gb", 2.1)
+    V.push_global_(sim, "gc", 4)
+    V.snapshot.write_snapshot(sim, path)
+    reference_raster_step(V, sim, t, ras)
+    V.snapshot.write_snapshot(sim, path, "Final state")
+    return sim
+
+
+def raster_edges(V, sim, t):
+    """(cell nr, agent nr, a) of every RasterEdge, sorted."""
+    to, fr, st = sim._read_edges(t["RasterEdge"])
+    nr = lambda x: (x & np.uint64(V._abi.VH_NR_MASK)).astype(np.int64)  # noqa: E731
+    return sorted(zip(nr(to).tolist(), nr(fr).tolist(), np.asarray(st["a"]).tolist()))

@@ -39,6 +39,26 @@ def test_served_results_equal_solo_results(vh, ora, gpu_ready):
                           np.ascontiguousarray(want).view(np.uint8)), (r.n, r.horizon)
 
 
+def test_served_config4_mix_equal_solo_results(vh, gpu_ready):
+    """Config c4 at its stated sizes: 24 Poisson requests, SIR or opinion,
+    N ~ logU[10^4, 10^7], horizons U[50, 500], HBM budget 0.9 x free; every
+    served result equals its one-at-a-time run (xxh3-128 digests)."""
+    import torch
+    S = vh.serve
+    reqs = S.make_requests(24, rate=20.0, seed=11)
+    assert max(r.n for r in reqs) > 10 ** 6 and min(r.n for r in reqs) < 10 ** 5
+    print(f"[gpu] serving c4 mix: {len(reqs)} requests, {sum(r.n * r.horizon for r in reqs):.3g} agent-steps",
+          flush=True)
+    free, _total = torch.cuda.mem_get_info()
+    srv = S.Server(vh, hbm_budget=int(0.9 * free), keep_results=False)
+    done, wall = srv.run(reqs)
+    srv.close()
+    seq, _ = S.run_sequential(vh, reqs)
+    assert len(done) == len(reqs)
+    assert {r.id: r.digest for r in done} == {r.id: r.digest for r in seq}
+    print(f"[gpu] serving c4 mix: {S.summarize(done, wall)}", flush=True)
+
+
 def test_serving_failures_leave_others_unchanged(vh, gpu_ready):
     """A request larger than the whole budget is rejected, one is cancelled
     after its first quantum, one fails to build (an injected device

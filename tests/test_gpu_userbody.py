@@ -1,9 +1,9 @@
 """Transition bodies loaded at run time (vh_register_body): bodies written
 against include/vahana_device.h in vahana.jl_amd/examples/bodies.hip, built
-into a code object, applied like built-in ones. The reference's own
-test/hdf5_snapshot.jl runs here with its own transitions (test/hdf5_common.jl:
-107-109, 129-134)."""
-import ctypes as C
+into a code object, applied like built-in ones and compared with their C
+restatements in the checker (oracle/example_bodies.c). The reference's own
+test/hdf5_snapshot.jl runs here with its own transitions
+(test/hdf5_common.jl:107-109, 129-134)."""
 import json
 import os
 
@@ -15,91 +15,59 @@ import scenarios as SC
 pytestmark = pytest.mark.gpu
 
 
-class AddF64(C.Structure):
-    _fields_ = [("offset", C.c_uint32), ("pad", C.c_uint32), ("delta", C.c_double)]
-
-
-class RasterP(C.Structure):
-    _fields_ = [("f_offset", C.c_uint32), ("pad", C.c_uint32), ("modulus", C.c_double),
-                ("threshold", C.c_double)]
-
-
-def _bodies(vh):
-    co = vh.example_bodies()
-    sub = vh.register_body(co, "vh_ex_add_f64")
-    ras = vh.register_body(co, "vh_ex_remove_some_rasternodes", adds_edges=True)
-    return sub, ras
-
-
-def test_user_body_field_update(vh, gpu_ready):
-    """state.f += delta on every agent, the states of the other fields kept."""
+def test_user_body_field_update(vh, ora, gpu_ready):
+    """state.f += delta on every agent, the states of the other fields kept;
+    equal to the checker."""
     print("[gpu] run-time body: add to a Float64 field", flush=True)
-    sub, _ras = _bodies(vh)
-    model, t = SC.snapshot_hint_model(vh, "default")
-    sim = SC.snapshot_hint_createsim(vh, model, t)
-    A = t["Agent"]
-    before = np.asarray(sim._read_agents(A)[0]).copy()
-    p = AddF64(A.dtype.fields["f"][1], 0, -2.0)
-    vh.apply_(sim, sub(params=p), [A], [A], [A])
-    after = np.asarray(sim._read_agents(A)[0])
-    assert np.array_equal(after["f"], before["f"] - 2.0)
-    assert np.array_equal(after["foo"], before["foo"]) and np.array_equal(after["inner_f"], before["inner_f"])
-
-
-def _runsim(vh, model, t, path, sub, ras):
-    """runsim test/hdf5_common.jl:101-127 with its own transitions."""
-    A, RA, RE = t["Agent"], t["RasterAgent"], t["RasterEdge"]
-    sim = SC.snapshot_hint_createsim(vh, model, t)
-    vh.snapshot.write_snapshot(sim, path, "Initial state")
-    vh.apply_(sim, sub(params=AddF64(A.dtype.fields["f"][1], 0, -2.0)), [A], [A], [A])
-    vh.push_global_(sim, "gb", 2.1)
-    vh.push_global_(sim, "gc", 4)
-    vh.snapshot.write_snapshot(sim, path)
-    _raster_step(vh, sim, t, ras)
-    vh.snapshot.write_snapshot(sim, path, "Final state")
-    return sim
-
-
-def _raster_step(vh, sim, t, ras):
-    A, RA, RE = t["Agent"], t["RasterAgent"], t["RasterEdge"]
-    p = RasterP(A.dtype.fields["f"][1], 0, 8.0, 4.5)
-    vh.apply_(sim, ras(role_edges=[RE], params=p), [RA], [RA, RE, A], [RA, RE])
-
-
-def _raster_edges(sim, t):
-    to, fr, st = sim._read_edges(t["RasterEdge"])
-    nr = lambda x: (x & np.uint64(0xFFFFFFFFF)).astype(np.int64)  # noqa: E731
-    return sorted(zip(nr(to).tolist(), nr(fr).tolist(), np.asarray(st["a"]).tolist()))
+    out = []
+    for V in (vh, ora):
+        sub, _ras = SC.reference_bodies(V)
+        model, t = SC.snapshot_hint_model(V, "default")
+        sim = SC.snapshot_hint_createsim(V, model, t)
+        A = t["Agent"]
+        before = np.asarray(sim._read_agents(A)[0]).copy()
+        V.apply_(sim, sub(params=SC.AddF64Params(A.dtype.fields["f"][1], 0, -2.0)), [A], [A], [A])
+        after = np.asarray(sim._read_agents(A)[0])
+        assert np.array_equal(after["f"], before["f"] - 2.0)
+        assert np.array_equal(after["foo"], before["foo"]) and np.array_equal(after["inner_f"], before["inner_f"])
+        out.append(after.tobytes())
+    assert out[0] == out[1]
 
 
 @pytest.mark.parametrize("variant", ["default", "immortal", "neighbors"])
-def test_hdf5_snapshot_with_reference_transitions(vh, gpu_ready, tmp_path, variant):
+def test_hdf5_snapshot_with_reference_transitions(vh, ora, gpu_ready, tmp_path, variant):
     """test/hdf5_snapshot.jl:5-27 with test/hdf5_common.jl's transitions as
-    run-time bodies: the restored simulation equals the one that wrote the
-    snapshot, before and after both apply remove_some_rasternodes again; the
-    raster cells whose first in-neighbour has mod1(f, 8) > 4.5 (cells 5-8)
-    die, the survivors keep one edge with the state doubled each time."""
+    run-time bodies: device and checker write the same snapshot records; the
+    restored simulation equals the one that wrote the snapshot, before and
+    after both apply remove_some_rasternodes again; the raster cells whose
+    first in-neighbour has mod1(f, 8) > 4.5 (cells 5-8) die, the survivors
+    keep one edge with the state doubled each time."""
     print(f"[gpu] reference snapshot test with run-time bodies ({variant})", flush=True)
-    sub, ras = _bodies(vh)
-    model, t = SC.snapshot_hint_model(vh, variant)
-    path = str(tmp_path / variant)
-    sim = _runsim(vh, model, t, path, sub, ras)
-    RA = t["RasterAgent"]
-    with open(os.path.join(path, "index.json")) as fh:
-        idx = json.load(fh)
-    assert [s[0] for s in idx["snapshots"]] == [0, 1, 2]
-    assert idx["agents"]["Agent"] == [[0, 0], [1, 1]] and idx["agents"]["RasterAgent"] == [[0, 0], [2, 2]]
-    assert idx["edges"]["RasterEdge"] == [[0, 0], [2, 2]] and idx["edges"]["EdgeState"] == [[0, 0]]
-    st, died = sim._read_agents(RA)
-    assert np.asarray(st["foo"])[~died].tolist() == [1, 2, 3, 4]
-    # cell c's first in-edge comes from agent c (state a = c % 8 + 1)
-    assert _raster_edges(sim, t) == [(c, c, 2 * (c % 8 + 1)) for c in (1, 2, 3, 4)]
-    restored = SC.snapshot_hint_restore(vh, model, path)
-    assert SC.snapshot_hint_state(vh, sim, t) == SC.snapshot_hint_state(vh, restored, t)
-    _raster_step(vh, sim, t, ras)
-    _raster_step(vh, restored, t, ras)
-    assert SC.snapshot_hint_state(vh, sim, t) == SC.snapshot_hint_state(vh, restored, t)
-    assert _raster_edges(restored, t) == [(c, c, 4 * (c % 8 + 1)) for c in (1, 2, 3, 4)]
+    res = {}
+    for V, tag in ((vh, "g"), (ora, "c")):
+        bodies = SC.reference_bodies(V)
+        model, t = SC.snapshot_hint_model(V, variant)
+        path = str(tmp_path / f"{variant}_{tag}")
+        sim = SC.reference_runsim(V, model, t, path, bodies)
+        with open(os.path.join(path, "index.json")) as fh:
+            idx = json.load(fh)
+        assert [s[0] for s in idx["snapshots"]] == [0, 1, 2]
+        assert idx["agents"]["Agent"] == [[0, 0], [1, 1]] and idx["agents"]["RasterAgent"] == [[0, 0], [2, 2]]
+        assert idx["edges"]["RasterEdge"] == [[0, 0], [2, 2]] and idx["edges"]["EdgeState"] == [[0, 0]]
+        st, died = sim._read_agents(t["RasterAgent"])
+        assert np.asarray(st["foo"])[~died].tolist() == [1, 2, 3, 4]
+        # cell c's first in-edge comes from agent c (state a = c % 8 + 1)
+        assert SC.raster_edges(V, sim, t) == [(c, c, 2 * (c % 8 + 1)) for c in (1, 2, 3, 4)]
+        restored = SC.snapshot_hint_restore(V, model, path)
+        first = SC.snapshot_hint_state(V, sim, t)
+        assert first == SC.snapshot_hint_state(V, restored, t)
+        SC.reference_raster_step(V, sim, t, bodies[1])
+        SC.reference_raster_step(V, restored, t, bodies[1])
+        second = SC.snapshot_hint_state(V, sim, t)
+        assert second == SC.snapshot_hint_state(V, restored, t)
+        assert SC.raster_edges(V, restored, t) == [(c, c, 4 * (c % 8 + 1)) for c in (1, 2, 3, 4)]
+        res[tag] = (first, second, idx)
+    assert res["g"] == res["c"]
 
 
 def test_register_body_errors(vh, gpu_ready, tmp_path):

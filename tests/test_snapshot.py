@@ -184,3 +184,25 @@ def test_param_values_restored(ora, tmp_path):
     assert r.globals_last_change == 3
     r2 = SC.snapshot_hint_restore(ora, model, path, ignore_params=True)
     assert r2.params["pb"] == 0 and r2._value(ora._abi.VALUE_PARAM, "pb") == 0
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_reference_transitions_checker(ora, tmp_path, variant):
+    """test/hdf5_snapshot.jl with test/hdf5_common.jl's own transitions (the
+    checker's restatements in oracle/example_bodies.c): the known answers of
+    remove_some_rasternodes (cells 5-8 die, the survivors keep one edge with
+    the state doubled), the records it writes, and restore = original."""
+    bodies = SC.reference_bodies(ora)
+    model, t = SC.snapshot_hint_model(ora, variant)
+    path = str(tmp_path / variant)
+    sim = SC.reference_runsim(ora, model, t, path, bodies)
+    st, died = sim._read_agents(t["RasterAgent"])
+    assert np.asarray(st["foo"])[~died].tolist() == [1, 2, 3, 4]
+    assert SC.raster_edges(ora, sim, t) == [(c, c, 2 * (c % 8 + 1)) for c in (1, 2, 3, 4)]
+    assert np.asarray(sim._read_agents(t["Agent"])[0]["f"]).tolist() == [float(i) for i in range(1, 17)]
+    restored = SC.snapshot_hint_restore(ora, model, path)
+    assert SC.snapshot_hint_state(ora, sim, t) == SC.snapshot_hint_state(ora, restored, t)
+    SC.reference_raster_step(ora, sim, t, bodies[1])
+    SC.reference_raster_step(ora, restored, t, bodies[1])
+    assert SC.snapshot_hint_state(ora, sim, t) == SC.snapshot_hint_state(ora, restored, t)
+    assert SC.raster_edges(ora, restored, t) == [(c, c, 4 * (c % 8 + 1)) for c in (1, 2, 3, 4)]

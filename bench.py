@@ -119,8 +119,8 @@ def cpu_baseline_all_cores(workload, f32, relabel=True, n=None):
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--workload", workload]
     if n:
         cmd += ["--n", str(max(1000, n // P))]
-    if not relabel:
-        cmd.append("--no-relabel")
+    if relabel:
+        cmd.append("--relabel")
     procs = [subprocess.Popen(cmd + ["--rank", str(r)], stdout=subprocess.PIPE, env=env)
              for r in range(P)]
     res = []
@@ -140,7 +140,7 @@ def cpu_baseline_all_cores(workload, f32, relabel=True, n=None):
 
 def cpu_worker(args):
     f32 = args.workload == "opinion32"
-    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank, relabel=not args.no_relabel,
+    n, ne, times = _cpu_sample(args.workload, f32, seed=1 + args.rank, relabel=args.relabel,
                                n=args.n if args.n != 10_000_000 else None)
     print(json.dumps({"n": n, "edges": ne, "median_s": times[len(times) // 2]}), flush=True)
 
@@ -161,11 +161,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-classes", action="store_true",
                     help="serve: also measure capacity per size class (30 requests per decade)")
-    ap.add_argument("--no-relabel", action="store_true",
-                    help="opinion: keep the generation order of the agents (default: renumbered "
-                         "breadth-first at finish_init!)")
+    ap.add_argument("--relabel", action="store_true",
+                    help="opinion: renumber the agents breadth-first at finish_init! (default: "
+                         "generation order)")
     ap.add_argument("--no-compare", action="store_true",
-                    help="opinion: skip the second timing in generation order")
+                    help="opinion: skip the second timing in the other agent order")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--rank", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calib", action="store_true",
@@ -198,7 +198,7 @@ def main():
         # finish_init! (the same graph the host generator gives; the CPU
         # baseline generates its copy on the host)
         edges = None
-        relabel = not args.no_relabel
+        relabel = args.relabel
         m = M.build_opinion(vh, n, seed=1, f32=f32, relabel=relabel, device_graph=True)
         step = lambda: M.step_opinion(vh, m, eps=0.2)  # noqa: E731
         nbytes = opinion_bytes(n, m.nedges, f32)
@@ -286,18 +286,19 @@ def main():
     achieved = nbytes / (ev_mean * 1e-3) / 1e9
     log(f"step wall {ms:.3f} ms, kernel events mean {ev_mean:.3f} ms (min {ev[0]:.3f} max {ev[-1]:.3f})")
 
-    gen_order = None
-    if args.workload.startswith("opinion") and not args.no_relabel and not args.no_compare:
-        # the same graph and states in generation order (no renumbering at
-        # finish_init!), timed the same way
-        x_relabel = m
-        m2 = M.build_opinion(vh, n, seed=1, f32=f32, relabel=False, device_graph=True)
+    other_order = None
+    if args.workload.startswith("opinion") and not args.no_compare:
+        # the same graph and states in the other agent order (breadth-first
+        # numbering at finish_init! when the headline keeps the generation
+        # order, and the reverse), timed the same way
+        m2 = M.build_opinion(vh, n, seed=1, f32=f32, relabel=not args.relabel, device_graph=True)
         ms2, ev2, _ = timed(m2.sim, lambda: M.step_opinion(vh, m2, eps=0.2))
-        gen_order = {"ms_per_step": ms2, "kernel_ms": ev2, "value": n / (ms2 * 1e-3),
-                     "roofline_frac": nbytes / (ev2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        log(f"generation order: step wall {ms2:.3f} ms, kernel events mean {ev2:.3f} ms")
+        other_order = {"ms_per_step": ms2, "kernel_ms": ev2, "value": n / (ms2 * 1e-3),
+                       "roofline_frac": nbytes / (ev2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"{'generation' if args.relabel else 'breadth-first'} order: step wall {ms2:.3f} ms, "
+            f"kernel events mean {ev2:.3f} ms")
         m2.sim._destroy()
-        del m2, x_relabel
+        del m2
 
     traffic = None
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
@@ -308,15 +309,15 @@ def main():
     two_pass = 0 <= pb <= n * 8
     pmc = {("opinion", 10_000_000, False, False): "k_opinion_f64_ba10M_traffic.json",
            ("opinion", 10_000_000, True, False): "k_opinion_f64_ba10M_bfs_traffic.json",
-           ("opinion", 10_000_000, True, True): "k_opinion_f64_ba10M_bfs_plan_traffic.json"}.get(
-        (args.workload, n, not args.no_relabel, two_pass))
+           ("opinion", 10_000_000, False, True): "k_opinion_f64_ba10M_plan_traffic.json"}.get(
+        (args.workload, n, args.relabel, two_pass))
     if pmc and os.path.exists(os.path.join(ROOT, "profiles", pmc)):
         traffic = json.load(open(os.path.join(ROOT, "profiles", pmc))).get("bytes_per_launch")
 
     drift = None
     if f32:
         # the Float32 variant against the Float64 run of the same inputs
-        m64 = M.build_opinion(vh, n, seed=1, f32=False, relabel=not args.no_relabel, device_graph=True)
+        m64 = M.build_opinion(vh, n, seed=1, f32=False, relabel=args.relabel, device_graph=True)
         for _ in range(1 + args.warmup + args.steps - 1):
             M.step_opinion(vh, m64, eps=0.2)
         x64 = np.asarray(m64.sim._read_agents(m64.T)[0]["opinion"])
@@ -334,11 +335,11 @@ def main():
         opin = args.workload.startswith("opinion")
         m.sim._destroy()
         t0 = time.perf_counter()
-        cpu = cpu_baseline(args.workload, f32, relabel=not args.no_relabel, n=n if opin else None,
+        cpu = cpu_baseline(args.workload, f32, relabel=args.relabel, n=n if opin else None,
                            edges=edges if opin else None)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {cpu}")
         t0 = time.perf_counter()
-        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32, relabel=not args.no_relabel,
+        cpu["all_cores"] = cpu_baseline_all_cores(args.workload, f32, relabel=args.relabel,
                                                   n=n if opin else None)
         log(f"cpu baseline, all cores {time.perf_counter() - t0:.1f}s: {cpu['all_cores']}")
 
@@ -377,8 +378,8 @@ def main():
                      "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},
         "cpu_baseline": cpu,
     }
-    if gen_order is not None:
-        out["generation_order"] = gen_order
+    if other_order is not None:
+        out["generation_order" if args.relabel else "breadth_first_order"] = other_order
     if ens is not None:
         out["ensemble"] = ens
     if drift is not None:

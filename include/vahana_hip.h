@@ -433,8 +433,9 @@ typedef struct vh_options {
     int64_t opinion_plan_bytes;   /* opinion states gathered in two passes through a plan from a
                                      state vector of at least this many bytes (< 0: never);
                                      default 16 MiB */
-    int64_t opinion_part_bytes;   /* source partition of the plan in bytes (0: 2 MiB, widened up
-                                     to 1.5x to make at most 32 partitions) */
+    int64_t opinion_part_bytes;   /* source window of the plan in bytes (0 or more than 160 KiB:
+                                     160 KiB, one CU's LDS; widened to make at most 1024
+                                     windows) */
     int32_t sir_expose;           /* SIR exposure pass: 1 on (default), 0 off */
     uint32_t pad;
     int64_t sir_expose_min;       /* agents from which a SIR simulation or ensemble uses the

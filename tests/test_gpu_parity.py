@@ -63,10 +63,10 @@ def test_sir_parity(vh, ora, gpu_ready, expose, opts):
 
 
 # neighbour-state gather (options opinion_plan_bytes, opinion_part_bytes):
-# one pass; two passes through a plan with 2 KiB source partitions (16 for
-# Float64, 8 for Float32 states), with 4 KiB partitions, with the default
-# 2 MiB (one partition at this size), and with 1-byte partitions requested
-# (widened to the most the lists allow: 32 partitions of 125 agents)
+# one pass; two passes through a plan with 2 KiB source windows (16 for
+# Float64, 8 for Float32 states), with 4 KiB windows, with the default
+# 160 KiB (one window at this size), and with 1-byte windows requested
+# (widened to the most a block's run table holds: 1000 windows of 4 agents)
 GATHER_MODES = {"direct": (-1, 0), "plan": (0, 2048), "plan_4k": (0, 4096), "plan_auto": (0, 0),
                 "plan_1src": (0, 1)}
 
@@ -86,6 +86,34 @@ def test_opinion_parity(vh, ora, gpu_ready, f32, mode, opts):
     g = M.build_opinion(vh, n, seed=8, f32=f32, edges=edges)
     c = M.build_opinion(ora, n, seed=8, f32=f32, edges=edges)
     for step in range(8):
+        M.step_opinion(vh, g, eps=0.2)
+        M.step_opinion(ora, c, eps=0.2)
+        x, _ = _agents(g)
+        y, _ = _agents(c)
+        assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
+
+
+@pytest.mark.parametrize("mode", ["direct", "plan", "plan_1src"])
+@pytest.mark.parametrize("f32", [False, True])
+def test_opinion_parity_hub(vh, ora, gpu_ready, f32, mode, opts):
+    """Every in-degree class of the gather: a Barabasi-Albert graph (degrees
+    up to a few hundred: agents summed by their own thread and by a wave)
+    with two hubs of 1500 and 3000 undirected neighbours added (in-degree
+    above 1024: the chunk lists)."""
+    plan_bytes, part_bytes = GATHER_MODES[mode]
+    opts(opinion_plan_bytes=plan_bytes, opinion_part_bytes=part_bytes)
+    print(f"[gpu] opinion parity with hubs f32={f32} gather={mode}", flush=True)
+    M = vh.models
+    n = 4000
+    a, b = vh.inputs.graph_ba(n, 5, 8)
+    ha = np.concatenate([np.full(3000, 17, np.uint32), np.full(1500, 900, np.uint32)])
+    hb = np.concatenate([np.arange(1000, 4000, dtype=np.uint32), np.arange(2, 1502, dtype=np.uint32)])
+    edges = vh.inputs.symmetric_weighted(np.concatenate([a, ha]), np.concatenate([b, hb]), 9)
+    deg = np.bincount(edges[1], minlength=n + 1)
+    assert deg.max() > 3000 and ((deg > 64) & (deg <= 1024)).sum() > 5
+    g = M.build_opinion(vh, n, seed=8, f32=f32, edges=edges)
+    c = M.build_opinion(ora, n, seed=8, f32=f32, edges=edges)
+    for step in range(6):
         M.step_opinion(vh, g, eps=0.2)
         M.step_opinion(ora, c, eps=0.2)
         x, _ = _agents(g)

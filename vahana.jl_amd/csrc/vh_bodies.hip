@@ -577,173 +577,48 @@ __global__ __launch_bounds__(256) void k_opinion_multi(const OpMember* __restric
 
 // ----------------------------------------------------------------- gather plan
 // For a large state vector the neighbour states of a static edge store are
-// gathered in two passes. The plan lists the CSR slots sorted stably by
-// source partition (`psize` consecutive sources, at most 32 partitions, sized
-// so that one XCD's L2 holds a partition's states), light rows' slots before
-// heavy rows' within a partition. Pass 1 (k_plan_gather) walks the plan, each
-// XCD one contiguous eighth of it (its gathers hit the partitions its L2
-// holds), and writes y[j] = x[psrc[j] - 1] sequentially. Within a partition
-// the plan keeps CSR order, so the light in-edges of a tile of consecutive
-// agents are one run of y per partition. Pass 2 (k_opinion_plan) reads those
-// runs through per-tile run tables and adds each agent's terms in canonical
-// order; the values are exactly x[src[k]], so the results are the one-pass
-// kernel's.
-constexpr int GP_U = 4;   // plan entries per thread in pass 1 (measured: 4 beats 1, 2, 8, 16)
-
+// gathered in two passes through a plan built once per edge store.
+//
+// Windows: the sources are cut into windows of `wsz` consecutive agents, at
+// most 160 KB of states (one CU's LDS) and at most WIN_RUNS windows. Blocks:
+// the agents are cut into blocks of consecutive agents with fewer than TB
+// staged in-edges, TB being the slots a block stages in LDS; an agent's
+// in-edges are staged when its in-degree is at most WIN_HMAX (rows above go
+// through chunk lists). An agent costs its staged in-degree + 1; block b
+// holds the agents whose exclusive cost prefix lies in [b S, (b + 1) S),
+// S = TB - WIN_HMAX - 1. The plan orders the CSR slots by (window, block),
+// the chunk-list rows' slots after the staged ones of their window, keeping
+// CSR order within: the staged in-edges of a block from one window are one
+// run of the plan.
+//
+// Pass 1 (k_win_gather) streams the plan: each block of the launch takes a
+// contiguous share, stages each window it meets in LDS and writes
+// y[j] = x[src(j)] for its share, so the random reads are LDS reads and HBM
+// sees sequential streams (the plan's local source indices, y, the windows).
+//
+// Pass 2 (k_win_sum) takes the agent blocks from a queue. A block's staged
+// entries are listed by (window, slot): entry e holds its slot (bit 15 set on
+// the first entry of a run) and its weight, and its value is
+// y[prun[r] + e - pfirst[r]] for its run r, counted from the run starts
+// before its 64-entry chunk (cbase) and a ballot of the start bits. Values and
+// weights are staged in LDS by slot; then each agent of in-degree <= 64 adds
+// its slots in order, and a wave each agent of in-degree 65..WIN_HMAX (lane l
+// its slots l, l + 64, ..., then the xor butterfly). Agents above WIN_HMAX go
+// first, one wave each, through their chunk lists. The values are exactly
+// x[src[k]] and the sums keep the canonical order, so the results are the
+// one-pass kernel's.
 template <class T>
-struct TileCfg {
-    static constexpr int TILE = sizeof(T) == 8 ? 1024 : 1536;   // light slots per tile
-    static constexpr int CHUNK = TILE / 4;                       // heavy-row slots staged per wave
-    static constexpr int TILES = (64 * 256 + TILE - 1) / TILE;   // tiles of a light block at most
+struct WinCfg {
+    static constexpr uint32_t WSZ = 163840 / sizeof(T);             // states per window at most
+    static constexpr uint32_t TB = sizeof(T) == 8 ? 12288 : 16384;  // staged slots per block at most
+    static constexpr uint32_t QU = TB / 1024;                       // list entries per thread
+    static constexpr uint32_t CPB = TB / 64;                        // 64-entry chunks per block
+    static constexpr uint32_t CHUNK = sizeof(T) == 8 ? 256 : 384;   // heavy-row slots staged per wave
 };
-constexpr int OP_TILE_BITS = 4;   // bits of a tile number within a light block (list sort keys)
-constexpr int OPP_RUNS = 32;      // partitions at most (5 bits of a list entry beside the 11-bit slot)
-
-// Light lists: the light in-edges of a 256-agent block are numbered from
-// lbase[block] in agent order, in tiles of TILE slots; every tile lists its
-// entries by (partition, slot). Entry e: tpk[e] = slot | partition << 11 and
-// twt[e] its weight. Its value is y[prun[t][p] + (e - pfirst[t][p])] for its
-// tile t and partition p (the tile's run table). Per tile: the owners mark
-// their slots, the entries' terms are computed in parallel and staged in LDS
-// by slot, then each thread adds its own agent's slots in order.
-template <class T>
-struct PlanLDS {
-    T num[TileCfg<T>::TILE];     // numerator term by slot (heavy waves: a quarter each for x_j)
-    float den[TileCfg<T>::TILE]; // weight term by slot
-    uint8_t own[TileCfg<T>::TILE];
-    T xi[256];                   // agent states
-    uint32_t wsum[4];
-    uint32_t prun[OPP_RUNS];     // run table of the tile
-    uint16_t pfirst[OPP_RUNS];
-};
-
-template <class T>
-__device__ __forceinline__ void opinion_block_plan(PlanLDS<T>& sm, const uint32_t* __restrict__ off,
-                                                   const T* __restrict__ x, const T* __restrict__ y,
-                                                   T* __restrict__ xo, uint64_t n, uint64_t nrows, T eps,
-                                                   uint32_t blk, const uint32_t* __restrict__ lbase,
-                                                   const uint16_t* __restrict__ tpk,
-                                                   const float* __restrict__ twt, uint32_t nblkp,
-                                                   const uint32_t* __restrict__ prun,
-                                                   const uint16_t* __restrict__ pfirst) {
-    constexpr uint32_t TL = TileCfg<T>::TILE, NT = TileCfg<T>::TILES;
-    constexpr int QU = TL / 256;   // entries of a tile per thread
-    const int tid = threadIdx.x;
-    const uint64_t i = (uint64_t)blk * 256 + tid;
-    uint32_t d = 0;
-    T xi = (T)0;
-    if (i < n) {
-        xi = __builtin_nontemporal_load(x + i);
-        if (i < nrows) d = __builtin_nontemporal_load(off + i + 1) - __builtin_nontemporal_load(off + i);
-    }
-    const bool inblk = blk < nblkp;
-    const uint32_t lb = inblk ? lbase[blk] : 0u;
-    const uint32_t le = inblk ? lbase[blk + 1] : 0u;
-    // packed list entries e = tid + 256 u of the next tile, fetched one tile
-    // ahead (the first during the block scan); 0xFFFF past the tile. The
-    // weights are fetched at each tile's start: fetched one tile ahead too,
-    // they cost Float32 its 8th wave per SIMD (71 VGPRs) and measured 0.785
-    // against 0.729 ms per step, Float64 0.930 against 0.911 ms.
-    uint32_t npk[QU];
-    {
-        const uint32_t c0 = (le - lb) < TL ? (le - lb) : TL;
-#pragma unroll
-        for (int u = 0; u < QU; u++) {
-            const uint32_t q = (uint32_t)(tid + 256 * u);
-            npk[u] = 0xFFFFu;
-            if (q < c0) npk[u] = __builtin_nontemporal_load(tpk + lb + q);
-        }
-    }
-    uint32_t nx_run = 0, nx_first = 0;   // run table entry `tid` of the next tile
-    if (tid < OPP_RUNS && inblk) {
-        nx_run = prun[(uint64_t)blk * NT * OPP_RUNS + tid];
-        nx_first = pfirst[(uint64_t)blk * NT * OPP_RUNS + tid];
-    }
-    const bool light = i < n && d <= 64u;
-    const uint32_t ld = light ? d : 0u;
-    uint32_t total;
-    const uint32_t P = op_light_scan(ld, sm.wsum, total);
-    sm.xi[tid] = xi;
-    T num = (T)0, den = (T)0;
-    for (uint32_t t0 = 0; t0 < total; t0 += TL) {
-        const uint32_t lo = P > t0 ? P : t0;
-        const uint32_t hi = (P + ld) < (t0 + TL) ? (P + ld) : (t0 + TL);
-        const uint32_t cnt = (total - t0) < TL ? (total - t0) : TL;
-        uint32_t pk[QU];
-        float wq[QU];
-#pragma unroll
-        for (int u = 0; u < QU; u++) {
-            const uint32_t q = (uint32_t)(tid + 256 * u);
-            pk[u] = npk[u];
-            wq[u] = q < cnt ? __builtin_nontemporal_load(twt + lb + t0 + q) : 0.0f;
-        }
-        {
-            const uint32_t t1 = t0 + TL;
-            const uint32_t c1 = total > t1 ? ((total - t1) < TL ? (total - t1) : TL) : 0u;
-#pragma unroll
-            for (int u = 0; u < QU; u++) {
-                const uint32_t q = (uint32_t)(tid + 256 * u);
-                npk[u] = 0xFFFFu;
-                if (q < c1) npk[u] = __builtin_nontemporal_load(tpk + lb + t1 + q);
-            }
-        }
-        for (uint32_t q = lo; q < hi; q++) sm.own[q - t0] = (uint8_t)tid;
-        if (tid < OPP_RUNS) {
-            sm.prun[tid] = nx_run;
-            sm.pfirst[tid] = (uint16_t)nx_first;
-            if (t0 + TL < total) {
-                const uint64_t g = (uint64_t)blk * NT + t0 / TL + 1;
-                nx_run = prun[g * OPP_RUNS + tid];
-                nx_first = pfirst[g * OPP_RUNS + tid];
-            }
-        }
-        __syncthreads();
-        // every entry's plan position first (LDS only), then the loads, then
-        // each entry's terms at its slot
-        uint32_t jj[QU];
-#pragma unroll
-        for (int u = 0; u < QU; u++) {
-            const uint32_t q = (uint32_t)(tid + 256 * u);
-            const uint32_t pp = (pk[u] >> 11) & 31u;
-            jj[u] = q < cnt ? sm.prun[pp] + (q - sm.pfirst[pp]) : 0u;
-        }
-        T v[QU];
-#pragma unroll
-        for (int u = 0; u < QU; u++) v[u] = pk[u] != 0xFFFFu ? y[jj[u]] : (T)0;
-#pragma unroll
-        for (int u = 0; u < QU; u++) {
-            if (pk[u] != 0xFFFFu) {
-                const uint32_t sl = pk[u] & 2047u;
-                const bool acc = fabs(v[u] - sm.xi[sm.own[sl]]) <= eps;
-                sm.num[sl] = acc ? (T)wq[u] * v[u] : (T)0;
-                sm.den[sl] = acc ? wq[u] : 0.0f;
-            }
-        }
-        __syncthreads();
-        // this thread's slots in order; four LDS reads issued together
-        uint32_t q = lo;
-        for (; q + 4 <= hi; q += 4) {
-            const T a0 = sm.num[q - t0], a1 = sm.num[q + 1 - t0], a2 = sm.num[q + 2 - t0],
-                    a3 = sm.num[q + 3 - t0];
-            const float d0 = sm.den[q - t0], d1 = sm.den[q + 1 - t0], d2 = sm.den[q + 2 - t0],
-                        d3 = sm.den[q + 3 - t0];
-            num = num + a0;
-            num = num + a1;
-            num = num + a2;
-            num = num + a3;
-            den = den + (T)d0;
-            den = den + (T)d1;
-            den = den + (T)d2;
-            den = den + (T)d3;
-        }
-        for (; q < hi; q++) {
-            num = num + sm.num[q - t0];
-            den = den + (T)sm.den[q - t0];
-        }
-        __syncthreads();
-    }
-    if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
-}
+constexpr uint32_t WIN_RUNS = 1024;   // windows at most (runs of a block)
+constexpr uint32_t WIN_HMAX = 1024;   // in-degrees staged within blocks (larger: chunk lists)
+constexpr uint32_t WIN_MED = 256;     // agents of in-degree 65..WIN_HMAX per block at most
+constexpr uint32_t WIN_SLOT = 0x3FFFu, WIN_START = 0x8000u;
 
 // A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
 // h are numbered from hbase[h] in CSR order, in chunks of CHUNK slots, each
@@ -756,7 +631,7 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
                                                    const float* __restrict__ w, T* __restrict__ xo,
                                                    const uint4 hi, T eps, const uint32_t* __restrict__ hty,
                                                    const uint16_t* __restrict__ hslot, T* buf) {
-    constexpr uint32_t CH = TileCfg<T>::CHUNK;
+    constexpr uint32_t CH = WinCfg<T>::CHUNK;
     constexpr int HU = CH / 64;
     const uint32_t lane = threadIdx.x & 63;
     // hi: row, CSR start, degree, first chunk-list entry
@@ -806,173 +681,471 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
     if (lane == 0) xo[r] = den > (T)0 ? num / den : xi;
 }
 
-// Pass 2: blocks [0, nh) run one wave per heavy agent (the longest work
-// starts first), the others are light blocks. Light blocks of one XCD take
-// one contiguous range of agents (blocks are dispatched round-robin over the
-// 8 XCDs; the grid holds 8 * ceil(nlight / 8) light blocks), so that they
-// read y as a few sequential streams.
-template <class T>
-__global__ __launch_bounds__(256) void k_opinion_plan(const uint32_t* __restrict__ off,
-                                                      const T* __restrict__ x, const T* __restrict__ y,
-                                                      const float* __restrict__ w, T* __restrict__ xo,
-                                                      uint64_t n, uint64_t nrows, T eps,
-                                                      const uint4* __restrict__ hinfo, uint64_t nheavy,
-                                                      uint32_t nh, uint32_t nlight,
-                                                      const uint32_t* __restrict__ lbase,
-                                                      const uint16_t* __restrict__ tpk,
-                                                      const float* __restrict__ twt, uint32_t nblkp,
-                                                      const uint32_t* __restrict__ prun,
-                                                      const uint16_t* __restrict__ pfirst,
-                                                      const uint32_t* __restrict__ hty,
-                                                      const uint16_t* __restrict__ hslot) {
-    __shared__ __align__(16) unsigned char smraw[sizeof(PlanLDS<T>)];
-    PlanLDS<T>& sm = *reinterpret_cast<PlanLDS<T>*>(smraw);
-    if (blockIdx.x >= nh) {
-        uint32_t l = blockIdx.x - nh;
-        const uint32_t per = (nlight + 7) / 8;
-        l = (l % 8) * per + l / 8;
-        if (l >= nlight) return;   // whole block
-        opinion_block_plan<T>(sm, off, x, y, xo, n, nrows, eps, l, lbase, tpk, twt, nblkp, prun, pfirst);
-    } else {
-        const uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        if (wv < nheavy)
-            opinion_heavy_plan<T>(x, y, w, xo, hinfo[wv], eps, hty, hslot,
-                                  sm.num + (threadIdx.x >> 6) * TileCfg<T>::CHUNK);
-    }
-}
 
-// Pass 1: grid 8 * ceil(nb / 8) blocks of 256, nb = ceil(ne / (256 GP_U));
-// logical block l takes plan entries [l * 1024, (l + 1) * 1024). Float64:
-// thread t the entries l * 1024 + t + 256 u, u < 4 (four 4-byte source
-// loads, four gathers, four 8-byte stores; measured faster than a contiguous
-// 2 x 2 layout, 470 against 536 us at 10^8 entries). Float32: the 4
-// consecutive entries from l * 1024 + 4 t (one 16-byte load of their
-// sources, four gathers, one 16-byte store; 0.731 against 0.739 ms per step
-// with the Float64 layout). The buffers are 256-byte aligned; a partial group
-// at the end goes entry by entry.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Pass 1: block l of the grid (one per CU) takes plan entries
+// [ne l / G, ne (l + 1) / G). For each window its share meets: the window's
+// states go to LDS (16-byte loads), then wave v, step u writes the 256
+// entries from c = g0 + 256 (v + 16 u) in two halves, lane l entries
+// c + 2l, c + 2l + 1 and c + 128 + 2l, c + 129 + 2l (one 4-byte load of their
+// two source indices, one 2-state store: each store instruction writes one
+// contiguous range; measured 236 us against 308 us with 4 entries per lane
+// at 10^8 entries).
 typedef uint32_t pu32x4 __attribute__((ext_vector_type(4)));
-static_assert(GP_U == 4, "k_plan_gather takes four plan entries per thread");
+
 template <class T>
-__global__ __launch_bounds__(256) void k_plan_gather(const uint32_t* __restrict__ psrc, const T* __restrict__ x,
-                                                     T* __restrict__ y, uint64_t ne) {
-    const uint32_t per = gridDim.x / 8;
-    const uint32_t l = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    const uint64_t base = (uint64_t)l * (256 * GP_U);
-    if constexpr (sizeof(T) == 4) {
-        const uint64_t j0 = base + (uint64_t)threadIdx.x * 4;
-        if (j0 >= ne) return;
-        if (j0 + 4 <= ne) {
-            const pu32x4 sv = __builtin_nontemporal_load((const pu32x4*)(psrc + j0));
-            const f32x4 v = {(float)x[sv.x - 1], (float)x[sv.y - 1], (float)x[sv.z - 1], (float)x[sv.w - 1]};
-            __builtin_nontemporal_store(v, (f32x4*)(y + j0));
-        } else {
-            for (uint64_t j = j0; j < ne; j++) y[j] = x[psrc[j] - 1];
+struct Pair;
+template <>
+struct Pair<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_win_gather(const uint16_t* __restrict__ psl, const T* __restrict__ x,
+                                                     T* __restrict__ y, const uint32_t* __restrict__ wstart,
+                                                     uint32_t W, uint32_t wsz, uint64_t nsrc, uint64_t ne) {
+    typedef typename Pair<T>::type T2;
+    constexpr uint32_t PER16 = 16 / sizeof(T);
+    constexpr int U = 2;
+    __shared__ __align__(16) T win[WinCfg<T>::WSZ];
+    const uint32_t G = gridDim.x, l = blockIdx.x;
+    const uint64_t j0 = ne * l / G, j1 = ne * (l + 1) / G;
+    if (j0 >= j1) return;   // whole block
+    uint32_t lo = 0, hi = W;   // the window of j0: wstart[lo] <= j0 < wstart[lo + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (wstart[mid] <= j0) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t w = lo;
+    uint64_t j = j0;
+    while (j < j1) {
+        const uint64_t we = wstart[w + 1];
+        if (we <= j) {   // empty window
+            w++;
+            continue;
         }
-    } else {
-        const uint64_t j0 = base + threadIdx.x;
-        uint32_t sj[GP_U];
-        T v[GP_U];
-#pragma unroll
-        for (int u = 0; u < GP_U; u++) {
-            const uint64_t j = j0 + (uint64_t)u * 256;
-            sj[u] = j < ne ? __builtin_nontemporal_load(psrc + j) : 0u;
+        const uint64_t s0 = (uint64_t)w * wsz;
+        const uint32_t wn = (uint32_t)((nsrc - s0) < wsz ? nsrc - s0 : wsz);
+        for (uint32_t q = threadIdx.x * PER16; q < wn; q += 1024 * PER16) {
+            if (q + PER16 <= wn) {
+                *(pu32x4*)(win + q) = __builtin_nontemporal_load((const pu32x4*)(x + s0 + q));
+            } else {
+                for (uint32_t t = q; t < wn; t++) win[t] = x[s0 + t];
+            }
         }
+        __syncthreads();
+        const uint64_t je = we < j1 ? we : j1;
+        for (uint64_t g0 = j & ~3ull; g0 < je; g0 += 4096ull * U) {
+            uint32_t p0[U], p1[U];
+            bool f0[U], f1[U];
 #pragma unroll
-        for (int u = 0; u < GP_U; u++) v[u] = sj[u] ? x[sj[u] - 1] : (T)0;
+            for (int u = 0; u < U; u++) {
+                const uint64_t ga = g0 + 256ull * (wv + 16 * u) + 2 * lane, gb = ga + 128;
+                f0[u] = ga >= j && ga + 2 <= je;
+                f1[u] = gb >= j && gb + 2 <= je;
+                p0[u] = f0[u] ? __builtin_nontemporal_load((const uint32_t*)(psl + ga)) : 0u;
+                p1[u] = f1[u] ? __builtin_nontemporal_load((const uint32_t*)(psl + gb)) : 0u;
+            }
 #pragma unroll
-        for (int u = 0; u < GP_U; u++) {
-            const uint64_t j = j0 + (uint64_t)u * 256;
-            if (j < ne) __builtin_nontemporal_store(v[u], y + j);
+            for (int u = 0; u < U; u++) {
+                const uint64_t ga = g0 + 256ull * (wv + 16 * u) + 2 * lane, gb = ga + 128;
+                if (f0[u]) {
+                    const T2 v = {win[p0[u] & 0xFFFFu], win[p0[u] >> 16]};
+                    __builtin_nontemporal_store(v, (T2*)(y + ga));
+                } else {
+                    for (uint64_t t = ga; t < ga + 2; t++)
+                        if (t >= j && t < je) y[t] = win[psl[t]];
+                }
+                if (f1[u]) {
+                    const T2 v = {win[p1[u] & 0xFFFFu], win[p1[u] >> 16]};
+                    __builtin_nontemporal_store(v, (T2*)(y + gb));
+                } else {
+                    for (uint64_t t = gb; t < gb + 2; t++)
+                        if (t >= j && t < je) y[t] = win[psl[t]];
+                }
+            }
         }
+        __syncthreads();
+        j = je;
+        w++;
     }
 }
 
-// plan sort key of each CSR slot: source partition, then 1 for the slots of
-// a heavy row (degree > 64), so that within a partition the light slots come
-// first, in CSR order
-__global__ void k_plan_keys(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t nrows,
-                            uint32_t psize, uint32_t* __restrict__ keys) {
+template <class T>
+struct WinLDS {
+    T val[WinCfg<T>::TB];          // values by slot (heavy waves: CHUNK each)
+    float wgt[WinCfg<T>::TB];      // weights by slot
+    uint32_t rp[1][WIN_RUNS];      // runs of the block: plan position, list index
+    uint16_t rf[1][WIN_RUNS];
+    uint32_t wsum[WinCfg<T>::TB / 1024 + 1][16];
+    uint32_t nextq[1];             // the queue item taken
+    uint32_t nmed;                 // agents of in-degree 65..WIN_HMAX of the block
+    uint32_t mslot[WIN_MED], mdeg[WIN_MED], magent[WIN_MED];
+    T mx[WIN_MED];
+};
+
+// The round-0 and round-1 agents of a light block (a0 + tid, a0 + 1024 +
+// tid): in-degrees and states, loaded one block ahead
+template <class T>
+struct WinAgents {
+    uint32_t d0, d1;
+    T x0, x1;
+};
+
+template <class T>
+__device__ __forceinline__ WinAgents<T> win_agents(const uint32_t* __restrict__ off, const T* __restrict__ x,
+                                                   uint64_t nrows, uint32_t a0, uint32_t a1, uint32_t tid) {
+    WinAgents<T> g = {0u, 0u, (T)0, (T)0};
+    const uint32_t i0 = a0 + tid, i1 = i0 + 1024;
+    if (i0 < a1) {
+        g.x0 = x[i0];
+        if (i0 < nrows) g.d0 = off[i0 + 1] - off[i0];
+    }
+    if (i1 < a1) {
+        g.x1 = x[i1];
+        if (i1 < nrows) g.d1 = off[i1 + 1] - off[i1];
+    }
+    return g;
+}
+
+// The agents of a block, in rounds of 1024 (agent a0 + 1024 r + tid): one
+// exclusive scan of the staged in-degrees over all rounds (one barrier, which
+// also publishes the staged values); each agent of in-degree <= 64 adds its
+// slots in order, the others of the block are listed and summed by a wave
+// each (lane-strided partials, xor butterfly). Rounds 0 and 1 use the agents
+// loaded ahead (g), later rounds load theirs.
+template <class T>
+__device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __restrict__ off,
+                                               const T* __restrict__ x, T* __restrict__ xo, uint64_t nrows, T eps,
+                                               uint32_t a0, uint32_t a1, const WinAgents<T>& g) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t K = (a1 - a0 + 1023) / 1024;
+    uint32_t inclk[2] = {0u, 0u}, ldk[2] = {0u, 0u};
+    for (uint32_t r = 0; r < K; r++) {
+        const uint32_t i = a0 + r * 1024 + tid;
+        uint32_t d = 0;
+        if (i < a1) d = r == 0 ? g.d0 : r == 1 ? g.d1 : (i < nrows ? off[i + 1] - off[i] : 0u);
+        const uint32_t ld = d <= WIN_HMAX ? d : 0u;
+        uint32_t incl = ld;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) sm.wsum[r][wv] = incl;
+        if (r < 2) {
+            inclk[r] = incl;
+            ldk[r] = ld;
+        }
+    }
+    if (tid == 0) sm.nmed = 0;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t r = 0; r < K; r++) {
+        const uint32_t i = a0 + r * 1024 + tid;
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint32_t v = sm.wsum[r][q];
+            pre += q < wv ? v : 0u;
+            tot += v;
+        }
+        uint32_t d, incl, ld;
+        T xi;
+        if (r < 2) {
+            d = i < a1 ? (r == 0 ? g.d0 : g.d1) : 0u;
+            incl = inclk[r];
+            ld = ldk[r];
+            xi = r == 0 ? g.x0 : g.x1;
+        } else {
+            d = (i < a1 && i < nrows) ? off[i + 1] - off[i] : 0u;
+            ld = d <= WIN_HMAX ? d : 0u;
+            incl = ld;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += v;
+            }
+            xi = i < a1 ? x[i] : (T)0;
+        }
+        const uint32_t P = base + pre + incl - ld;
+        if (i < a1 && d <= 64u) {
+            T num = (T)0, den = (T)0;
+            uint32_t q = 0;
+            for (; q + 4 <= d; q += 4) {
+                const T y0 = sm.val[P + q], y1 = sm.val[P + q + 1], y2 = sm.val[P + q + 2], y3 = sm.val[P + q + 3];
+                const float w0 = sm.wgt[P + q], w1 = sm.wgt[P + q + 1], w2 = sm.wgt[P + q + 2],
+                            w3 = sm.wgt[P + q + 3];
+                op_accum(y0, w0, xi, eps, num, den);
+                op_accum(y1, w1, xi, eps, num, den);
+                op_accum(y2, w2, xi, eps, num, den);
+                op_accum(y3, w3, xi, eps, num, den);
+            }
+            for (; q < d; q++) op_accum(sm.val[P + q], sm.wgt[P + q], xi, eps, num, den);
+            __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
+        } else if (i < a1 && d <= WIN_HMAX) {
+            const uint32_t m = atomicAdd(&sm.nmed, 1u);   // < WIN_MED (each costs > 64 of < TB)
+            sm.mslot[m] = P;
+            sm.mdeg[m] = d;
+            sm.magent[m] = i;
+            sm.mx[m] = xi;
+        }
+        base += tot;
+    }
+    __syncthreads();
+    const uint32_t nmed = sm.nmed;
+    for (uint32_t m = (uint32_t)wv; m < nmed; m += 16) {
+        const uint32_t P = sm.mslot[m], d = sm.mdeg[m];
+        const T xi = sm.mx[m];
+        T num = (T)0, den = (T)0;
+        for (uint32_t q = (uint32_t)lane; q < d; q += 64) op_accum(sm.val[P + q], sm.wgt[P + q], xi, eps, num, den);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            num = num + __shfl_xor(num, o, 64);
+            den = den + __shfl_xor(den, o, 64);
+        }
+        if (lane == 0) xo[sm.magent[m]] = den > (T)0 ? num / den : xi;
+    }
+}
+
+// Pass 2: one block of 1024 per CU. The waves first take the heavy agents
+// (wave g of the grid agents g, g + 16 G, ...), then the blocks take light
+// blocks from a queue (queue[0]: next light block, queue[1]: blocks done;
+// the last block to finish sets both to 0 for the next launch). Measured
+// slower at 10^7 agents (0.865 ms per step here): pipelining the phases of
+// consecutive light blocks (0.916 ms), heavy agents in queued groups of 16
+// longest first with 512-slot chunks (0.890 ms).
+template <class T>
+__global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ off, const T* __restrict__ x,
+                                                  const T* __restrict__ y, const float* __restrict__ w,
+                                                  T* __restrict__ xo, uint64_t nrows, T eps,
+                                                  const uint4* __restrict__ hinfo, uint64_t nheavy,
+                                                  const uint32_t* __restrict__ hty,
+                                                  const uint16_t* __restrict__ hslot,
+                                                  const uint32_t* __restrict__ ablk,
+                                                  const uint32_t* __restrict__ lbase,
+                                                  const uint16_t* __restrict__ tpk, const float* __restrict__ twt,
+                                                  const uint32_t* __restrict__ rbase,
+                                                  const uint32_t* __restrict__ prun,
+                                                  const uint16_t* __restrict__ pfirst,
+                                                  const uint16_t* __restrict__ cbase, uint32_t nb,
+                                                  uint32_t* __restrict__ queue) {
+    constexpr uint32_t QU = WinCfg<T>::QU, CPB = WinCfg<T>::CPB;
+    __shared__ __align__(16) unsigned char smraw[sizeof(WinLDS<T>)];
+    WinLDS<T>& sm = *reinterpret_cast<WinLDS<T>*>(smraw);
+    const uint32_t tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t lemask = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    // heavy agents: wave g of the grid takes heavy agents g, g + 16 G, ...
+    for (uint64_t h = (uint64_t)blockIdx.x * 16 + (uint32_t)wv; h < nheavy; h += (uint64_t)gridDim.x * 16)
+        opinion_heavy_plan<T>(x, y, w, xo, hinfo[h], eps, hty, hslot, sm.val + wv * WinCfg<T>::CHUNK);
+    __syncthreads();
+    // light blocks, each in three phases: its list, weights, runs and
+    // agents; its values, staged by slot; the sums
+    for (;;) {
+        if (tid == 0) sm.nextq[0] = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t b = sm.nextq[0];
+        if (b >= nb) break;
+        const uint32_t a0 = ablk[b], a1 = ablk[b + 1];
+        const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
+        const uint32_t rb = rbase[b], R = rbase[b + 1] - rb;
+        if ((uint32_t)tid < R) {
+            sm.rp[0][tid] = prun[rb + tid];
+            sm.rf[0][tid] = pfirst[rb + tid];
+        }
+        uint32_t pk[QU], cb[QU];
+        float wt[QU];
+#pragma unroll
+        for (uint32_t u = 0; u < QU; u++) {
+            const uint32_t e = tid + 1024 * u;
+            pk[u] = e < L ? (uint32_t)__builtin_nontemporal_load(tpk + lb + e) : 0xFFFFFFFFu;
+            wt[u] = e < L ? __builtin_nontemporal_load(twt + lb + e) : 0.0f;
+        }
+        // chunk run counts before the barrier: a 16-bit load is a vector load,
+        // and a wait for it among the value loads would wait for all of them
+#pragma unroll
+        for (uint32_t u = 0; u < QU; u++) cb[u] = cbase[(uint64_t)b * CPB + u * 16 + wv];
+        const WinAgents<T> ag = win_agents<T>(off, x, nrows, a0, a1, tid);
+        __syncthreads();
+        T v[QU];
+#pragma unroll
+        for (uint32_t u = 0; u < QU; u++) {
+            const uint32_t e = tid + 1024 * u;
+            const bool ok = pk[u] != 0xFFFFFFFFu;
+            const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
+            const uint32_t run = cb[u] + __popcll(m & lemask) - 1u;   // (entries of the list only)
+            v[u] = (T)0;
+            if (ok) v[u] = y[sm.rp[0][run] + (e - sm.rf[0][run])];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < QU; u++)
+            if (pk[u] != 0xFFFFFFFFu) {
+                sm.val[pk[u] & WIN_SLOT] = v[u];
+                sm.wgt[pk[u] & WIN_SLOT] = wt[u];
+            }
+        win_sum_agents<T>(sm, off, x, xo, nrows, eps, a0, a1, ag);   // (its barrier publishes the values)
+        __syncthreads();
+    }
+    if (tid == 0 && atomicAdd(queue + 1, 1u) == gridDim.x - 1) {
+        queue[0] = 0;
+        queue[1] = 0;
+    }
+}
+
+// consistency of a built plan, one thread per block: its agents, list and
+// runs in range, every run's first entry and only those marked, each run
+// inside y, each chunk's run count; any failure sets *bad (the plan is then
+// refused: pass 2 trusts these bounds)
+__global__ void k_win_check(const uint32_t* __restrict__ ablk, const uint32_t* __restrict__ lbase,
+                            const uint32_t* __restrict__ rbase, const uint32_t* __restrict__ prun,
+                            const uint16_t* __restrict__ pfirst, const uint16_t* __restrict__ tpk,
+                            const uint16_t* __restrict__ cbase, uint32_t nb, uint64_t n, uint64_t nl,
+                            uint64_t nruns, uint64_t ne, uint32_t TB, uint32_t cpb, uint32_t* __restrict__ bad) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t a0 = ablk[b], a1 = ablk[b + 1], lb = lbase[b], lb1 = lbase[b + 1];
+    const uint32_t rb = rbase[b], rb1 = rbase[b + 1];
+    bool ok = a0 < a1 && a1 <= n && lb <= lb1 && lb1 <= nl && lb1 - lb < TB && rb <= rb1 && rb1 <= nruns &&
+              rb1 - rb <= WIN_RUNS && (lb1 > lb) == (rb1 > rb);
+    if (ok) {
+        const uint32_t L = lb1 - lb;
+        uint32_t prev = 0;
+        for (uint32_t r = rb; r < rb1 && ok; r++) {
+            const uint32_t f = pfirst[r], f1 = r + 1 < rb1 ? pfirst[r + 1] : L;
+            ok = (r == rb ? f == 0 : f > prev) && f1 > f && f1 <= L && (uint64_t)prun[r] + (f1 - f) <= ne;
+            for (uint32_t e = f; e < f1 && ok; e++) {
+                const uint32_t pk = tpk[lb + e];
+                ok = ((pk & WIN_START) != 0) == (e == f) && (pk & WIN_SLOT) < L;
+            }
+            prev = f;
+        }
+        uint32_t r = rb;
+        for (uint32_t c = 0; c < cpb && 64u * c < L && ok; c++) {
+            while (r < rb1 && pfirst[r] < 64u * c) r++;
+            ok = cbase[b * cpb + c] == r - rb;
+        }
+    }
+    if (!ok) atomicOr(bad, 1u);
+}
+
+// plan build: staged in-degree + 1 of each agent (rows past nrows: 1)
+__global__ void k_win_cost(const uint32_t* __restrict__ off, uint64_t n, uint64_t nrows,
+                           uint32_t* __restrict__ cost) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t d = i < nrows ? off[i + 1] - off[i] : 0u;
+    cost[i] = (d <= WIN_HMAX ? d : 0u) + 1u;
+}
+
+// first agent and first list entry of each block (C: exclusive cost prefix,
+// n + 1 entries; the thread of agent n writes the ends)
+__global__ void k_win_blocks(const uint32_t* __restrict__ C, uint64_t n, uint32_t S, uint32_t nb,
+                             uint32_t* __restrict__ ablk, uint32_t* __restrict__ lbase) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        ablk[nb] = (uint32_t)n;
+        lbase[nb] = C[n] - (uint32_t)n;
+        return;
+    }
+    const uint32_t b = C[i] / S;
+    if (i == 0 || C[i - 1] / S != b) {
+        ablk[b] = (uint32_t)i;
+        lbase[b] = C[i] - (uint32_t)i;
+    }
+}
+
+// plan sort key of each CSR slot: window * (nb + 1) + (chunk-list row ? nb : block)
+__global__ void k_win_keys(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t nrows,
+                           const uint32_t* __restrict__ C, uint32_t S, uint32_t nb, uint32_t wsz,
+                           uint32_t* __restrict__ keys) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
-    const uint32_t b = off[r], e = off[r + 1], hv = e - b > 64u ? 1u : 0u;
-    for (uint32_t k = b; k < e; k++) keys[k] = ((src[k] - 1u) / psize) << 1 | hv;
+    const uint32_t b = off[r], e = off[r + 1];
+    const uint32_t col = e - b > WIN_HMAX ? nb : C[r] / S;
+    for (uint32_t k = b; k < e; k++) keys[k] = ((src[k] - 1u) / wsz) * (nb + 1u) + col;
 }
 
-__global__ void k_plan_fill(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint64_t ne,
-                            uint32_t* __restrict__ psrc, uint32_t* __restrict__ pinv) {
+// per plan entry j: local source index, inverse permutation; the first plan
+// entry of every window (wstart, W + 1 entries)
+__global__ void k_win_fill(const uint32_t* __restrict__ src, const uint32_t* __restrict__ keys,
+                           const uint32_t* __restrict__ perm, uint64_t ne, uint32_t nb, uint32_t wsz, uint32_t W,
+                           uint16_t* __restrict__ psl, uint32_t* __restrict__ pinv, uint32_t* __restrict__ wstart) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ne) return;
-    const uint32_t k = perm[j];
-    psrc[j] = src[k];
+    const uint32_t k = perm[j], w = keys[j] / (nb + 1u);
+    psl[j] = (uint16_t)((src[k] - 1u) - w * wsz);
     pinv[k] = (uint32_t)j;
+    const int64_t wp = j ? (int64_t)(keys[j - 1] / (nb + 1u)) : -1;
+    for (int64_t q = wp + 1; q <= (int64_t)w; q++) wstart[q] = (uint32_t)j;
+    if (j == ne - 1)
+        for (uint32_t q = w + 1; q <= W; q++) wstart[q] = (uint32_t)ne;
 }
 
-// light-edge total of each 256-row block
-__global__ __launch_bounds__(256) void k_light_totals(const uint32_t* __restrict__ off, uint64_t nrows,
-                                                      uint32_t* __restrict__ tot) {
-    __shared__ uint32_t s_w[4];
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint32_t d = i < nrows ? off[i + 1] - off[i] : 0u;
-    uint32_t v = d <= 64u ? d : 0u;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) tot[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-}
-
-// per light edge (number lbase[block] + position in the block): sort key
-// (block, tile, source partition), CSR slot and slot within the tile
-__global__ __launch_bounds__(256) void k_light_keys(const uint32_t* __restrict__ off,
-                                                    const uint32_t* __restrict__ src, uint64_t nrows,
-                                                    const uint32_t* __restrict__ lbase, uint32_t psize,
-                                                    uint32_t pbits, uint32_t tile, uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ kidx, uint16_t* __restrict__ tsl) {
-    __shared__ uint32_t s_w[4];
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    uint32_t b = 0, d = 0;
-    if (i < nrows) {
-        b = off[i];
-        d = off[i + 1] - b;
-    }
-    const uint32_t ld = d <= 64u ? d : 0u;
-    uint32_t total;
-    const uint32_t P = op_light_scan(ld, s_w, total);
-    const uint32_t lb = lbase[blockIdx.x];
-    const uint32_t hi = blockIdx.x << (OP_TILE_BITS + pbits);
-    for (uint32_t e = 0; e < ld; e++) {
-        const uint32_t q = P + e, idx = lb + q;
-        keys[idx] = hi | ((q / tile) << pbits) | ((src[b + e] - 1u) / psize);
-        kidx[idx] = b + e;
-        tsl[idx] = (uint16_t)(q % tile);
+// list sort key of each staged in-edge (numbered by the staged prefix
+// C[r] - r + position): block * W + window; its CSR slot
+__global__ void k_win_lkeys(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t nrows,
+                            const uint32_t* __restrict__ C, uint32_t S, uint32_t W, uint32_t wsz,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ kidx) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const uint32_t b = off[r], d = off[r + 1] - b;
+    if (d > WIN_HMAX) return;
+    const uint32_t blk = C[r] / S, l0 = C[r] - (uint32_t)r;
+    for (uint32_t q = 0; q < d; q++) {
+        keys[l0 + q] = blk * W + (src[b + q] - 1u) / wsz;
+        kidx[l0 + q] = b + q;
     }
 }
 
-// packed light lists: per entry of the sorted lists, slot | partition << 11
-// and its weight (0 without 4-byte edge states); for tile t of block b
-// and each partition p present, the plan position of its run's first entry
-// (prun[(b * ntiles + t) * OPP_RUNS + p]) and that entry's index in the tile
-// (pfirst[...])
-__global__ void k_pack_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
-                            const uint32_t* __restrict__ kidx, const uint16_t* __restrict__ tsl,
-                            const uint32_t* __restrict__ pinv,
-                            const uint32_t* __restrict__ lbase, uint32_t pbits, uint32_t tile, uint32_t ntiles,
-                            uint64_t nl, uint16_t* __restrict__ tpk, uint32_t* __restrict__ prun,
-                            uint16_t* __restrict__ pfirst, const float* __restrict__ w, float* __restrict__ twt) {
+// 1 at the first list entry of each run (sorted keys)
+__global__ void k_win_starts(const uint32_t* __restrict__ keys, uint64_t nl, uint32_t* __restrict__ flag) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nl) return;
-    const uint32_t key = keys[e], p = key & ((1u << pbits) - 1u);
-    const uint32_t blk = key >> (OP_TILE_BITS + pbits), t = (key >> pbits) & ((1u << OP_TILE_BITS) - 1u);
-    const uint32_t t0 = lbase[blk] + t * tile;
-    const uint32_t idx = order[e];
-    tpk[e] = (uint16_t)(tsl[idx] | (p << 11));
-    twt[e] = w ? w[kidx[idx]] : 0.0f;
-    if (e == t0 || keys[e - 1] != key) {
-        const uint64_t g = (uint64_t)blk * ntiles + t;
-        prun[g * OPP_RUNS + p] = pinv[kidx[idx]];
-        pfirst[g * OPP_RUNS + p] = (uint16_t)(e - t0);
+    flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1u : 0u;
+}
+
+// packed lists: slot | run start, weight in list order; per run its plan
+// position and list index in the block (ridx: exclusive run count)
+__global__ void k_win_pack(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+                           const uint32_t* __restrict__ kidx, const uint32_t* __restrict__ flag,
+                           const uint32_t* __restrict__ ridx, const uint32_t* __restrict__ pinv,
+                           const uint32_t* __restrict__ lbase, uint32_t W, uint64_t nl, const float* __restrict__ w,
+                           uint16_t* __restrict__ tpk, float* __restrict__ twt, uint32_t* __restrict__ prun,
+                           uint16_t* __restrict__ pfirst) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nl) return;
+    const uint32_t idx = order[e], blk = keys[e] / W, k = kidx[idx];
+    const uint32_t st = flag[e];
+    tpk[e] = (uint16_t)((idx - lbase[blk]) | (st ? WIN_START : 0u));
+    twt[e] = w ? w[k] : 0.0f;
+    if (st) {
+        prun[ridx[e]] = pinv[k];
+        pfirst[ridx[e]] = (uint16_t)(e - lbase[blk]);
     }
+}
+
+// first run of each block (nb + 1 entries) and each chunk's run count
+__global__ void k_win_rbase(const uint32_t* __restrict__ lbase, const uint32_t* __restrict__ ridx, uint32_t nb,
+                            uint32_t cpb, uint32_t* __restrict__ rbase, uint16_t* __restrict__ cbase) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = (uint32_t)(t / cpb), c = (uint32_t)(t % cpb);
+    if (b > nb) return;
+    if (b == nb) {
+        if (c == 0) rbase[nb] = ridx[lbase[nb]];
+        return;
+    }
+    const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
+    if (c == 0) rbase[b] = ridx[lb];
+    cbase[t] = (uint16_t)(64u * c < L ? ridx[lb + 64u * c] - ridx[lb] : 0u);
 }
 
 // heavy chunk lists: plan position and chunk slot per entry
@@ -996,7 +1169,7 @@ __global__ void k_heavy_sizes(const uint32_t* __restrict__ off, const uint32_t* 
     chunks[h] = (d + chunk - 1) / chunk;
 }
 
-// one wave per heavy row: sort key (chunk, source partition), CSR slot and
+// one wave per heavy row: sort key (chunk, source window), CSR slot and
 // slot within the chunk of each of its in-edges (numbered from hbase[h])
 __global__ __launch_bounds__(256) void k_heavy_keys(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ src,
@@ -1483,32 +1656,57 @@ int build_heavy_bins(Sim& s, EdgeStore& E, int t) {
     return VH_OK;
 }
 
-// The gather plan of the in-edges of E (one target type at row 0, sources
-// 1-based numbers <= nsrc < 2^32, at most OPP_RUNS partitions) and its lists:
-// psrc in plan order; the light lists (lbase, tpk, twt) and their run tables
-// (prun, pfirst); the heavy rows' chunk lists (hbase, hty, hslot, hinfo). E.plan_ok stays false
-// when a sort key would need more than 32 bits (the one-pass kernel runs).
-int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint32_t tile) {
-    if (E.plan_version == E.version && E.plan_psize == psize && E.plan_tile == tile) return VH_OK;
+// The gather plan of the in-edges of E (one target type at row 0, n agents,
+// sources 1-based numbers <= n < 2^32) for states of `tb` bytes and windows
+// of `wsz` sources: the plan's local source indices (psrc) and windows
+// (wstart); the blocks (ablk, lbase), their light lists (tpk, twt), runs
+// (rbase, prun, pfirst) and chunk run counts (cbase); the heavy rows' chunk
+// lists (hbase, hty, hslot, hinfo). E.plan_ok stays false when a sort key
+// would need more than 32 bits (the one-pass kernel runs).
+int build_gather_plan(Sim& s, EdgeStore& E, uint32_t wsz, uint64_t n, uint32_t tb) {
+    if (E.plan_version == E.version && E.plan_wsz == wsz && E.plan_n == n && E.plan_tb == tb) return VH_OK;
     hipStream_t st = s.stream;
     E.plan_ok = false;
     E.plan_version = E.version;
-    E.plan_psize = psize;
-    E.plan_tile = tile;
+    E.plan_wsz = wsz;
+    E.plan_n = n;
+    E.plan_tb = tb;
     const uint64_t ne = E.nedges, nrows = E.rows;
-    const uint64_t nparts = (nsrc + psize - 1) / psize;
-    const uint32_t nblk = (uint32_t)((nrows + 255) / 256);
-    const uint32_t chunk = tile / 4, ntiles = (64 * 256 + tile - 1) / tile;
-    uint32_t pbits = 1, bbits = 1;
-    while ((1ull << pbits) < nparts) pbits++;
-    while ((1ull << bbits) < nblk) bbits++;
-    if (!ne || ne >= 0xFFFFFFFFull || nparts > (uint64_t)OPP_RUNS || bbits + OP_TILE_BITS + pbits > 32 ||
-        ntiles > (1u << OP_TILE_BITS))
-        return VH_OK;
+    const uint32_t TB = tb == 8 ? WinCfg<double>::TB : WinCfg<float>::TB;
+    const uint32_t CPB = TB / 64, S = TB - WIN_HMAX - 1;
+    const uint32_t chunk = tb == 8 ? WinCfg<double>::CHUNK : WinCfg<float>::CHUNK;
+    const uint64_t W = (n + wsz - 1) / wsz;
+    if (!ne || ne + n >= 0xFFFFFFFFull || W > WIN_RUNS || nrows > n) return VH_OK;
     const uint32_t* off = E.off.as<uint32_t>();
     const uint32_t* src = E.src.as<uint32_t>();
-    // plan order: stable sort of the CSR slots by (partition, heavy row)
-    RC(E.psrc.ensure(ne * 4 + 8, st));
+    // blocks
+    uint32_t* C = (uint32_t*)s.scratch.take((n + 1) * 4);
+    uint32_t* cost = (uint32_t*)s.scratch.take(n * 4 + 4);
+    if (!C || !cost) return set_error(VH_ERR_OOM, "scratch");
+    hipLaunchKernelGGL(k_win_cost, dim3(grid_for(n, 256)), dim3(256), 0, st, off, n, nrows, cost);
+    if (scan_excl_u32(cost, C, n, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+    uint32_t cl[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(cl, C + n - 1, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const uint32_t nb = cl[0] / S + 1;
+    const uint64_t nl = (uint64_t)cl[1] - n;
+    if (W * (nb + 1ull) > 0xFFFFFFFFull) return VH_OK;
+    // the build's pieces in one block (the sorts take many), released at the
+    // next reset when large
+    const uint64_t nh_e = ne - nl;
+    if (s.scratch.room(24 * ne + 28 * nl + 22 * nh_e + 32 * (uint64_t)nb + ne / 4 + ((uint64_t)256 << 20)))
+        return set_error(VH_ERR_OOM, "scratch for the gather plan");
+    s.scratch.shrink = true;
+    uint32_t kbits = 1, lbits = 1;
+    while ((1ull << kbits) < W * (nb + 1ull)) kbits++;
+    while ((1ull << lbits) < W * (uint64_t)nb) lbits++;
+    RC(E.ablk.ensure(((uint64_t)nb + 1) * 4 + 8, st));
+    RC(E.lbase.ensure(((uint64_t)nb + 1) * 4 + 8, st));
+    hipLaunchKernelGGL(k_win_blocks, dim3(grid_for(n + 1, 256)), dim3(256), 0, st, C, n, S, nb,
+                       E.ablk.as<uint32_t>(), E.lbase.as<uint32_t>());
+    // plan order: stable sort of the CSR slots by (window, block or heavy)
+    RC(E.psrc.ensure(ne * 2 + 16, st));
+    RC(E.wstart.ensure((W + 1) * 4 + 8, st));
     uint32_t* pinv = (uint32_t*)s.scratch.take(ne * 4 + 4);
     {
         uint32_t* keys = (uint32_t*)s.scratch.take(ne * 4);
@@ -1516,68 +1714,111 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
         uint32_t* ka = (uint32_t*)s.scratch.take(ne * 4);
         uint32_t* va = (uint32_t*)s.scratch.take(ne * 4);
         if (!pinv || !keys || !vals || !ka || !va) return set_error(VH_ERR_OOM, "scratch");
-        hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(nrows, 256)), dim3(256), 0, st, off, src, nrows, psize, keys);
-        if (radix_sort_pairs(&keys, &vals, ka, va, ne, pbits + 1, true, s.scratch, st))
+        hipLaunchKernelGGL(k_win_keys, dim3(grid_for(nrows, 256)), dim3(256), 0, st, off, src, nrows, C, S, nb, wsz,
+                           keys);
+        if (radix_sort_pairs(&keys, &vals, ka, va, ne, kbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "plan sort");
-        hipLaunchKernelGGL(k_plan_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, src, vals, ne,
-                           E.psrc.as<uint32_t>(), pinv);
+        hipLaunchKernelGGL(k_win_fill, dim3(grid_for(ne, 256)), dim3(256), 0, st, src, keys, vals, ne, nb, wsz,
+                           (uint32_t)W, E.psrc.as<uint16_t>(), pinv, E.wstart.as<uint32_t>());
         HIPCHECK(hipGetLastError());
     }
-    // light lists: the light edges of each 256-row block numbered from
-    // lbase[block], sorted by (block, tile, partition) (stable: slot order
-    // within a partition); the blocks' degree orders
-    RC(E.lbase.ensure(((uint64_t)nblk + 1) * 4 + 8, st));
-    uint32_t* tot = (uint32_t*)s.scratch.take((uint64_t)nblk * 4 + 4);
-    if (!tot) return set_error(VH_ERR_OOM, "scratch");
-    hipLaunchKernelGGL(k_light_totals, dim3(nblk), dim3(256), 0, st, off, nrows, tot);
-    if (scan_excl_u32(tot, E.lbase.as<uint32_t>(), nblk, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
-    uint32_t nl = 0;
-    HIPCHECK(hipMemcpyAsync(&nl, E.lbase.as<uint32_t>() + nblk, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    const uint64_t nt = (uint64_t)nblk * ntiles * OPP_RUNS;
-    RC(E.tpk.ensure((uint64_t)nl * 2 + 8, st));
-    RC(E.twt.ensure((uint64_t)nl * 4 + 8, st));
-    RC(E.prun.ensure(nt * 4 + 8, st));
-    RC(E.pfirst.ensure(nt * 2 + 8, st));
+    // light lists: each block's light in-edges by (window, slot)
+    RC(E.tpk.ensure(nl * 2 + 8, st));
+    RC(E.twt.ensure(nl * 4 + 8, st));
+    RC(E.rbase.ensure(((uint64_t)nb + 1) * 4 + 8, st));
+    RC(E.cbase.ensure((uint64_t)nb * CPB * 2 + 8, st));
     const float* w4 = E.bytes == 4 ? E.st.as<float>() : nullptr;
-    if (nl) {
-        uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
-        uint32_t* vals = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
-        uint32_t* ka = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
-        uint32_t* va = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
-        uint32_t* kidx = (uint32_t*)s.scratch.take((uint64_t)nl * 4 + 4);
-        uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nl * 2 + 4);
-        if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
-        hipLaunchKernelGGL(k_light_keys, dim3(nblk), dim3(256), 0, st, off, src, nrows, E.lbase.as<uint32_t>(),
-                           psize, pbits, tile, keys, kidx, tsl);
-        if (radix_sort_pairs(&keys, &vals, ka, va, nl, bbits + OP_TILE_BITS + pbits, true, s.scratch, st))
-            return set_error(VH_ERR_DEVICE, "tile sort");
-        hipLaunchKernelGGL(k_pack_fill, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, tsl, pinv,
-                           E.lbase.as<uint32_t>(), pbits, tile, ntiles, (uint64_t)nl, E.tpk.as<uint16_t>(),
-                           E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(), w4, E.twt.as<float>());
+    {
+        uint32_t* ridx = (uint32_t*)s.scratch.take((nl + 1) * 4);
+        if (!ridx) return set_error(VH_ERR_OOM, "scratch");
+        uint32_t nruns = 0;
+        if (nl) {
+            uint32_t* keys = (uint32_t*)s.scratch.take(nl * 4 + 4);
+            uint32_t* vals = (uint32_t*)s.scratch.take(nl * 4 + 4);
+            uint32_t* ka = (uint32_t*)s.scratch.take(nl * 4 + 4);
+            uint32_t* va = (uint32_t*)s.scratch.take(nl * 4 + 4);
+            uint32_t* kidx = (uint32_t*)s.scratch.take(nl * 4 + 4);
+            uint32_t* flag = (uint32_t*)s.scratch.take(nl * 4 + 4);
+            if (!keys || !vals || !ka || !va || !kidx || !flag) return set_error(VH_ERR_OOM, "scratch");
+            hipLaunchKernelGGL(k_win_lkeys, dim3(grid_for(nrows, 256)), dim3(256), 0, st, off, src, nrows, C, S,
+                               (uint32_t)W, wsz, keys, kidx);
+            if (radix_sort_pairs(&keys, &vals, ka, va, nl, lbits, true, s.scratch, st))
+                return set_error(VH_ERR_DEVICE, "list sort");
+            hipLaunchKernelGGL(k_win_starts, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, nl, flag);
+            if (scan_excl_u32(flag, ridx, nl, s.scratch, st)) return set_error(VH_ERR_DEVICE, "scan");
+            HIPCHECK(hipMemcpyAsync(&nruns, ridx + nl, 4, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            RC(E.prun.ensure((uint64_t)nruns * 4 + 8, st));
+            RC(E.pfirst.ensure((uint64_t)nruns * 2 + 8, st));
+            hipLaunchKernelGGL(k_win_pack, dim3(grid_for(nl, 256)), dim3(256), 0, st, keys, vals, kidx, flag, ridx,
+                               pinv, E.lbase.as<uint32_t>(), (uint32_t)W, nl, w4, E.tpk.as<uint16_t>(),
+                               E.twt.as<float>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>());
+        } else {
+            fill_u32(ridx, 0, 1, st);
+        }
+        hipLaunchKernelGGL(k_win_rbase, dim3(grid_for(((uint64_t)nb + 1) * CPB, 256)), dim3(256), 0, st,
+                           E.lbase.as<uint32_t>(), ridx, nb, CPB, E.rbase.as<uint32_t>(), E.cbase.as<uint16_t>());
         HIPCHECK(hipGetLastError());
     }
-    // heavy rows (E.heavy, ascending): chunk lists
-    const uint64_t nheavy = E.nheavy;
+    RC(E.wq.ensure(16, st));
+    fill_u32(E.wq.as<uint32_t>(), 0, 3, st);   // queue (2) and the check flag
+    {
+        uint32_t nruns = 0, bad = 0;
+        HIPCHECK(hipMemcpyAsync(&nruns, E.rbase.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        hipLaunchKernelGGL(k_win_check, dim3(grid_for(nb, 64)), dim3(64), 0, st, E.ablk.as<uint32_t>(),
+                           E.lbase.as<uint32_t>(), E.rbase.as<uint32_t>(), E.prun.as<uint32_t>(),
+                           E.pfirst.as<uint16_t>(), E.tpk.as<uint16_t>(), E.cbase.as<uint16_t>(), nb, n, nl,
+                           (uint64_t)nruns, ne, TB, CPB, E.wq.as<uint32_t>() + 2);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(&bad, E.wq.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (bad) return set_error(VH_ERR_DEVICE, "gather plan inconsistent");
+    }
+    // rows of in-degree above WIN_HMAX (from E.heavy, ascending): chunk lists
+    // sorted by (chunk, window)
+    uint64_t nheavy = 0;
+    if (E.nheavy) {
+        uint32_t* hd = (uint32_t*)s.scratch.take(E.nheavy * 4 + 4);
+        uint32_t* hc = (uint32_t*)s.scratch.take(E.nheavy * 4 + 4);
+        if (!hd || !hc) return set_error(VH_ERR_OOM, "scratch");
+        hipLaunchKernelGGL(k_heavy_sizes, dim3(grid_for(E.nheavy, 256)), dim3(256), 0, st, off,
+                           E.heavy.as<uint32_t>(), E.nheavy, chunk, hd, hc);
+        std::vector<uint32_t> rows(E.nheavy), deg(E.nheavy);
+        HIPCHECK(hipMemcpyAsync(rows.data(), E.heavy.p, E.nheavy * 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(deg.data(), hd, E.nheavy * 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<uint32_t> huge;
+        for (uint64_t h = 0; h < E.nheavy; h++)
+            if (deg[h] > WIN_HMAX) huge.push_back(rows[h]);
+        nheavy = huge.size();
+        if (nheavy) {
+            RC(E.huge.ensure(nheavy * 4 + 8, st));
+            HIPCHECK(hipMemcpyAsync(E.huge.p, huge.data(), nheavy * 4, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        }
+    }
+    E.plan_nhuge = nheavy;
     if (nheavy) {
         uint32_t* hd = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
         uint32_t* hc = (uint32_t*)s.scratch.take(nheavy * 4 + 4);
-        uint32_t* cbase = (uint32_t*)s.scratch.take(nheavy * 4 + 8);
-        if (!hd || !hc || !cbase) return set_error(VH_ERR_OOM, "scratch");
+        uint32_t* hcb = (uint32_t*)s.scratch.take(nheavy * 4 + 8);
+        if (!hd || !hc || !hcb) return set_error(VH_ERR_OOM, "scratch");
         RC(E.hbase.ensure(nheavy * 4 + 8, st));
         hipLaunchKernelGGL(k_heavy_sizes, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off,
-                           E.heavy.as<uint32_t>(), nheavy, chunk, hd, hc);
+                           E.huge.as<uint32_t>(), nheavy, chunk, hd, hc);
         if (scan_excl_u32(hd, E.hbase.as<uint32_t>(), nheavy, s.scratch, st) ||
-            scan_excl_u32(hc, cbase, nheavy, s.scratch, st))
+            scan_excl_u32(hc, hcb, nheavy, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "scan");
         uint32_t htot[2] = {0, 0};
         HIPCHECK(hipMemcpyAsync(&htot[0], E.hbase.as<uint32_t>() + nheavy, 4, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(&htot[1], cbase + nheavy, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&htot[1], hcb + nheavy, 4, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         const uint32_t nhe = htot[0], nch = htot[1];
-        uint32_t cbits = 1;
+        uint32_t cbits = 1, wbits = 1;
         while ((1ull << cbits) < nch) cbits++;
-        if (cbits + pbits > 32 || (uint64_t)nl + nhe != ne) return VH_OK;
+        while ((1ull << wbits) < W) wbits++;
+        if (cbits + wbits > 32 || nl + nhe != ne) return VH_OK;
         RC(E.hty.ensure((uint64_t)nhe * 4 + 8, st));
         RC(E.hslot.ensure((uint64_t)nhe * 2 + 8, st));
         uint32_t* keys = (uint32_t*)s.scratch.take((uint64_t)nhe * 4);
@@ -1588,21 +1829,48 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t psize, uint64_t nsrc, uint3
         uint16_t* tsl = (uint16_t*)s.scratch.take((uint64_t)nhe * 2);
         if (!keys || !vals || !ka || !va || !kidx || !tsl) return set_error(VH_ERR_OOM, "scratch");
         hipLaunchKernelGGL(k_heavy_keys, dim3(grid_for(nheavy * 64, 256)), dim3(256), 0, st, off, src,
-                           E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), cbase, psize, pbits, chunk,
-                           keys, kidx, tsl);
-        if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + pbits, true, s.scratch, st))
+                           E.huge.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), hcb, wsz, wbits, chunk, keys,
+                           kidx, tsl);
+        if (radix_sort_pairs(&keys, &vals, ka, va, nhe, cbits + wbits, true, s.scratch, st))
             return set_error(VH_ERR_DEVICE, "chunk sort");
         hipLaunchKernelGGL(k_heavy_fill, dim3(grid_for(nhe, 256)), dim3(256), 0, st, vals, kidx, tsl, pinv,
                            (uint64_t)nhe, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
         RC(E.hinfo.ensure(nheavy * 16 + 16, st));
         hipLaunchKernelGGL(k_heavy_info, dim3(grid_for(nheavy, 256)), dim3(256), 0, st, off,
-                           E.heavy.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), E.hinfo.as<uint4>());
+                           E.huge.as<uint32_t>(), nheavy, E.hbase.as<uint32_t>(), E.hinfo.as<uint4>());
         HIPCHECK(hipGetLastError());
+        // longest rows first: pass 2's waves take them round-robin, so every
+        // wave's first row is among the longest
+        std::vector<uint4> hi(nheavy);
+        HIPCHECK(hipMemcpyAsync(hi.data(), E.hinfo.p, nheavy * 16, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::stable_sort(hi.begin(), hi.end(), [](const uint4& a, const uint4& b) { return a.z > b.z; });
+        HIPCHECK(hipMemcpyAsync(E.hinfo.p, hi.data(), nheavy * 16, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
     } else if (nl != ne) {
         return VH_OK;
     }
+    E.plan_nb = nb;
+    E.plan_w = (uint32_t)W;
     E.plan_ok = true;
     return VH_OK;
+}
+
+// compute units of the current device (the persistent plan kernels run one
+// block per CU)
+static uint32_t device_cus() {
+    static uint32_t cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) {
+            (void)hipGetLastError();
+            v = 256;
+        }
+        cus[dev] = (uint32_t)v;
+    }
+    return cus[dev];
 }
 
 int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step) {
@@ -1618,25 +1886,25 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
     const uint64_t nheavy = E.rows ? E.nheavy : 0;
     const uint32_t nlight = grid_for(A.nslots, 256);
     // two-pass gather through a plan from a state vector of at least
-    // opinion_plan_bytes (< 0: never), with source partitions of
-    // opinion_part_bytes (0: 2 MiB, widened up to 1.5x to make at most 32
-    // partitions), never more than 32 partitions
+    // opinion_plan_bytes (< 0: never), with source windows of
+    // opinion_part_bytes (0 or more than 160 KB: 160 KB; widened to make at
+    // most WIN_RUNS windows, a multiple of 4 sources)
     const int64_t plan_min = g_opts.opinion_plan_bytes;
     bool plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
                 E.rowbase[t] == 0 && nrows == E.rows && A.nslots < 0xFFFFFFFFull &&
                 (uint64_t)A.nslots * tb >= (uint64_t)plan_min;
-    const uint32_t tile = f32 ? (uint32_t)TileCfg<float>::TILE : (uint32_t)TileCfg<double>::TILE;
     if (plan) {
-        const bool auto_part = g_opts.opinion_part_bytes == 0;
-        uint64_t psize = (auto_part ? (2ull << 20) : (uint64_t)g_opts.opinion_part_bytes) / tb;
-        if (psize < 1) psize = 1;
-        const uint64_t np = (A.nslots + psize - 1) / psize;
-        if (auto_part && np > OPP_RUNS && np * 2 <= OPP_RUNS * 3) psize = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;
-        const uint64_t pmin = (A.nslots + OPP_RUNS - 1) / OPP_RUNS;   // at most OPP_RUNS partitions
-        if (psize < pmin) psize = pmin;
-        if (psize > 0x7FFFFFFFull) psize = 0x7FFFFFFFull;
-        RC(build_gather_plan(s, E, (uint32_t)psize, A.nslots, tile));
-        plan = E.plan_ok;
+        const uint64_t cap = f32 ? WinCfg<float>::WSZ : WinCfg<double>::WSZ;
+        uint64_t wsz = g_opts.opinion_part_bytes <= 0 ? cap : (uint64_t)g_opts.opinion_part_bytes / tb;
+        if (wsz > cap) wsz = cap;
+        const uint64_t wmin = (A.nslots + WIN_RUNS - 1) / WIN_RUNS;
+        if (wsz < wmin) wsz = wmin;
+        wsz = (wsz + 3) / 4 * 4;
+        if (wsz > cap) plan = false;   // more sources than WIN_RUNS windows hold
+        if (plan) {
+            RC(build_gather_plan(s, E, (uint32_t)wsz, A.nslots, tb));
+            plan = E.plan_ok;
+        }
     }
     const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
     if (!plan && g_opinion_defer && g_opinion_defer->stream == s.stream) {
@@ -1670,31 +1938,27 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
         HIPCHECK(hipGetLastError());
         return VH_OK;
     }
-    RC(E.pval.ensure(E.nedges * tb + 8, s.stream));
-    const uint64_t nb = (E.nedges + 256 * GP_U - 1) / (256 * GP_U);
-    const dim3 gg((uint32_t)((nb + 7) / 8 * 8)), bb(256);
-    // light blocks: 8 * ceil(nlight / 8) (XCD-aware order), heavy blocks a multiple of 8
-    const uint32_t nh8 = (nh + 7) / 8 * 8, nl8 = (nlight + 7) / 8 * 8;
-    const uint32_t nblkp = (uint32_t)((nrows + 255) / 256);
-    const dim3 g2(nh8 + nl8);
+    RC(E.pval.ensure(E.nedges * tb + 16, s.stream));
+    const dim3 g(device_cus()), bb(1024);
     if (f32) {
-        hipLaunchKernelGGL(k_plan_gather<float>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
-                           (const float*)A.rstate(), E.pval.as<float>(), E.nedges);
-        hipLaunchKernelGGL(k_opinion_plan<float>, g2, bb, 0, s.stream, E.off.as<uint32_t>(),
-                           (const float*)A.rstate(), E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(),
-                           A.nslots, nrows, (float)p.eps, E.hinfo.as<uint4>(), nheavy, nh8, nlight,
-                           E.lbase.as<uint32_t>(), E.tpk.as<uint16_t>(), E.twt.as<float>(), nblkp,
-                           E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
-                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
+        hipLaunchKernelGGL(k_win_gather<float>, g, bb, 0, s.stream, E.psrc.as<uint16_t>(), (const float*)A.rstate(),
+                           E.pval.as<float>(), E.wstart.as<uint32_t>(), E.plan_w, E.plan_wsz, A.nslots, E.nedges);
+        hipLaunchKernelGGL(k_win_sum<float>, g, bb, 0, s.stream, E.off.as<uint32_t>(), (const float*)A.rstate(),
+                           E.pval.as<float>(), E.st.as<float>(), (float*)A.wstate(), nrows, (float)p.eps,
+                           E.hinfo.as<uint4>(), E.plan_nhuge, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(),
+                           E.ablk.as<uint32_t>(), E.lbase.as<uint32_t>(), E.tpk.as<uint16_t>(), E.twt.as<float>(),
+                           E.rbase.as<uint32_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
+                           E.cbase.as<uint16_t>(), E.plan_nb, E.wq.as<uint32_t>());
     } else {
-        hipLaunchKernelGGL(k_plan_gather<double>, gg, bb, 0, s.stream, E.psrc.as<uint32_t>(),
-                           (const double*)A.rstate(), E.pval.as<double>(), E.nedges);
-        hipLaunchKernelGGL(k_opinion_plan<double>, g2, bb, 0, s.stream, E.off.as<uint32_t>(),
-                           (const double*)A.rstate(), E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(),
-                           A.nslots, nrows, p.eps, E.hinfo.as<uint4>(), nheavy, nh8, nlight,
-                           E.lbase.as<uint32_t>(), E.tpk.as<uint16_t>(), E.twt.as<float>(), nblkp,
-                           E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
-                           E.hty.as<uint32_t>(), E.hslot.as<uint16_t>());
+        hipLaunchKernelGGL(k_win_gather<double>, g, bb, 0, s.stream, E.psrc.as<uint16_t>(),
+                           (const double*)A.rstate(), E.pval.as<double>(), E.wstart.as<uint32_t>(), E.plan_w,
+                           E.plan_wsz, A.nslots, E.nedges);
+        hipLaunchKernelGGL(k_win_sum<double>, g, bb, 0, s.stream, E.off.as<uint32_t>(), (const double*)A.rstate(),
+                           E.pval.as<double>(), E.st.as<float>(), (double*)A.wstate(), nrows, p.eps,
+                           E.hinfo.as<uint4>(), E.plan_nhuge, E.hty.as<uint32_t>(), E.hslot.as<uint16_t>(),
+                           E.ablk.as<uint32_t>(), E.lbase.as<uint32_t>(), E.tpk.as<uint16_t>(), E.twt.as<float>(),
+                           E.rbase.as<uint32_t>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>(),
+                           E.cbase.as<uint16_t>(), E.plan_nb, E.wq.as<uint32_t>());
     }
     HIPCHECK(hipGetLastError());
     return VH_OK;

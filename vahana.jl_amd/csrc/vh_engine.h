@@ -75,17 +75,22 @@ struct EdgeStore {
     DevBuf heavy;                  // uint32 rows with degree > 64
     uint64_t nheavy = 0, bins_version = ~0ull;
     // gather plan of the opinion kernels (build_gather_plan), valid for
-    // `plan_version` with partitions of plan_psize sources and tiles of
-    // plan_tile light slots
-    DevBuf psrc, pval;             // per plan entry: source; gathered states
-    DevBuf lbase, tpk, twt;        // light lists: first entry per block; packed entry, weight
-    DevBuf prun, pfirst;           // run table per block and tile: plan position, list index
-    DevBuf hbase, hty, hslot;      // heavy rows' chunk lists: first entry per row; plan position, slot
+    // `plan_version` with windows of plan_wsz sources, plan_n agents and
+    // states of plan_tb bytes
+    DevBuf psrc, pval;             // per plan entry: local source index (uint16); gathered states
+    DevBuf wstart;                 // first plan entry of each window
+    DevBuf ablk, lbase;            // per block: first agent; first light-list entry
+    DevBuf tpk, twt;               // light lists: slot | run start, weight
+    DevBuf rbase, prun, pfirst;    // runs: first run per block; plan position, list index in the block
+    DevBuf cbase;                  // run starts before each 64-entry chunk of a block's list
+    DevBuf wq;                     // pass-2 queue: next block, finished launch blocks
+    DevBuf huge;                   // rows of in-degree above the plan's staging limit
+    DevBuf hbase, hty, hslot;      // their chunk lists: first entry per row; plan position, slot
     DevBuf hinfo;                  // per heavy row: row, CSR start, degree, hbase (one 16-byte load)
     bool plan_ok = false;          // the lists fit their sort keys (else the one-pass kernel)
-    uint64_t plan_version = ~0ull;
-    uint32_t plan_psize = 0;       // sources per partition
-    uint32_t plan_tile = 0;        // light slots per tile of the lists
+    uint64_t plan_version = ~0ull, plan_n = 0;
+    uint32_t plan_wsz = 0, plan_tb = 0, plan_nb = 0, plan_w = 0;
+    uint64_t plan_nhuge = 0;
     // out-edges of the SIR exposure pass (build_out_csr), valid for
     // `out_version`; `expo`: one bit per agent, set when it has an infected
     // in-neighbour in this step; `expo_cnt`: infected counts by step parity
@@ -94,8 +99,10 @@ struct EdgeStore {
     // every device buffer of the store (release, accounting)
     template <class F>
     void each_buf(F f) {
-        DevBuf* all[] = {&off,   &src,   &st,  &heavy, &psrc,  &pval,    &lbase,   &tpk,  &twt,      &prun,
-                         &pfirst, &hbase, &hty, &hslot, &hinfo, &out_off, &out_dst, &expo, &expo_cnt, &ba_pairs};
+        DevBuf* all[] = {&off,  &src,   &st,    &heavy, &psrc, &pval,    &wstart,  &ablk, &lbase,    &tpk,
+                         &twt,  &rbase, &prun,  &pfirst, &cbase, &wq,     &huge,    &hbase, &hty,     &hslot,
+                         &hinfo,
+                         &out_off, &out_dst, &expo, &expo_cnt, &ba_pairs};
         for (DevBuf* b : all) f(*b);
     }
     bool writeable = false;

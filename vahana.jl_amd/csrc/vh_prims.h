@@ -24,11 +24,17 @@ struct Scratch {
     size_t cap = 0, used = 0, spilled = 0;
     void* extra[64];
     int nextra = 0;
+    char* xb = nullptr;                  // spill block handed out piecewise (room())
+    size_t xcap = 0, xused = 0;
+    bool shrink = false;                 // at the next reset: release when more than 1 GiB is held
     void* take(size_t bytes);            // nullptr on allocation failure
     void reset(hipStream_t st);
     // after reset: grow the main block to at least `bytes` (work that takes
     // many pieces before its next reset); 0 or -3
     int reserve(size_t bytes, hipStream_t st);
+    // at any point: room for `bytes` more in one piece (the main block's rest,
+    // or one spill block that the following takes share); 0 or -3
+    int room(size_t bytes);
     void release();
     size_t bytes_held() const { return cap + spilled; }
 };

@@ -27,6 +27,11 @@ void* Scratch::take(size_t bytes) {
         used += bytes;
         return r;
     }
+    if (xb && xused + bytes <= xcap) {
+        void* r = xb + xused;
+        xused += bytes;
+        return r;
+    }
     if (nextra >= 64) return nullptr;
     void* p = nullptr;
     if (device_alloc(&p, bytes) != hipSuccess) return nullptr;
@@ -34,7 +39,30 @@ void* Scratch::take(size_t bytes) {
     spilled += bytes;
     return p;
 }
+int Scratch::room(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (used + bytes <= cap || (xb && xused + bytes <= xcap)) return 0;
+    if (nextra >= 64) return -3;
+    void* p = nullptr;
+    if (device_alloc(&p, bytes) != hipSuccess) return -3;
+    extra[nextra++] = p;
+    spilled += bytes;
+    xb = (char*)p;
+    xcap = bytes;
+    xused = 0;
+    return 0;
+}
 void Scratch::reset(hipStream_t st) {
+    xb = nullptr;
+    xcap = xused = 0;
+    if (shrink) {
+        shrink = false;
+        if (bytes_held() > ((size_t)1 << 30)) {
+            hipStreamSynchronize(st);
+            release();
+            return;
+        }
+    }
     if (nextra) {
         // fold the spill into one main block sized for the whole round
         size_t need = used + spilled;
@@ -61,6 +89,8 @@ int Scratch::reserve(size_t bytes, hipStream_t st) {
     return 0;
 }
 void Scratch::release() {
+    xb = nullptr;
+    xcap = xused = 0;
     for (int i = 0; i < nextra; i++) hipFree(extra[i]);
     nextra = 0;
     spilled = 0;

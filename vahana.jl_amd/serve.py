@@ -58,10 +58,10 @@ def footprint(req):
     # two Float64 state buffers, offsets, sources + weights, generator scratch
     fp = req.n * 16 + req.n * 4 + e * 8 + req.n * 8
     if req.n * 8 >= OPINION_PLAN_MIN_BYTES:
-        # the two-pass gather: plan sources, positions and states, tile lists
-        # with their weights and run tables; the scratch of their one-time
-        # sorts
-        fp += e * 28 + req.n * 8 + e * 40
+        # the two-pass gather: local source indices, gathered states, lists
+        # with their weights, run tables; the scratch of the one-time plan
+        # build (its sorts)
+        fp += e * 17 + req.n * 8 + e * 53
     return int(fp)
 
 

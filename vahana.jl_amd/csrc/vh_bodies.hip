@@ -780,17 +780,17 @@ template <class T>
 struct WinLDS {
     T val[WinCfg<T>::TB];          // values by slot (heavy waves: CHUNK each)
     float wgt[WinCfg<T>::TB];      // weights by slot
-    uint32_t rp[1][WIN_RUNS];      // runs of the block: plan position, list index
-    uint16_t rf[1][WIN_RUNS];
+    uint32_t rp[WIN_RUNS];         // runs of the block: plan position, list index
+    uint16_t rf[WIN_RUNS];
     uint32_t wsum[WinCfg<T>::TB / 1024 + 1][16];
-    uint32_t nextq[1];             // the queue item taken
+    uint32_t next;                 // the queue item taken
     uint32_t nmed;                 // agents of in-degree 65..WIN_HMAX of the block
     uint32_t mslot[WIN_MED], mdeg[WIN_MED], magent[WIN_MED];
     T mx[WIN_MED];
 };
 
-// The round-0 and round-1 agents of a light block (a0 + tid, a0 + 1024 +
-// tid): in-degrees and states, loaded one block ahead
+// The round-0 and round-1 agents of a block (a0 + tid, a0 + 1024 + tid):
+// in-degrees and states, loaded with the block's list
 template <class T>
 struct WinAgents {
     uint32_t d0, d1;
@@ -818,7 +818,7 @@ __device__ __forceinline__ WinAgents<T> win_agents(const uint32_t* __restrict__ 
 // also publishes the staged values); each agent of in-degree <= 64 adds its
 // slots in order, the others of the block are listed and summed by a wave
 // each (lane-strided partials, xor butterfly). Rounds 0 and 1 use the agents
-// loaded ahead (g), later rounds load theirs.
+// loaded early (g), later rounds load theirs.
 template <class T>
 __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __restrict__ off,
                                                const T* __restrict__ x, T* __restrict__ xo, uint64_t nrows, T eps,
@@ -950,16 +950,16 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
     // light blocks, each in three phases: its list, weights, runs and
     // agents; its values, staged by slot; the sums
     for (;;) {
-        if (tid == 0) sm.nextq[0] = atomicAdd(queue, 1u);
+        if (tid == 0) sm.next = atomicAdd(queue, 1u);
         __syncthreads();
-        const uint32_t b = sm.nextq[0];
+        const uint32_t b = sm.next;
         if (b >= nb) break;
         const uint32_t a0 = ablk[b], a1 = ablk[b + 1];
         const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
         const uint32_t rb = rbase[b], R = rbase[b + 1] - rb;
         if ((uint32_t)tid < R) {
-            sm.rp[0][tid] = prun[rb + tid];
-            sm.rf[0][tid] = pfirst[rb + tid];
+            sm.rp[tid] = prun[rb + tid];
+            sm.rf[tid] = pfirst[rb + tid];
         }
         uint32_t pk[QU], cb[QU];
         float wt[QU];
@@ -983,7 +983,7 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
             const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
             const uint32_t run = cb[u] + __popcll(m & lemask) - 1u;   // (entries of the list only)
             v[u] = (T)0;
-            if (ok) v[u] = y[sm.rp[0][run] + (e - sm.rf[0][run])];
+            if (ok) v[u] = y[sm.rp[run] + (e - sm.rf[run])];
         }
 #pragma unroll
         for (uint32_t u = 0; u < QU; u++)

@@ -793,23 +793,23 @@ struct WinLDS {
 // in-degrees and states, loaded with the block's list
 template <class T>
 struct WinAgents {
-    uint32_t d0, d1;
+    uint32_t o0, o1, o2, o3;   // raw loads; selected where used
     T x0, x1;
 };
 
 template <class T>
 __device__ __forceinline__ WinAgents<T> win_agents(const uint32_t* __restrict__ off, const T* __restrict__ x,
                                                    uint64_t nrows, uint32_t a0, uint32_t a1, uint32_t tid) {
-    WinAgents<T> g = {0u, 0u, (T)0, (T)0};
+    WinAgents<T> g;
     const uint32_t i0 = a0 + tid, i1 = i0 + 1024;
-    if (i0 < a1) {
-        g.x0 = x[i0];
-        if (i0 < nrows) g.d0 = off[i0 + 1] - off[i0];
-    }
-    if (i1 < a1) {
-        g.x1 = x[i1];
-        if (i1 < nrows) g.d1 = off[i1 + 1] - off[i1];
-    }
+    const uint32_t j0 = i0 < a1 ? i0 : a0, j1 = i1 < a1 ? i1 : a0;
+    const uint32_t r0 = j0 < nrows ? j0 : 0u, r1 = j1 < nrows ? j1 : 0u;
+    g.x0 = x[j0];
+    g.x1 = x[j1];
+    g.o0 = off[r0];
+    g.o1 = off[r0 + 1];
+    g.o2 = off[r1];
+    g.o3 = off[r1 + 1];
     return g;
 }
 
@@ -830,7 +830,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
     for (uint32_t r = 0; r < K; r++) {
         const uint32_t i = a0 + r * 1024 + tid;
         uint32_t d = 0;
-        if (i < a1) d = r == 0 ? g.d0 : r == 1 ? g.d1 : (i < nrows ? off[i + 1] - off[i] : 0u);
+        if (i < a1 && i < nrows) d = r == 0 ? g.o1 - g.o0 : r == 1 ? g.o3 - g.o2 : off[i + 1] - off[i];
         const uint32_t ld = d <= WIN_HMAX ? d : 0u;
         uint32_t incl = ld;
 #pragma unroll
@@ -859,10 +859,10 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
         uint32_t d, incl, ld;
         T xi;
         if (r < 2) {
-            d = i < a1 ? (r == 0 ? g.d0 : g.d1) : 0u;
+            d = (i < a1 && i < nrows) ? (r == 0 ? g.o1 - g.o0 : g.o3 - g.o2) : 0u;
             incl = inclk[r];
             ld = ldk[r];
-            xi = r == 0 ? g.x0 : g.x1;
+            xi = i < a1 ? (r == 0 ? g.x0 : g.x1) : (T)0;
         } else {
             d = (i < a1 && i < nrows) ? off[i + 1] - off[i] : 0u;
             ld = d <= WIN_HMAX ? d : 0u;
@@ -947,52 +947,93 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
     for (uint64_t h = (uint64_t)blockIdx.x * 16 + (uint32_t)wv; h < nheavy; h += (uint64_t)gridDim.x * 16)
         opinion_heavy_plan<T>(x, y, w, xo, hinfo[h], eps, hty, hslot, sm.val + wv * WinCfg<T>::CHUNK);
     __syncthreads();
-    // light blocks, each in three phases: its list, weights, runs and
-    // agents; its values, staged by slot; the sums
-    for (;;) {
-        if (tid == 0) sm.next = atomicAdd(queue, 1u);
-        __syncthreads();
-        const uint32_t b = sm.next;
-        if (b >= nb) break;
-        const uint32_t a0 = ablk[b], a1 = ablk[b + 1];
+    // light blocks: while block b's values are in flight and summed, the
+    // list, runs and agents of the next block are loading (loads complete in
+    // issue order, so those are issued after b's values; every load is
+    // branch-free, a load under a branch makes the compiler wait for all
+    // loads in flight where the branch joins)
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t ltid = tid;
+    auto load_list = [&](uint32_t b, uint32_t* pk) __attribute__((always_inline)) {
         const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
+#pragma unroll
+        for (uint32_t u = 0; u < QU; u++) {
+            const uint32_t e = ltid + 1024 * u;
+            pk[u] = __builtin_nontemporal_load(tpk + lb + (e < L ? e : 0u));   // (past the list: masked where used)
+        }
+    };
+    auto load_runs = [&](uint32_t b, uint32_t& a, uint32_t& f) __attribute__((always_inline)) {
         const uint32_t rb = rbase[b], R = rbase[b + 1] - rb;
-        if ((uint32_t)tid < R) {
-            sm.rp[tid] = prun[rb + tid];
-            sm.rf[tid] = pfirst[rb + tid];
+        const uint32_t r = rb + (ltid < R ? ltid : 0u);   // (R == 0: the slack after the runs)
+        a = prun[r];
+        f = pfirst[r];
+    };
+    if (tid == 0) sm.next = atomicAdd(queue, 1u);
+    __syncthreads();
+    uint32_t b = __builtin_amdgcn_readfirstlane(sm.next);
+    __syncthreads();
+    if (b < nb) {
+        uint32_t pk[QU];
+        WinAgents<T> ag;
+        {
+            uint32_t ra, rfv;
+            load_runs(b, ra, rfv);
+            load_list(b, pk);
+            ag = win_agents<T>(off, x, nrows, ablk[b], ablk[b + 1], ltid);
+            sm.rp[tid] = ra;
+            sm.rf[tid] = (uint16_t)rfv;
+            if (tid == 0) sm.next = atomicAdd(queue, 1u);
+            __syncthreads();
         }
-        uint32_t pk[QU], cb[QU];
-        float wt[QU];
+        for (;;) {
+            asm volatile("" : "+v"(ltid));
+            const uint32_t it = __builtin_amdgcn_readfirstlane(sm.next);
+            const uint32_t bn = it < nb ? it : NONE;
+            const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
+            // the wave's 16 chunk run counts: one scalar 32-byte load
+            const uint32_t* cw = (const uint32_t*)(cbase + ((uint64_t)b * 256 + (uint32_t)wv * 16));
+            uint32_t cwv[8];
 #pragma unroll
-        for (uint32_t u = 0; u < QU; u++) {
-            const uint32_t e = tid + 1024 * u;
-            pk[u] = e < L ? (uint32_t)__builtin_nontemporal_load(tpk + lb + e) : 0xFFFFFFFFu;
-            wt[u] = e < L ? __builtin_nontemporal_load(twt + lb + e) : 0.0f;
-        }
-        // chunk run counts before the barrier: a 16-bit load is a vector load,
-        // and a wait for it among the value loads would wait for all of them
+            for (int q = 0; q < 8; q++) cwv[q] = cw[q];
+            T v[QU];
+            float wt[QU];
 #pragma unroll
-        for (uint32_t u = 0; u < QU; u++) cb[u] = cbase[(uint64_t)b * CPB + u * 16 + wv];
-        const WinAgents<T> ag = win_agents<T>(off, x, nrows, a0, a1, tid);
-        __syncthreads();
-        T v[QU];
-#pragma unroll
-        for (uint32_t u = 0; u < QU; u++) {
-            const uint32_t e = tid + 1024 * u;
-            const bool ok = pk[u] != 0xFFFFFFFFu;
-            const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
-            const uint32_t run = cb[u] + __popcll(m & lemask) - 1u;   // (entries of the list only)
-            v[u] = (T)0;
-            if (ok) v[u] = y[sm.rp[run] + (e - sm.rf[run])];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < QU; u++)
-            if (pk[u] != 0xFFFFFFFFu) {
-                sm.val[pk[u] & WIN_SLOT] = v[u];
-                sm.wgt[pk[u] & WIN_SLOT] = wt[u];
+            for (uint32_t u = 0; u < QU; u++) {
+                const uint32_t e = ltid + 1024 * u;
+                const bool ok = e < L;
+                const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
+                const uint32_t cb = (cwv[u >> 1] >> ((u & 1) * 16)) & 0xFFFFu;
+                const uint32_t run = ok ? cb + __popcll(m & lemask) - 1u : 0u;
+                const uint32_t pos = ok ? sm.rp[run] + (e - sm.rf[run]) : 0u;
+                v[u] = y[pos];
+                wt[u] = __builtin_nontemporal_load(twt + lb + (ok ? e : 0u));
             }
-        win_sum_agents<T>(sm, off, x, xo, nrows, eps, a0, a1, ag);   // (its barrier publishes the values)
-        __syncthreads();
+            uint32_t pkn[QU];
+            uint32_t ra = 0, rfv = 0;
+            WinAgents<T> agn = ag;
+            if (bn != NONE) {
+                load_list(bn, pkn);
+                load_runs(bn, ra, rfv);
+                agn = win_agents<T>(off, x, nrows, ablk[bn], ablk[bn + 1], ltid);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < QU; u++)
+                if (ltid + 1024 * u < L) {
+                    sm.val[pk[u] & WIN_SLOT] = v[u];
+                    sm.wgt[pk[u] & WIN_SLOT] = wt[u];
+                }
+            win_sum_agents<T>(sm, off, x, xo, nrows, eps, ablk[b], ablk[b + 1], ag);   // (its barrier publishes the values)
+            __syncthreads();
+            if (bn == NONE) break;
+            sm.rp[tid] = ra;
+            sm.rf[tid] = (uint16_t)rfv;
+            if (tid == 0) sm.next = atomicAdd(queue, 1u);
+            __syncthreads();
+            b = bn;
+#pragma unroll
+            for (uint32_t u = 0; u < QU; u++) pk[u] = pkn[u];
+            ag = agn;
+        }
     }
     if (tid == 0 && atomicAdd(queue + 1, 1u) == gridDim.x - 1) {
         queue[0] = 0;
@@ -1030,7 +1071,7 @@ __global__ void k_win_check(const uint32_t* __restrict__ ablk, const uint32_t* _
         uint32_t r = rb;
         for (uint32_t c = 0; c < cpb && 64u * c < L && ok; c++) {
             while (r < rb1 && pfirst[r] < 64u * c) r++;
-            ok = cbase[b * cpb + c] == r - rb;
+            ok = cbase[b * 256 + (c % 16) * 16 + c / 16] == r - rb;
         }
     }
     if (!ok) atomicOr(bad, 1u);
@@ -1133,7 +1174,9 @@ __global__ void k_win_pack(const uint32_t* __restrict__ keys, const uint32_t* __
     }
 }
 
-// first run of each block (nb + 1 entries) and each chunk's run count
+// first run of each block (nb + 1 entries) and each chunk's run count, at
+// cbase[b 256 + (c % 16) 16 + c / 16]: the run counts of wave w's chunks
+// (c = w, w + 16, ...) are 32 contiguous bytes
 __global__ void k_win_rbase(const uint32_t* __restrict__ lbase, const uint32_t* __restrict__ ridx, uint32_t nb,
                             uint32_t cpb, uint32_t* __restrict__ rbase, uint16_t* __restrict__ cbase) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1145,7 +1188,7 @@ __global__ void k_win_rbase(const uint32_t* __restrict__ lbase, const uint32_t* 
     }
     const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
     if (c == 0) rbase[b] = ridx[lb];
-    cbase[t] = (uint16_t)(64u * c < L ? ridx[lb + 64u * c] - ridx[lb] : 0u);
+    cbase[(uint64_t)b * 256 + (c % 16) * 16 + c / 16] = (uint16_t)(64u * c < L ? ridx[lb + 64u * c] - ridx[lb] : 0u);
 }
 
 // heavy chunk lists: plan position and chunk slot per entry
@@ -1726,7 +1769,7 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t wsz, uint64_t n, uint32_t t
     RC(E.tpk.ensure(nl * 2 + 8, st));
     RC(E.twt.ensure(nl * 4 + 8, st));
     RC(E.rbase.ensure(((uint64_t)nb + 1) * 4 + 8, st));
-    RC(E.cbase.ensure((uint64_t)nb * CPB * 2 + 8, st));
+    RC(E.cbase.ensure((uint64_t)nb * 256 * 2 + 64, st));
     const float* w4 = E.bytes == 4 ? E.st.as<float>() : nullptr;
     {
         uint32_t* ridx = (uint32_t*)s.scratch.take((nl + 1) * 4);
@@ -1755,6 +1798,8 @@ int build_gather_plan(Sim& s, EdgeStore& E, uint32_t wsz, uint64_t n, uint32_t t
                                E.twt.as<float>(), E.prun.as<uint32_t>(), E.pfirst.as<uint16_t>());
         } else {
             fill_u32(ridx, 0, 1, st);
+            RC(E.prun.ensure(8, st));
+            RC(E.pfirst.ensure(8, st));
         }
         hipLaunchKernelGGL(k_win_rbase, dim3(grid_for(((uint64_t)nb + 1) * CPB, 256)), dim3(256), 0, st,
                            E.lbase.as<uint32_t>(), ridx, nb, CPB, E.rbase.as<uint32_t>(), E.cbase.as<uint16_t>());

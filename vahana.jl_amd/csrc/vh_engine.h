@@ -82,7 +82,7 @@ struct EdgeStore {
     DevBuf ablk, lbase;            // per block: first agent; first light-list entry
     DevBuf tpk, twt;               // light lists: slot | run start, weight
     DevBuf rbase, prun, pfirst;    // runs: first run per block; plan position, list index in the block
-    DevBuf cbase;                  // run starts before each 64-entry chunk of a block's list
+    DevBuf cbase;                  // run starts before each 64-entry chunk of a block's list (by wave)
     DevBuf wq;                     // pass-2 queue: next block, finished launch blocks
     DevBuf huge;                   // rows of in-degree above the plan's staging limit
     DevBuf hbase, hty, hslot;      // their chunk lists: first entry per row; plan position, slot

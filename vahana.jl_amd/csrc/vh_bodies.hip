@@ -617,7 +617,6 @@ struct WinCfg {
 };
 constexpr uint32_t WIN_RUNS = 1024;   // windows at most (runs of a block)
 constexpr uint32_t WIN_HMAX = 1024;   // in-degrees staged within blocks (larger: chunk lists)
-constexpr uint32_t WIN_MED = 256;     // agents of in-degree 65..WIN_HMAX per block at most
 constexpr uint32_t WIN_SLOT = 0x3FFFu, WIN_START = 0x8000u;
 
 // A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
@@ -780,13 +779,13 @@ template <class T>
 struct WinLDS {
     T val[WinCfg<T>::TB];          // values by slot (heavy waves: CHUNK each)
     float wgt[WinCfg<T>::TB];      // weights by slot
-    uint32_t rp[WIN_RUNS];         // runs of the block: plan position, list index
-    uint16_t rf[WIN_RUNS];
+    uint32_t rp[2][WIN_RUNS];      // runs of this block and the next: plan position, list index
+    uint16_t rf[2][WIN_RUNS];
     uint32_t wsum[WinCfg<T>::TB / 1024 + 1][16];
-    uint32_t next;                 // the queue item taken
-    uint32_t nmed;                 // agents of in-degree 65..WIN_HMAX of the block
-    uint32_t mslot[WIN_MED], mdeg[WIN_MED], magent[WIN_MED];
-    T mx[WIN_MED];
+    uint32_t nextq[2];             // queue items taken ahead
+    uint32_t nmed;                 // agents of in-degree 65..WIN_HMAX of the block (< TB / 64)
+    uint32_t mpk[WinCfg<T>::TB / 64], magent[WinCfg<T>::TB / 64];   // slot | in-degree << 16
+    T mx[WinCfg<T>::TB / 64];
 };
 
 // The round-0 and round-1 agents of a block (a0 + tid, a0 + 1024 + tid):
@@ -890,9 +889,8 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
             for (; q < d; q++) op_accum(sm.val[P + q], sm.wgt[P + q], xi, eps, num, den);
             __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
         } else if (i < a1 && d <= WIN_HMAX) {
-            const uint32_t m = atomicAdd(&sm.nmed, 1u);   // < WIN_MED (each costs > 64 of < TB)
-            sm.mslot[m] = P;
-            sm.mdeg[m] = d;
+            const uint32_t m = atomicAdd(&sm.nmed, 1u);   // < TB / 64 (each costs > 65 of < TB)
+            sm.mpk[m] = P | d << 16;
             sm.magent[m] = i;
             sm.mx[m] = xi;
         }
@@ -901,7 +899,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
     __syncthreads();
     const uint32_t nmed = sm.nmed;
     for (uint32_t m = (uint32_t)wv; m < nmed; m += 16) {
-        const uint32_t P = sm.mslot[m], d = sm.mdeg[m];
+        const uint32_t P = sm.mpk[m] & 0xFFFFu, d = sm.mpk[m] >> 16;
         const T xi = sm.mx[m];
         T num = (T)0, den = (T)0;
         for (uint32_t q = (uint32_t)lane; q < d; q += 64) op_accum(sm.val[P + q], sm.wgt[P + q], xi, eps, num, den);
@@ -968,26 +966,29 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
         a = prun[r];
         f = pfirst[r];
     };
-    if (tid == 0) sm.next = atomicAdd(queue, 1u);
+    if (tid == 0) sm.nextq[0] = atomicAdd(queue, 1u);
     __syncthreads();
-    uint32_t b = __builtin_amdgcn_readfirstlane(sm.next);
+    uint32_t b = __builtin_amdgcn_readfirstlane(sm.nextq[0]);
     __syncthreads();
     if (b < nb) {
         uint32_t pk[QU];
         WinAgents<T> ag;
+        int cur = 0, par = 1;
         {
             uint32_t ra, rfv;
             load_runs(b, ra, rfv);
             load_list(b, pk);
             ag = win_agents<T>(off, x, nrows, ablk[b], ablk[b + 1], ltid);
-            sm.rp[tid] = ra;
-            sm.rf[tid] = (uint16_t)rfv;
-            if (tid == 0) sm.next = atomicAdd(queue, 1u);
+            sm.rp[0][tid] = ra;
+            sm.rf[0][tid] = (uint16_t)rfv;
+            if (tid == 0) sm.nextq[1] = atomicAdd(queue, 1u);
             __syncthreads();
         }
+        // per block two barriers: the scan's (publishing the staged values,
+        // the next block's runs and the queue item after it) and the end's
         for (;;) {
             asm volatile("" : "+v"(ltid));
-            const uint32_t it = __builtin_amdgcn_readfirstlane(sm.next);
+            const uint32_t it = __builtin_amdgcn_readfirstlane(sm.nextq[par]);
             const uint32_t bn = it < nb ? it : NONE;
             const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
             // the wave's 16 chunk run counts: one scalar 32-byte load
@@ -1004,7 +1005,7 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
                 const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
                 const uint32_t cb = (cwv[u >> 1] >> ((u & 1) * 16)) & 0xFFFFu;
                 const uint32_t run = ok ? cb + __popcll(m & lemask) - 1u : 0u;
-                const uint32_t pos = ok ? sm.rp[run] + (e - sm.rf[run]) : 0u;
+                const uint32_t pos = ok ? sm.rp[cur][run] + (e - sm.rf[cur][run]) : 0u;
                 v[u] = y[pos];
                 wt[u] = __builtin_nontemporal_load(twt + lb + (ok ? e : 0u));
             }
@@ -1022,17 +1023,20 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
                     sm.val[pk[u] & WIN_SLOT] = v[u];
                     sm.wgt[pk[u] & WIN_SLOT] = wt[u];
                 }
-            win_sum_agents<T>(sm, off, x, xo, nrows, eps, ablk[b], ablk[b + 1], ag);   // (its barrier publishes the values)
+            if (bn != NONE) {
+                sm.rp[cur ^ 1][tid] = ra;
+                sm.rf[cur ^ 1][tid] = (uint16_t)rfv;
+                if (tid == 0) sm.nextq[par ^ 1] = atomicAdd(queue, 1u);
+            }
+            win_sum_agents<T>(sm, off, x, xo, nrows, eps, ablk[b], ablk[b + 1], ag);   // (its barrier publishes)
             __syncthreads();
             if (bn == NONE) break;
-            sm.rp[tid] = ra;
-            sm.rf[tid] = (uint16_t)rfv;
-            if (tid == 0) sm.next = atomicAdd(queue, 1u);
-            __syncthreads();
             b = bn;
 #pragma unroll
             for (uint32_t u = 0; u < QU; u++) pk[u] = pkn[u];
             ag = agn;
+            cur ^= 1;
+            par ^= 1;
         }
     }
     if (tid == 0 && atomicAdd(queue + 1, 1u) == gridDim.x - 1) {

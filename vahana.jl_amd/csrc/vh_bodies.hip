@@ -650,9 +650,9 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
             jj[u] = 0u;
             sl[u] = 0xFFFFFFFFu;
             if (q < cnt) {
-                wq[u] = __builtin_nontemporal_load(w + b + c0 + q);
-                jj[u] = __builtin_nontemporal_load(hty + hb + c0 + q);
-                sl[u] = __builtin_nontemporal_load(hslot + hb + c0 + q);
+                wq[u] = w[b + c0 + q];
+                jj[u] = hty[hb + c0 + q];
+                sl[u] = hslot[hb + c0 + q];
             }
         }
 #pragma unroll
@@ -755,14 +755,14 @@ __global__ __launch_bounds__(1024) void k_win_gather(const uint16_t* __restrict_
                 const uint64_t ga = g0 + 256ull * (wv + 16 * u) + 2 * lane, gb = ga + 128;
                 if (f0[u]) {
                     const T2 v = {win[p0[u] & 0xFFFFu], win[p0[u] >> 16]};
-                    __builtin_nontemporal_store(v, (T2*)(y + ga));
+                    *(T2*)(y + ga) = v;
                 } else {
                     for (uint64_t t = ga; t < ga + 2; t++)
                         if (t >= j && t < je) y[t] = win[psl[t]];
                 }
                 if (f1[u]) {
                     const T2 v = {win[p1[u] & 0xFFFFu], win[p1[u] >> 16]};
-                    __builtin_nontemporal_store(v, (T2*)(y + gb));
+                    *(T2*)(y + gb) = v;
                 } else {
                     for (uint64_t t = gb; t < gb + 2; t++)
                         if (t >= j && t < je) y[t] = win[psl[t]];
@@ -887,7 +887,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
                 op_accum(y3, w3, xi, eps, num, den);
             }
             for (; q < d; q++) op_accum(sm.val[P + q], sm.wgt[P + q], xi, eps, num, den);
-            __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
+            xo[i] = den > (T)0 ? num / den : xi;
         } else if (i < a1 && d <= WIN_HMAX) {
             const uint32_t m = atomicAdd(&sm.nmed, 1u);   // < TB / 64 (each costs > 65 of < TB)
             sm.mpk[m] = P | d << 16;
@@ -957,7 +957,7 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
 #pragma unroll
         for (uint32_t u = 0; u < QU; u++) {
             const uint32_t e = ltid + 1024 * u;
-            pk[u] = __builtin_nontemporal_load(tpk + lb + (e < L ? e : 0u));   // (past the list: masked where used)
+            pk[u] = tpk[lb + (e < L ? e : 0u)];   // (past the list: masked where used)
         }
     };
     auto load_runs = [&](uint32_t b, uint32_t& a, uint32_t& f) __attribute__((always_inline)) {
@@ -1007,7 +1007,7 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
                 const uint32_t run = ok ? cb + __popcll(m & lemask) - 1u : 0u;
                 const uint32_t pos = ok ? sm.rp[cur][run] + (e - sm.rf[cur][run]) : 0u;
                 v[u] = y[pos];
-                wt[u] = __builtin_nontemporal_load(twt + lb + (ok ? e : 0u));
+                wt[u] = twt[lb + (ok ? e : 0u)];
             }
             uint32_t pkn[QU];
             uint32_t ra = 0, rfv = 0;

@@ -88,14 +88,22 @@ class Server:
     """`solo_agents`: SIR requests with at least this many agents run in an
     ensemble of their own (their `quantum` steps back to back, so that their
     graph stays in the 256 MB Infinity Cache from step to step); smaller ones
-    share one ensemble whose launches advance all of them at once."""
+    share one ensemble whose launches advance all of them at once.
+    `large_concurrency`: how many large requests (those with kernels of
+    their own: solo SIR, opinion through the two-pass plan) advance in one
+    quantum, oldest admitted first (None: all). The small ones always all
+    advance. Large requests take nearly all GPU time, so sharing it among
+    them only delays each; in admission order they finish as early as one at
+    a time, while the small ones no longer wait behind them."""
 
-    def __init__(self, vh, hbm_budget, quantum=10, keep_results=False, solo_agents=1_000_000):
+    def __init__(self, vh, hbm_budget, quantum=10, keep_results=False, solo_agents=1_000_000,
+                 large_concurrency=1):
         self.vh = vh
         self.budget = int(hbm_budget)
         self.quantum = quantum
         self.keep = keep_results
         self.solo_agents = solo_agents
+        self.large_concurrency = large_concurrency
         # Two streams in all: every step launch goes to `compute`, every
         # graph build and result read to `io`. Simulations do not get streams
         # of their own: with a handful of hardware queues per process, many
@@ -146,9 +154,15 @@ class Server:
         self.stats = ph
         q = self.quantum
         while qi < len(pending) or active or finished:
-            # (1) one quantum for every active request
+            # (1) one quantum for every active small request and the oldest
+            # large ones
             ta = time.perf_counter()
             stepping = list(active.values())
+            if self.large_concurrency is not None:
+                large = sorted((r for r in stepping if self._large(r)), key=lambda r: (r.admitted, r.id))
+                if len(large) > self.large_concurrency:
+                    held = set(id(r) for r in large[self.large_concurrency:])
+                    stepping = [r for r in stepping if id(r) not in held]
             if any(r.kind == "sir" and r.ens is self.ens for r in stepping):
                 self.ens.step(q, use_graph=True)
             for r in stepping:
@@ -237,6 +251,10 @@ class Server:
                     ph["idle"] += min(wait, 0.05)
         wall = time.perf_counter() - t0
         return done, wall
+
+    def _large(self, r):
+        return ((r.kind == "sir" and r.ens is not self.ens) or
+                (r.kind == "opinion" and r.n * 8 >= OPINION_PLAN_MIN_BYTES))
 
     def _build(self, vh, r):
         """Graph and state of a request on the device; on failure nothing of

@@ -452,10 +452,10 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
     uint32_t b = 0, d = 0;
     T xi = (T)0;
     if (i < n) {
-        xi = __builtin_nontemporal_load(x + i);
+        xi = (*(x + i));
         if (i < nrows) {
-            b = __builtin_nontemporal_load(off + rowbase + i);
-            d = __builtin_nontemporal_load(off + rowbase + i + 1) - b;
+            b = (*(off + rowbase + i));
+            d = (*(off + rowbase + i + 1)) - b;
         }
     }
     const bool light = i < n && d <= 64u;
@@ -484,8 +484,8 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
             }
 #pragma unroll
             for (int u = 0; u < OP_U; u++) {
-                sj[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(src + kk[u]) : 1u;
-                wk[u] = kk[u] != 0xFFFFFFFFu ? __builtin_nontemporal_load(w + kk[u]) : 0.0f;
+                sj[u] = kk[u] != 0xFFFFFFFFu ? (*(src + kk[u])) : 1u;
+                wk[u] = kk[u] != 0xFFFFFFFFu ? (*(w + kk[u])) : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < OP_U; u++) xj[u] = kk[u] != 0xFFFFFFFFu ? x[sj[u] - 1] : (T)0;
@@ -502,7 +502,7 @@ __device__ __forceinline__ void opinion_block(const uint32_t* __restrict__ off,
         for (uint32_t p = lo; p < hi; p++) op_accum(s_x[p - t0], s_w[p - t0], xi, eps, num, den);
         __syncthreads();
     }
-    if (light) __builtin_nontemporal_store(den > (T)0 ? num / den : xi, xo + i);
+    if (light) *(xo + i) = den > (T)0 ? num / den : xi;
 }
 
 template <class T>

@@ -604,9 +604,10 @@ __global__ __launch_bounds__(256) void k_opinion_multi(const OpMember* __restric
 // weights are staged in LDS by slot; then each agent of in-degree <= 64 adds
 // its slots in order, and a wave each agent of in-degree 65..WIN_HMAX (lane l
 // its slots l, l + 64, ..., then the xor butterfly). Agents above WIN_HMAX go
-// first, one wave each, through their chunk lists. The values are exactly
-// x[src[k]] and the sums keep the canonical order, so the results are the
-// one-pass kernel's.
+// first, one wave each, through their chunk lists. The next block's list,
+// runs and agents load while a block's values are in flight. The values are
+// exactly x[src[k]] and the sums keep the canonical order, so the results
+// are the one-pass kernel's.
 template <class T>
 struct WinCfg {
     static constexpr uint32_t WSZ = 163840 / sizeof(T);             // states per window at most
@@ -622,7 +623,7 @@ constexpr uint32_t WIN_SLOT = 0x3FFFu, WIN_START = 0x8000u;
 // A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
 // h are numbered from hbase[h] in CSR order, in chunks of CHUNK slots, each
 // chunk listed in plan order (hty: plan position, hslot: slot in the chunk).
-// The wave copies a chunk's values into its LDS quarter `buf` by slot, then
+// The wave copies a chunk's values into its LDS region `buf` by slot, then
 // lane l adds slots l, l + 64, ... in order: the canonical lane-strided
 // partials, then the xor butterfly.
 template <class T>

@@ -121,6 +121,32 @@ def test_opinion_parity_hub(vh, ora, gpu_ready, f32, mode, opts):
         assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
 
 
+@pytest.mark.parametrize("mode", ["plan", "plan_auto"])
+@pytest.mark.parametrize("f32", [False, True])
+def test_opinion_parity_isolated(vh, ora, gpu_ready, f32, mode, opts):
+    """Agents without in-edges: a 3000-agent Barabasi-Albert graph inside
+    40000 agents, so that the gather plan's blocks of isolated agents hold
+    up to ~12000 agents (many rounds of the block scan) and no staged edges
+    (no runs), next to blocks with both."""
+    plan_bytes, part_bytes = GATHER_MODES[mode]
+    opts(opinion_plan_bytes=plan_bytes, opinion_part_bytes=part_bytes)
+    print(f"[gpu] opinion parity with isolated agents f32={f32} gather={mode}", flush=True)
+    M = vh.models
+    n = 40_000
+    a, b = vh.inputs.graph_ba(3000, 5, 8)
+    edges = vh.inputs.symmetric_weighted(a + 17_000, b + 17_000, 9)   # agents 17001..20000
+    deg = np.bincount(edges[1], minlength=n + 1)
+    assert (deg[1:] == 0).sum() > 30_000
+    g = M.build_opinion(vh, n, seed=8, f32=f32, edges=edges)
+    c = M.build_opinion(ora, n, seed=8, f32=f32, edges=edges)
+    for step in range(4):
+        M.step_opinion(vh, g, eps=0.2)
+        M.step_opinion(ora, c, eps=0.2)
+        x, _ = _agents(g)
+        y, _ = _agents(c)
+        assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
+
+
 @pytest.mark.parametrize("f32", [False, True])
 def test_opinion_plan_after_edge_change(vh, ora, gpu_ready, f32, opts):
     """The two-pass plan (its lists and the edge weights copied into list

@@ -341,6 +341,14 @@ int vh_batch_apply(vh_batch_t b, const vh_sim_t* sims, uint32_t n, const vh_appl
                    const uint32_t* steps);
 /* wait for the simulation's stream */
 int vh_sync(vh_sim_t s);
+/* build, on the simulation's stream and waiting for it, what the first
+ * apply! of the opinion transition (VH_TR_OPINION_F64, or _F32 with f32 != 0)
+ * of agent type t over edge type et would build: its degree bins and, for a
+ * state that takes the two-pass gather, its gather plan (no reference
+ * counterpart: for servers that build simulations on one stream and step
+ * them on another, so that the plan's host round trips stay off the step
+ * stream). Kept until the edges change; no effect on results. */
+int vh_opinion_prepare(vh_sim_t s, uint8_t t, uint8_t et, int f32);
 /* attach an external hipStream_t (NULL: the library's own stream) */
 int vh_set_stream(vh_sim_t s, void* hip_stream);
 /* a non-blocking hipStream_t at default priority for vh_set_stream,

@@ -148,6 +148,31 @@ def test_opinion_parity_isolated(vh, ora, gpu_ready, f32, mode, opts):
 
 
 @pytest.mark.parametrize("f32", [False, True])
+def test_opinion_prepare(vh, ora, gpu_ready, f32, opts):
+    """vh_opinion_prepare builds the plan ahead of the first apply! (twice in
+    a row: the second keeps it); the steps after it match the checker's, and
+    a state type that does not match the transition's is refused."""
+    opts(opinion_plan_bytes=0, opinion_part_bytes=2048)
+    print(f"[gpu] opinion prepare f32={f32}", flush=True)
+    M = vh.models
+    n = 4000
+    a, b = vh.inputs.graph_ba(n, 5, 8)
+    edges = vh.inputs.symmetric_weighted(a, b, 9)
+    g = M.build_opinion(vh, n, seed=8, f32=f32, edges=edges)
+    c = M.build_opinion(ora, n, seed=8, f32=f32, edges=edges)
+    with pytest.raises(vh.VahanaError):
+        g.sim._opinion_prepare(g.T, g.E, not f32)
+    g.sim._opinion_prepare(g.T, g.E, f32)
+    g.sim._opinion_prepare(g.T, g.E, f32)
+    for step in range(3):
+        M.step_opinion(vh, g, eps=0.2)
+        M.step_opinion(ora, c, eps=0.2)
+        x, _ = _agents(g)
+        y, _ = _agents(c)
+        assert np.array_equal(_bytes(x), _bytes(y)), f"step {step}"
+
+
+@pytest.mark.parametrize("f32", [False, True])
 def test_opinion_plan_after_edge_change(vh, ora, gpu_ready, f32, opts):
     """The two-pass plan (its lists and the edge weights copied into list
     order) follows a change of the edge store: after three steps every agent

@@ -503,6 +503,12 @@ class Simulation:
         _check(self.lib.apply_n(self.handle, C.byref(d), steps), self.lib)
         return self
 
+    def _opinion_prepare(self, T, E, f32=False):
+        """Build what the first opinion apply! of agent type T over edge type
+        E needs (degree bins, the two-pass gather plan) now, on the
+        simulation's stream (vh_opinion_prepare)."""
+        _check(self.lib.opinion_prepare(self.handle, self._aid(T), self._eid(E), 1 if f32 else 0), self.lib)
+
     def _set_streams(self, stream, io_stream=None):
         """Run apply! on `stream` (a raw hipStream_t, None: own stream) and
         reads on `io_stream` (vh_set_io_stream)."""

@@ -174,6 +174,7 @@ DEVICE_ONLY = [
                                     C.POINTER(C.c_int64), C.c_double, C.c_int, C.c_int,
                                     C.POINTER(C.c_uint8)]),
     ("sync", C.c_int, [_vp]),
+    ("opinion_prepare", C.c_int, [_vp, C.c_uint8, C.c_uint8, C.c_int]),
     ("set_stream", C.c_int, [_vp, _vp]),
     ("set_io_stream", C.c_int, [_vp, _vp]),
     ("stream_create", C.c_int, [C.POINTER(_vp)]),

@@ -1923,26 +1923,23 @@ static uint32_t device_cus() {
     return cus[dev];
 }
 
-int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step) {
-    (void)step;
-    RC(check_model_layout(s, t, et, f32 ? 4 : 8, 4, "opinion"));
+// the degree bins and, when the step takes the two-pass gather, its plan and
+// gathered-value buffer for an opinion step of agent type t over edge type et
+// (kept while the edges and the state type stay the same); `plan` tells which
+static int opinion_plan(Sim& s, int t, int et, bool f32, bool& plan) {
     AgentStore& A = s.a[t];
     EdgeStore& E = s.e[et];
-    if (!A.writeable) return set_error(VH_ERR_ASSERT, "opinion: the agent type must be in `write`");
-    if (!A.nslots) return VH_OK;
     const uint32_t tb = f32 ? 4 : 8;
     const uint64_t nrows = E.rows ? E.nrows[t] : 0;
     if (E.rows) RC(build_heavy_bins(s, E, t));
-    const uint64_t nheavy = E.rows ? E.nheavy : 0;
-    const uint32_t nlight = grid_for(A.nslots, 256);
     // two-pass gather through a plan from a state vector of at least
     // opinion_plan_bytes (< 0: never), with source windows of
     // opinion_part_bytes (0 or more than 160 KB: 160 KB; widened to make at
     // most WIN_RUNS windows, a multiple of 4 sources)
     const int64_t plan_min = g_opts.opinion_plan_bytes;
-    bool plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
-                E.rowbase[t] == 0 && nrows == E.rows && A.nslots < 0xFFFFFFFFull &&
-                (uint64_t)A.nslots * tb >= (uint64_t)plan_min;
+    plan = plan_min >= 0 && E.rows && E.nedges && !E.src64 && E.src_type == (uint32_t)t &&
+           E.rowbase[t] == 0 && nrows == E.rows && A.nslots < 0xFFFFFFFFull &&
+           (uint64_t)A.nslots * tb >= (uint64_t)plan_min;
     if (plan) {
         const uint64_t cap = f32 ? WinCfg<float>::WSZ : WinCfg<double>::WSZ;
         uint64_t wsz = g_opts.opinion_part_bytes <= 0 ? cap : (uint64_t)g_opts.opinion_part_bytes / tb;
@@ -1956,6 +1953,22 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
             plan = E.plan_ok;
         }
     }
+    if (plan) RC(E.pval.ensure(E.nedges * tb + 16, s.stream));
+    return VH_OK;
+}
+
+int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, uint64_t step) {
+    (void)step;
+    RC(check_model_layout(s, t, et, f32 ? 4 : 8, 4, "opinion"));
+    AgentStore& A = s.a[t];
+    EdgeStore& E = s.e[et];
+    if (!A.writeable) return set_error(VH_ERR_ASSERT, "opinion: the agent type must be in `write`");
+    if (!A.nslots) return VH_OK;
+    const uint64_t nrows = E.rows ? E.nrows[t] : 0;
+    bool plan = false;
+    RC(opinion_plan(s, t, et, f32, plan));
+    const uint64_t nheavy = E.rows ? E.nheavy : 0;
+    const uint32_t nlight = grid_for(A.nslots, 256);
     const uint32_t nh = (uint32_t)((nheavy + 3) / 4);
     if (!plan && g_opinion_defer && g_opinion_defer->stream == s.stream) {
         OpMember om;
@@ -1988,7 +2001,6 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
         HIPCHECK(hipGetLastError());
         return VH_OK;
     }
-    RC(E.pval.ensure(E.nedges * tb + 16, s.stream));
     const dim3 g(device_cus()), bb(1024);
     if (f32) {
         hipLaunchKernelGGL(k_win_gather<float>, g, bb, 0, s.stream, E.psrc.as<uint16_t>(), (const float*)A.rstate(),
@@ -2011,6 +2023,25 @@ int launch_opinion(Sim& s, int t, int et, const vh_opinion_params& p, bool f32, 
                            E.cbase.as<uint16_t>(), E.plan_nb, E.wq.as<uint32_t>());
     }
     HIPCHECK(hipGetLastError());
+    return VH_OK;
+}
+
+// the plan of an opinion step built ahead of its first apply!, on the
+// simulation's stream, which the call waits for; the build's scratch is
+// released when large
+extern "C" int vh_opinion_prepare(vh_sim_t h, uint8_t t, uint8_t et, int f32) {
+    if (!h) return set_error(VH_ERR_ARG, "null simulation");
+    Sim& s = h->s;
+    if (!s.initialized) return set_error(VH_ERR_ASSERT, "opinion_prepare after finish_init! only");
+    if (t < 1 || t > s.m->na) return set_error(VH_ERR_UNREGISTERED, "unregistered agent type");
+    if (et >= s.m->ne) return set_error(VH_ERR_UNREGISTERED, "unregistered edge type");
+    RC(check_model_layout(s, t, et, f32 ? 4 : 8, 4, "opinion"));
+    if (!s.a[t].nslots) return VH_OK;
+    s.scratch.reset(s.stream);
+    bool plan = false;
+    RC(opinion_plan(s, t, et, f32 != 0, plan));
+    s.scratch.reset(s.stream);
+    HIPCHECK(hipStreamSynchronize(s.stream));
     return VH_OK;
 }
 

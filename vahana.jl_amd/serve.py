@@ -277,6 +277,9 @@ class Server:
             m = None
             try:
                 m = _build_opinion(vh, r, stream=self.io, keep=lambda mm: setattr(r, "_partial", mm))
+                # the gather plan's host round trips on the build stream,
+                # not on the step stream the other requests share
+                m.sim._opinion_prepare(m.T, m.E, m.f32)
                 m.sim._set_streams(self.compute, self.io)
             except Exception:
                 part = getattr(r, "_partial", None)

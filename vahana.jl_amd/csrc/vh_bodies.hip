@@ -604,8 +604,9 @@ __global__ __launch_bounds__(256) void k_opinion_multi(const OpMember* __restric
 // weights are staged in LDS by slot; then each agent of in-degree <= 64 adds
 // its slots in order, and a wave each agent of in-degree 65..WIN_HMAX (lane l
 // its slots l, l + 64, ..., then the xor butterfly). Agents above WIN_HMAX go
-// first, one wave each, through their chunk lists. The next block's list,
-// runs and agents load while a block's values are in flight. The values are
+// first, one wave each, through their chunk lists. Most of the next block's
+// values, and its list, runs and agents, load while a block is summed. The
+// values are
 // exactly x[src[k]] and the sums keep the canonical order, so the results
 // are the one-pass kernel's.
 template <class T>
@@ -613,6 +614,7 @@ struct WinCfg {
     static constexpr uint32_t WSZ = 163840 / sizeof(T);             // states per window at most
     static constexpr uint32_t TB = sizeof(T) == 8 ? 12288 : 16384;  // staged slots per block at most
     static constexpr uint32_t QU = TB / 1024;                       // list entries per thread
+    static constexpr uint32_t QA = sizeof(T) == 8 ? 8 : 12;         // of them, loaded during the previous sums
     static constexpr uint32_t CPB = TB / 64;                        // 64-entry chunks per block
     static constexpr uint32_t CHUNK = sizeof(T) == 8 ? 256 : 384;   // heavy-row slots staged per wave
 };
@@ -818,11 +820,13 @@ __device__ __forceinline__ WinAgents<T> win_agents(const uint32_t* __restrict__ 
 // also publishes the staged values); each agent of in-degree <= 64 adds its
 // slots in order, the others of the block are listed and summed by a wave
 // each (lane-strided partials, xor butterfly). Rounds 0 and 1 use the agents
-// loaded early (g), later rounds load theirs.
-template <class T>
+// loaded early (g); later rounds load theirs when LD (without LD the block
+// has at most two rounds, and no global load is issued). after_scan() runs
+// right after the scan's barrier.
+template <class T, bool LD, class F>
 __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __restrict__ off,
                                                const T* __restrict__ x, T* __restrict__ xo, uint64_t nrows, T eps,
-                                               uint32_t a0, uint32_t a1, const WinAgents<T>& g) {
+                                               uint32_t a0, uint32_t a1, const WinAgents<T>& g, F after_scan) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t K = (a1 - a0 + 1023) / 1024;
@@ -830,7 +834,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
     for (uint32_t r = 0; r < K; r++) {
         const uint32_t i = a0 + r * 1024 + tid;
         uint32_t d = 0;
-        if (i < a1 && i < nrows) d = r == 0 ? g.o1 - g.o0 : r == 1 ? g.o3 - g.o2 : off[i + 1] - off[i];
+        if (i < a1 && i < nrows) d = r == 0 ? g.o1 - g.o0 : r == 1 ? g.o3 - g.o2 : LD ? off[i + 1] - off[i] : 0u;
         const uint32_t ld = d <= WIN_HMAX ? d : 0u;
         uint32_t incl = ld;
 #pragma unroll
@@ -846,6 +850,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
     }
     if (tid == 0) sm.nmed = 0;
     __syncthreads();
+    after_scan();
     uint32_t base = 0;
     for (uint32_t r = 0; r < K; r++) {
         const uint32_t i = a0 + r * 1024 + tid;
@@ -858,7 +863,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
         }
         uint32_t d, incl, ld;
         T xi;
-        if (r < 2) {
+        if (!LD || r < 2) {
             d = (i < a1 && i < nrows) ? (r == 0 ? g.o1 - g.o0 : g.o3 - g.o2) : 0u;
             incl = inclk[r];
             ld = ldk[r];
@@ -916,10 +921,14 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
 // Pass 2: one block of 1024 per CU. The waves first take the heavy agents
 // (wave g of the grid agents g, g + 16 G, ...), then the blocks take light
 // blocks from a queue (queue[0]: next light block, queue[1]: blocks done;
-// the last block to finish sets both to 0 for the next launch). Measured
-// slower at 10^7 agents (0.865 ms per step here): pipelining the phases of
-// consecutive light blocks (0.916 ms), heavy agents in queued groups of 16
-// longest first with 512-slot chunks (0.890 ms).
+// the last block to finish sets both to 0 for the next launch). Each thread
+// stages QU list entries of a block; the values of its first QA entries load
+// while the previous block is summed (in registers: the block's LDS is in
+// use), the other QU - QA after the sums. Measured slower at 10^7 agents:
+// all QU loaded during the sums (past the 128 VGPRs of a 1024-thread block:
+// spills), 512 threads with 256 VGPRs and all QU during the sums, two
+// half-size blocks in LDS, each agent's first slot from the plan instead of
+// the block scan; QA = 6 or 10 of 12 (FP64) was no faster than 8.
 template <class T>
 __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ off, const T* __restrict__ x,
                                                   const T* __restrict__ y, const float* __restrict__ w,
@@ -935,26 +944,24 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
                                                   const uint16_t* __restrict__ pfirst,
                                                   const uint16_t* __restrict__ cbase, uint32_t nb,
                                                   uint32_t* __restrict__ queue) {
-    constexpr uint32_t QU = WinCfg<T>::QU, CPB = WinCfg<T>::CPB;
+    constexpr uint32_t QU = WinCfg<T>::QU, QA = WinCfg<T>::QA, QB = QU - QA;
     __shared__ __align__(16) unsigned char smraw[sizeof(WinLDS<T>)];
     WinLDS<T>& sm = *reinterpret_cast<WinLDS<T>*>(smraw);
     const uint32_t tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t lemask = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    // heavy agents: wave g of the grid takes heavy agents g, g + 16 G, ...
     for (uint64_t h = (uint64_t)blockIdx.x * 16 + (uint32_t)wv; h < nheavy; h += (uint64_t)gridDim.x * 16)
         opinion_heavy_plan<T>(x, y, w, xo, hinfo[h], eps, hty, hslot, sm.val + wv * WinCfg<T>::CHUNK);
-    __syncthreads();
-    // light blocks: while block b's values are in flight and summed, the
-    // list, runs and agents of the next block are loading (loads complete in
-    // issue order, so those are issued after b's values; every load is
-    // branch-free, a load under a branch makes the compiler wait for all
-    // loads in flight where the branch joins)
+    // every load is branch-free (lanes past a list read a valid dummy, an
+    // inactive list reads as empty): a load under a branch makes the compiler
+    // wait for all loads in flight where the branch joins; and loads complete
+    // in issue order, so the sums issue no global load (blocks of at most two
+    // rounds of agents, the others take a path that does)
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     uint32_t ltid = tid;
-    auto load_list = [&](uint32_t b, uint32_t* pk) __attribute__((always_inline)) {
-        const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
+    auto load_list = [&](uint32_t b, bool act, uint32_t* pk) __attribute__((always_inline)) {
+        const uint32_t lb = lbase[b], L = act ? lbase[b + 1] - lb : 0u;
 #pragma unroll
         for (uint32_t u = 0; u < QU; u++) {
             const uint32_t e = ltid + 1024 * u;
@@ -967,72 +974,105 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
         a = prun[r];
         f = pfirst[r];
     };
-    if (tid == 0) sm.nextq[0] = atomicAdd(queue, 1u);
+    // values and weights of entries u in [U0, U0 + QH) of block b (its runs in
+    // sm.rp[c]; the wave's chunk run counts by scalar loads)
+    auto load_vals = [&](auto NN, uint32_t b, bool act, uint32_t c, uint32_t U0, const uint32_t* pk, T* v,
+                         float* wt) __attribute__((always_inline)) {
+        constexpr uint32_t QH = decltype(NN)::value;
+        const uint32_t lb = lbase[b], L = act ? lbase[b + 1] - lb : 0u;
+        const uint32_t* cw = (const uint32_t*)(cbase + ((uint64_t)b * 256 + (uint32_t)wv * 16));
+        uint32_t cwv[QH / 2];
+#pragma unroll
+        for (uint32_t q = 0; q < QH / 2; q++) cwv[q] = cw[U0 / 2 + q];
+#pragma unroll
+        for (uint32_t k = 0; k < QH; k++) {
+            const uint32_t u = U0 + k;
+            const uint32_t e = ltid + 1024 * u;
+            const bool ok = e < L;
+            const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
+            const uint32_t cb = (cwv[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu;
+            const uint32_t run = ok ? cb + __popcll(m & lemask) - 1u : 0u;
+            const uint32_t pos = ok ? sm.rp[c][run] + (e - sm.rf[c][run]) : 0u;
+            v[k] = y[pos];
+            wt[k] = twt[lb + (ok ? e : 0u)];
+        }
+    };
+    auto scatter = [&](auto NN, uint32_t L, uint32_t U0, const uint32_t* pk, const T* v, const float* wt)
+                       __attribute__((always_inline)) {
+        constexpr uint32_t QH = decltype(NN)::value;
+#pragma unroll
+        for (uint32_t k = 0; k < QH; k++)
+            if (ltid + 1024 * (U0 + k) < L) {
+                sm.val[pk[U0 + k] & WIN_SLOT] = v[k];
+                sm.wgt[pk[U0 + k] & WIN_SLOT] = wt[k];
+            }
+    };
+    if (tid == 0) {
+        sm.nextq[0] = atomicAdd(queue, 1u);
+        sm.nextq[1] = atomicAdd(queue, 1u);
+    }
+    __syncthreads();   // (also ends the heavy agents' use of sm.val)
+    const uint32_t q0 = __builtin_amdgcn_readfirstlane(sm.nextq[0]), q1 = __builtin_amdgcn_readfirstlane(sm.nextq[1]);
+    uint32_t b = q0 < nb ? q0 : NONE, bn = q1 < nb ? q1 : NONE;
     __syncthreads();
-    uint32_t b = __builtin_amdgcn_readfirstlane(sm.nextq[0]);
-    __syncthreads();
-    if (b < nb) {
+    if (b != NONE) {
         uint32_t pk[QU];
+        T vA[QA];
+        float wA[QA];
         WinAgents<T> ag;
-        int cur = 0, par = 1;
+        int cur = 0, par = 0;
         {
             uint32_t ra, rfv;
             load_runs(b, ra, rfv);
-            load_list(b, pk);
+            load_list(b, true, pk);
             ag = win_agents<T>(off, x, nrows, ablk[b], ablk[b + 1], ltid);
             sm.rp[0][tid] = ra;
             sm.rf[0][tid] = (uint16_t)rfv;
-            if (tid == 0) sm.nextq[1] = atomicAdd(queue, 1u);
             __syncthreads();
+            load_vals(std::integral_constant<uint32_t, QA>(), b, true, 0, 0, pk, vA, wA);
         }
-        // per block two barriers: the scan's (publishing the staged values,
-        // the next block's runs and the queue item after it) and the end's
+        // per block: the rest of its values; the next block's list, runs and
+        // agents; all its values staged; the next queue item; the sums, whose
+        // scan barrier publishes the staged values and the next runs, after
+        // which the first QA values of the next block are issued (in flight
+        // during the sums); a barrier
         for (;;) {
             asm volatile("" : "+v"(ltid));
-            const uint32_t it = __builtin_amdgcn_readfirstlane(sm.nextq[par]);
-            const uint32_t bn = it < nb ? it : NONE;
-            const uint32_t lb = lbase[b], L = lbase[b + 1] - lb;
-            // the wave's 16 chunk run counts: one scalar 32-byte load
-            const uint32_t* cw = (const uint32_t*)(cbase + ((uint64_t)b * 256 + (uint32_t)wv * 16));
-            uint32_t cwv[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) cwv[q] = cw[q];
-            T v[QU];
-            float wt[QU];
-#pragma unroll
-            for (uint32_t u = 0; u < QU; u++) {
-                const uint32_t e = ltid + 1024 * u;
-                const bool ok = e < L;
-                const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
-                const uint32_t cb = (cwv[u >> 1] >> ((u & 1) * 16)) & 0xFFFFu;
-                const uint32_t run = ok ? cb + __popcll(m & lemask) - 1u : 0u;
-                const uint32_t pos = ok ? sm.rp[cur][run] + (e - sm.rf[cur][run]) : 0u;
-                v[u] = y[pos];
-                wt[u] = twt[lb + (ok ? e : 0u)];
-            }
+            const bool hasn = bn != NONE;
+            const uint32_t bnc = hasn ? bn : b;
+            const uint32_t L = lbase[b + 1] - lbase[b];
+            T vB[QB];
+            float wB[QB];
+            load_vals(std::integral_constant<uint32_t, QB>(), b, true, cur, QA, pk, vB, wB);
             uint32_t pkn[QU];
-            uint32_t ra = 0, rfv = 0;
-            WinAgents<T> agn = ag;
-            if (bn != NONE) {
-                load_list(bn, pkn);
-                load_runs(bn, ra, rfv);
-                agn = win_agents<T>(off, x, nrows, ablk[bn], ablk[bn + 1], ltid);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < QU; u++)
-                if (ltid + 1024 * u < L) {
-                    sm.val[pk[u] & WIN_SLOT] = v[u];
-                    sm.wgt[pk[u] & WIN_SLOT] = wt[u];
+            uint32_t ra, rfv;
+            load_list(bnc, hasn, pkn);
+            load_runs(bnc, ra, rfv);
+            const WinAgents<T> agn = win_agents<T>(off, x, nrows, ablk[bnc], ablk[bnc + 1], ltid);
+            scatter(std::integral_constant<uint32_t, QA>(), L, 0, pk, vA, wA);
+            scatter(std::integral_constant<uint32_t, QB>(), L, QA, pk, vB, wB);
+            sm.rp[cur ^ 1][tid] = ra;
+            sm.rf[cur ^ 1][tid] = (uint16_t)rfv;
+            if (tid == 0) sm.nextq[par] = atomicAdd(queue, 1u);
+            auto after_scan = [&]() __attribute__((always_inline)) {
+                load_vals(std::integral_constant<uint32_t, QA>(), bnc, hasn, cur ^ 1, 0, pkn, vA, wA);
+            };
+            {
+                // a block of at most two rounds sums without a global load,
+                // so the next block's values stay in flight
+                const uint32_t a0 = ablk[b], a1 = ablk[b + 1];
+                if (a1 - a0 <= 2048u)
+                    win_sum_agents<T, false>(sm, off, x, xo, nrows, eps, a0, a1, ag, after_scan);
+                else {
+                    win_sum_agents<T, true>(sm, off, x, xo, nrows, eps, a0, a1, ag, after_scan);
+                    __builtin_amdgcn_s_waitcnt(0);   // (its loads have drained the counter anyway)
                 }
-            if (bn != NONE) {
-                sm.rp[cur ^ 1][tid] = ra;
-                sm.rf[cur ^ 1][tid] = (uint16_t)rfv;
-                if (tid == 0) sm.nextq[par ^ 1] = atomicAdd(queue, 1u);
             }
-            win_sum_agents<T>(sm, off, x, xo, nrows, eps, ablk[b], ablk[b + 1], ag);   // (its barrier publishes)
             __syncthreads();
-            if (bn == NONE) break;
+            if (!hasn) break;
+            const uint32_t it = __builtin_amdgcn_readfirstlane(sm.nextq[par]);
             b = bn;
+            bn = it < nb ? it : NONE;
 #pragma unroll
             for (uint32_t u = 0; u < QU; u++) pk[u] = pkn[u];
             ag = agn;

@@ -407,16 +407,23 @@ __device__ __forceinline__ void op_accum(T xj, float w, T xi, T eps, T& num, T& 
     den = den + (acc ? (T)w : (T)0);
 }
 
+// inclusive sum over the lanes of a wave up to each lane (DPP: shifts within
+// rows of 16 lanes, then the row totals broadcast to the rows after them)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    return v;
+}
+
 // block exclusive scan of the light degrees: the thread's first slot in the
 // block's light edges and their total
 __device__ __forceinline__ uint32_t op_light_scan(uint32_t ld, uint32_t* s_wsum, uint32_t& total) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint32_t incl = ld;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y1 = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y1;
-    }
+    const uint32_t incl = wave_incl_scan(ld);
     if (lane == 63) s_wsum[wv] = incl;
     __syncthreads();
     uint32_t pre = 0;
@@ -836,12 +843,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
         uint32_t d = 0;
         if (i < a1 && i < nrows) d = r == 0 ? g.o1 - g.o0 : r == 1 ? g.o3 - g.o2 : LD ? off[i + 1] - off[i] : 0u;
         const uint32_t ld = d <= WIN_HMAX ? d : 0u;
-        uint32_t incl = ld;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t v = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += v;
-        }
+        const uint32_t incl = wave_incl_scan(ld);
         if (lane == 63) sm.wsum[r][wv] = incl;
         if (r < 2) {
             inclk[r] = incl;
@@ -871,12 +873,7 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
         } else {
             d = (i < a1 && i < nrows) ? off[i + 1] - off[i] : 0u;
             ld = d <= WIN_HMAX ? d : 0u;
-            incl = ld;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t v = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += v;
-            }
+            incl = wave_incl_scan(ld);
             xi = i < a1 ? x[i] : (T)0;
         }
         const uint32_t P = base + pre + incl - ld;

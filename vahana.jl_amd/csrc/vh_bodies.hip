@@ -698,7 +698,10 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
 // c + 2l, c + 2l + 1 and c + 128 + 2l, c + 129 + 2l (one 4-byte load of their
 // two source indices, one 2-state store: each store instruction writes one
 // contiguous range; measured 236 us against 308 us with 4 entries per lane
-// at 10^8 entries).
+// at 10^8 entries), for U steps u at a time: the loads of all U issued
+// before the stores (10^7 agents, 10^8 entries, ms per opinion step, FP64 /
+// FP32: U = 2 0.689-0.692 / 0.515-0.520, U = 4 0.670-0.674 / 0.472-0.477,
+// U = 6 0.660-0.672 / 0.455-0.461, U = 8 0.664-0.673 / 0.456-0.460).
 typedef uint32_t pu32x4 __attribute__((ext_vector_type(4)));
 
 template <class T>
@@ -718,7 +721,7 @@ __global__ __launch_bounds__(1024) void k_win_gather(const uint16_t* __restrict_
                                                      uint32_t W, uint32_t wsz, uint64_t nsrc, uint64_t ne) {
     typedef typename Pair<T>::type T2;
     constexpr uint32_t PER16 = 16 / sizeof(T);
-    constexpr int U = 2;
+    constexpr int U = 6;
     __shared__ __align__(16) T win[WinCfg<T>::WSZ];
     const uint32_t G = gridDim.x, l = blockIdx.x;
     const uint64_t j0 = ne * l / G, j1 = ne * (l + 1) / G;

@@ -945,6 +945,7 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
                                                   const uint16_t* __restrict__ cbase, uint32_t nb,
                                                   uint32_t* __restrict__ queue) {
     constexpr uint32_t QU = WinCfg<T>::QU, QA = WinCfg<T>::QA, QB = QU - QA;
+    static_assert(QA % 2 == 0 && QB % 2 == 0 && QB > 0, "chunk run counts are read in 16-bit pairs");
     __shared__ __align__(16) unsigned char smraw[sizeof(WinLDS<T>)];
     WinLDS<T>& sm = *reinterpret_cast<WinLDS<T>*>(smraw);
     const uint32_t tid = threadIdx.x;

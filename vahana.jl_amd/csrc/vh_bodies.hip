@@ -985,6 +985,9 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
         uint32_t cwv[QH / 2];
 #pragma unroll
         for (uint32_t q = 0; q < QH / 2; q++) cwv[q] = cw[U0 / 2 + q];
+        // the run table reads of all QH entries first (unconditional: run 0
+        // for lanes past the list), then the global loads
+        uint32_t rpv[QH], rfv[QH];
 #pragma unroll
         for (uint32_t k = 0; k < QH; k++) {
             const uint32_t u = U0 + k;
@@ -993,7 +996,14 @@ __global__ __launch_bounds__(1024) void k_win_sum(const uint32_t* __restrict__ o
             const uint64_t m = __ballot(ok && (pk[u] & WIN_START));
             const uint32_t cb = (cwv[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu;
             const uint32_t run = ok ? cb + __popcll(m & lemask) - 1u : 0u;
-            const uint32_t pos = ok ? sm.rp[c][run] + (e - sm.rf[c][run]) : 0u;
+            rpv[k] = sm.rp[c][run];
+            rfv[k] = sm.rf[c][run];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < QH; k++) {
+            const uint32_t e = ltid + 1024 * (U0 + k);
+            const bool ok = e < L;
+            const uint32_t pos = ok ? rpv[k] + (e - rfv[k]) : 0u;
             v[k] = y[pos];
             wt[k] = twt[lb + (ok ? e : 0u)];
         }

@@ -613,9 +613,8 @@ __global__ __launch_bounds__(256) void k_opinion_multi(const OpMember* __restric
 // its slots l, l + 64, ..., then the xor butterfly). Agents above WIN_HMAX go
 // first, one wave each, through their chunk lists. Most of the next block's
 // values, and its list, runs and agents, load while a block is summed. The
-// values are
-// exactly x[src[k]] and the sums keep the canonical order, so the results
-// are the one-pass kernel's.
+// values are exactly x[src[k]] and the sums keep the canonical order, so the
+// results are the one-pass kernel's.
 template <class T>
 struct WinCfg {
     static constexpr uint32_t WSZ = 163840 / sizeof(T);             // states per window at most

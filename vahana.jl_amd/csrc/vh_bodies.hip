@@ -419,6 +419,38 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
+// the value of lane l ^ O (O = 1, 2: DPP quad permutations; 4, 8, 16:
+// ds_swizzle in bitmask mode; 32: ds_bpermute)
+template <int O>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+    if constexpr (O == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (O == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (O <= 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (O << 10) | 0x1F);
+    else return (uint32_t)__shfl_xor((int)v, O, 64);
+}
+template <int O, class T>
+__device__ __forceinline__ T lane_xor_t(T v) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t r = (uint64_t)lane_xor<O>((uint32_t)u) | (uint64_t)lane_xor<O>((uint32_t)(u >> 32)) << 32;
+        return __builtin_bit_cast(T, r);
+    } else {
+        return __builtin_bit_cast(T, lane_xor<O>(__builtin_bit_cast(uint32_t, v)));
+    }
+}
+// p[l] + p[l ^ o] for o = 32, 16, ..., 1: every lane ends with the sum in
+// the canonical order of the wave-per-agent path
+template <class T>
+__device__ __forceinline__ T xor_butterfly(T v) {
+    v = v + lane_xor_t<32>(v);
+    v = v + lane_xor_t<16>(v);
+    v = v + lane_xor_t<8>(v);
+    v = v + lane_xor_t<4>(v);
+    v = v + lane_xor_t<2>(v);
+    v = v + lane_xor_t<1>(v);
+    return v;
+}
+
 // block exclusive scan of the light degrees: the thread's first slot in the
 // block's light edges and their total
 __device__ __forceinline__ uint32_t op_light_scan(uint32_t ld, uint32_t* s_wsum, uint32_t& total) {
@@ -534,11 +566,8 @@ __device__ __forceinline__ void opinion_heavy(const uint32_t* __restrict__ off,
         op_accum(x3, w3, xi, eps, num, den);
     }
     for (; k < e; k += 64) op_accum(x[src[k] - 1], w[k], xi, eps, num, den);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        num = num + __shfl_xor(num, o, 64);
-        den = den + __shfl_xor(den, o, 64);
-    }
+    num = xor_butterfly(num);
+    den = xor_butterfly(den);
     if (lane == 0) xo[i] = den > (T)0 ? num / den : xi;
 }
 
@@ -681,11 +710,8 @@ __device__ __forceinline__ void opinion_heavy_plan(const T* __restrict__ x, cons
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        num = num + __shfl_xor(num, o, 64);
-        den = den + __shfl_xor(den, o, 64);
-    }
+    num = xor_butterfly(num);
+    den = xor_butterfly(den);
     if (lane == 0) xo[r] = den > (T)0 ? num / den : xi;
 }
 
@@ -908,11 +934,8 @@ __device__ __forceinline__ void win_sum_agents(WinLDS<T>& sm, const uint32_t* __
         const T xi = sm.mx[m];
         T num = (T)0, den = (T)0;
         for (uint32_t q = (uint32_t)lane; q < d; q += 64) op_accum(sm.val[P + q], sm.wgt[P + q], xi, eps, num, den);
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            num = num + __shfl_xor(num, o, 64);
-            den = den + __shfl_xor(den, o, 64);
-        }
+        num = xor_butterfly(num);
+        den = xor_butterfly(den);
         if (lane == 0) xo[sm.magent[m]] = den > (T)0 ? num / den : xi;
     }
 }

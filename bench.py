@@ -166,6 +166,9 @@ def main():
                          "generation order)")
     ap.add_argument("--no-compare", action="store_true",
                     help="opinion: skip the second timing in the other agent order")
+    ap.add_argument("--one-pass", action="store_true",
+                    help="opinion: gather in one pass (option opinion_plan_bytes = -1) instead of "
+                         "through the two-pass plan")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--rank", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calib", action="store_true",
@@ -185,6 +188,8 @@ def main():
     vh = load_package()
     M = vh.models
     torch.cuda.init()
+    if args.one_pass:
+        vh.set_options(opinion_plan_bytes=-1)
     if args.workload == "ensemble":
         return bench_ensemble(vh, args)
     if args.workload == "serve":
@@ -372,7 +377,9 @@ def main():
         "dtype": dtype,
         "data": "synthetic (seeded counter-hash graph and states)",
         "config": {"workload": workload, "agents": n, "edges": m.nedges, "parallelism": "single GPU",
-                   "init_s": round(init_s, 3)},
+                   "init_s": round(init_s, 3),
+                   **({"gather": "two-pass" if two_pass else "one-pass"} if args.workload.startswith("opinion")
+                      else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": nbytes, "kernel_ms": ev_mean},

@@ -654,7 +654,7 @@ struct WinCfg {
     static constexpr uint32_t CHUNK = sizeof(T) == 8 ? 256 : 384;   // heavy-row slots staged per wave
 };
 constexpr uint32_t WIN_RUNS = 1024;   // windows at most (runs of a block)
-constexpr uint32_t WIN_HMAX = 1024;   // in-degrees staged within blocks (larger: chunk lists)
+constexpr uint32_t WIN_HMAX = 512;    // in-degrees staged within blocks (larger: chunk lists)
 constexpr uint32_t WIN_SLOT = 0x3FFFu, WIN_START = 0x8000u;
 
 // A heavy agent (one wave) from its chunk lists: the in-edges of heavy agent
